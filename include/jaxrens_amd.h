@@ -1,0 +1,205 @@
+/*
+ * jaxrens_amd -- C ABI of the MI355X (gfx950) nested-sampling inner walk.
+ *
+ * Plain pointers and sizes only; every entry point returns 0 on success and a
+ * negative JR_E_* code on failure, with a message from jr_last_error() (or
+ * jr_global_error() when no handle exists yet).  No C++ exceptions cross the
+ * ABI.  A handle owns all device buffers and one HIP stream; it is not
+ * thread-safe.  Host buffers are owned by the caller.
+ *
+ * Reference interfaces replaced (nunglert/jaxrens, src/jaxrens/...):
+ *   jr_eval_energy_forces  <- backends/lj.py:125-204 (LJBackend.__call__),
+ *                             backends/base.py:74-113 (eval_energy_and_forces),
+ *                             backends/hamiltonian.py:112-153,
+ *                             backends/ensemble.py:34-133 (EnsembleTerm)
+ *   jr_create / jr_set_population / jr_set_run_params
+ *                          <- sampling/mwg.py:40-223 (build_mwg, init_fn),
+ *                             sampling/nested_sampling.py:260-383 (init_ns)
+ *   jr_ns_iterations       <- sampling/nested_sampling.py:391-686 (ns_step,
+ *                             one call per iteration from run_loop.py:511-516)
+ *   jr_replica_exchange    <- sampling/moves/replica_exchange.py:318-436
+ *                             (replica_exchange_step + EnsembleSwap),
+ *                             sampling/inter_re_manager.py:218-270
+ *   jr_trial_moves         <- sampling/adaptation/stepsize_handler.py:215-261
+ *                             (trial moves of adjust_step_size)
+ */
+#ifndef JAXRENS_AMD_H
+#define JAXRENS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JR_OK 0
+#define JR_E_ARG (-1)
+#define JR_E_HIP (-2)
+#define JR_E_STATE (-3)
+#define JR_E_UNSUPPORTED (-4)
+
+/* move types (cli/schema/moves.py: gmc|galilean, volume, hmc,
+ * alchemical_morph) */
+#define JR_MOVE_GMC 0
+#define JR_MOVE_VOLUME 1
+#define JR_MOVE_HMC 2
+#define JR_MOVE_MORPH 3
+
+#define JR_MAX_MOVES 8
+#define JR_MAX_SPECIES 8
+#define JR_NCOUNT 7 /* proposed, rr0..rr3, n_evaluations, n_grad_evaluations */
+
+#define JR_PREC_FP64 0
+#define JR_PREC_FP32_PAIRS 1
+
+typedef struct jr_handle jr_handle;
+
+/* Pair potential: per species pair (a, b) at index a*S+b. */
+typedef struct {
+  int32_t n_species;   /* S, 1..JR_MAX_SPECIES */
+  int32_t precision;   /* JR_PREC_* */
+  double rc2;          /* squared cutoff, +inf for none */
+  const double* eps4;  /* [S*S] 4*eps_ab  */
+  const double* eps24; /* [S*S] 24*eps_ab */
+  const double* sig2;  /* [S*S] sigma_ab^2 */
+} jr_pot;
+
+typedef struct {
+  int32_t type;        /* JR_MOVE_* */
+  int32_t n_reflect;   /* gmc */
+  int32_t n_leapfrog;  /* hmc */
+  int32_t flat_v_prior;/* volume */
+  int32_t n_species;   /* alchemical_morph */
+  int32_t pad_;
+  double perturb_angle;/* gmc */
+  double max_vpa;      /* volume: cell-geometry guard */
+  double min_vpa;
+  double min_aspect;
+} jr_move;
+
+typedef struct {
+  int32_t n_replicas;    /* R */
+  int32_t n_walkers;     /* K (n_live) */
+  int32_t n_atoms_max;   /* Nmax, <= 256 */
+  int32_t n_moves;
+  int32_t n_mcmc_steps;
+  int32_t n_extra;       /* 0 .. K-2 */
+  int32_t n_cull;        /* must be 1 */
+  int32_t n_swap_cycles;
+  int32_t re_interval;   /* RE fires when i > 0 and i % re_interval == 0; 0 = never */
+  int32_t device;
+  int32_t waves_per_block; /* 0 = library default / tuned */
+  int32_t pad_;
+  uint64_t seed;
+  jr_pot pot;
+  const jr_move* moves;      /* [n_moves] */
+  const double* move_weights;/* [n_moves] relative weights */
+} jr_config;
+
+/* Host-side draw tape for n_iter iterations (parity / replay).  Layouts:
+ *   sel_u     [n_iter][R][K]
+ *   clone_u   [n_iter][R]
+ *   extra_u   [n_iter][R][K]               (NULL allowed when n_extra == 0)
+ *   schedule  [n_iter][n_steps]            move index per MWG sub-step
+ *   normals   [n_iter][R][n_walk][n_steps][3][Nmax]
+ *   uniforms  [n_iter][R][n_walk][n_steps][4]
+ *   re_u      [n_iter][n_cycles][2][R/2][2] (NULL: RE draws unused)       */
+typedef struct {
+  const double* sel_u;
+  const double* clone_u;
+  const double* extra_u;
+  const int32_t* schedule;
+  const double* normals;
+  const double* uniforms;
+  const double* re_u;
+} jr_tape;
+
+/* Per-iteration outputs for a block of n_iter iterations (any may be NULL). */
+typedef struct {
+  double* emax;         /* [n_iter][R] contour = dead energy */
+  int32_t* worst;       /* [n_iter][R] */
+  int32_t* clone;       /* [n_iter][R] */
+  double* dead_volume;  /* [n_iter][R] */
+  double* dead_pos;     /* [n_iter][R][Nmax][3] */
+  int32_t* dead_types;  /* [n_iter][R][Nmax] */
+  double* dead_cell;    /* [n_iter][R][9] */
+  int32_t* counters;    /* [n_iter][R][n_moves][JR_NCOUNT] */
+  int32_t* re_acc;      /* [n_iter][R-1] */
+  int32_t* re_att;      /* [n_iter][R-1] */
+  int32_t* walk;        /* [n_iter][R][n_walk] */
+} jr_iter_out;
+
+const char* jr_global_error(void);
+const char* jr_last_error(const jr_handle* h);
+
+/* Batched energy (+forces) of B configurations, host buffers.
+ * pos [B][Nmax][3], types [B][Nmax], n_atoms [B], cell [B][9],
+ * pressure [B] (NULL = 0), mu [B][S] (NULL = 0).
+ * Outputs: U [B] model energy, H [B] = U + P V - mu.N, F [B][Nmax][3]
+ * (NULL = energy only). */
+int jr_eval_energy_forces(const jr_pot* pot, int32_t device, int32_t B,
+                          int32_t Nmax, const double* pos,
+                          const int32_t* types, const int32_t* n_atoms,
+                          const double* cell, const double* pressure,
+                          const double* mu, double* U, double* H, double* F);
+
+int jr_create(const jr_config* cfg, jr_handle** out);
+void jr_destroy(jr_handle* h);
+
+/* Population upload; energies are evaluated on the device.  dir may be NULL
+ * (zero GMC direction).  Shapes as jr_tape with [R*K] leading. */
+int jr_set_population(jr_handle* h, const double* pos, const int32_t* types,
+                      const int32_t* n_atoms, const double* cell,
+                      const double* dir);
+int jr_get_population(jr_handle* h, double* pos, int32_t* types,
+                      int32_t* n_atoms, double* cell, double* energy,
+                      double* dir);
+/* pressure [R], mu [R][S] (NULL = 0), step_sizes [R][n_moves] */
+int jr_set_run_params(jr_handle* h, const double* pressure, const double* mu,
+                      const double* step_sizes);
+int jr_get_step_sizes(jr_handle* h, double* step_sizes);
+/* pair_valid [R-1]: RE pair (k, k+1) allowed (default: equal atom counts). */
+int jr_set_pair_valid(jr_handle* h, const int32_t* pair_valid);
+/* Outer-loop iteration index i used for the RE firing rule. */
+int jr_set_iteration(jr_handle* h, int64_t i);
+int jr_get_emax(jr_handle* h, double* emax);
+
+/* Run n_iter NS iterations (select -> clone -> walk -> RE).  tape == NULL
+ * draws everything on the device (Philox4x32-10).  One host sync at the end. */
+int jr_ns_iterations(jr_handle* h, int32_t n_iter, const jr_tape* tape,
+                     jr_iter_out* out);
+/* Same, replayed from a captured HIP graph of one iteration (device draws). */
+int jr_ns_iterations_graph(jr_handle* h, int32_t n_iter, jr_iter_out* out);
+
+/* Device draws of the most recent iteration, in jr_tape layout (n_iter=1). */
+int jr_get_last_tape(jr_handle* h, double* sel_u, double* clone_u,
+                     double* extra_u, int32_t* schedule, double* normals,
+                     double* uniforms, double* re_u);
+
+/* One replica-exchange pass on the resident population using the current
+ * emax; re_u [n_cycles][2][R/2][2] or NULL for device draws. */
+int jr_replica_exchange(jr_handle* h, const double* re_u, int32_t* acc,
+                        int32_t* att);
+
+/* One trial move per sample (population untouched).  sample_idx [R][n],
+ * trial_step [R], emax [R] (NULL = resident emax), normals
+ * [R][n][3][Nmax], uniforms [R][n][4] (both NULL = device draws).
+ * accepted/reason [R][n]. */
+int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
+                   const double* emax, int32_t n_samples,
+                   const int32_t* sample_idx, const double* normals,
+                   const double* uniforms, int32_t* accepted,
+                   int32_t* reason);
+
+/* Walk-kernel timing accumulated since the last reset (HIP events on the
+ * handle's stream): total milliseconds and launch count. */
+int jr_walk_timing(jr_handle* h, double* total_ms, int64_t* launches,
+                   int32_t reset);
+/* Number of FP64 pair evaluations the walk kernel performed since reset. */
+int jr_get_info(jr_handle* h, int32_t* n_cu, int32_t* blocks,
+                int32_t* waves_per_block);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,196 @@
+"""ctypes binding of ``libjrhip.so`` (the C ABI declared in include/jaxrens_amd.h).
+
+The library is the product path: importing this module raises if the compiled
+HIP library is missing, and no call here falls back to host arithmetic.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libjrhip.so"
+
+JR_MOVE_GMC = 0
+JR_MOVE_VOLUME = 1
+JR_MOVE_HMC = 2
+JR_MOVE_MORPH = 3
+JR_MAX_MOVES = 8
+JR_MAX_SPECIES = 8
+JR_NCOUNT = 7
+JR_PREC_FP64 = 0
+JR_PREC_FP32_PAIRS = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class JrPot(C.Structure):
+    _fields_ = [
+        ("n_species", C.c_int32),
+        ("precision", C.c_int32),
+        ("rc2", C.c_double),
+        ("eps4", _dp),
+        ("eps24", _dp),
+        ("sig2", _dp),
+    ]
+
+
+class JrMove(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("n_reflect", C.c_int32),
+        ("n_leapfrog", C.c_int32),
+        ("flat_v_prior", C.c_int32),
+        ("n_species", C.c_int32),
+        ("pad_", C.c_int32),
+        ("perturb_angle", C.c_double),
+        ("max_vpa", C.c_double),
+        ("min_vpa", C.c_double),
+        ("min_aspect", C.c_double),
+    ]
+
+
+class JrConfig(C.Structure):
+    _fields_ = [
+        ("n_replicas", C.c_int32),
+        ("n_walkers", C.c_int32),
+        ("n_atoms_max", C.c_int32),
+        ("n_moves", C.c_int32),
+        ("n_mcmc_steps", C.c_int32),
+        ("n_extra", C.c_int32),
+        ("n_cull", C.c_int32),
+        ("n_swap_cycles", C.c_int32),
+        ("re_interval", C.c_int32),
+        ("device", C.c_int32),
+        ("waves_per_block", C.c_int32),
+        ("pad_", C.c_int32),
+        ("seed", C.c_uint64),
+        ("pot", JrPot),
+        ("moves", C.POINTER(JrMove)),
+        ("move_weights", _dp),
+    ]
+
+
+class JrTape(C.Structure):
+    _fields_ = [
+        ("sel_u", _dp),
+        ("clone_u", _dp),
+        ("extra_u", _dp),
+        ("schedule", _ip),
+        ("normals", _dp),
+        ("uniforms", _dp),
+        ("re_u", _dp),
+    ]
+
+
+class JrIterOut(C.Structure):
+    _fields_ = [
+        ("emax", _dp),
+        ("worst", _ip),
+        ("clone", _ip),
+        ("dead_volume", _dp),
+        ("dead_pos", _dp),
+        ("dead_types", _ip),
+        ("dead_cell", _dp),
+        ("counters", _ip),
+        ("re_acc", _ip),
+        ("re_att", _ip),
+        ("walk", _ip),
+    ]
+
+
+# (name, restype, argtypes) of every exported symbol; the header declares
+# exactly these.
+SIGNATURES = [
+    ("jr_global_error", C.c_char_p, []),
+    ("jr_last_error", C.c_char_p, [C.c_void_p]),
+    ("jr_eval_energy_forces", C.c_int,
+     [C.POINTER(JrPot), C.c_int32, C.c_int32, C.c_int32, _dp, _ip, _ip, _dp,
+      _dp, _dp, _dp, _dp, _dp]),
+    ("jr_create", C.c_int, [C.POINTER(JrConfig), C.POINTER(C.c_void_p)]),
+    ("jr_destroy", None, [C.c_void_p]),
+    ("jr_set_population", C.c_int, [C.c_void_p, _dp, _ip, _ip, _dp, _dp]),
+    ("jr_get_population", C.c_int,
+     [C.c_void_p, _dp, _ip, _ip, _dp, _dp, _dp]),
+    ("jr_set_run_params", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("jr_get_step_sizes", C.c_int, [C.c_void_p, _dp]),
+    ("jr_set_pair_valid", C.c_int, [C.c_void_p, _ip]),
+    ("jr_set_iteration", C.c_int, [C.c_void_p, C.c_int64]),
+    ("jr_get_emax", C.c_int, [C.c_void_p, _dp]),
+    ("jr_ns_iterations", C.c_int,
+     [C.c_void_p, C.c_int32, C.POINTER(JrTape), C.POINTER(JrIterOut)]),
+    ("jr_ns_iterations_graph", C.c_int,
+     [C.c_void_p, C.c_int32, C.POINTER(JrIterOut)]),
+    ("jr_get_last_tape", C.c_int,
+     [C.c_void_p, _dp, _dp, _dp, _ip, _dp, _dp, _dp]),
+    ("jr_replica_exchange", C.c_int, [C.c_void_p, _dp, _ip, _ip]),
+    ("jr_trial_moves", C.c_int,
+     [C.c_void_p, C.c_int32, _dp, _dp, C.c_int32, _ip, _dp, _dp, _ip, _ip]),
+    ("jr_walk_timing", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]),
+    ("jr_get_info", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+      C.POINTER(C.c_int32)]),
+]
+
+_LIB = None
+
+
+def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libjrhip.so and declare every symbol; raises if it is missing."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise ImportError(
+            f"{p} not found: the HIP library is required (no host fallback). "
+            "Build it with `python -m jaxrens_amd.build`."
+        )
+    lib = C.CDLL(str(p))
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+class JrError(RuntimeError):
+    pass
+
+
+def check(rc: int, handle=None) -> None:
+    if rc != 0:
+        lib = load_library()
+        msg = (lib.jr_last_error(handle) if handle is not None
+               else lib.jr_global_error())
+        raise JrError(f"libjrhip error {rc}: {msg.decode() if msg else ''}")
+
+
+def dptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+def f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def i32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))

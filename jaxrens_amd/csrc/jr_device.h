@@ -1,0 +1,190 @@
+// Device-side building blocks for the gfx950 nested-sampling walk.
+//
+// Wave64 layout: one wavefront owns one walker; ring position p = lane + 64*s
+// (s < A slots per lane) holds atom p.  All reductions over atoms use the
+// same canonical order: per-lane partial over slots in ascending order, then
+// an xor butterfly over m = 1, 2, 4, 8, 16, 32.  Arithmetic is compiled with
+// -ffp-contract=off; only the fma() calls written here fuse.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define JR_WAVE 64
+
+namespace jr {
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ __forceinline__ double dot3(double a0, double a1, double a2,
+                                       double b0, double b1, double b2) {
+  return fma(a2, b2, fma(a1, b1, a0 * b0));
+}
+
+// Butterfly sum over the wave; every lane ends with the same value.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 1; m < JR_WAVE; m <<= 1) v = v + __shfl_xor(v, m, JR_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double uniform_d(double v) {
+  int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+  int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+// dst[lane] = src[(lane + 1) % 64]  (DPP wave_rol:1, a VALU move).
+__device__ __forceinline__ double wave_rol1(double v) {
+  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x134, 0xF, 0xF,
+                                       false);
+  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x134, 0xF, 0xF,
+                                       false);
+  return __hiloint2double(hi, lo);
+}
+
+// Ring rotation over positions p < N: new[p] = old[(p + 1) % N].
+template <int A>
+__device__ __forceinline__ void ring_rot1(double (&v)[A], int lane, int N) {
+  const bool partial = (N & 63) != 0;
+  double w0 = 0.0;
+  if (partial) w0 = readlane_d(v[0], 0);
+  double r[A];
+#pragma unroll
+  for (int s = 0; s < A; ++s) r[s] = wave_rol1(v[s]);
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    if (A > 1)
+      v[s] = (lane == 63) ? r[(s + 1) % A] : r[s];
+    else
+      v[s] = r[s];
+  }
+  if (partial) {
+    const int ln = (N - 1) & 63, sn = (N - 1) >> 6;
+#pragma unroll
+    for (int s = 0; s < A; ++s)
+      if (s == sn && lane == ln) v[s] = w0;
+  }
+}
+
+// Orders LDS traffic of one wavefront (LDS ops of a wave complete in order;
+// this keeps the compiler from moving them across the point).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------- cell math
+__device__ __forceinline__ double det3(const double* c) {
+  double m0 = c[4] * c[8] - c[5] * c[7];
+  double m1 = c[3] * c[8] - c[5] * c[6];
+  double m2 = c[3] * c[7] - c[4] * c[6];
+  return c[0] * m0 - c[1] * m1 + c[2] * m2;
+}
+
+__device__ __forceinline__ void inv3(const double* c, double det,
+                                     double* inv) {
+  double idet = 1.0 / det;
+  inv[0] = (c[4] * c[8] - c[5] * c[7]) * idet;
+  inv[1] = (c[2] * c[7] - c[1] * c[8]) * idet;
+  inv[2] = (c[1] * c[5] - c[2] * c[4]) * idet;
+  inv[3] = (c[5] * c[6] - c[3] * c[8]) * idet;
+  inv[4] = (c[0] * c[8] - c[2] * c[6]) * idet;
+  inv[5] = (c[2] * c[3] - c[0] * c[5]) * idet;
+  inv[6] = (c[3] * c[7] - c[4] * c[6]) * idet;
+  inv[7] = (c[1] * c[6] - c[0] * c[7]) * idet;
+  inv[8] = (c[0] * c[4] - c[1] * c[3]) * idet;
+}
+
+// Cube root: integer guess on the IEEE bits, five Newton steps
+// y <- (2y + x/y^2)/3.  Deterministic (IEEE +,*,/ only).
+__device__ __forceinline__ double cbrt_c(double x) {
+  if (!(x > 0.0) || __builtin_isinf(x)) return x;
+  const long long bias = (long long)1023 << 52;
+  long long b = __double_as_longlong(x);
+  long long g = (b - bias) / 3 + bias;
+  double y = __longlong_as_double(g);
+#pragma unroll
+  for (int it = 0; it < 5; ++it) {
+    double t = y * y;
+    y = (2.0 * y + x / t) / 3.0;
+  }
+  return y;
+}
+
+__device__ __forceinline__ double ipow_c(double x, int n) {
+  double r = 1.0, b = x;
+  while (n > 0) {
+    if (n & 1) r = r * b;
+    b = b * b;
+    n >>= 1;
+  }
+  return r;
+}
+
+// utils/cell.py:80-125 (reference): volume-per-atom band and min aspect.
+__device__ __forceinline__ bool check_cell_shape(const double* c, int n_atoms,
+                                                 double max_vpa,
+                                                 double min_vpa,
+                                                 double min_aspect) {
+  double vol = fabs(det3(c));
+  double vpa = vol / (double)n_atoms;
+  double vc = cbrt_c(vol);
+  const int P0[3] = {0, 1, 0}, P1[3] = {1, 2, 2}, P2[3] = {2, 0, 1};
+  double aspect = 0.0;
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const double* a = c + 3 * P0[p];
+    const double* b = c + 3 * P1[p];
+    const double* k = c + 3 * P2[p];
+    double cx = a[1] * b[2] - a[2] * b[1];
+    double cy = a[2] * b[0] - a[0] * b[2];
+    double cz = a[0] * b[1] - a[1] * b[0];
+    double nrm = sqrt(dot3(cx, cy, cz, cx, cy, cz));
+    double h = fabs(dot3(cx / nrm, cy / nrm, cz / nrm, k[0], k[1], k[2]));
+    double ratio = h / vc;
+    aspect = (p == 0) ? ratio : fmin(aspect, ratio);
+  }
+  return (vpa <= max_vpa) && (vpa >= min_vpa) && (aspect >= min_aspect);
+}
+
+// ----------------------------------------------------------- Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11.  Counter-based: (counter, key) -> 4 x 32 bits.
+__host__ __device__ __forceinline__ void philox4x32_10(uint32_t c[4],
+                                                       uint32_t k0,
+                                                       uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += W0;
+      k1 += W1;
+    }
+    uint64_t p0 = (uint64_t)M0 * c[0];
+    uint64_t p1 = (uint64_t)M1 * c[2];
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c[1] ^ k0;
+    uint32_t n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+  }
+}
+
+// 53-bit uniform in [0, 1) from two 32-bit words.
+__host__ __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
+  uint64_t v = (((uint64_t)hi << 32) | (uint64_t)lo) >> 11;
+  return (double)v * 0x1.0p-53;
+}
+
+}  // namespace jr

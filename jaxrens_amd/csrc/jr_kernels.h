@@ -1,0 +1,700 @@
+// Kernels of one NS iteration: draws (Philox), select/clone, persistent walk,
+// replica exchange, and the batched energy evaluator.
+#pragma once
+
+#include "jr_moves.h"
+
+namespace jr {
+
+// Device-resident iteration state, read by every kernel of an iteration so a
+// captured graph can be replayed without parameter updates.
+struct IterState {
+  long long i;         // outer-loop iteration index (RE firing rule)
+  long long rng_iter;  // monotone counter feeding the Philox counter
+  int block_pos;       // slot in the per-block output buffers
+  int re_interval;
+};
+
+struct PopPtrs {
+  double* pos;
+  int* types;
+  int* n_atoms;
+  double* cell;
+  double* energy;
+  double* dir;
+};
+
+struct WalkArgs {
+  PopPtrs pop;
+  const double* pressure;
+  const double* mu;
+  const double* step_sizes;
+  const double* emax;
+  const int* walk;
+  int n_walk, R, K, Nmax, S, n_steps, n_moves;
+  const int* schedule;
+  const double* normals;
+  const double* uniforms;
+  int* counters;  // [R][n_moves][JR_NCOUNT]
+  int* queue;
+  int total;
+  const int* order;  // replica processing order (largest N first)
+  double rc2;
+  const double* eps4;
+  const double* eps24;
+  const double* sig2;
+  const DevMove* moves;  // [n_moves], global (uniform scalar loads)
+  // trial mode (step-size adaptation): one move per sampled walker
+  int trial;
+  int trial_move;
+  int n_samples;
+  const int* sample_idx;
+  const double* trial_step;
+  int* trial_reason;
+};
+
+__host__ __device__ inline size_t wave_lds_bytes(int Nmax, int n_moves) {
+  (void)n_moves;
+  size_t b = (size_t)Nmax * (6 * sizeof(double) + sizeof(int)) +
+             (size_t)JR_MAX_MOVES * JR_NCOUNT * sizeof(int);
+  return (b + 15) & ~(size_t)15;
+}
+
+template <int A, bool MULTI>
+__global__ void __launch_bounds__(256)
+    walk_kernel(WalkArgs a) {
+  extern __shared__ double lds_raw[];
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const int nwave_blk = blockDim.x >> 6;
+  const size_t per_wave = wave_lds_bytes(a.Nmax, a.n_moves);
+  char* base = reinterpret_cast<char*>(lds_raw) + per_wave * wave;
+  double* tabs = reinterpret_cast<double*>(
+      reinterpret_cast<char*>(lds_raw) + per_wave * nwave_blk);
+  WaveLds L;
+  L.x = reinterpret_cast<double*>(base);
+  L.y = L.x + a.Nmax;
+  L.z = L.y + a.Nmax;
+  L.hx = L.z + a.Nmax;
+  L.hy = L.hx + a.Nmax;
+  L.hz = L.hy + a.Nmax;
+  L.t = reinterpret_cast<int*>(L.hz + a.Nmax);
+  int* lctr = L.t + a.Nmax;
+
+  // pair table in LDS (block-shared)
+  const int SS = a.S * a.S;
+  for (int i = threadIdx.x; i < SS; i += blockDim.x) {
+    tabs[i] = a.eps4[i];
+    tabs[SS + i] = a.eps24[i];
+    tabs[2 * SS + i] = a.sig2[i];
+  }
+  for (int i = lane; i < JR_MAX_MOVES * JR_NCOUNT; i += 64) lctr[i] = 0;
+  __syncthreads();
+
+  WalkCtx C;
+  C.L = L;
+  C.tab.rc2 = a.rc2;
+  C.tab.eps4 = tabs;
+  C.tab.eps24 = tabs + SS;
+  C.tab.sig2 = tabs + 2 * SS;
+  C.tab.S = a.S;
+  C.Nmax = a.Nmax;
+  C.S = a.S;
+
+  for (;;) {
+    int q = 0;
+    if (lane == 0) q = atomicAdd(a.queue, 1);
+    q = __builtin_amdgcn_readfirstlane(q);
+    if (q >= a.total) break;
+    int r, slot, tq;
+    if (a.trial) {
+      r = q / a.n_samples;
+      slot = r * a.K + a.sample_idx[q];
+      tq = q;
+    } else {
+      const int rr = q / a.n_walk, c = q - rr * a.n_walk;
+      r = a.order ? a.order[rr] : rr;
+      slot = r * a.K + a.walk[r * a.n_walk + c];
+      tq = r * a.n_walk + c;
+    }
+    const int N = a.pop.n_atoms[slot];
+    C.N = N;
+    C.pressure = a.pressure[r];
+    C.mu = a.mu + (size_t)r * a.S;
+    C.emax = a.emax[r];
+
+    WalkerRegs<A> W;
+    const double* P = a.pop.pos + (size_t)slot * a.Nmax * 3;
+    const double* D = a.pop.dir + (size_t)slot * a.Nmax * 3;
+    const int* T = a.pop.types + (size_t)slot * a.Nmax;
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      const int at = lane + 64 * s;
+      if (at < N) {
+        W.x[s] = P[3 * at + 0];
+        W.y[s] = P[3 * at + 1];
+        W.z[s] = P[3 * at + 2];
+        W.dx[s] = D[3 * at + 0];
+        W.dy[s] = D[3 * at + 1];
+        W.dz[s] = D[3 * at + 2];
+        W.t[s] = T[at];
+      } else {
+        W.x[s] = W.y[s] = W.z[s] = 0.0;
+        W.dx[s] = W.dy[s] = W.dz[s] = 0.0;
+        W.t[s] = 0;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) W.cell[i] = a.pop.cell[(size_t)slot * 9 + i];
+    W.E = a.pop.energy[slot];
+    species_counts<A>(N, a.S, W.t, W.counts);
+
+    const int n_steps = a.trial ? 1 : a.n_steps;
+    int last_reason = 0;
+    for (int t = 0; t < n_steps; ++t) {
+      const int m = a.trial ? a.trial_move : a.schedule[t];
+      const DevMove mv = a.moves[m];
+      const double step =
+          a.trial ? a.trial_step[r] : a.step_sizes[(size_t)r * a.n_moves + m];
+      const double* xi =
+          a.normals + ((size_t)tq * n_steps + t) * 3 * (size_t)a.Nmax;
+      const double* u = a.uniforms + ((size_t)tq * n_steps + t) * 4;
+      int reason = 0, n_eval = 1, n_grad = 0;
+      switch (mv.type) {
+        case JR_MOVE_GMC:
+          reason = gmc_move<A, MULTI>(C, W, step, mv, xi) ? 0 : 1;
+          n_eval = n_grad = mv.n_reflect;
+          break;
+        case JR_MOVE_VOLUME:
+          reason = volume_move<A, MULTI>(C, W, step, mv, xi[0], u[0]);
+          break;
+        case JR_MOVE_HMC:
+          reason = hmc_move<A, MULTI>(C, W, step, mv, xi) ? 0 : 1;
+          n_eval = 2 * mv.n_leapfrog + 1;
+          n_grad = 2 * mv.n_leapfrog;
+          break;
+        case JR_MOVE_MORPH:
+          reason = morph_move<A, MULTI>(C, W, mv, u[0], u[1]) ? 0 : 1;
+          break;
+        default:
+          break;
+      }
+      last_reason = reason;
+      if (lane == 0) {
+        int* c = lctr + m * JR_NCOUNT;
+        c[0] += 1;
+        c[1 + reason] += 1;
+        c[5] += n_eval;
+        c[6] += n_grad;
+      }
+    }
+    if (a.trial) {
+      if (lane == 0) a.trial_reason[q] = last_reason;
+      continue;
+    }
+    double* Po = a.pop.pos + (size_t)slot * a.Nmax * 3;
+    double* Do = a.pop.dir + (size_t)slot * a.Nmax * 3;
+    int* To = a.pop.types + (size_t)slot * a.Nmax;
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      const int at = lane + 64 * s;
+      if (at < N) {
+        Po[3 * at + 0] = W.x[s];
+        Po[3 * at + 1] = W.y[s];
+        Po[3 * at + 2] = W.z[s];
+        Do[3 * at + 0] = W.dx[s];
+        Do[3 * at + 1] = W.dy[s];
+        Do[3 * at + 2] = W.dz[s];
+        To[at] = W.t[s];
+      }
+    }
+    if (lane < 9) {
+      double cv = W.cell[0];
+#pragma unroll
+      for (int i = 1; i < 9; ++i)
+        if (lane == i) cv = W.cell[i];
+      a.pop.cell[(size_t)slot * 9 + lane] = cv;
+    }
+    if (lane == 0) a.pop.energy[slot] = W.E;
+    // counters are per replica: flush this chain's tallies
+    wave_lds_sync();
+    for (int i = lane; i < a.n_moves * JR_NCOUNT; i += 64) {
+      int v = lctr[i];
+      if (v) atomicAdd(a.counters + (size_t)r * a.n_moves * JR_NCOUNT + i, v);
+      lctr[i] = 0;
+    }
+    wave_lds_sync();
+  }
+}
+
+// Batched energy (+forces): one wave per configuration.
+struct EvalArgs {
+  const double* pos;
+  const int* types;
+  const int* n_atoms;
+  const double* cell;
+  const double* pressure;
+  const double* mu;
+  double* U;
+  double* H;
+  double* F;
+  int B, Nmax, S;
+  double rc2;
+  const double* eps4;
+  const double* eps24;
+  const double* sig2;
+};
+
+template <int A, bool MULTI>
+__global__ void __launch_bounds__(256) eval_kernel(EvalArgs a) {
+  extern __shared__ double lds_raw[];
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const int nwave_blk = blockDim.x >> 6;
+  const size_t per_wave = wave_lds_bytes(a.Nmax, 0);
+  char* base = reinterpret_cast<char*>(lds_raw) + per_wave * wave;
+  double* tabs = reinterpret_cast<double*>(
+      reinterpret_cast<char*>(lds_raw) + per_wave * nwave_blk);
+  WaveLds L;
+  L.x = reinterpret_cast<double*>(base);
+  L.y = L.x + a.Nmax;
+  L.z = L.y + a.Nmax;
+  L.hx = L.z + a.Nmax;
+  L.hy = L.hx + a.Nmax;
+  L.hz = L.hy + a.Nmax;
+  L.t = reinterpret_cast<int*>(L.hz + a.Nmax);
+  const int SS = a.S * a.S;
+  for (int i = threadIdx.x; i < SS; i += blockDim.x) {
+    tabs[i] = a.eps4[i];
+    tabs[SS + i] = a.eps24[i];
+    tabs[2 * SS + i] = a.sig2[i];
+  }
+  __syncthreads();
+  PairTab tab;
+  tab.rc2 = a.rc2;
+  tab.eps4 = tabs;
+  tab.eps24 = tabs + SS;
+  tab.sig2 = tabs + 2 * SS;
+  tab.S = a.S;
+
+  const int b = blockIdx.x * nwave_blk + wave;
+  if (b >= a.B) return;
+  const int N = a.n_atoms[b];
+  double x[A], y[A], z[A];
+  int t[A];
+  const double* P = a.pos + (size_t)b * a.Nmax * 3;
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    const int at = lane + 64 * s;
+    if (at < N) {
+      x[s] = P[3 * at];
+      y[s] = P[3 * at + 1];
+      z[s] = P[3 * at + 2];
+      t[s] = a.types[(size_t)b * a.Nmax + at];
+    } else {
+      x[s] = y[s] = z[s] = 0.0;
+      t[s] = 0;
+    }
+  }
+  double cell[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) cell[i] = a.cell[(size_t)b * 9 + i];
+  Geom g;
+  make_geom(cell, g);
+  double el, fx[A], fy[A], fz[A];
+  if (a.F)
+    wave_eval<A, true, MULTI>(L, N, g, tab, x, y, z, t, el, fx, fy, fz);
+  else
+    wave_eval<A, false, MULTI>(L, N, g, tab, x, y, z, t, el, fx, fy, fz);
+  double U = wave_sum((lane < N) ? el : 0.0);
+  int counts[JR_MAX_SPECIES];
+  species_counts<A>(N, a.S, t, counts);
+  const double pr = a.pressure ? a.pressure[b] : 0.0;
+  double Hv;
+  if (a.mu) {
+    Hv = U + ensemble_term(cell, pr, a.mu + (size_t)b * a.S, a.S, counts);
+  } else {
+    Hv = U + pr * fabs(det3(cell));
+  }
+  if (lane == 0) {
+    if (a.U) a.U[b] = U;
+    if (a.H) a.H[b] = Hv;
+  }
+  if (a.F) {
+    double* Fo = a.F + (size_t)b * a.Nmax * 3;
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      const int at = lane + 64 * s;
+      if (at < N) {
+        Fo[3 * at] = fx[s];
+        Fo[3 * at + 1] = fy[s];
+        Fo[3 * at + 2] = fz[s];
+      } else if (at < a.Nmax) {
+        Fo[3 * at] = Fo[3 * at + 1] = Fo[3 * at + 2] = 0.0;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ select
+struct SelArgs {
+  PopPtrs pop;
+  int R, K, Nmax, n_extra, n_walk, n_moves, Kpad;
+  const double* sel_u;
+  const double* clone_u;
+  const double* extra_u;
+  double* emax;
+  int* walk;
+  int* counters;
+  int* queue;
+  IterState* st;
+  // per-block outputs (indexed by st->block_pos)
+  double* o_emax;
+  int* o_worst;
+  int* o_clone;
+  double* o_dead_volume;
+  double* o_dead_pos;
+  int* o_dead_types;
+  double* o_dead_cell;
+};
+
+__device__ __forceinline__ bool key_before(double ka, int ia, double kb,
+                                           int ib) {
+  return (ka > kb) || (ka == kb && ia < ib);
+}
+
+// nested_sampling.py:86-133,151-186 (n_cull = 1) and :486-518 (dead record,
+// clone into the culled slot).  One block per replica.
+__global__ void __launch_bounds__(1024) select_kernel(SelArgs a) {
+  extern __shared__ double sh[];
+  double* skey = sh;                                   // [Kpad]
+  int* sidx = reinterpret_cast<int*>(sh + a.Kpad);     // [Kpad]
+  __shared__ double red_d[32];
+  __shared__ int red_i[32];
+  __shared__ int s_worst, s_clone;
+  __shared__ double s_mx;
+  const int r = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  const int lane = tid & 63, wv = tid >> 6, nwv = nt >> 6;
+  const double* E = a.pop.energy + (size_t)r * a.K;
+
+  // 1. contour = max energy
+  double mx = -__builtin_inf();
+  for (int k = tid; k < a.K; k += nt) mx = fmax(mx, E[k]);
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) mx = fmax(mx, __shfl_xor(mx, m, 64));
+  if (lane == 0) red_d[wv] = mx;
+  __syncthreads();
+  if (tid == 0) {
+    double v = red_d[0];
+    for (int w = 1; w < nwv; ++w) v = fmax(v, red_d[w]);
+    s_mx = v;
+  }
+  __syncthreads();
+  mx = s_mx;
+  // 2. uniform tie-break among exact maxima (first index on equal noise)
+  const double* su = a.sel_u + (size_t)r * a.K;
+  double bk = -__builtin_inf();
+  int bi = 0x7fffffff;
+  for (int k = tid; k < a.K; k += nt) {
+    if (E[k] == mx) {
+      double kk = su[k];
+      if (key_before(kk, k, bk, bi)) {
+        bk = kk;
+        bi = k;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    double ok = __shfl_xor(bk, m, 64);
+    int oi = __shfl_xor(bi, m, 64);
+    if (key_before(ok, oi, bk, bi)) {
+      bk = ok;
+      bi = oi;
+    }
+  }
+  if (lane == 0) {
+    red_d[wv] = bk;
+    red_i[wv] = bi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double k0 = red_d[0];
+    int i0 = red_i[0];
+    for (int w = 1; w < nwv; ++w)
+      if (key_before(red_d[w], red_i[w], k0, i0)) {
+        k0 = red_d[w];
+        i0 = red_i[w];
+      }
+    int w = i0;
+    int cl = (int)(a.clone_u[r] * (double)(a.K - 1));
+    if (cl > a.K - 2) cl = a.K - 2;
+    if (cl >= w) cl += 1;
+    s_worst = w;
+    s_clone = cl;
+  }
+  __syncthreads();
+  const int w = s_worst, cl = s_clone;
+  // 3. extras: stable descending sort of noise with worst/clone excluded
+  if (a.n_extra > 0) {
+    const double* eu = a.extra_u + (size_t)r * a.K;
+    for (int k = tid; k < a.Kpad; k += nt) {
+      if (k < a.K) {
+        skey[k] = (k == w || k == cl) ? -__builtin_inf() : eu[k];
+        sidx[k] = k;
+      } else {
+        skey[k] = -__builtin_inf();
+        sidx[k] = k;
+      }
+    }
+    __syncthreads();
+    for (int size = 2; size <= a.Kpad; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int i = tid; i < a.Kpad; i += nt) {
+          int j = i ^ stride;
+          if (j > i) {
+            bool up = ((i & size) == 0);
+            double ki = skey[i], kj = skey[j];
+            int ii = sidx[i], ij = sidx[j];
+            bool swap = up ? key_before(kj, ij, ki, ii)
+                           : key_before(ki, ii, kj, ij);
+            if (swap) {
+              skey[i] = kj;
+              skey[j] = ki;
+              sidx[i] = ij;
+              sidx[j] = ii;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int e = tid; e < a.n_extra; e += nt)
+      a.walk[(size_t)r * a.n_walk + 1 + e] = sidx[e];
+  }
+  const int bp = a.st->block_pos;
+  const size_t ws = (size_t)r * a.K + w, cs = (size_t)r * a.K + cl;
+  if (tid == 0) {
+    a.walk[(size_t)r * a.n_walk] = w;
+    a.emax[r] = mx;
+    const size_t o = (size_t)bp * a.R + r;
+    a.o_emax[o] = mx;
+    a.o_worst[o] = w;
+    a.o_clone[o] = cl;
+    a.o_dead_volume[o] = fabs(det3(a.pop.cell + ws * 9));
+  }
+  for (int i = tid; i < a.n_moves * JR_NCOUNT; i += nt)
+    a.counters[(size_t)r * a.n_moves * JR_NCOUNT + i] = 0;
+  if (r == 0 && tid == 0) *a.queue = 0;
+  if (a.o_dead_pos) {
+    const size_t o = (size_t)bp * a.R + r;
+    for (int i = tid; i < a.Nmax * 3; i += nt)
+      a.o_dead_pos[o * a.Nmax * 3 + i] = a.pop.pos[ws * a.Nmax * 3 + i];
+    for (int i = tid; i < a.Nmax; i += nt)
+      a.o_dead_types[o * a.Nmax + i] = a.pop.types[ws * a.Nmax + i];
+    for (int i = tid; i < 9; i += nt)
+      a.o_dead_cell[o * 9 + i] = a.pop.cell[ws * 9 + i];
+  }
+  __syncthreads();
+  // 4. clone the survivor into the culled slot (all walker-bound fields)
+  for (int i = tid; i < a.Nmax * 3; i += nt) {
+    a.pop.pos[ws * a.Nmax * 3 + i] = a.pop.pos[cs * a.Nmax * 3 + i];
+    a.pop.dir[ws * a.Nmax * 3 + i] = a.pop.dir[cs * a.Nmax * 3 + i];
+  }
+  for (int i = tid; i < a.Nmax; i += nt)
+    a.pop.types[ws * a.Nmax + i] = a.pop.types[cs * a.Nmax + i];
+  if (tid < 9) a.pop.cell[ws * 9 + tid] = a.pop.cell[cs * 9 + tid];
+  if (tid == 0) {
+    a.pop.energy[ws] = a.pop.energy[cs];
+    a.pop.n_atoms[ws] = a.pop.n_atoms[cs];
+  }
+}
+
+// ------------------------------------------------------ replica exchange
+struct ReArgs {
+  PopPtrs pop;
+  int R, K, Nmax, S, n_cycles, max_pairs;
+  const double* pressure;
+  const double* mu;
+  const double* emax;
+  const int* pair_valid;
+  const double* re_u;
+  int* acc;  // [R-1] (per-block output row or standalone)
+  int* att;
+  IterState* st;  // NULL: always fire (standalone pass)
+  int* o_acc;     // per-block outputs
+  int* o_att;
+};
+
+__device__ double walker_ensemble(const PopPtrs& p, size_t slot, int Nmax,
+                                  int S, double P, const double* mu) {
+  int counts[JR_MAX_SPECIES];
+#pragma unroll
+  for (int s = 0; s < JR_MAX_SPECIES; ++s) counts[s] = 0;
+  const int N = p.n_atoms[slot];
+  for (int i = 0; i < N; ++i) {
+    int ty = p.types[slot * Nmax + i];
+#pragma unroll
+    for (int s = 0; s < JR_MAX_SPECIES; ++s) counts[s] += (ty == s) ? 1 : 0;
+  }
+  return ensemble_term(p.cell + slot * 9, P, mu, S, counts);
+}
+
+// sampling/moves/replica_exchange.py:318-436 with EnsembleSwap (:120-164).
+__global__ void __launch_bounds__(256) re_kernel(ReArgs a) {
+  __shared__ int s_ok, s_si, s_sj;
+  __shared__ double s_ei, s_ej;
+  const int tid = threadIdx.x;
+  int* acc = a.acc;
+  int* att = a.att;
+  if (a.st) {
+    const long long i = a.st->i;
+    const bool fires = a.st->re_interval > 0 && i > 0 &&
+                       (i % a.st->re_interval) == 0;
+    const int bp = a.st->block_pos;
+    acc = a.o_acc + (size_t)bp * (a.R - 1);
+    att = a.o_att + (size_t)bp * (a.R - 1);
+    for (int k = tid; k < a.R - 1; k += blockDim.x) {
+      acc[k] = 0;
+      att[k] = 0;
+    }
+    __syncthreads();
+    if (!fires) return;
+  } else {
+    for (int k = tid; k < a.R - 1; k += blockDim.x) {
+      acc[k] = 0;
+      att[k] = 0;
+    }
+    __syncthreads();
+  }
+  for (int cyc = 0; cyc < a.n_cycles; ++cyc) {
+    for (int ph = 0; ph < 2; ++ph) {
+      int pi = 0;
+      for (int i = ph; i + 1 < a.R; i += 2, ++pi) {
+        const int j = i + 1;
+        if (!a.pair_valid[i]) continue;
+        if (tid == 0) {
+          const double* u =
+              a.re_u + (((size_t)cyc * 2 + ph) * a.max_pairs + pi) * 2;
+          int wi = (int)(u[0] * (double)a.K);
+          if (wi > a.K - 1) wi = a.K - 1;
+          int wj = (int)(u[1] * (double)a.K);
+          if (wj > a.K - 1) wj = a.K - 1;
+          const size_t si = (size_t)i * a.K + wi, sj = (size_t)j * a.K + wj;
+          const double* mui = a.mu + (size_t)i * a.S;
+          const double* muj = a.mu + (size_t)j * a.S;
+          double ui = a.pop.energy[sj] - walker_ensemble(a.pop, sj, a.Nmax,
+                                                         a.S, a.pressure[j],
+                                                         muj);
+          double into_i = ui + walker_ensemble(a.pop, sj, a.Nmax, a.S,
+                                               a.pressure[i], mui);
+          double uj = a.pop.energy[si] - walker_ensemble(a.pop, si, a.Nmax,
+                                                         a.S, a.pressure[i],
+                                                         mui);
+          double into_j = uj + walker_ensemble(a.pop, si, a.Nmax, a.S,
+                                               a.pressure[j], muj);
+          int ok = (into_i < a.emax[i]) && (into_j < a.emax[j]);
+          att[i] += 1;
+          acc[i] += ok;
+          s_ok = ok;
+          s_si = (int)si;
+          s_sj = (int)sj;
+          s_ei = into_i;
+          s_ej = into_j;
+        }
+        __syncthreads();
+        if (s_ok) {
+          const size_t si = (size_t)s_si, sj = (size_t)s_sj;
+          for (int k = tid; k < a.Nmax * 3; k += blockDim.x) {
+            double p0 = a.pop.pos[si * a.Nmax * 3 + k];
+            a.pop.pos[si * a.Nmax * 3 + k] = a.pop.pos[sj * a.Nmax * 3 + k];
+            a.pop.pos[sj * a.Nmax * 3 + k] = p0;
+            double d0 = a.pop.dir[si * a.Nmax * 3 + k];
+            a.pop.dir[si * a.Nmax * 3 + k] = a.pop.dir[sj * a.Nmax * 3 + k];
+            a.pop.dir[sj * a.Nmax * 3 + k] = d0;
+          }
+          for (int k = tid; k < a.Nmax; k += blockDim.x) {
+            int t0 = a.pop.types[si * a.Nmax + k];
+            a.pop.types[si * a.Nmax + k] = a.pop.types[sj * a.Nmax + k];
+            a.pop.types[sj * a.Nmax + k] = t0;
+          }
+          if (tid < 9) {
+            double c0 = a.pop.cell[si * 9 + tid];
+            a.pop.cell[si * 9 + tid] = a.pop.cell[sj * 9 + tid];
+            a.pop.cell[sj * 9 + tid] = c0;
+          }
+          if (tid == 0) {
+            int n0 = a.pop.n_atoms[si];
+            a.pop.n_atoms[si] = a.pop.n_atoms[sj];
+            a.pop.n_atoms[sj] = n0;
+            a.pop.energy[si] = s_ei;
+            a.pop.energy[sj] = s_ej;
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ draws
+// Philox counter = (element-pair index lo, hi, rng iteration, stream).
+__global__ void fill_uniform_kernel(double* out, long long n,
+                                    unsigned long long seed,
+                                    const IterState* st, long long it_fixed,
+                                    unsigned stream) {
+  const long long it = st ? st->rng_iter : it_fixed;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+       2 * p < n; p += (long long)gridDim.x * blockDim.x) {
+    uint32_t c[4] = {(uint32_t)p, (uint32_t)(p >> 32), (uint32_t)it, stream};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    out[2 * p] = u53(c[0], c[1]);
+    if (2 * p + 1 < n) out[2 * p + 1] = u53(c[2], c[3]);
+  }
+}
+
+__global__ void fill_normal_kernel(double* out, long long n,
+                                   unsigned long long seed,
+                                   const IterState* st, long long it_fixed,
+                                   unsigned stream) {
+  const long long it = st ? st->rng_iter : it_fixed;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+       2 * p < n; p += (long long)gridDim.x * blockDim.x) {
+    uint32_t c[4] = {(uint32_t)p, (uint32_t)(p >> 32), (uint32_t)it, stream};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    double u1 = 1.0 - u53(c[0], c[1]);  // (0, 1]
+    double u2 = u53(c[2], c[3]);
+    double rad = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincospi(2.0 * u2, &sn, &cs);
+    out[2 * p] = rad * cs;
+    if (2 * p + 1 < n) out[2 * p + 1] = rad * sn;
+  }
+}
+
+// Shared move schedule: m = first index with u < cum[m] (sampling/mwg.py:243).
+__global__ void schedule_kernel(int* sched, int n_steps,
+                                unsigned long long seed, const IterState* st,
+                                const double* cum, int n_moves) {
+  const long long it = st->rng_iter;
+  for (int t = threadIdx.x; t < n_steps; t += blockDim.x) {
+    uint32_t c[4] = {(uint32_t)t, 0u, (uint32_t)it, 4u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    double u = u53(c[0], c[1]);
+    int m = 0;
+    while (m < n_moves - 1 && !(u < cum[m])) ++m;
+    sched[t] = m;
+  }
+}
+
+__global__ void advance_kernel(IterState* st) {
+  if (threadIdx.x == 0) {
+    st->i += 1;
+    st->rng_iter += 1;
+    st->block_pos += 1;
+  }
+}
+
+}  // namespace jr

@@ -1,0 +1,1085 @@
+// Host side of libjrhip: handle, device buffers, launches, C ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "jr_kernels.h"
+
+using namespace jr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_global_err(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct jr_handle {
+  jr_config cfg;
+  std::string err;
+  int n_walk = 1;
+  int max_pairs = 0;
+  int A = 1;
+  bool multi = false;
+  hipStream_t stream = nullptr;
+  // population
+  DBuf<double> pos, dir, cell, energy;
+  DBuf<int> types, n_atoms;
+  bool have_pop = false;
+  // run params
+  DBuf<double> pressure, mu, step_sizes, emax;
+  std::vector<double> h_pressure, h_mu, h_step;
+  // potential
+  DBuf<double> eps4, eps24, sig2;
+  DBuf<DevMove> moves;
+  DBuf<double> cum;  // cumulative move probabilities
+  // iteration work
+  DBuf<int> walk, counters, queue, pair_valid, order, sched;
+  DBuf<double> sel_u, clone_u, extra_u, normals, uniforms, re_u;
+  DBuf<IterState> st;
+  // per-block outputs
+  int out_cap = 0;
+  DBuf<double> o_emax, o_dead_volume, o_dead_pos, o_dead_cell;
+  DBuf<int> o_worst, o_clone, o_dead_types, o_counters, o_re_acc, o_re_att,
+      o_walk;
+  // trial buffers
+  DBuf<int> t_idx, t_reason;
+  DBuf<double> t_step, t_emax, t_normals, t_uniforms;
+  long long trial_rng = 0;
+  // timing
+  std::vector<hipEvent_t> ev;
+  double walk_ms = 0.0;
+  long long walk_launches = 0;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  // launch geometry
+  int n_cu = 0;
+  int wpb = 4;
+  int blocks = 0;
+  size_t lds_walk = 0;
+  long long iter_i = 0;
+  long long rng_iter = 0;
+
+  int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+};
+
+#define HIPCHK(h, call)                                                   \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess)                                                 \
+      return (h)->fail(JR_E_HIP, "%s failed: %s (%s:%d)", #call,          \
+                       hipGetErrorString(e_), __FILE__, __LINE__);        \
+  } while (0)
+
+#define HIPCHK_G(call)                                                    \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess)                                                 \
+      return set_global_err(JR_E_HIP, "%s failed: %s", #call,             \
+                            hipGetErrorString(e_));                       \
+  } while (0)
+
+namespace {
+
+int atoms_per_lane(int nmax) { return nmax <= 64 ? 1 : (nmax <= 128 ? 2 : 4); }
+
+template <int A, bool M>
+void launch_walk_t(const WalkArgs& a, int blocks, int wpb, size_t lds,
+                   hipStream_t s) {
+  hipLaunchKernelGGL((walk_kernel<A, M>), dim3(blocks), dim3(64 * wpb), lds,
+                     s, a);
+}
+
+void launch_walk(int A, bool multi, const WalkArgs& a, int blocks, int wpb,
+                 size_t lds, hipStream_t s) {
+  if (A == 1) {
+    if (multi) launch_walk_t<1, true>(a, blocks, wpb, lds, s);
+    else launch_walk_t<1, false>(a, blocks, wpb, lds, s);
+  } else if (A == 2) {
+    if (multi) launch_walk_t<2, true>(a, blocks, wpb, lds, s);
+    else launch_walk_t<2, false>(a, blocks, wpb, lds, s);
+  } else {
+    if (multi) launch_walk_t<4, true>(a, blocks, wpb, lds, s);
+    else launch_walk_t<4, false>(a, blocks, wpb, lds, s);
+  }
+}
+
+const void* walk_fn(int A, bool multi) {
+  if (A == 1) return multi ? (const void*)walk_kernel<1, true> : (const void*)walk_kernel<1, false>;
+  if (A == 2) return multi ? (const void*)walk_kernel<2, true> : (const void*)walk_kernel<2, false>;
+  return multi ? (const void*)walk_kernel<4, true> : (const void*)walk_kernel<4, false>;
+}
+
+template <int A, bool M>
+void launch_eval_t(const EvalArgs& a, hipStream_t s) {
+  const int wpb = 4;
+  size_t lds = wave_lds_bytes(a.Nmax, 0) * wpb + 3 * sizeof(double) * a.S * a.S;
+  int blocks = (a.B + wpb - 1) / wpb;
+  hipLaunchKernelGGL((eval_kernel<A, M>), dim3(blocks), dim3(64 * wpb), lds, s,
+                     a);
+}
+
+void launch_eval(int A, bool multi, const EvalArgs& a, hipStream_t s) {
+  if (A == 1) {
+    if (multi) launch_eval_t<1, true>(a, s);
+    else launch_eval_t<1, false>(a, s);
+  } else if (A == 2) {
+    if (multi) launch_eval_t<2, true>(a, s);
+    else launch_eval_t<2, false>(a, s);
+  } else {
+    if (multi) launch_eval_t<4, true>(a, s);
+    else launch_eval_t<4, false>(a, s);
+  }
+}
+
+int validate_pot(const jr_pot* pot) {
+  if (!pot) return set_global_err(JR_E_ARG, "pot is NULL");
+  if (pot->n_species < 1 || pot->n_species > JR_MAX_SPECIES)
+    return set_global_err(JR_E_ARG, "n_species must be in [1, %d], got %d",
+                          JR_MAX_SPECIES, pot->n_species);
+  if (!pot->eps4 || !pot->eps24 || !pot->sig2)
+    return set_global_err(JR_E_ARG, "pair tables must not be NULL");
+  if (pot->precision != JR_PREC_FP64)
+    return set_global_err(JR_E_UNSUPPORTED, "precision %d not built",
+                          pot->precision);
+  return JR_OK;
+}
+
+long long pow2_at_least(long long n) {
+  long long p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* jr_global_error(void) { return g_err.c_str(); }
+
+const char* jr_last_error(const jr_handle* h) {
+  return h ? h->err.c_str() : g_err.c_str();
+}
+
+int jr_eval_energy_forces(const jr_pot* pot, int32_t device, int32_t B,
+                          int32_t Nmax, const double* pos,
+                          const int32_t* types, const int32_t* n_atoms,
+                          const double* cell, const double* pressure,
+                          const double* mu, double* U, double* H, double* F) {
+  int rc = validate_pot(pot);
+  if (rc) return rc;
+  if (B < 0 || Nmax < 1 || Nmax > 256)
+    return set_global_err(JR_E_ARG, "need B >= 0 and 1 <= Nmax <= 256");
+  if (!pos || !types || !n_atoms || !cell)
+    return set_global_err(JR_E_ARG, "pos/types/n_atoms/cell must be set");
+  if (B == 0) return JR_OK;
+  for (int b = 0; b < B; ++b) {
+    if (n_atoms[b] < 0 || n_atoms[b] > Nmax)
+      return set_global_err(JR_E_ARG, "n_atoms[%d]=%d outside [0, %d]", b,
+                            n_atoms[b], Nmax);
+    for (int i = 0; i < n_atoms[b]; ++i) {
+      int t = types[(size_t)b * Nmax + i];
+      if (t < 0 || t >= pot->n_species)
+        return set_global_err(JR_E_ARG, "species %d out of range", t);
+    }
+  }
+  HIPCHK_G(hipSetDevice(device));
+  const int S = pot->n_species;
+  DBuf<double> d_pos, d_cell, d_pr, d_mu, d_U, d_H, d_F, d_e4, d_e24, d_s2;
+  DBuf<int> d_t, d_n;
+  HIPCHK_G(d_pos.alloc((size_t)B * Nmax * 3));
+  HIPCHK_G(d_cell.alloc((size_t)B * 9));
+  HIPCHK_G(d_t.alloc((size_t)B * Nmax));
+  HIPCHK_G(d_n.alloc((size_t)B));
+  HIPCHK_G(d_U.alloc((size_t)B));
+  HIPCHK_G(d_H.alloc((size_t)B));
+  HIPCHK_G(d_e4.alloc((size_t)S * S));
+  HIPCHK_G(d_e24.alloc((size_t)S * S));
+  HIPCHK_G(d_s2.alloc((size_t)S * S));
+  if (pressure) HIPCHK_G(d_pr.alloc((size_t)B));
+  if (mu) HIPCHK_G(d_mu.alloc((size_t)B * S));
+  if (F) HIPCHK_G(d_F.alloc((size_t)B * Nmax * 3));
+  hipStream_t s;
+  HIPCHK_G(hipStreamCreate(&s));
+  auto h2d = [&](void* d, const void* hsrc, size_t bytes) {
+    return hipMemcpyAsync(d, hsrc, bytes, hipMemcpyHostToDevice, s);
+  };
+  HIPCHK_G(h2d(d_pos.p, pos, sizeof(double) * B * Nmax * 3));
+  HIPCHK_G(h2d(d_cell.p, cell, sizeof(double) * B * 9));
+  HIPCHK_G(h2d(d_t.p, types, sizeof(int) * B * Nmax));
+  HIPCHK_G(h2d(d_n.p, n_atoms, sizeof(int) * B));
+  HIPCHK_G(h2d(d_e4.p, pot->eps4, sizeof(double) * S * S));
+  HIPCHK_G(h2d(d_e24.p, pot->eps24, sizeof(double) * S * S));
+  HIPCHK_G(h2d(d_s2.p, pot->sig2, sizeof(double) * S * S));
+  if (pressure) HIPCHK_G(h2d(d_pr.p, pressure, sizeof(double) * B));
+  if (mu) HIPCHK_G(h2d(d_mu.p, mu, sizeof(double) * B * S));
+  EvalArgs a;
+  a.pos = d_pos.p;
+  a.types = d_t.p;
+  a.n_atoms = d_n.p;
+  a.cell = d_cell.p;
+  a.pressure = pressure ? d_pr.p : nullptr;
+  a.mu = mu ? d_mu.p : nullptr;
+  a.U = d_U.p;
+  a.H = d_H.p;
+  a.F = F ? d_F.p : nullptr;
+  a.B = B;
+  a.Nmax = Nmax;
+  a.S = S;
+  a.rc2 = pot->rc2;
+  a.eps4 = d_e4.p;
+  a.eps24 = d_e24.p;
+  a.sig2 = d_s2.p;
+  launch_eval(atoms_per_lane(Nmax), S > 1, a, s);
+  HIPCHK_G(hipGetLastError());
+  if (U) HIPCHK_G(hipMemcpyAsync(U, d_U.p, sizeof(double) * B, hipMemcpyDeviceToHost, s));
+  if (H) HIPCHK_G(hipMemcpyAsync(H, d_H.p, sizeof(double) * B, hipMemcpyDeviceToHost, s));
+  if (F)
+    HIPCHK_G(hipMemcpyAsync(F, d_F.p, sizeof(double) * B * Nmax * 3,
+                            hipMemcpyDeviceToHost, s));
+  HIPCHK_G(hipStreamSynchronize(s));
+  HIPCHK_G(hipStreamDestroy(s));
+  return JR_OK;
+}
+
+void jr_destroy(jr_handle* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->cfg.device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  for (auto e : h->ev) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;  // DBuf members release in their owners' scope below
+}
+
+}  // extern "C"
+
+namespace {
+
+void release_all(jr_handle* h) {
+  h->pos.release(); h->dir.release(); h->cell.release(); h->energy.release();
+  h->types.release(); h->n_atoms.release();
+  h->pressure.release(); h->mu.release(); h->step_sizes.release(); h->emax.release();
+  h->eps4.release(); h->eps24.release(); h->sig2.release(); h->moves.release(); h->cum.release();
+  h->walk.release(); h->counters.release(); h->queue.release(); h->pair_valid.release();
+  h->order.release(); h->sched.release();
+  h->sel_u.release(); h->clone_u.release(); h->extra_u.release(); h->normals.release();
+  h->uniforms.release(); h->re_u.release(); h->st.release();
+  h->o_emax.release(); h->o_dead_volume.release(); h->o_dead_pos.release(); h->o_dead_cell.release();
+  h->o_worst.release(); h->o_clone.release(); h->o_dead_types.release(); h->o_counters.release();
+  h->o_re_acc.release(); h->o_re_att.release(); h->o_walk.release();
+  h->t_idx.release(); h->t_reason.release(); h->t_step.release(); h->t_emax.release();
+  h->t_normals.release(); h->t_uniforms.release();
+}
+
+int ensure_out(jr_handle* h, int n_iter) {
+  if (n_iter <= h->out_cap) return JR_OK;
+  const auto& c = h->cfg;
+  const int R = c.n_replicas, Nm = c.n_atoms_max;
+  size_t n = (size_t)n_iter;
+  HIPCHK(h, h->o_emax.alloc(n * R));
+  HIPCHK(h, h->o_dead_volume.alloc(n * R));
+  HIPCHK(h, h->o_dead_pos.alloc(n * R * Nm * 3));
+  HIPCHK(h, h->o_dead_cell.alloc(n * R * 9));
+  HIPCHK(h, h->o_worst.alloc(n * R));
+  HIPCHK(h, h->o_clone.alloc(n * R));
+  HIPCHK(h, h->o_dead_types.alloc(n * R * Nm));
+  HIPCHK(h, h->o_counters.alloc(n * R * c.n_moves * JR_NCOUNT));
+  HIPCHK(h, h->o_re_acc.alloc(n * std::max(R - 1, 1)));
+  HIPCHK(h, h->o_re_att.alloc(n * std::max(R - 1, 1)));
+  HIPCHK(h, h->o_walk.alloc(n * R * h->n_walk));
+  h->out_cap = n_iter;
+  if (h->graph_exec) {
+    (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+  }
+  if (h->graph) {
+    (void)hipGraphDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return JR_OK;
+}
+
+// population energies on the device (needs run params)
+int recompute_energies(jr_handle* h) {
+  const auto& c = h->cfg;
+  const int R = c.n_replicas, K = c.n_walkers;
+  const int S = c.pot.n_species;
+  // per-walker pressure/mu expanded on host
+  std::vector<double> pr((size_t)R * K), mu((size_t)R * K * S);
+  for (int r = 0; r < R; ++r)
+    for (int k = 0; k < K; ++k) {
+      pr[(size_t)r * K + k] = h->h_pressure[r];
+      for (int s = 0; s < S; ++s) mu[((size_t)r * K + k) * S + s] = h->h_mu[(size_t)r * S + s];
+    }
+  DBuf<double> dpr, dmu;
+  HIPCHK(h, dpr.alloc(pr.size()));
+  HIPCHK(h, dmu.alloc(mu.size()));
+  HIPCHK(h, hipMemcpyAsync(dpr.p, pr.data(), pr.size() * 8, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dmu.p, mu.data(), mu.size() * 8, hipMemcpyHostToDevice, h->stream));
+  EvalArgs a;
+  a.pos = h->pos.p;
+  a.types = h->types.p;
+  a.n_atoms = h->n_atoms.p;
+  a.cell = h->cell.p;
+  a.pressure = dpr.p;
+  a.mu = dmu.p;
+  a.U = nullptr;
+  a.H = h->energy.p;
+  a.F = nullptr;
+  a.B = R * K;
+  a.Nmax = c.n_atoms_max;
+  a.S = S;
+  a.rc2 = c.pot.rc2;
+  a.eps4 = h->eps4.p;
+  a.eps24 = h->eps24.p;
+  a.sig2 = h->sig2.p;
+  launch_eval(h->A, h->multi, a, h->stream);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return JR_OK;
+}
+
+WalkArgs walk_args(jr_handle* h) {
+  const auto& c = h->cfg;
+  WalkArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.pop = PopPtrs{h->pos.p, h->types.p, h->n_atoms.p, h->cell.p, h->energy.p, h->dir.p};
+  a.pressure = h->pressure.p;
+  a.mu = h->mu.p;
+  a.step_sizes = h->step_sizes.p;
+  a.emax = h->emax.p;
+  a.walk = h->walk.p;
+  a.n_walk = h->n_walk;
+  a.R = c.n_replicas;
+  a.K = c.n_walkers;
+  a.Nmax = c.n_atoms_max;
+  a.S = c.pot.n_species;
+  a.n_steps = c.n_mcmc_steps;
+  a.n_moves = c.n_moves;
+  a.schedule = h->sched.p;
+  a.normals = h->normals.p;
+  a.uniforms = h->uniforms.p;
+  a.counters = h->counters.p;
+  a.queue = h->queue.p;
+  a.total = c.n_replicas * h->n_walk;
+  a.order = h->order.p;
+  a.rc2 = c.pot.rc2;
+  a.eps4 = h->eps4.p;
+  a.eps24 = h->eps24.p;
+  a.sig2 = h->sig2.p;
+  a.moves = h->moves.p;
+  return a;
+}
+
+SelArgs sel_args(jr_handle* h) {
+  const auto& c = h->cfg;
+  SelArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.pop = PopPtrs{h->pos.p, h->types.p, h->n_atoms.p, h->cell.p, h->energy.p, h->dir.p};
+  a.R = c.n_replicas;
+  a.K = c.n_walkers;
+  a.Nmax = c.n_atoms_max;
+  a.n_extra = c.n_extra;
+  a.n_walk = h->n_walk;
+  a.n_moves = c.n_moves;
+  a.Kpad = c.n_extra > 0 ? (int)pow2_at_least(c.n_walkers) : 0;
+  a.sel_u = h->sel_u.p;
+  a.clone_u = h->clone_u.p;
+  a.extra_u = h->extra_u.p;
+  a.emax = h->emax.p;
+  a.walk = h->walk.p;
+  a.counters = h->counters.p;
+  a.queue = h->queue.p;
+  a.st = h->st.p;
+  a.o_emax = h->o_emax.p;
+  a.o_worst = h->o_worst.p;
+  a.o_clone = h->o_clone.p;
+  a.o_dead_volume = h->o_dead_volume.p;
+  a.o_dead_pos = h->o_dead_pos.p;
+  a.o_dead_types = h->o_dead_types.p;
+  a.o_dead_cell = h->o_dead_cell.p;
+  return a;
+}
+
+ReArgs re_args(jr_handle* h, IterState* st) {
+  const auto& c = h->cfg;
+  ReArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.pop = PopPtrs{h->pos.p, h->types.p, h->n_atoms.p, h->cell.p, h->energy.p, h->dir.p};
+  a.R = c.n_replicas;
+  a.K = c.n_walkers;
+  a.Nmax = c.n_atoms_max;
+  a.S = c.pot.n_species;
+  a.n_cycles = c.n_swap_cycles;
+  a.max_pairs = h->max_pairs;
+  a.pressure = h->pressure.p;
+  a.mu = h->mu.p;
+  a.emax = h->emax.p;
+  a.pair_valid = h->pair_valid.p;
+  a.re_u = h->re_u.p;
+  a.st = st;
+  a.o_acc = h->o_re_acc.p;
+  a.o_att = h->o_re_att.p;
+  return a;
+}
+
+size_t n_normals(const jr_handle* h) {
+  const auto& c = h->cfg;
+  return (size_t)c.n_replicas * h->n_walk * c.n_mcmc_steps * 3 * c.n_atoms_max;
+}
+size_t n_uniforms(const jr_handle* h) {
+  const auto& c = h->cfg;
+  return (size_t)c.n_replicas * h->n_walk * c.n_mcmc_steps * 4;
+}
+size_t n_re_u(const jr_handle* h) {
+  return (size_t)std::max(h->cfg.n_swap_cycles, 1) * 2 * std::max(h->max_pairs, 1) * 2;
+}
+
+unsigned grid_for(size_t n) {
+  size_t pairs = (n + 1) / 2;
+  size_t b = (pairs + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > 8192) b = 8192;
+  return (unsigned)b;
+}
+
+// Device draws for the current iteration (Philox keyed by seed, counter
+// (index, rng_iter, stream)).
+void enqueue_draws(jr_handle* h) {
+  const auto& c = h->cfg;
+  const int R = c.n_replicas, K = c.n_walkers;
+  hipStream_t s = h->stream;
+  IterState* st = h->st.p;
+  unsigned long long seed = c.seed;
+  hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for((size_t)R * K)), dim3(256), 0, s,
+                     h->sel_u.p, (long long)R * K, seed, st, 0LL, 1u);
+  hipLaunchKernelGGL(fill_uniform_kernel, dim3(1), dim3(256), 0, s, h->clone_u.p,
+                     (long long)R, seed, st, 0LL, 2u);
+  if (c.n_extra > 0)
+    hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for((size_t)R * K)), dim3(256), 0, s,
+                       h->extra_u.p, (long long)R * K, seed, st, 0LL, 3u);
+  hipLaunchKernelGGL(schedule_kernel, dim3(1), dim3(64), 0, s, h->sched.p,
+                     c.n_mcmc_steps, seed, st, h->cum.p, c.n_moves);
+  hipLaunchKernelGGL(fill_normal_kernel, dim3(grid_for(n_normals(h))), dim3(256), 0, s,
+                     h->normals.p, (long long)n_normals(h), seed, st, 0LL, 5u);
+  hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(n_uniforms(h))), dim3(256), 0, s,
+                     h->uniforms.p, (long long)n_uniforms(h), seed, st, 0LL, 6u);
+  if (c.n_replicas > 1)
+    hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(n_re_u(h))), dim3(256), 0, s,
+                       h->re_u.p, (long long)n_re_u(h), seed, st, 0LL, 7u);
+}
+
+// select -> walk -> RE -> advance, all on the handle's stream.
+void enqueue_iteration_core(jr_handle* h, bool timed, int ev_idx) {
+  const auto& c = h->cfg;
+  SelArgs sa = sel_args(h);
+  size_t sel_lds = sa.Kpad ? (size_t)sa.Kpad * (sizeof(double) + sizeof(int)) : 0;
+  hipLaunchKernelGGL(select_kernel, dim3(c.n_replicas), dim3(1024), sel_lds, h->stream, sa);
+  WalkArgs wa = walk_args(h);
+  if (timed) (void)hipEventRecord(h->ev[2 * ev_idx], h->stream);
+  launch_walk(h->A, h->multi, wa, h->blocks, h->wpb, h->lds_walk, h->stream);
+  if (timed) (void)hipEventRecord(h->ev[2 * ev_idx + 1], h->stream);
+  if (c.n_replicas > 1) {
+    ReArgs ra = re_args(h, h->st.p);
+    hipLaunchKernelGGL(re_kernel, dim3(1), dim3(256), 0, h->stream, ra);
+  }
+  // copy this iteration's walk list + counters into the block outputs
+  hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, h->stream, h->st.p);
+}
+
+}  // namespace
+
+// counters/walk snapshots are copied device-to-device after each iteration
+// (outside the graph path they are plain async copies).
+static int snapshot_iter(jr_handle* h, int it) {
+  const auto& c = h->cfg;
+  const int R = c.n_replicas;
+  size_t nc = (size_t)R * c.n_moves * JR_NCOUNT;
+  HIPCHK(h, hipMemcpyAsync(h->o_counters.p + (size_t)it * nc, h->counters.p,
+                           nc * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
+  size_t nw = (size_t)R * h->n_walk;
+  HIPCHK(h, hipMemcpyAsync(h->o_walk.p + (size_t)it * nw, h->walk.p,
+                           nw * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
+  return JR_OK;
+}
+
+extern "C" {
+
+int jr_create(const jr_config* cfg, jr_handle** out) {
+  if (!cfg || !out) return set_global_err(JR_E_ARG, "cfg/out must not be NULL");
+  *out = nullptr;
+  int rc = validate_pot(&cfg->pot);
+  if (rc) return rc;
+  const jr_config& c = *cfg;
+  if (c.n_replicas < 1) return set_global_err(JR_E_ARG, "n_replicas must be >= 1");
+  if (c.n_walkers < 2 || c.n_walkers > 4096)
+    return set_global_err(JR_E_ARG, "n_walkers must be in [2, 4096]");
+  if (c.n_atoms_max < 1 || c.n_atoms_max > 256)
+    return set_global_err(JR_E_ARG, "n_atoms_max must be in [1, 256]");
+  if (c.n_moves < 1 || c.n_moves > JR_MAX_MOVES || !c.moves || !c.move_weights)
+    return set_global_err(JR_E_ARG, "need 1..%d moves with weights", JR_MAX_MOVES);
+  if (c.n_mcmc_steps < 1) return set_global_err(JR_E_ARG, "n_mcmc_steps must be >= 1");
+  if (c.n_cull != 1)
+    return set_global_err(JR_E_UNSUPPORTED, "n_cull=%d: only n_cull=1 is built", c.n_cull);
+  if (c.n_extra < 0 || c.n_extra > c.n_walkers - 2)
+    return set_global_err(JR_E_ARG, "n_extra must be in [0, K-2]");
+  if (c.n_swap_cycles < 0) return set_global_err(JR_E_ARG, "n_swap_cycles must be >= 0");
+  double wsum = 0.0;
+  for (int m = 0; m < c.n_moves; ++m) {
+    const jr_move& mv = c.moves[m];
+    if (mv.type < 0 || mv.type > JR_MOVE_MORPH)
+      return set_global_err(JR_E_ARG, "move %d: unknown type %d", m, mv.type);
+    if (mv.type == JR_MOVE_MORPH && (mv.n_species < 2 || mv.n_species > c.pot.n_species))
+      return set_global_err(JR_E_ARG, "alchemical_morph needs 2 <= n_species <= S");
+    if (mv.type == JR_MOVE_GMC && mv.n_reflect < 1)
+      return set_global_err(JR_E_ARG, "gmc needs n_reflect >= 1");
+    if (c.move_weights[m] < 0.0) return set_global_err(JR_E_ARG, "negative move weight");
+    wsum += c.move_weights[m];
+  }
+  if (!(wsum > 0.0)) return set_global_err(JR_E_ARG, "move weights sum to zero");
+  HIPCHK_G(hipSetDevice(c.device));
+
+  jr_handle* h = new jr_handle();
+  h->cfg = c;
+  h->cfg.moves = nullptr;
+  h->cfg.move_weights = nullptr;
+  h->cfg.pot.eps4 = h->cfg.pot.eps24 = h->cfg.pot.sig2 = nullptr;
+  h->n_walk = 1 + c.n_extra;
+  h->max_pairs = c.n_replicas / 2;
+  h->A = atoms_per_lane(c.n_atoms_max);
+  h->multi = c.pot.n_species > 1;
+  const int R = c.n_replicas, K = c.n_walkers, Nm = c.n_atoms_max, S = c.pot.n_species;
+  auto fail = [&](int code) {
+    g_err = h->err;
+    release_all(h);
+    jr_destroy(h);
+    return code;
+  };
+#define TRY(call)                                                        \
+  do {                                                                   \
+    hipError_t e_ = (call);                                              \
+    if (e_ != hipSuccess) {                                              \
+      h->err = std::string(#call) + ": " + hipGetErrorString(e_);        \
+      return fail(JR_E_HIP);                                             \
+    }                                                                    \
+  } while (0)
+  TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  const size_t W = (size_t)R * K;
+  TRY(h->pos.alloc(W * Nm * 3));
+  TRY(h->dir.alloc(W * Nm * 3));
+  TRY(h->cell.alloc(W * 9));
+  TRY(h->energy.alloc(W));
+  TRY(h->types.alloc(W * Nm));
+  TRY(h->n_atoms.alloc(W));
+  TRY(h->pressure.alloc(R));
+  TRY(h->mu.alloc((size_t)R * S));
+  TRY(h->step_sizes.alloc((size_t)R * c.n_moves));
+  TRY(h->emax.alloc(R));
+  TRY(h->eps4.alloc((size_t)S * S));
+  TRY(h->eps24.alloc((size_t)S * S));
+  TRY(h->sig2.alloc((size_t)S * S));
+  TRY(h->moves.alloc(c.n_moves));
+  TRY(h->cum.alloc(c.n_moves));
+  TRY(h->walk.alloc((size_t)R * h->n_walk));
+  TRY(h->counters.alloc((size_t)R * c.n_moves * JR_NCOUNT));
+  TRY(h->queue.alloc(1));
+  TRY(h->pair_valid.alloc(std::max(R - 1, 1)));
+  TRY(h->order.alloc(R));
+  TRY(h->sched.alloc(c.n_mcmc_steps));
+  TRY(h->sel_u.alloc(W));
+  TRY(h->clone_u.alloc(R));
+  TRY(h->extra_u.alloc(W));
+  TRY(h->normals.alloc(n_normals(h)));
+  TRY(h->uniforms.alloc(n_uniforms(h)));
+  TRY(h->re_u.alloc(n_re_u(h)));
+  TRY(h->st.alloc(1));
+
+  TRY(hipMemcpy(h->eps4.p, c.pot.eps4, sizeof(double) * S * S, hipMemcpyHostToDevice));
+  TRY(hipMemcpy(h->eps24.p, c.pot.eps24, sizeof(double) * S * S, hipMemcpyHostToDevice));
+  TRY(hipMemcpy(h->sig2.p, c.pot.sig2, sizeof(double) * S * S, hipMemcpyHostToDevice));
+  std::vector<DevMove> dm(c.n_moves);
+  std::vector<double> cum(c.n_moves);
+  double acc = 0.0;
+  for (int m = 0; m < c.n_moves; ++m) {
+    const jr_move& mv = c.moves[m];
+    dm[m] = DevMove{mv.type, mv.n_reflect, mv.n_leapfrog, mv.flat_v_prior, mv.n_species, 0,
+                    mv.perturb_angle, mv.max_vpa, mv.min_vpa, mv.min_aspect};
+    acc += c.move_weights[m] / wsum;
+    cum[m] = acc;
+  }
+  cum[c.n_moves - 1] = 2.0;  // last bucket absorbs rounding
+  TRY(hipMemcpy(h->moves.p, dm.data(), sizeof(DevMove) * c.n_moves, hipMemcpyHostToDevice));
+  TRY(hipMemcpy(h->cum.p, cum.data(), sizeof(double) * c.n_moves, hipMemcpyHostToDevice));
+  std::vector<int> pv(std::max(R - 1, 1), 1), ord(R);
+  for (int r = 0; r < R; ++r) ord[r] = r;
+  TRY(hipMemcpy(h->pair_valid.p, pv.data(), sizeof(int) * pv.size(), hipMemcpyHostToDevice));
+  TRY(hipMemcpy(h->order.p, ord.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+  h->h_pressure.assign(R, 0.0);
+  h->h_mu.assign((size_t)R * S, 0.0);
+  h->h_step.assign((size_t)R * c.n_moves, 0.1);
+  TRY(hipMemcpy(h->pressure.p, h->h_pressure.data(), sizeof(double) * R, hipMemcpyHostToDevice));
+  TRY(hipMemcpy(h->mu.p, h->h_mu.data(), sizeof(double) * R * S, hipMemcpyHostToDevice));
+  TRY(hipMemcpy(h->step_sizes.p, h->h_step.data(), sizeof(double) * R * c.n_moves,
+                hipMemcpyHostToDevice));
+  TRY(hipMemset(h->energy.p, 0, sizeof(double) * W));
+  TRY(hipMemset(h->emax.p, 0, sizeof(double) * R));
+  std::vector<double> inf(R, INFINITY);
+  TRY(hipMemcpy(h->emax.p, inf.data(), sizeof(double) * R, hipMemcpyHostToDevice));
+  IterState st0{0, 0, 0, c.re_interval};
+  TRY(hipMemcpy(h->st.p, &st0, sizeof(st0), hipMemcpyHostToDevice));
+
+  // launch geometry: persistent grid sized by occupancy
+  int dev = c.device;
+  hipDeviceProp_t prop;
+  TRY(hipGetDeviceProperties(&prop, dev));
+  h->n_cu = prop.multiProcessorCount;
+  h->wpb = c.waves_per_block > 0 ? std::min(c.waves_per_block, 4) : 4;
+  h->lds_walk = wave_lds_bytes(Nm, c.n_moves) * h->wpb + 3 * sizeof(double) * S * S;
+  int occ = 0;
+  TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, walk_fn(h->A, h->multi),
+                                                    64 * h->wpb, h->lds_walk));
+  if (occ < 1) occ = 1;
+  const int total = R * h->n_walk;
+  const int need = (total + h->wpb - 1) / h->wpb;
+  h->blocks = std::max(1, std::min(need, occ * h->n_cu));
+  int ret = ensure_out(h, 1);
+  if (ret) return fail(ret);
+  *out = h;
+  return JR_OK;
+#undef TRY
+}
+
+int jr_set_population(jr_handle* h, const double* pos, const int32_t* types,
+                      const int32_t* n_atoms, const double* cell,
+                      const double* dir) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  if (!pos || !types || !n_atoms || !cell)
+    return h->fail(JR_E_ARG, "pos/types/n_atoms/cell must not be NULL");
+  const auto& c = h->cfg;
+  const size_t W = (size_t)c.n_replicas * c.n_walkers;
+  const int Nm = c.n_atoms_max, S = c.pot.n_species;
+  for (size_t w = 0; w < W; ++w) {
+    if (n_atoms[w] < 1 || n_atoms[w] > Nm)
+      return h->fail(JR_E_ARG, "n_atoms[%zu]=%d outside [1, %d]", w, n_atoms[w], Nm);
+    for (int i = 0; i < n_atoms[w]; ++i) {
+      int t = types[w * Nm + i];
+      if (t < 0 || t >= S) return h->fail(JR_E_ARG, "species %d out of range", t);
+    }
+  }
+  HIPCHK(h, hipSetDevice(c.device));
+  HIPCHK(h, hipMemcpyAsync(h->pos.p, pos, sizeof(double) * W * Nm * 3, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->types.p, types, sizeof(int) * W * Nm, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->n_atoms.p, n_atoms, sizeof(int) * W, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->cell.p, cell, sizeof(double) * W * 9, hipMemcpyHostToDevice, h->stream));
+  if (dir)
+    HIPCHK(h, hipMemcpyAsync(h->dir.p, dir, sizeof(double) * W * Nm * 3, hipMemcpyHostToDevice, h->stream));
+  else
+    HIPCHK(h, hipMemsetAsync(h->dir.p, 0, sizeof(double) * W * Nm * 3, h->stream));
+  // default RE validity: equal atom counts per replica (first walker)
+  const int R = c.n_replicas, K = c.n_walkers;
+  std::vector<int> pv(std::max(R - 1, 1), 1);
+  std::vector<std::pair<int, int>> byn(R);
+  for (int r = 0; r < R; ++r) byn[r] = {-n_atoms[(size_t)r * K], r};
+  for (int r = 0; r + 1 < R; ++r) pv[r] = n_atoms[(size_t)r * K] == n_atoms[(size_t)(r + 1) * K];
+  std::stable_sort(byn.begin(), byn.end());
+  std::vector<int> ord(R);
+  for (int r = 0; r < R; ++r) ord[r] = byn[r].second;
+  HIPCHK(h, hipMemcpyAsync(h->pair_valid.p, pv.data(), sizeof(int) * pv.size(), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->order.p, ord.data(), sizeof(int) * R, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->have_pop = true;
+  return recompute_energies(h);
+}
+
+int jr_get_population(jr_handle* h, double* pos, int32_t* types,
+                      int32_t* n_atoms, double* cell, double* energy,
+                      double* dir) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  const auto& c = h->cfg;
+  const size_t W = (size_t)c.n_replicas * c.n_walkers;
+  const int Nm = c.n_atoms_max;
+  HIPCHK(h, hipSetDevice(c.device));
+  if (pos) HIPCHK(h, hipMemcpyAsync(pos, h->pos.p, sizeof(double) * W * Nm * 3, hipMemcpyDeviceToHost, h->stream));
+  if (types) HIPCHK(h, hipMemcpyAsync(types, h->types.p, sizeof(int) * W * Nm, hipMemcpyDeviceToHost, h->stream));
+  if (n_atoms) HIPCHK(h, hipMemcpyAsync(n_atoms, h->n_atoms.p, sizeof(int) * W, hipMemcpyDeviceToHost, h->stream));
+  if (cell) HIPCHK(h, hipMemcpyAsync(cell, h->cell.p, sizeof(double) * W * 9, hipMemcpyDeviceToHost, h->stream));
+  if (energy) HIPCHK(h, hipMemcpyAsync(energy, h->energy.p, sizeof(double) * W, hipMemcpyDeviceToHost, h->stream));
+  if (dir) HIPCHK(h, hipMemcpyAsync(dir, h->dir.p, sizeof(double) * W * Nm * 3, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return JR_OK;
+}
+
+int jr_set_run_params(jr_handle* h, const double* pressure, const double* mu,
+                      const double* step_sizes) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  const auto& c = h->cfg;
+  const int R = c.n_replicas, S = c.pot.n_species;
+  HIPCHK(h, hipSetDevice(c.device));
+  if (pressure) h->h_pressure.assign(pressure, pressure + R);
+  if (mu) h->h_mu.assign(mu, mu + (size_t)R * S);
+  if (step_sizes) h->h_step.assign(step_sizes, step_sizes + (size_t)R * c.n_moves);
+  HIPCHK(h, hipMemcpyAsync(h->pressure.p, h->h_pressure.data(), sizeof(double) * R, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->mu.p, h->h_mu.data(), sizeof(double) * R * S, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->step_sizes.p, h->h_step.data(), sizeof(double) * R * c.n_moves, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (h->have_pop && (pressure || mu)) return recompute_energies(h);
+  return JR_OK;
+}
+
+int jr_get_step_sizes(jr_handle* h, double* step_sizes) {
+  if (!h || !step_sizes) return set_global_err(JR_E_ARG, "NULL argument");
+  std::copy(h->h_step.begin(), h->h_step.end(), step_sizes);
+  return JR_OK;
+}
+
+int jr_set_pair_valid(jr_handle* h, const int32_t* pair_valid) {
+  if (!h || !pair_valid) return set_global_err(JR_E_ARG, "NULL argument");
+  const int R = h->cfg.n_replicas;
+  if (R < 2) return JR_OK;
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  HIPCHK(h, hipMemcpy(h->pair_valid.p, pair_valid, sizeof(int) * (R - 1), hipMemcpyHostToDevice));
+  return JR_OK;
+}
+
+int jr_set_iteration(jr_handle* h, int64_t i) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  h->iter_i = i;
+  return JR_OK;
+}
+
+int jr_get_emax(jr_handle* h, double* emax) {
+  if (!h || !emax) return set_global_err(JR_E_ARG, "NULL argument");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  HIPCHK(h, hipMemcpy(emax, h->emax.p, sizeof(double) * h->cfg.n_replicas, hipMemcpyDeviceToHost));
+  return JR_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int copy_outputs(jr_handle* h, int n_iter, jr_iter_out* out) {
+  if (!out) return JR_OK;
+  const auto& c = h->cfg;
+  const size_t R = c.n_replicas, Nm = c.n_atoms_max;
+  const size_t n = (size_t)n_iter;
+  hipStream_t s = h->stream;
+  auto d2h = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+  };
+  if (out->emax) HIPCHK(h, d2h(out->emax, h->o_emax.p, sizeof(double) * n * R));
+  if (out->worst) HIPCHK(h, d2h(out->worst, h->o_worst.p, sizeof(int) * n * R));
+  if (out->clone) HIPCHK(h, d2h(out->clone, h->o_clone.p, sizeof(int) * n * R));
+  if (out->dead_volume) HIPCHK(h, d2h(out->dead_volume, h->o_dead_volume.p, sizeof(double) * n * R));
+  if (out->dead_pos) HIPCHK(h, d2h(out->dead_pos, h->o_dead_pos.p, sizeof(double) * n * R * Nm * 3));
+  if (out->dead_types) HIPCHK(h, d2h(out->dead_types, h->o_dead_types.p, sizeof(int) * n * R * Nm));
+  if (out->dead_cell) HIPCHK(h, d2h(out->dead_cell, h->o_dead_cell.p, sizeof(double) * n * R * 9));
+  if (out->counters)
+    HIPCHK(h, d2h(out->counters, h->o_counters.p, sizeof(int) * n * R * c.n_moves * JR_NCOUNT));
+  if (R > 1) {
+    if (out->re_acc) HIPCHK(h, d2h(out->re_acc, h->o_re_acc.p, sizeof(int) * n * (R - 1)));
+    if (out->re_att) HIPCHK(h, d2h(out->re_att, h->o_re_att.p, sizeof(int) * n * (R - 1)));
+  }
+  if (out->walk) HIPCHK(h, d2h(out->walk, h->o_walk.p, sizeof(int) * n * R * h->n_walk));
+  return JR_OK;
+}
+
+int begin_block(jr_handle* h, int n_iter) {
+  int rc = ensure_out(h, n_iter);
+  if (rc) return rc;
+  IterState st{h->iter_i, h->rng_iter, 0, h->cfg.re_interval};
+  HIPCHK(h, hipMemcpyAsync(h->st.p, &st, sizeof(st), hipMemcpyHostToDevice, h->stream));
+  if ((int)h->ev.size() < 2 * n_iter) {
+    size_t old = h->ev.size();
+    h->ev.resize(2 * n_iter);
+    for (size_t i = old; i < h->ev.size(); ++i) HIPCHK(h, hipEventCreate(&h->ev[i]));
+  }
+  return JR_OK;
+}
+
+int upload_tape(jr_handle* h, const jr_tape* t, int it) {
+  const auto& c = h->cfg;
+  const size_t R = c.n_replicas, K = c.n_walkers;
+  hipStream_t s = h->stream;
+  auto h2d = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+  };
+  if (!t->sel_u || !t->clone_u || !t->schedule || !t->normals || !t->uniforms)
+    return h->fail(JR_E_ARG, "tape: sel_u/clone_u/schedule/normals/uniforms required");
+  if (c.n_extra > 0 && !t->extra_u) return h->fail(JR_E_ARG, "tape: extra_u required");
+  HIPCHK(h, h2d(h->sel_u.p, t->sel_u + (size_t)it * R * K, sizeof(double) * R * K));
+  HIPCHK(h, h2d(h->clone_u.p, t->clone_u + (size_t)it * R, sizeof(double) * R));
+  if (c.n_extra > 0)
+    HIPCHK(h, h2d(h->extra_u.p, t->extra_u + (size_t)it * R * K, sizeof(double) * R * K));
+  for (int s2 = 0; s2 < c.n_mcmc_steps; ++s2) {
+    int m = t->schedule[(size_t)it * c.n_mcmc_steps + s2];
+    if (m < 0 || m >= c.n_moves) return h->fail(JR_E_ARG, "tape: move index %d out of range", m);
+  }
+  HIPCHK(h, h2d(h->sched.p, t->schedule + (size_t)it * c.n_mcmc_steps, sizeof(int) * c.n_mcmc_steps));
+  HIPCHK(h, h2d(h->normals.p, t->normals + (size_t)it * n_normals(h), sizeof(double) * n_normals(h)));
+  HIPCHK(h, h2d(h->uniforms.p, t->uniforms + (size_t)it * n_uniforms(h), sizeof(double) * n_uniforms(h)));
+  if (t->re_u && R > 1)
+    HIPCHK(h, h2d(h->re_u.p, t->re_u + (size_t)it * n_re_u(h), sizeof(double) * n_re_u(h)));
+  return JR_OK;
+}
+
+int finish_block(jr_handle* h, int n_iter, bool timed, jr_iter_out* out) {
+  int rc = copy_outputs(h, n_iter, out);
+  if (rc) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (timed) {
+    for (int it = 0; it < n_iter; ++it) {
+      float ms = 0.f;
+      HIPCHK(h, hipEventElapsedTime(&ms, h->ev[2 * it], h->ev[2 * it + 1]));
+      h->walk_ms += ms;
+      h->walk_launches += 1;
+    }
+  }
+  h->iter_i += n_iter;
+  h->rng_iter += n_iter;
+  return JR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jr_ns_iterations(jr_handle* h, int32_t n_iter, const jr_tape* tape,
+                     jr_iter_out* out) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  if (!h->have_pop) return h->fail(JR_E_STATE, "population not set");
+  if (n_iter < 0) return h->fail(JR_E_ARG, "n_iter < 0");
+  if (n_iter == 0) return JR_OK;
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  int rc = begin_block(h, n_iter);
+  if (rc) return rc;
+  for (int it = 0; it < n_iter; ++it) {
+    if (tape) {
+      rc = upload_tape(h, tape, it);
+      if (rc) return rc;
+    } else {
+      enqueue_draws(h);
+    }
+    enqueue_iteration_core(h, true, it);
+    rc = snapshot_iter(h, it);
+    if (rc) return rc;
+  }
+  HIPCHK(h, hipGetLastError());
+  return finish_block(h, n_iter, true, out);
+}
+
+int jr_ns_iterations_graph(jr_handle* h, int32_t n_iter, jr_iter_out* out) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  if (!h->have_pop) return h->fail(JR_E_STATE, "population not set");
+  if (n_iter <= 0) return n_iter == 0 ? JR_OK : h->fail(JR_E_ARG, "n_iter < 0");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  int rc = begin_block(h, n_iter);
+  if (rc) return rc;
+  if (!h->graph_exec) {
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_draws(h);
+    enqueue_iteration_core(h, false, 0);
+    hipError_t e = hipStreamEndCapture(h->stream, &h->graph);
+    if (e != hipSuccess) return h->fail(JR_E_HIP, "graph capture: %s", hipGetErrorString(e));
+    HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+  }
+  for (int it = 0; it < n_iter; ++it) {
+    HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+    rc = snapshot_iter(h, it);
+    if (rc) return rc;
+  }
+  return finish_block(h, n_iter, false, out);
+}
+
+int jr_get_last_tape(jr_handle* h, double* sel_u, double* clone_u,
+                     double* extra_u, int32_t* schedule, double* normals,
+                     double* uniforms, double* re_u) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  const auto& c = h->cfg;
+  const size_t R = c.n_replicas, K = c.n_walkers;
+  HIPCHK(h, hipSetDevice(c.device));
+  hipStream_t s = h->stream;
+  auto d2h = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+  };
+  if (sel_u) HIPCHK(h, d2h(sel_u, h->sel_u.p, sizeof(double) * R * K));
+  if (clone_u) HIPCHK(h, d2h(clone_u, h->clone_u.p, sizeof(double) * R));
+  if (extra_u && c.n_extra > 0) HIPCHK(h, d2h(extra_u, h->extra_u.p, sizeof(double) * R * K));
+  if (schedule) HIPCHK(h, d2h(schedule, h->sched.p, sizeof(int) * c.n_mcmc_steps));
+  if (normals) HIPCHK(h, d2h(normals, h->normals.p, sizeof(double) * n_normals(h)));
+  if (uniforms) HIPCHK(h, d2h(uniforms, h->uniforms.p, sizeof(double) * n_uniforms(h)));
+  if (re_u && R > 1) HIPCHK(h, d2h(re_u, h->re_u.p, sizeof(double) * n_re_u(h)));
+  HIPCHK(h, hipStreamSynchronize(s));
+  return JR_OK;
+}
+
+int jr_replica_exchange(jr_handle* h, const double* re_u, int32_t* acc,
+                        int32_t* att) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  const auto& c = h->cfg;
+  const int R = c.n_replicas;
+  if (R < 2) return JR_OK;
+  HIPCHK(h, hipSetDevice(c.device));
+  DBuf<int> dacc, datt;
+  HIPCHK(h, dacc.alloc(R - 1));
+  HIPCHK(h, datt.alloc(R - 1));
+  if (re_u) {
+    HIPCHK(h, hipMemcpyAsync(h->re_u.p, re_u, sizeof(double) * n_re_u(h), hipMemcpyHostToDevice, h->stream));
+  } else {
+    // independent stream id so standalone passes do not reuse iteration draws
+    hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(n_re_u(h))), dim3(256), 0, h->stream,
+                       h->re_u.p, (long long)n_re_u(h), (unsigned long long)c.seed,
+                       (const IterState*)nullptr, (long long)(h->trial_rng++), 9u);
+  }
+  ReArgs ra = re_args(h, nullptr);
+  ra.acc = dacc.p;
+  ra.att = datt.p;
+  hipLaunchKernelGGL(re_kernel, dim3(1), dim3(256), 0, h->stream, ra);
+  HIPCHK(h, hipGetLastError());
+  if (acc) HIPCHK(h, hipMemcpyAsync(acc, dacc.p, sizeof(int) * (R - 1), hipMemcpyDeviceToHost, h->stream));
+  if (att) HIPCHK(h, hipMemcpyAsync(att, datt.p, sizeof(int) * (R - 1), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return JR_OK;
+}
+
+int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
+                   const double* emax, int32_t n_samples,
+                   const int32_t* sample_idx, const double* normals,
+                   const double* uniforms, int32_t* accepted,
+                   int32_t* reason) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  const auto& c = h->cfg;
+  if (!h->have_pop) return h->fail(JR_E_STATE, "population not set");
+  if (move_idx < 0 || move_idx >= c.n_moves) return h->fail(JR_E_ARG, "bad move index");
+  if (n_samples < 1 || !sample_idx || !trial_step)
+    return h->fail(JR_E_ARG, "need n_samples >= 1, sample_idx, trial_step");
+  if ((normals == nullptr) != (uniforms == nullptr))
+    return h->fail(JR_E_ARG, "normals and uniforms must both be given or both NULL");
+  const int R = c.n_replicas, Nm = c.n_atoms_max;
+  const size_t T = (size_t)R * n_samples;
+  for (size_t q = 0; q < T; ++q)
+    if (sample_idx[q] < 0 || sample_idx[q] >= c.n_walkers)
+      return h->fail(JR_E_ARG, "sample index out of range");
+  HIPCHK(h, hipSetDevice(c.device));
+  if (h->t_idx.n < T) {
+    HIPCHK(h, h->t_idx.alloc(T));
+    HIPCHK(h, h->t_reason.alloc(T));
+    HIPCHK(h, h->t_normals.alloc(T * 3 * Nm));
+    HIPCHK(h, h->t_uniforms.alloc(T * 4));
+  }
+  if (!h->t_step.p) {
+    HIPCHK(h, h->t_step.alloc(R));
+    HIPCHK(h, h->t_emax.alloc(R));
+  }
+  hipStream_t s = h->stream;
+  HIPCHK(h, hipMemcpyAsync(h->t_idx.p, sample_idx, sizeof(int) * T, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->t_step.p, trial_step, sizeof(double) * R, hipMemcpyHostToDevice, s));
+  if (emax) HIPCHK(h, hipMemcpyAsync(h->t_emax.p, emax, sizeof(double) * R, hipMemcpyHostToDevice, s));
+  else HIPCHK(h, hipMemcpyAsync(h->t_emax.p, h->emax.p, sizeof(double) * R, hipMemcpyDeviceToDevice, s));
+  if (normals) {
+    HIPCHK(h, hipMemcpyAsync(h->t_normals.p, normals, sizeof(double) * T * 3 * Nm, hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->t_uniforms.p, uniforms, sizeof(double) * T * 4, hipMemcpyHostToDevice, s));
+  } else {
+    long long it = h->trial_rng++;
+    hipLaunchKernelGGL(fill_normal_kernel, dim3(grid_for(T * 3 * Nm)), dim3(256), 0, s,
+                       h->t_normals.p, (long long)(T * 3 * Nm), (unsigned long long)c.seed,
+                       (const IterState*)nullptr, it, 10u);
+    hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(T * 4)), dim3(256), 0, s,
+                       h->t_uniforms.p, (long long)(T * 4), (unsigned long long)c.seed,
+                       (const IterState*)nullptr, it, 11u);
+  }
+  HIPCHK(h, hipMemsetAsync(h->queue.p, 0, sizeof(int), s));
+  WalkArgs a = walk_args(h);
+  a.emax = h->t_emax.p;
+  a.normals = h->t_normals.p;
+  a.uniforms = h->t_uniforms.p;
+  a.total = (int)T;
+  a.trial = 1;
+  a.trial_move = move_idx;
+  a.n_samples = n_samples;
+  a.sample_idx = h->t_idx.p;
+  a.trial_step = h->t_step.p;
+  a.trial_reason = h->t_reason.p;
+  a.counters = nullptr;
+  int blocks = std::max(1, std::min((int)((T + h->wpb - 1) / h->wpb), h->blocks));
+  launch_walk(h->A, h->multi, a, blocks, h->wpb, h->lds_walk, s);
+  HIPCHK(h, hipGetLastError());
+  std::vector<int> rs(T);
+  HIPCHK(h, hipMemcpyAsync(rs.data(), h->t_reason.p, sizeof(int) * T, hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  for (size_t q = 0; q < T; ++q) {
+    if (accepted) accepted[q] = rs[q] == 0;
+    if (reason) reason[q] = rs[q];
+  }
+  return JR_OK;
+}
+
+int jr_walk_timing(jr_handle* h, double* total_ms, int64_t* launches,
+                   int32_t reset) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  if (total_ms) *total_ms = h->walk_ms;
+  if (launches) *launches = h->walk_launches;
+  if (reset) {
+    h->walk_ms = 0.0;
+    h->walk_launches = 0;
+  }
+  return JR_OK;
+}
+
+int jr_get_info(jr_handle* h, int32_t* n_cu, int32_t* blocks,
+                int32_t* waves_per_block) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  if (n_cu) *n_cu = h->n_cu;
+  if (blocks) *blocks = h->blocks;
+  if (waves_per_block) *waves_per_block = h->wpb;
+  return JR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,281 @@
+// MWG move kernels for one walker per wavefront (register-resident state).
+#pragma once
+
+#include "jr_walk.h"
+
+namespace jr {
+
+template <int A>
+struct WalkerRegs {
+  double x[A], y[A], z[A];     // positions
+  double dx[A], dy[A], dz[A];  // GMC persistent direction
+  int t[A];                    // species
+  double cell[9];
+  double E;                    // full Hamiltonian U + P V - mu.N
+  int counts[JR_MAX_SPECIES];
+};
+
+struct WalkCtx {
+  WaveLds L;
+  PairTab tab;
+  int N, Nmax, S;
+  double pressure;
+  const double* mu;  // [S]
+  double emax;
+};
+
+__device__ __forceinline__ double tape_xi(const double* xi, int Nmax, int c,
+                                          int a) {
+  return xi[c * Nmax + a];
+}
+
+// sampling/moves/galilean.py:142-300 (use_forces=True, unscoped).
+template <int A, bool MULTI>
+__device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
+                         const DevMove& mv, const double* xi) {
+  const int lane = lane_id();
+  const int N = C.N;
+  double dn = sqrt(wave_dot<A>(N, W.dx, W.dy, W.dz, W.dx, W.dy, W.dz));
+  const bool fresh = dn < 1e-8;
+  double vx[A], vy[A], vz[A];
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    const int a = lane + 64 * s;
+    double zx = 0.0, zy = 0.0, zz = 0.0;
+    if (a < N) {
+      zx = tape_xi(xi, C.Nmax, 0, a);
+      zy = tape_xi(xi, C.Nmax, 1, a);
+      zz = tape_xi(xi, C.Nmax, 2, a);
+    }
+    if (fresh) {
+      vx[s] = zx;
+      vy[s] = zy;
+      vz[s] = zz;
+    } else {
+      vx[s] = W.dx[s] + mv.perturb_angle * zx;
+      vy[s] = W.dy[s] + mv.perturb_angle * zy;
+      vz[s] = W.dz[s] + mv.perturb_angle * zz;
+    }
+  }
+  double vn = sqrt(wave_dot<A>(N, vx, vy, vz, vx, vy, vz));
+  double den = fmax(vn, 1e-10);
+  double x0[A], y0[A], z0[A];
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    vx[s] = vx[s] / den;
+    vy[s] = vy[s] / den;
+    vz[s] = vz[s] / den;
+    W.dx[s] = vx[s];
+    W.dy[s] = vy[s];
+    W.dz[s] = vz[s];
+    x0[s] = W.x[s];
+    y0[s] = W.y[s];
+    z0[s] = W.z[s];
+  }
+  Geom g;
+  make_geom(W.cell, g);
+  const double ens = ensemble_term(W.cell, C.pressure, C.mu, C.S, W.counts);
+  double H = W.E;
+  for (int r = 0; r < mv.n_reflect; ++r) {
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      W.x[s] = W.x[s] + step * W.dx[s];
+      W.y[s] = W.y[s] + step * W.dy[s];
+      W.z[s] = W.z[s] + step * W.dz[s];
+    }
+    double el, fx[A], fy[A], fz[A];
+    wave_eval<A, true, MULTI>(C.L, N, g, C.tab, W.x, W.y, W.z, W.t, el, fx,
+                              fy, fz);
+    double U = wave_sum((lane < N) ? el : 0.0);
+    double Hn = U + ens;
+    if (__builtin_isnan(Hn)) Hn = __builtin_inf();
+    double fsq = wave_dot<A>(N, fx, fy, fz, fx, fy, fz);
+    double fdt = wave_dot<A>(N, fx, fy, fz, W.dx, W.dy, W.dz);
+    if (Hn >= C.emax) {
+      double fn = fmax(sqrt(fsq), 1e-10);
+      double tt = 2.0 * (fdt / fn);
+#pragma unroll
+      for (int s = 0; s < A; ++s) {
+        W.dx[s] = fma(-tt, fx[s] / fn, W.dx[s]);
+        W.dy[s] = fma(-tt, fy[s] / fn, W.dy[s]);
+        W.dz[s] = fma(-tt, fz[s] / fn, W.dz[s]);
+      }
+    }
+    H = Hn;
+  }
+  const bool accepted = H < C.emax;
+  if (accepted) {
+    W.E = H;
+  } else {
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      W.x[s] = x0[s];
+      W.y[s] = y0[s];
+      W.z[s] = z0[s];
+      W.dx[s] = -vx[s];
+      W.dy[s] = -vy[s];
+      W.dz[s] = -vz[s];
+    }
+  }
+  return accepted;
+}
+
+// sampling/moves/volume.py:49-109 + _common.py:16-93; returns reject reason.
+template <int A, bool MULTI>
+__device__ int volume_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
+                           const DevMove& mv, double xi0, double u0) {
+  const int lane = lane_id();
+  const int N = C.N;
+  double oldV = fabs(det3(W.cell));
+  double dV = (step * (double)N) * xi0;
+  double newV = fabs(oldV + dV);
+  double ratio = newV / oldV;
+  double scale = cbrt_c(ratio);
+  double nc[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) nc[i] = W.cell[i] * scale;
+  double nx[A], ny[A], nz[A];
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    nx[s] = W.x[s] * scale;
+    ny[s] = W.y[s] * scale;
+    nz[s] = W.z[s] * scale;
+  }
+  Geom g;
+  make_geom(nc, g);
+  double el, fx[A], fy[A], fz[A];
+  wave_eval<A, false, MULTI>(C.L, N, g, C.tab, nx, ny, nz, W.t, el, fx, fy,
+                             fz);
+  double U = wave_sum((lane < N) ? el : 0.0);
+  double Hn = U + ensemble_term(nc, C.pressure, C.mu, C.S, W.counts);
+  bool cell_ok =
+      check_cell_shape(nc, N, mv.max_vpa, mv.min_vpa, mv.min_aspect);
+  double p_acc = mv.flat_v_prior ? 1.0 : fmin(1.0, ipow_c(ratio, N));
+  bool energy_ok = Hn < C.emax;
+  bool prior_ok = u0 < p_acc;
+  if (energy_ok && cell_ok && prior_ok) {
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      W.x[s] = nx[s];
+      W.y[s] = ny[s];
+      W.z[s] = nz[s];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) W.cell[i] = nc[i];
+    W.E = Hn;
+    return 0;
+  }
+  return !energy_ok ? 1 : (!cell_ok ? 2 : 3);
+}
+
+// sampling/moves/hmc.py:51-152; forces at a position are evaluated once and
+// reused by the following half kick (identical values to re-evaluating).
+template <int A, bool MULTI>
+__device__ bool hmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
+                         const DevMove& mv, const double* xi) {
+  const int lane = lane_id();
+  const int N = C.N;
+  double px[A], py[A], pz[A], qx[A], qy[A], qz[A];
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    const int a = lane + 64 * s;
+    px[s] = py[s] = pz[s] = 0.0;
+    if (a < N) {
+      px[s] = tape_xi(xi, C.Nmax, 0, a);
+      py[s] = tape_xi(xi, C.Nmax, 1, a);
+      pz[s] = tape_xi(xi, C.Nmax, 2, a);
+    }
+    qx[s] = W.x[s];
+    qy[s] = W.y[s];
+    qz[s] = W.z[s];
+  }
+  double K0 = 0.5 * wave_dot<A>(N, px, py, pz, px, py, pz);
+  Geom g;
+  make_geom(W.cell, g);
+  const double ens = ensemble_term(W.cell, C.pressure, C.mu, C.S, W.counts);
+  const double h = 0.5 * step;
+  double el, fx[A], fy[A], fz[A];
+  wave_eval<A, true, MULTI>(C.L, N, g, C.tab, qx, qy, qz, W.t, el, fx, fy,
+                            fz);
+  for (int l = 0; l < mv.n_leapfrog; ++l) {
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      px[s] = px[s] + h * fx[s];
+      py[s] = py[s] + h * fy[s];
+      pz[s] = pz[s] + h * fz[s];
+      qx[s] = qx[s] + step * px[s];
+      qy[s] = qy[s] + step * py[s];
+      qz[s] = qz[s] + step * pz[s];
+    }
+    wave_eval<A, true, MULTI>(C.L, N, g, C.tab, qx, qy, qz, W.t, el, fx, fy,
+                              fz);
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      px[s] = px[s] + h * fx[s];
+      py[s] = py[s] + h * fy[s];
+      pz[s] = pz[s] + h * fz[s];
+    }
+  }
+  double U = wave_sum((lane < N) ? el : 0.0);
+  double Hn = U + ens;
+  double K1 = 0.5 * wave_dot<A>(N, px, py, pz, px, py, pz);
+  double dH = (Hn + K1) - (W.E + K0);
+  bool accepted = (Hn < C.emax) && (dH < 1.0);
+  if (accepted) {
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      W.x[s] = qx[s];
+      W.y[s] = qy[s];
+      W.z[s] = qz[s];
+    }
+    W.E = Hn;
+  }
+  return accepted;
+}
+
+// sampling/moves/alchemical.py:40-79 (build_morph_kernel).
+template <int A, bool MULTI>
+__device__ bool morph_move(const WalkCtx& C, WalkerRegs<A>& W,
+                           const DevMove& mv, double u0, double u1) {
+  const int lane = lane_id();
+  const int N = C.N;
+  const int S = mv.n_species;
+  int atom = (int)(u0 * (double)N);
+  if (atom > N - 1) atom = N - 1;
+  int cand = (int)(u1 * (double)(S - 1));
+  if (cand > S - 2) cand = S - 2;
+  const int al = atom & 63, as = atom >> 6;
+  int curv = 0;
+#pragma unroll
+  for (int s = 0; s < A; ++s)
+    if (s == as) curv = W.t[s];
+  const int cur = __builtin_amdgcn_readlane(curv, al);
+  const int nt = cand + (cand >= cur ? 1 : 0);
+#pragma unroll
+  for (int s = 0; s < A; ++s)
+    if (s == as && lane == al) W.t[s] = nt;
+#pragma unroll
+  for (int sp = 0; sp < JR_MAX_SPECIES; ++sp)
+    W.counts[sp] += (sp == nt ? 1 : 0) - (sp == cur ? 1 : 0);
+  Geom g;
+  make_geom(W.cell, g);
+  double el, fx[A], fy[A], fz[A];
+  wave_eval<A, false, MULTI>(C.L, N, g, C.tab, W.x, W.y, W.z, W.t, el, fx,
+                             fy, fz);
+  double U = wave_sum((lane < N) ? el : 0.0);
+  double Hn = U + ensemble_term(W.cell, C.pressure, C.mu, C.S, W.counts);
+  bool accepted = Hn < C.emax;
+  if (accepted) {
+    W.E = Hn;
+  } else {
+#pragma unroll
+    for (int s = 0; s < A; ++s)
+      if (s == as && lane == al) W.t[s] = cur;
+#pragma unroll
+    for (int sp = 0; sp < JR_MAX_SPECIES; ++sp)
+      W.counts[sp] += (sp == cur ? 1 : 0) - (sp == nt ? 1 : 0);
+  }
+  return accepted;
+}
+
+}  // namespace jr

@@ -1,0 +1,359 @@
+"""Device-resident nested-sampling engine: the drop-in for the reference's
+``ns_step`` / ``replica_exchange_step`` / step-size trials, on one GPU.
+
+Reference call sites replaced: ``sampling/run_loop.py:419-551`` (jit'd
+``ns_step`` per iteration, RE pass, adaptation), ``sampling/nested_sampling.py:
+391-686`` (``ns_step``), ``sampling/moves/replica_exchange.py:318-436``.
+Replica axis P of the reference's ``PmapVmapRuns(G=1, P)`` is the leading
+axis R here; the population lives in HBM between calls.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .backends import Hamiltonian, LJBackend, unwrap_model
+from .draws import Tape
+from .moves import MWG, CellSpec
+from .prior_mass import dead_log_weight, final_log_evidence
+
+
+@dataclass
+class RunConfig:
+    n_replicas: int
+    n_walkers: int
+    n_atoms: int | list[int]          # per replica
+    n_mcmc_steps: int = 20
+    n_extra: int = 0
+    n_cull: int = 1
+    seed: int = 42
+    pressures: list[float] | None = None
+    chemical_potentials: list[list[float]] | None = None
+    re_interval: int = 0
+    n_swap_cycles: int = 1
+    waves_per_block: int = 0
+    device: int = 0
+
+
+def _model_of(backend) -> tuple[LJBackend, float | None, np.ndarray | None]:
+    model = unwrap_model(backend)
+    if not isinstance(model, LJBackend):
+        raise TypeError("the device walk needs an LJBackend model")
+    pressure, mu = None, None
+    if isinstance(backend, Hamiltonian):
+        pm = backend.ensemble_params_of(None)
+        if pm is None:
+            raise TypeError("only EnsembleTerm terms are fused on the device")
+        pressure, mu = pm
+    return model, pressure, mu
+
+
+class NSEngine:
+    """One handle of ``libjrhip``: population, run parameters and draws."""
+
+    def __init__(self, backend, moves, run: RunConfig, cell: CellSpec | dict | None = None):
+        self.lib = _lib.load_library()
+        self.model, p_default, mu_default = _model_of(backend)
+        self.mwg = MWG(moves, cell)
+        self.run = run
+        R, K = run.n_replicas, run.n_walkers
+        n_atoms = run.n_atoms
+        self.n_atoms_rep = (np.full(R, int(n_atoms), np.int32)
+                            if np.isscalar(n_atoms) else np.asarray(n_atoms, np.int32))
+        if self.n_atoms_rep.shape != (R,):
+            raise ValueError("n_atoms must be an int or one value per replica")
+        self.Nmax = int(self.n_atoms_rep.max())
+        self.S = self.model.n_species
+        self.n_replicas, self.n_walkers = R, K
+        self.n_walk = run.n_cull + run.n_extra
+        self.n_moves = self.mwg.n_moves
+        pot, self._pot_keep = self.model.table.as_struct()
+        self._moves_arr = self.mwg.device_table()
+        self._weights = _lib.f64(self.mwg.weights)
+        cfg = _lib.JrConfig()
+        cfg.n_replicas = R
+        cfg.n_walkers = K
+        cfg.n_atoms_max = self.Nmax
+        cfg.n_moves = self.n_moves
+        cfg.n_mcmc_steps = run.n_mcmc_steps
+        cfg.n_extra = run.n_extra
+        cfg.n_cull = run.n_cull
+        cfg.n_swap_cycles = run.n_swap_cycles
+        cfg.re_interval = run.re_interval
+        cfg.device = run.device
+        cfg.waves_per_block = run.waves_per_block
+        cfg.seed = run.seed & 0xFFFFFFFFFFFFFFFF
+        cfg.pot = pot
+        cfg.moves = C.cast(self._moves_arr, C.POINTER(_lib.JrMove))
+        cfg.move_weights = _lib.dptr(self._weights)
+        h = C.c_void_p()
+        _lib.check(self.lib.jr_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.iteration = 0
+        self.log_evidence = np.full(R, -np.inf)
+        pressures = (run.pressures if run.pressures is not None
+                     else [0.0 if p_default is None else p_default] * R)
+        mu = run.chemical_potentials
+        if mu is None:
+            mu = np.zeros((R, self.S))
+            if mu_default is not None:
+                mu[:, : len(mu_default)] = mu_default
+        self.pressure = _lib.f64(pressures)
+        self.mu = _lib.f64(np.asarray(mu, dtype=np.float64).reshape(R, self.S))
+        self.step_sizes = _lib.f64(np.tile(self.mwg.default_step_sizes, (R, 1)))
+        self.set_run_params()
+
+    # ------------------------------------------------------------ lifetime
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.jr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        _lib.check(rc, self.h)
+
+    # ---------------------------------------------------------- parameters
+    def set_run_params(self, pressure=None, mu=None, step_sizes=None):
+        if pressure is not None:
+            self.pressure = _lib.f64(pressure)
+        if mu is not None:
+            self.mu = _lib.f64(np.asarray(mu).reshape(self.n_replicas, self.S))
+        if step_sizes is not None:
+            self.step_sizes = _lib.f64(
+                np.asarray(step_sizes).reshape(self.n_replicas, self.n_moves))
+        self._check(self.lib.jr_set_run_params(
+            self.h, _lib.dptr(self.pressure), _lib.dptr(self.mu),
+            _lib.dptr(self.step_sizes)))
+
+    # ---------------------------------------------------------- population
+    def set_population(self, positions, types, cell, n_atoms=None, direction=None):
+        R, K, Nm = self.n_replicas, self.n_walkers, self.Nmax
+        pos = np.zeros((R, K, Nm, 3))
+        p = np.asarray(positions, dtype=np.float64)
+        pos[:, :, : p.shape[2]] = p
+        ty = np.zeros((R, K, Nm), np.int32)
+        t = np.asarray(types)
+        ty[:, :, : t.shape[2]] = t
+        na = (np.broadcast_to(self.n_atoms_rep[:, None], (R, K)) if n_atoms is None
+              else np.asarray(n_atoms))
+        na = _lib.i32(na)
+        cl = _lib.f64(np.asarray(cell, dtype=np.float64).reshape(R, K, 9))
+        d = None
+        if direction is not None:
+            d = np.zeros((R, K, Nm, 3))
+            dd = np.asarray(direction, dtype=np.float64)
+            d[:, :, : dd.shape[2]] = dd
+            d = _lib.f64(d)
+        self._check(self.lib.jr_set_population(
+            self.h, _lib.dptr(_lib.f64(pos)), _lib.iptr(ty), _lib.iptr(na),
+            _lib.dptr(cl), _lib.dptr(d)))
+
+    def get_population(self) -> dict:
+        R, K, Nm = self.n_replicas, self.n_walkers, self.Nmax
+        pos = np.empty((R, K, Nm, 3))
+        d = np.empty((R, K, Nm, 3))
+        ty = np.empty((R, K, Nm), np.int32)
+        na = np.empty((R, K), np.int32)
+        cl = np.empty((R, K, 3, 3))
+        e = np.empty((R, K))
+        self._check(self.lib.jr_get_population(
+            self.h, _lib.dptr(pos), _lib.iptr(ty), _lib.iptr(na), _lib.dptr(cl),
+            _lib.dptr(e), _lib.dptr(d)))
+        return {"positions": pos, "types": ty, "n_atoms": na, "cell": cl,
+                "energy": e, "direction": d}
+
+    def init_walkers(self, rng: np.random.Generator, v_lo_per_atom: float,
+                     v_hi_per_atom: float, flat_V_prior: bool = True,
+                     species_p=None):
+        """Cubic cells from the volume prior and uniform positions
+        (``init/cells.py:82-137``, ``init/positions.py:15-32``); species
+        Bernoulli/categorical per atom with probabilities ``species_p``."""
+        R, K = self.n_replicas, self.n_walkers
+        Nm = self.Nmax
+        pos = np.zeros((R, K, Nm, 3))
+        cell = np.zeros((R, K, 3, 3))
+        ty = np.zeros((R, K, Nm), np.int32)
+        for r in range(R):
+            n = int(self.n_atoms_rep[r])
+            u = rng.random(K)
+            v_lo, v_hi = n * v_lo_per_atom, n * v_hi_per_atom
+            if flat_V_prior:
+                vol = v_lo + u * (v_hi - v_lo)
+            else:
+                p = float(n + 1)
+                a = (v_lo / v_hi) ** p if v_hi > 0 else 0.0
+                vol = v_hi * (a + u * (1.0 - a)) ** (1.0 / p)
+            lc = vol ** (1.0 / 3.0)
+            cell[r] = lc[:, None, None] * np.eye(3)
+            frac = rng.random((K, n, 3))
+            pos[r, :, :n] = frac * lc[:, None, None]
+            if species_p is not None:
+                ty[r, :, :n] = rng.choice(len(species_p), size=(K, n), p=species_p)
+        self.set_population(pos, ty, cell)
+
+    # ------------------------------------------------------------- draws
+    @staticmethod
+    def _tape_struct(t: Tape):
+        keep = [_lib.f64(t.sel_u), _lib.f64(t.clone_u), _lib.f64(t.extra_u),
+                _lib.i32(t.schedule), _lib.f64(t.normals), _lib.f64(t.uniforms),
+                _lib.f64(t.re_u)]
+        s = _lib.JrTape(_lib.dptr(keep[0]), _lib.dptr(keep[1]), _lib.dptr(keep[2]),
+                        _lib.iptr(keep[3]), _lib.dptr(keep[4]), _lib.dptr(keep[5]),
+                        _lib.dptr(keep[6]))
+        return s, keep
+
+    def last_tape(self) -> Tape:
+        R, K, Nm, T = self.n_replicas, self.n_walkers, self.Nmax, self.run.n_mcmc_steps
+        nc = max(self.run.n_swap_cycles, 1)
+        t = Tape(np.zeros((1, R, K)), np.zeros((1, R)), np.zeros((1, R, K)),
+                 np.zeros((1, T), np.int32),
+                 np.zeros((1, R, self.n_walk, T, 3, Nm)),
+                 np.zeros((1, R, self.n_walk, T, 4)),
+                 np.zeros((1, nc, 2, max(R // 2, 1), 2)))
+        self._check(self.lib.jr_get_last_tape(
+            self.h, _lib.dptr(t.sel_u), _lib.dptr(t.clone_u), _lib.dptr(t.extra_u),
+            _lib.iptr(t.schedule), _lib.dptr(t.normals), _lib.dptr(t.uniforms),
+            _lib.dptr(t.re_u)))
+        return t
+
+    # -------------------------------------------------------- iterations
+    def ns_iterations(self, n_iter: int, tape: Tape | None = None,
+                      graph: bool = False, dead_records: bool = False) -> dict:
+        """Run ``n_iter`` NS iterations; returns per-iteration outputs and
+        updates ``log_evidence`` (float64 closed-form weights)."""
+        R, Nm, nm = self.n_replicas, self.Nmax, self.n_moves
+        out = {
+            "emax": np.empty((n_iter, R)),
+            "worst": np.empty((n_iter, R), np.int32),
+            "clone": np.empty((n_iter, R), np.int32),
+            "dead_volume": np.empty((n_iter, R)),
+            "counters": np.empty((n_iter, R, nm, _lib.JR_NCOUNT), np.int32),
+            "re_acc": np.empty((n_iter, max(R - 1, 1)), np.int32),
+            "re_att": np.empty((n_iter, max(R - 1, 1)), np.int32),
+            "walk": np.empty((n_iter, R, self.n_walk), np.int32),
+        }
+        if dead_records:
+            out["dead_pos"] = np.empty((n_iter, R, Nm, 3))
+            out["dead_types"] = np.empty((n_iter, R, Nm), np.int32)
+            out["dead_cell"] = np.empty((n_iter, R, 3, 3))
+        o = _lib.JrIterOut()
+        for k in ("emax", "dead_volume", "dead_pos", "dead_cell"):
+            if k in out:
+                setattr(o, k, _lib.dptr(out[k]))
+        for k in ("worst", "clone", "counters", "re_acc", "re_att", "walk",
+                  "dead_types"):
+            if k in out:
+                setattr(o, k, _lib.iptr(out[k]))
+        self._check(self.lib.jr_set_iteration(self.h, self.iteration))
+        if graph:
+            if tape is not None:
+                raise ValueError("graph replay draws on the device")
+            self._check(self.lib.jr_ns_iterations_graph(self.h, n_iter, C.byref(o)))
+        else:
+            if tape is not None:
+                ts, keep = self._tape_struct(tape)
+                self._check(self.lib.jr_ns_iterations(self.h, n_iter, C.byref(ts),
+                                                      C.byref(o)))
+                del keep
+            else:
+                self._check(self.lib.jr_ns_iterations(self.h, n_iter, None,
+                                                      C.byref(o)))
+        its = self.iteration + np.arange(n_iter)
+        w = dead_log_weight(its, self.n_walkers, self.run.n_cull)[:, 0]  # (n_iter,)
+        for i in range(n_iter):
+            self.log_evidence = np.logaddexp(self.log_evidence, w[i] - out["emax"][i])
+        self.iteration += n_iter
+        if R == 1:
+            out["re_acc"][:] = 0
+            out["re_att"][:] = 0
+        return out
+
+    def ns_step(self, tape: Tape | None = None) -> list[dict]:
+        """One iteration; one reference-shaped ``info`` dict per replica
+        (keys of ``sampling/nested_sampling.py:653-685``)."""
+        out = self.ns_iterations(1, tape=tape, dead_records=True)
+        infos = []
+        for r in range(self.n_replicas):
+            c = out["counters"][0, r]
+            n_acc = c[:, 1]
+            n_prop = c[:, 0]
+            infos.append({
+                "emax": out["emax"][0, r],
+                "hmax": out["emax"][0, r],
+                "worst_idx": int(out["worst"][0, r]),
+                "clone_idx": int(out["clone"][0, r]),
+                "worst_indices": out["worst"][0, r:r + 1],
+                "clone_indices": out["clone"][0, r:r + 1],
+                "acceptance_rate": n_acc.sum() / max(self.n_walk * self.run.n_mcmc_steps, 1),
+                "dead_walker": {"positions": out["dead_pos"][0, r],
+                                "types": out["dead_types"][0, r],
+                                "energy": out["emax"][0, r],
+                                "cell": out["dead_cell"][0, r]},
+                "n_accepted_per_move": n_acc,
+                "n_proposed_per_move": n_prop,
+                "reject_reason_counts_per_move": c[:, 1:5],
+                "n_evaluations_per_move": c[:, 5],
+                "n_grad_evaluations_per_move": c[:, 6],
+                "inter_re_acc": out["re_acc"][0],
+                "inter_re_att": out["re_att"][0],
+            })
+        return infos
+
+    def final_log_evidence(self) -> np.ndarray:
+        pop = self.get_population()
+        return final_log_evidence(self.log_evidence, self.iteration, pop["energy"],
+                                  self.n_walkers, self.run.n_cull)
+
+    def emax(self) -> np.ndarray:
+        e = np.empty(self.n_replicas)
+        self._check(self.lib.jr_get_emax(self.h, _lib.dptr(e)))
+        return e
+
+    def replica_exchange(self, re_u=None) -> dict:
+        R = self.n_replicas
+        acc = np.zeros(max(R - 1, 1), np.int32)
+        att = np.zeros(max(R - 1, 1), np.int32)
+        ru = None if re_u is None else _lib.f64(re_u)
+        self._check(self.lib.jr_replica_exchange(self.h, _lib.dptr(ru),
+                                                 _lib.iptr(acc), _lib.iptr(att)))
+        return {"n_accepted": int(acc.sum()), "n_attempted": int(att.sum()),
+                "n_energy_evals": 0, "n_accepted_per_pair": acc[: R - 1],
+                "n_attempted_per_pair": att[: R - 1]}
+
+    def trial_moves(self, move_idx: int, trial_step, sample_idx, emax=None,
+                    normals=None, uniforms=None):
+        R = self.n_replicas
+        idx = _lib.i32(sample_idx)
+        n = idx.shape[1]
+        acc = np.empty((R, n), np.int32)
+        rs = np.empty((R, n), np.int32)
+        ts = _lib.f64(np.broadcast_to(np.asarray(trial_step, dtype=np.float64), (R,)))
+        em = None if emax is None else _lib.f64(emax)
+        nn = None if normals is None else _lib.f64(normals)
+        uu = None if uniforms is None else _lib.f64(uniforms)
+        self._check(self.lib.jr_trial_moves(
+            self.h, int(move_idx), _lib.dptr(ts), _lib.dptr(em), n, _lib.iptr(idx),
+            _lib.dptr(nn), _lib.dptr(uu), _lib.iptr(acc), _lib.iptr(rs)))
+        return acc.astype(bool), rs
+
+    def walk_timing(self, reset: bool = False) -> tuple[float, int]:
+        ms = C.c_double()
+        n = C.c_int64()
+        self._check(self.lib.jr_walk_timing(self.h, C.byref(ms), C.byref(n), int(reset)))
+        return ms.value, n.value
+
+    def launch_info(self) -> dict:
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.jr_get_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"n_cu": a.value, "blocks": b.value, "waves_per_block": c.value}

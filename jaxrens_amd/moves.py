@@ -1,0 +1,168 @@
+"""Moves-YAML API of the reference, translated to the device move table.
+
+Field names and defaults follow ``cli/schema/moves.py`` (``BaseMoveSpec``
+:60-120, ``GMCMoveSpec`` :249-406 with the legacy ``galilean`` alias of
+``cli/schema/root.py:29-40``, ``HMCMoveSpec`` :409-447, ``VolumeMoveSpec``
+:628-682, ``AlchemicalMorphMoveSpec`` :791-828) and ``cli/schema/cell.py``
+for the cell-geometry guard of the volume move.  ``build_mwg`` mirrors
+``sampling/mwg.py:40-259``: weights are normalised to probabilities and each
+move carries its own step size.
+"""
+
+from __future__ import annotations
+
+from typing import Annotated, Literal, Union
+
+import numpy as np
+from pydantic import BaseModel, ConfigDict, Field, TypeAdapter
+
+from . import _lib
+
+
+class CellSpec(BaseModel):
+    """``cli/schema/cell.py``: volume-per-atom band, aspect guard, V prior."""
+
+    model_config = ConfigDict(extra="forbid", frozen=True)
+    max_volume_per_atom: float = 1e4
+    min_volume_per_atom: float = 1.0
+    initial_min_volume_per_atom: float | None = None
+    min_aspect_ratio: float = 0.8
+    flat_V_prior: bool = False
+
+
+class BaseMoveSpec(BaseModel):
+    model_config = ConfigDict(extra="forbid", frozen=True)
+    step_size: float = 0.1
+    weight: float = 1.0
+    adaptation_warmup: int = 100
+    target_acceptance: float = 0.5
+    name: str | None = None
+
+    @property
+    def move_type(self) -> str:
+        return self.type  # type: ignore[attr-defined]
+
+    def effective_name(self) -> str:
+        return self.name if self.name is not None else self.type  # type: ignore[attr-defined]
+
+
+class GMCMoveSpec(BaseMoveSpec):
+    """Galilean MC with force reflection (``sampling/moves/galilean.py``)."""
+
+    type: Literal["gmc"] = "gmc"
+    n_reflect: int = 5
+    species: tuple[str, ...] | None = None
+    refresh_direction: bool = False
+
+    def device_move(self, cell: CellSpec) -> _lib.JrMove:
+        if self.species is not None:
+            raise NotImplementedError("species-scoped gmc is not built")
+        if self.refresh_direction:
+            raise NotImplementedError("refresh_direction is not built")
+        if self.n_reflect < 1:
+            raise ValueError("n_reflect must be >= 1")
+        return _lib.JrMove(type=_lib.JR_MOVE_GMC, n_reflect=self.n_reflect,
+                           perturb_angle=0.1)
+
+
+class HMCMoveSpec(BaseMoveSpec):
+    """Leapfrog HMC with the hard ``dH < 1`` cut (``sampling/moves/hmc.py``)."""
+
+    type: Literal["hmc"] = "hmc"
+    n_leapfrog: int = 10
+
+    def device_move(self, cell: CellSpec) -> _lib.JrMove:
+        if self.n_leapfrog < 0:
+            raise ValueError("n_leapfrog must be >= 0")
+        return _lib.JrMove(type=_lib.JR_MOVE_HMC, n_leapfrog=self.n_leapfrog)
+
+
+class VolumeMoveSpec(BaseMoveSpec):
+    """Isotropic volume move (``sampling/moves/volume.py``)."""
+
+    type: Literal["volume"] = "volume"
+
+    def device_move(self, cell: CellSpec) -> _lib.JrMove:
+        return _lib.JrMove(type=_lib.JR_MOVE_VOLUME,
+                           flat_v_prior=int(cell.flat_V_prior),
+                           max_vpa=cell.max_volume_per_atom,
+                           min_vpa=cell.min_volume_per_atom,
+                           min_aspect=cell.min_aspect_ratio)
+
+
+class AlchemicalMorphMoveSpec(BaseMoveSpec):
+    """Random atom to a random other species (``sampling/moves/alchemical.py``)."""
+
+    type: Literal["alchemical_morph"] = "alchemical_morph"
+    n_species: int = 2
+
+    def device_move(self, cell: CellSpec) -> _lib.JrMove:
+        if self.n_species < 2:
+            raise ValueError("alchemical_morph needs n_species >= 2")
+        return _lib.JrMove(type=_lib.JR_MOVE_MORPH, n_species=self.n_species)
+
+
+MoveSpec = Annotated[
+    Union[GMCMoveSpec, HMCMoveSpec, VolumeMoveSpec, AlchemicalMorphMoveSpec],
+    Field(discriminator="type"),
+]
+_MOVE_ADAPTER = TypeAdapter(MoveSpec)
+
+_UNBUILT = {"random_walk", "single_atom", "single_atom_sweep", "species_swap",
+            "shear", "stretch"}
+
+
+def parse_move(spec) -> BaseMoveSpec:
+    """Validate one move dict (``type: galilean`` is accepted as ``gmc``)."""
+    if isinstance(spec, BaseMoveSpec):
+        return spec
+    d = dict(spec)
+    if d.get("type") == "galilean":
+        d["type"] = "gmc"
+    if d.get("type") in _UNBUILT:
+        raise NotImplementedError(f"move type {d['type']!r} is not built")
+    return _MOVE_ADAPTER.validate_python(d)
+
+
+def parse_moves(specs) -> list[BaseMoveSpec]:
+    moves = [parse_move(s) for s in specs]
+    if not moves:
+        raise ValueError("at least one move is required")
+    if len(moves) > _lib.JR_MAX_MOVES:
+        raise ValueError(f"at most {_lib.JR_MAX_MOVES} moves")
+    names = [m.effective_name() for m in moves]
+    if len(set(names)) != len(names):
+        raise ValueError(f"move names must be distinct, got {names}")
+    return moves
+
+
+class MWG:
+    """Device move table + normalised weights (``sampling/mwg.py:40-259``)."""
+
+    def __init__(self, moves, cell: CellSpec | dict | None = None):
+        self.moves = parse_moves(moves)
+        self.cell = (cell if isinstance(cell, CellSpec)
+                     else CellSpec(**(cell or {})))
+        w = np.array([m.weight for m in self.moves], dtype=np.float64)
+        if (w < 0).any() or not (w.sum() > 0):
+            raise ValueError("move weights must be >= 0 and not all zero")
+        self.weights = w
+        self.move_probs = w / w.sum()
+        self.default_step_sizes = np.array([m.step_size for m in self.moves])
+        self.names = [m.effective_name() for m in self.moves]
+        self.n_moves = len(self.moves)
+
+    def device_table(self):
+        arr = (_lib.JrMove * self.n_moves)()
+        for i, m in enumerate(self.moves):
+            arr[i] = m.device_move(self.cell)
+        return arr
+
+    def as_oracle_moves(self):
+        """Plain tuples for the CPU oracle (same fields as the device table)."""
+        return [m.device_move(self.cell) for m in self.moves]
+
+
+def build_mwg(backend, move_descriptors, cell=None) -> MWG:
+    """Reference-shaped entry point; ``backend`` is held by the engine."""
+    return MWG(move_descriptors, cell)

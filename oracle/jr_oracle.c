@@ -1,0 +1,737 @@
+/*
+ * CPU oracle (test infrastructure only) -- FP64 restatement of the reference
+ * nested-sampling inner walk.  See jr_oracle.h.
+ *
+ * Canonical evaluation order.  The reference leaves summation order to XLA;
+ * this file fixes one order and the device kernels follow the same order so
+ * that the two can be compared bit-for-bit:
+ *   - "lane partial" of a per-atom quantity q: lane p (0..63) holds
+ *     q[p] + q[p+64] + q[p+128] + ... (ascending, atoms < N only);
+ *   - "wave sum": butterfly over the 64 lane partials, v[l] += v[l ^ m] for
+ *     m = 1, 2, 4, 8, 16, 32 (lanes >= N hold +0.0);
+ *   - dot3(a, b) = fma(a2, b2, fma(a1, b1, a0*b0)).
+ * Compiled with -ffp-contract=off: only the fma() calls written here fuse.
+ */
+#include "jr_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define WAVE 64
+
+static inline double dot3(double a0, double a1, double a2, double b0,
+                          double b1, double b2) {
+  return fma(a2, b2, fma(a1, b1, a0 * b0));
+}
+
+/* Butterfly sum of lane partials built from per-atom values q[0..N). */
+static double wave_sum_atoms(const double* q, int N) {
+  double v[WAVE];
+  for (int l = 0; l < WAVE; ++l) {
+    double acc = 0.0;
+    int first = 1;
+    for (int a = l; a < N; a += WAVE) {
+      if (first) {
+        acc = q[a];
+        first = 0;
+      } else {
+        acc = acc + q[a];
+      }
+    }
+    v[l] = acc;
+  }
+  for (int m = 1; m < WAVE; m <<= 1) {
+    double t[WAVE];
+    for (int l = 0; l < WAVE; ++l) t[l] = v[l] + v[l ^ m];
+    memcpy(v, t, sizeof(v));
+  }
+  return v[0];
+}
+
+/* |det| of a 3x3 row-major cell; utils/cell.py:12-14 (get_volume). */
+double or_det3(const double* c) {
+  double m0 = c[4] * c[8] - c[5] * c[7];
+  double m1 = c[3] * c[8] - c[5] * c[6];
+  double m2 = c[3] * c[7] - c[4] * c[6];
+  return c[0] * m0 - c[1] * m1 + c[2] * m2;
+}
+
+/* Adjugate-over-determinant inverse (row-vector convention). */
+static void inv3(const double* c, double det, double* inv) {
+  double idet = 1.0 / det;
+  inv[0] = (c[4] * c[8] - c[5] * c[7]) * idet;
+  inv[1] = (c[2] * c[7] - c[1] * c[8]) * idet;
+  inv[2] = (c[1] * c[5] - c[2] * c[4]) * idet;
+  inv[3] = (c[5] * c[6] - c[3] * c[8]) * idet;
+  inv[4] = (c[0] * c[8] - c[2] * c[6]) * idet;
+  inv[5] = (c[2] * c[3] - c[0] * c[5]) * idet;
+  inv[6] = (c[3] * c[7] - c[4] * c[6]) * idet;
+  inv[7] = (c[1] * c[6] - c[0] * c[7]) * idet;
+  inv[8] = (c[0] * c[4] - c[1] * c[3]) * idet;
+}
+
+/* Cube root by an integer initial guess on the IEEE bits and five Newton
+ * steps y <- (2y + x/y^2)/3; used for the isotropic scale (V'/V)^(1/3) of
+ * sampling/moves/volume.py:60 and V^(1/3) of utils/cell.py:93. */
+double or_cbrt(double x) {
+  if (!(x > 0.0) || isinf(x)) return x; /* 0, negative (unused), nan, inf */
+  const int64_t bias = (int64_t)1023 << 52;
+  int64_t b;
+  memcpy(&b, &x, 8);
+  int64_t g = (b - bias) / 3 + bias;
+  double y;
+  memcpy(&y, &g, 8);
+  for (int it = 0; it < 5; ++it) {
+    double t = y * y;
+    y = (2.0 * y + x / t) / 3.0;
+  }
+  return y;
+}
+
+/* x**n for integer n >= 0 by binary exponentiation (volume.py:92). */
+double or_ipow(double x, int n) {
+  double r = 1.0, b = x;
+  while (n > 0) {
+    if (n & 1) r = r * b;
+    b = b * b;
+    n >>= 1;
+  }
+  return r;
+}
+
+/* utils/cell.py:80-125 (min_aspect_ratio, check_cell_shape). */
+int or_check_cell_shape(const double* c, int n_atoms, double max_vpa,
+                        double min_vpa, double min_aspect) {
+  double vol = fabs(or_det3(c));
+  double vpa = vol / (double)n_atoms;
+  double vc = or_cbrt(vol);
+  static const int pairs[3][3] = {{0, 1, 2}, {1, 2, 0}, {0, 2, 1}};
+  double aspect = 0.0;
+  for (int p = 0; p < 3; ++p) {
+    const double* a = c + 3 * pairs[p][0];
+    const double* b = c + 3 * pairs[p][1];
+    const double* k = c + 3 * pairs[p][2];
+    double cx = a[1] * b[2] - a[2] * b[1];
+    double cy = a[2] * b[0] - a[0] * b[2];
+    double cz = a[0] * b[1] - a[1] * b[0];
+    double nrm = sqrt(dot3(cx, cy, cz, cx, cy, cz));
+    double h = fabs(dot3(cx / nrm, cy / nrm, cz / nrm, k[0], k[1], k[2]));
+    double ratio = h / vc;
+    aspect = (p == 0) ? ratio : fmin(aspect, ratio);
+  }
+  return (vpa <= max_vpa) && (vpa >= min_vpa) && (aspect >= min_aspect);
+}
+
+/* P*V - mu.counts; backends/ensemble.py:34-59. */
+double or_ensemble_energy(int N, const int32_t* types, const double* cell,
+                          double pressure, const double* mu, int S) {
+  double e = pressure * fabs(or_det3(cell));
+  if (mu != NULL && S > 0) {
+    int counts[16] = {0};
+    for (int a = 0; a < N; ++a) counts[types[a]] += 1;
+    double acc = 0.0;
+    for (int s = 0; s < S; ++s) acc = acc + mu[s] * (double)counts[s];
+    e = e - acc;
+  }
+  return e;
+}
+
+/* One pair (i, j): dr = MIC(x_j - x_i); backends/lj.py:141-197. */
+typedef struct {
+  double e, fs, dx, dy, dz;
+} pair_t;
+
+static inline pair_t lj_pair(const double* xi, const double* xj,
+                             const double* cell, const double* inv, int ti,
+                             int tj, const or_pot* pot) {
+  pair_t r;
+  double dx = xj[0] - xi[0], dy = xj[1] - xi[1], dz = xj[2] - xi[2];
+  double f0 = fma(dz, inv[6], fma(dy, inv[3], dx * inv[0]));
+  double f1 = fma(dz, inv[7], fma(dy, inv[4], dx * inv[1]));
+  double f2 = fma(dz, inv[8], fma(dy, inv[5], dx * inv[2]));
+  double n0 = nearbyint(f0), n1 = nearbyint(f1), n2 = nearbyint(f2);
+  dx = fma(-n2, cell[6], fma(-n1, cell[3], fma(-n0, cell[0], dx)));
+  dy = fma(-n2, cell[7], fma(-n1, cell[4], fma(-n0, cell[1], dy)));
+  dz = fma(-n2, cell[8], fma(-n1, cell[5], fma(-n0, cell[2], dz)));
+  double r2 = dot3(dx, dy, dz, dx, dy, dz);
+  int S = pot->n_species;
+  int ab = ti * S + tj;
+  double inv2 = 1.0 / r2;
+  if (!(r2 < pot->rc2)) inv2 = 0.0;
+  double s2 = pot->sig2[ab] * inv2;
+  double s6 = (s2 * s2) * s2;
+  double s12 = s6 * s6;
+  r.e = pot->eps4[ab] * (s12 - s6);
+  r.fs = (pot->eps24[ab] * fma(2.0, s12, -s6)) * inv2;
+  r.dx = dx;
+  r.dy = dy;
+  r.dz = dz;
+  return r;
+}
+
+/* Model energy U (and forces F = -dU/dx when F != NULL), then
+ * H = U + ensemble term; backends/hamiltonian.py:112-153. */
+void or_energy_forces(int N, const double* pos, const int32_t* types,
+                      const double* cell_in, const or_pot* pot,
+                      double pressure, const double* mu, double* U_out,
+                      double* H_out, double* F) {
+  double det = or_det3(cell_in);
+  double cell[9];
+  if (det == 0.0) { /* lj.py:138-139: non-periodic stand-in */
+    memset(cell, 0, sizeof(cell));
+    cell[0] = cell[4] = cell[8] = 1.0e10;
+    det = or_det3(cell);
+  } else {
+    memcpy(cell, cell_in, sizeof(cell));
+  }
+  double inv[9];
+  inv3(cell, det, inv);
+
+  double* e = (double*)calloc((size_t)(N > 0 ? N : 1), sizeof(double));
+  double* own = (double*)calloc((size_t)(3 * (N > 0 ? N : 1)), sizeof(double));
+  double* trav = (double*)calloc((size_t)(3 * (N > 0 ? N : 1)), sizeof(double));
+  int kmax = (N - 1) / 2;
+  int half = (N >= 2) && (N % 2 == 0);
+  for (int k = 1; k <= kmax; ++k) {
+    for (int a = 0; a < N; ++a) {
+      int j = a + k;
+      if (j >= N) j -= N;
+      pair_t p = lj_pair(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
+                         types[j], pot);
+      e[a] = e[a] + p.e;
+      own[3 * a + 0] = fma(-p.fs, p.dx, own[3 * a + 0]);
+      own[3 * a + 1] = fma(-p.fs, p.dy, own[3 * a + 1]);
+      own[3 * a + 2] = fma(-p.fs, p.dz, own[3 * a + 2]);
+      trav[3 * j + 0] = fma(p.fs, p.dx, trav[3 * j + 0]);
+      trav[3 * j + 1] = fma(p.fs, p.dy, trav[3 * j + 1]);
+      trav[3 * j + 2] = fma(p.fs, p.dz, trav[3 * j + 2]);
+    }
+  }
+  if (half) {
+    int h = N / 2;
+    for (int a = 0; a < N; ++a) {
+      int j = a + h;
+      if (j >= N) j -= N;
+      pair_t p = lj_pair(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
+                         types[j], pot);
+      e[a] = e[a] + 0.5 * p.e;
+      own[3 * a + 0] = fma(-p.fs, p.dx, own[3 * a + 0]);
+      own[3 * a + 1] = fma(-p.fs, p.dy, own[3 * a + 1]);
+      own[3 * a + 2] = fma(-p.fs, p.dz, own[3 * a + 2]);
+    }
+  }
+  double U = wave_sum_atoms(e, N);
+  if (F != NULL) {
+    for (int a = 0; a < 3 * N; ++a) F[a] = own[a] + trav[a];
+  }
+  free(e);
+  free(own);
+  free(trav);
+  *U_out = U;
+  if (H_out != NULL)
+    *H_out = U + or_ensemble_energy(N, types, cell_in, pressure, mu,
+                                    pot->n_species);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Moves                                                                  */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+  int N, Nmax, S;
+  const or_pot* pot;
+  double pressure;
+  const double* mu;
+  double emax;
+} chain_ctx;
+
+static double wave_sum_dot(const double* a, const double* b, int N,
+                           double* scratch) {
+  for (int i = 0; i < N; ++i)
+    scratch[i] = dot3(a[3 * i], a[3 * i + 1], a[3 * i + 2], b[3 * i],
+                      b[3 * i + 1], b[3 * i + 2]);
+  return wave_sum_atoms(scratch, N);
+}
+
+/* sampling/moves/galilean.py:142-300 (use_forces=True, no species mask).
+ * Returns accepted. */
+static int gmc_move(const chain_ctx* c, double* x, const int32_t* types,
+                    const double* cell, double* E, double* d, double s,
+                    const or_move* mv, const double* xi) {
+  int N = c->N, Nmax = c->Nmax;
+  double* scr = (double*)malloc(sizeof(double) * (size_t)(N + 1));
+  double* v = (double*)malloc(sizeof(double) * 3 * (size_t)(N + 1));
+  double* x0 = (double*)malloc(sizeof(double) * 3 * (size_t)(N + 1));
+  double* F = (double*)malloc(sizeof(double) * 3 * (size_t)(N + 1));
+  double* fq = (double*)malloc(sizeof(double) * (size_t)(N + 1));
+  double* fd = (double*)malloc(sizeof(double) * (size_t)(N + 1));
+
+  /* direction: fresh if |d_prev| < 1e-8, else perturbed (galilean.py:160-168) */
+  double dn = sqrt(wave_sum_dot(d, d, N, scr));
+  int fresh = dn < 1e-8;
+  for (int a = 0; a < N; ++a)
+    for (int q = 0; q < 3; ++q) {
+      double z = xi[q * Nmax + a];
+      v[3 * a + q] = fresh ? z : d[3 * a + q] + mv->perturb_angle * z;
+    }
+  double vn = sqrt(wave_sum_dot(v, v, N, scr));
+  double den = fmax(vn, 1e-10);
+  for (int i = 0; i < 3 * N; ++i) d[i] = v[i] / den;
+  memcpy(x0, x, sizeof(double) * 3 * (size_t)N);
+  memcpy(v, d, sizeof(double) * 3 * (size_t)N); /* v := initial direction */
+
+  double H = *E;
+  for (int r = 0; r < mv->n_reflect; ++r) {
+    for (int i = 0; i < 3 * N; ++i) x[i] = x[i] + s * d[i];
+    double U, Hn;
+    or_energy_forces(N, x, types, cell, c->pot, c->pressure, c->mu, &U, &Hn,
+                     F);
+    if (isnan(Hn)) Hn = INFINITY;
+    for (int a = 0; a < N; ++a) {
+      fq[a] = dot3(F[3 * a], F[3 * a + 1], F[3 * a + 2], F[3 * a],
+                   F[3 * a + 1], F[3 * a + 2]);
+      fd[a] = dot3(F[3 * a], F[3 * a + 1], F[3 * a + 2], d[3 * a],
+                   d[3 * a + 1], d[3 * a + 2]);
+    }
+    double fsq = wave_sum_atoms(fq, N);
+    double fdot = wave_sum_atoms(fd, N);
+    if (Hn >= c->emax) {
+      double fn = fmax(sqrt(fsq), 1e-10);
+      double t = 2.0 * (fdot / fn);
+      for (int i = 0; i < 3 * N; ++i) d[i] = fma(-t, F[i] / fn, d[i]);
+    }
+    H = Hn;
+  }
+  int accepted = H < c->emax;
+  if (accepted) {
+    *E = H;
+  } else {
+    memcpy(x, x0, sizeof(double) * 3 * (size_t)N);
+    for (int i = 0; i < 3 * N; ++i) d[i] = -v[i];
+  }
+  free(scr);
+  free(v);
+  free(x0);
+  free(F);
+  free(fq);
+  free(fd);
+  return accepted;
+}
+
+/* sampling/moves/volume.py:49-109 + _common.py:16-93.  Returns reason. */
+static int volume_move(const chain_ctx* c, double* x, const int32_t* types,
+                       double* cell, double* E, double s, const or_move* mv,
+                       double xi, double u) {
+  int N = c->N;
+  double oldV = fabs(or_det3(cell));
+  double dV = (s * (double)N) * xi;
+  double newV = fabs(oldV + dV);
+  double ratio = newV / oldV;
+  double scale = or_cbrt(ratio);
+  double nc[9];
+  for (int i = 0; i < 9; ++i) nc[i] = cell[i] * scale;
+  double* nx = (double*)malloc(sizeof(double) * 3 * (size_t)(N + 1));
+  for (int i = 0; i < 3 * N; ++i) nx[i] = x[i] * scale;
+  double U, Hn;
+  or_energy_forces(N, nx, types, nc, c->pot, c->pressure, c->mu, &U, &Hn,
+                   NULL);
+  int cell_ok = or_check_cell_shape(nc, N, mv->max_vpa, mv->min_vpa,
+                                    mv->min_aspect);
+  double p_acc = mv->flat_v_prior ? 1.0 : fmin(1.0, or_ipow(ratio, N));
+  int energy_ok = Hn < c->emax;
+  int prior_ok = u < p_acc;
+  int reason;
+  if (energy_ok && cell_ok && prior_ok) {
+    reason = 0;
+    memcpy(x, nx, sizeof(double) * 3 * (size_t)N);
+    memcpy(cell, nc, sizeof(nc));
+    *E = Hn;
+  } else {
+    reason = !energy_ok ? 1 : (!cell_ok ? 2 : 3);
+  }
+  free(nx);
+  return reason;
+}
+
+/* sampling/moves/hmc.py:51-152.  Forces at a position are evaluated once and
+ * reused by the next half kick (bit-identical to re-evaluating). */
+static int hmc_move(const chain_ctx* c, double* x, const int32_t* types,
+                    const double* cell, double* E, double s,
+                    const or_move* mv, const double* xi) {
+  int N = c->N, Nmax = c->Nmax;
+  double* p = (double*)malloc(sizeof(double) * 3 * (size_t)(N + 1));
+  double* nx = (double*)malloc(sizeof(double) * 3 * (size_t)(N + 1));
+  double* F = (double*)malloc(sizeof(double) * 3 * (size_t)(N + 1));
+  double* scr = (double*)malloc(sizeof(double) * (size_t)(N + 1));
+  for (int a = 0; a < N; ++a)
+    for (int q = 0; q < 3; ++q) p[3 * a + q] = xi[q * Nmax + a];
+  double K0 = 0.5 * wave_sum_dot(p, p, N, scr);
+  memcpy(nx, x, sizeof(double) * 3 * (size_t)N);
+  double U, Hn;
+  double h = 0.5 * s;
+  or_energy_forces(N, nx, types, cell, c->pot, c->pressure, c->mu, &U, &Hn,
+                   F);
+  for (int l = 0; l < mv->n_leapfrog; ++l) {
+    for (int i = 0; i < 3 * N; ++i) p[i] = p[i] + h * F[i];
+    for (int i = 0; i < 3 * N; ++i) nx[i] = nx[i] + s * p[i];
+    or_energy_forces(N, nx, types, cell, c->pot, c->pressure, c->mu, &U,
+                     &Hn, F);
+    for (int i = 0; i < 3 * N; ++i) p[i] = p[i] + h * F[i];
+  }
+  double K1 = 0.5 * wave_sum_dot(p, p, N, scr);
+  double dH = (Hn + K1) - (*E + K0);
+  int accepted = (Hn < c->emax) && (dH < 1.0);
+  if (accepted) {
+    memcpy(x, nx, sizeof(double) * 3 * (size_t)N);
+    *E = Hn;
+  }
+  free(p);
+  free(nx);
+  free(F);
+  free(scr);
+  return accepted;
+}
+
+/* sampling/moves/alchemical.py:40-79 (build_morph_kernel). */
+static int morph_move(const chain_ctx* c, const double* x, int32_t* types,
+                      const double* cell, double* E, const or_move* mv,
+                      const double* u) {
+  int N = c->N;
+  int S = mv->n_species;
+  int atom = (int)(u[0] * (double)N);
+  if (atom > N - 1) atom = N - 1;
+  int cand = (int)(u[1] * (double)(S - 1));
+  if (cand > S - 2) cand = S - 2;
+  int cur = types[atom];
+  int nt = cand + (cand >= cur ? 1 : 0);
+  types[atom] = nt;
+  double U, Hn;
+  or_energy_forces(N, x, types, cell, c->pot, c->pressure, c->mu, &U, &Hn,
+                   NULL);
+  int accepted = Hn < c->emax;
+  if (accepted)
+    *E = Hn;
+  else
+    types[atom] = cur;
+  return accepted;
+}
+
+static void count(int32_t* ctr, int reason, int n_eval, int n_grad) {
+  ctr[0] += 1;
+  ctr[1 + reason] += 1;
+  ctr[5] += n_eval;
+  ctr[6] += n_grad;
+}
+
+/* sampling/mwg.py:108-150,227-257 + nested_sampling.py:533-609 (one chain). */
+void or_walk_chain(int N, int Nmax, double* pos, int32_t* types, double* cell,
+                   double* energy, double* dir, const or_pot* pot,
+                   double pressure, const double* mu, double emax,
+                   const double* step_sizes, const or_move* moves,
+                   int n_moves, const int32_t* schedule, int n_steps,
+                   const double* normals, const double* uniforms,
+                   int32_t* counters) {
+  chain_ctx c = {N, Nmax, pot->n_species, pot, pressure, mu, emax};
+  (void)n_moves;
+  for (int t = 0; t < n_steps; ++t) {
+    int m = schedule[t];
+    const or_move* mv = &moves[m];
+    const double* xi = normals + (size_t)t * 3 * Nmax;
+    const double* u = uniforms + (size_t)t * 4;
+    double s = step_sizes[m];
+    int32_t* ctr = counters + OR_NCOUNT * m;
+    switch (mv->type) {
+      case OR_MOVE_GMC: {
+        int acc = gmc_move(&c, pos, types, cell, energy, dir, s, mv, xi);
+        count(ctr, acc ? 0 : 1, mv->n_reflect, mv->n_reflect);
+        break;
+      }
+      case OR_MOVE_VOLUME: {
+        int reason = volume_move(&c, pos, types, cell, energy, s, mv, xi[0],
+                                 u[0]);
+        count(ctr, reason, 1, 0);
+        break;
+      }
+      case OR_MOVE_HMC: {
+        int acc = hmc_move(&c, pos, types, cell, energy, s, mv, xi);
+        count(ctr, acc ? 0 : 1, 2 * mv->n_leapfrog + 1, 2 * mv->n_leapfrog);
+        break;
+      }
+      case OR_MOVE_MORPH: {
+        int acc = morph_move(&c, pos, types, cell, energy, mv, u);
+        count(ctr, acc ? 0 : 1, 1, 0);
+        break;
+      }
+      default:
+        break;
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* NS bookkeeping                                                         */
+/* ---------------------------------------------------------------------- */
+
+static void copy_walker(or_pop* pop, int dst, int src) {
+  int Nm = pop->Nmax;
+  memcpy(pop->pos + (size_t)dst * Nm * 3, pop->pos + (size_t)src * Nm * 3,
+         sizeof(double) * 3 * (size_t)Nm);
+  memcpy(pop->dir + (size_t)dst * Nm * 3, pop->dir + (size_t)src * Nm * 3,
+         sizeof(double) * 3 * (size_t)Nm);
+  memcpy(pop->types + (size_t)dst * Nm, pop->types + (size_t)src * Nm,
+         sizeof(int32_t) * (size_t)Nm);
+  memcpy(pop->cell + (size_t)dst * 9, pop->cell + (size_t)src * 9,
+         sizeof(double) * 9);
+  pop->energy[dst] = pop->energy[src];
+  pop->n_atoms[dst] = pop->n_atoms[src];
+}
+
+typedef struct {
+  double key;
+  int idx;
+} keyed_t;
+
+static int cmp_desc(const void* a, const void* b) {
+  const keyed_t* x = (const keyed_t*)a;
+  const keyed_t* y = (const keyed_t*)b;
+  if (x->key > y->key) return -1;
+  if (x->key < y->key) return 1;
+  return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+/* nested_sampling.py:86-133,151-186 (n_cull = 1 path): worst with uniform
+ * tie-break among exact maxima, clone shifted past worst, extras = top
+ * n_extra of uniform noise excluding worst and clone. */
+static void select_replica(const double* E, int K, int n_extra,
+                           const double* sel_u, double clone_u,
+                           const double* extra_u, int* worst, int* clone,
+                           int* walk, double* emax) {
+  double mx = E[0];
+  for (int k = 1; k < K; ++k) mx = fmax(mx, E[k]);
+  int w = -1;
+  double best = -INFINITY;
+  for (int k = 0; k < K; ++k) {
+    if (E[k] == mx && (w < 0 || sel_u[k] > best)) {
+      best = sel_u[k];
+      w = k;
+    }
+  }
+  int cl = (int)(clone_u * (double)(K - 1));
+  if (cl > K - 2) cl = K - 2;
+  if (cl >= w) cl += 1;
+  walk[0] = w;
+  if (n_extra > 0) {
+    keyed_t* ks = (keyed_t*)malloc(sizeof(keyed_t) * (size_t)K);
+    for (int k = 0; k < K; ++k) {
+      ks[k].key = (k == w || k == cl) ? -INFINITY : extra_u[k];
+      ks[k].idx = k;
+    }
+    qsort(ks, (size_t)K, sizeof(keyed_t), cmp_desc);
+    for (int e = 0; e < n_extra; ++e) walk[1 + e] = ks[e].idx;
+    free(ks);
+  }
+  *worst = w;
+  *clone = cl;
+  *emax = mx;
+}
+
+static void rebase_pair(double E, const double* cell, const int32_t* types,
+                        int N, double Pf, const double* muf, double Pt,
+                        const double* mut, int S, double* out) {
+  double u = E - or_ensemble_energy(N, types, cell, Pf, muf, S);
+  *out = u + or_ensemble_energy(N, types, cell, Pt, mut, S);
+}
+
+/* sampling/moves/replica_exchange.py:318-436 with EnsembleSwap (:120-164). */
+void or_replica_exchange(or_pop* pop, const or_run* run, const double* emax,
+                         int n_swap_cycles, const int32_t* pair_valid,
+                         const double* re_u, int32_t* acc, int32_t* att) {
+  int R = pop->R, K = pop->K, S = pop->S, Nm = pop->Nmax;
+  int max_pairs = R / 2;
+  if (R < 2) return;
+  double* tmp_pos = (double*)malloc(sizeof(double) * 3 * (size_t)Nm);
+  double* tmp_dir = (double*)malloc(sizeof(double) * 3 * (size_t)Nm);
+  int32_t* tmp_t = (int32_t*)malloc(sizeof(int32_t) * (size_t)Nm);
+  for (int cyc = 0; cyc < n_swap_cycles; ++cyc) {
+    for (int ph = 0; ph < 2; ++ph) {
+      int pi = 0;
+      for (int i = ph; i + 1 < R; i += 2, ++pi) {
+        int j = i + 1;
+        const double* u = re_u + (((size_t)cyc * 2 + ph) * max_pairs + pi) * 2;
+        if (!pair_valid[i]) continue;
+        int wi = (int)(u[0] * (double)K);
+        if (wi > K - 1) wi = K - 1;
+        int wj = (int)(u[1] * (double)K);
+        if (wj > K - 1) wj = K - 1;
+        int si = i * K + wi, sj = j * K + wj;
+        int Ni = pop->n_atoms[si], Nj = pop->n_atoms[sj];
+        double into_i, into_j;
+        rebase_pair(pop->energy[sj], pop->cell + 9 * (size_t)sj,
+                    pop->types + (size_t)sj * Nm, Nj, run->pressure[j],
+                    run->mu + (size_t)j * S, run->pressure[i],
+                    run->mu + (size_t)i * S, S, &into_i);
+        rebase_pair(pop->energy[si], pop->cell + 9 * (size_t)si,
+                    pop->types + (size_t)si * Nm, Ni, run->pressure[i],
+                    run->mu + (size_t)i * S, run->pressure[j],
+                    run->mu + (size_t)j * S, S, &into_j);
+        int ok = (into_i < emax[i]) && (into_j < emax[j]);
+        att[i] += 1;
+        if (ok) {
+          acc[i] += 1;
+          /* swap walker-bound fields (state/mc_state.py:117-137) */
+          memcpy(tmp_pos, pop->pos + (size_t)si * Nm * 3,
+                 sizeof(double) * 3 * (size_t)Nm);
+          memcpy(pop->pos + (size_t)si * Nm * 3,
+                 pop->pos + (size_t)sj * Nm * 3,
+                 sizeof(double) * 3 * (size_t)Nm);
+          memcpy(pop->pos + (size_t)sj * Nm * 3, tmp_pos,
+                 sizeof(double) * 3 * (size_t)Nm);
+          memcpy(tmp_dir, pop->dir + (size_t)si * Nm * 3,
+                 sizeof(double) * 3 * (size_t)Nm);
+          memcpy(pop->dir + (size_t)si * Nm * 3,
+                 pop->dir + (size_t)sj * Nm * 3,
+                 sizeof(double) * 3 * (size_t)Nm);
+          memcpy(pop->dir + (size_t)sj * Nm * 3, tmp_dir,
+                 sizeof(double) * 3 * (size_t)Nm);
+          memcpy(tmp_t, pop->types + (size_t)si * Nm,
+                 sizeof(int32_t) * (size_t)Nm);
+          memcpy(pop->types + (size_t)si * Nm, pop->types + (size_t)sj * Nm,
+                 sizeof(int32_t) * (size_t)Nm);
+          memcpy(pop->types + (size_t)sj * Nm, tmp_t,
+                 sizeof(int32_t) * (size_t)Nm);
+          double tc[9];
+          memcpy(tc, pop->cell + 9 * (size_t)si, sizeof(tc));
+          memcpy(pop->cell + 9 * (size_t)si, pop->cell + 9 * (size_t)sj,
+                 sizeof(tc));
+          memcpy(pop->cell + 9 * (size_t)sj, tc, sizeof(tc));
+          pop->n_atoms[si] = Nj;
+          pop->n_atoms[sj] = Ni;
+          pop->energy[si] = into_i;
+          pop->energy[sj] = into_j;
+        }
+      }
+    }
+  }
+  free(tmp_pos);
+  free(tmp_dir);
+  free(tmp_t);
+}
+
+/* nested_sampling.py:391-686 for n_cull = 1, all replicas. */
+int or_ns_iteration(or_pop* pop, const or_pot* pot, const or_run* run,
+                    const or_move* moves, int n_moves, int n_steps,
+                    int n_extra, int n_swap_cycles, const int32_t* pair_valid,
+                    const or_draws* d, or_iter_out* out, int n_threads) {
+  int R = pop->R, K = pop->K, Nm = pop->Nmax, S = pop->S;
+  int n_walk = 1 + n_extra;
+  if (K < 2 || n_extra < 0 || n_extra > K - 2) return -1;
+  for (int r = 0; r < R; ++r) {
+    int w, cl;
+    double mx;
+    select_replica(pop->energy + (size_t)r * K, K, n_extra,
+                   d->sel_u + (size_t)r * K, d->clone_u[r],
+                   d->extra_u ? d->extra_u + (size_t)r * K : NULL, &w, &cl,
+                   out->walk + (size_t)r * n_walk, &mx);
+    out->worst[r] = w;
+    out->clone[r] = cl;
+    out->emax[r] = mx;
+    out->dead_energy[r] = pop->energy[(size_t)r * K + w];
+    out->dead_volume[r] = fabs(or_det3(pop->cell + 9 * ((size_t)r * K + w)));
+    copy_walker(pop, r * K + w, r * K + cl);
+  }
+  memset(out->counters, 0, sizeof(int32_t) * (size_t)R * n_moves * OR_NCOUNT);
+  int total = R * n_walk;
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+  (void)n_threads;
+#endif
+#pragma omp parallel
+  {
+    int32_t* local =
+        (int32_t*)calloc((size_t)R * n_moves * OR_NCOUNT, sizeof(int32_t));
+#pragma omp for schedule(dynamic, 1)
+    for (int q = 0; q < total; ++q) {
+      int r = q / n_walk, ci = q % n_walk;
+      int slot = r * K + out->walk[(size_t)r * n_walk + ci];
+      const double* xi = d->normals + ((size_t)q * n_steps) * 3 * Nm;
+      const double* u = d->uniforms + ((size_t)q * n_steps) * 4;
+      or_walk_chain(pop->n_atoms[slot], Nm, pop->pos + (size_t)slot * Nm * 3,
+                    pop->types + (size_t)slot * Nm, pop->cell + 9 * (size_t)slot,
+                    pop->energy + slot, pop->dir + (size_t)slot * Nm * 3, pot,
+                    run->pressure[r], run->mu + (size_t)r * S, out->emax[r],
+                    run->step_sizes + (size_t)r * n_moves, moves, n_moves,
+                    d->schedule, n_steps, xi, u,
+                    local + (size_t)r * n_moves * OR_NCOUNT);
+    }
+#pragma omp critical
+    {
+      for (int i = 0; i < R * n_moves * OR_NCOUNT; ++i)
+        out->counters[i] += local[i];
+    }
+    free(local);
+  }
+  if (d->re_u != NULL && R > 1) {
+    memset(out->re_acc, 0, sizeof(int32_t) * (size_t)(R - 1));
+    memset(out->re_att, 0, sizeof(int32_t) * (size_t)(R - 1));
+    or_replica_exchange(pop, run, out->emax, n_swap_cycles, pair_valid,
+                        d->re_u, out->re_acc, out->re_att);
+  }
+  return 0;
+}
+
+/* sampling/adaptation/stepsize_handler.py:215-261: one trial move on each
+ * sampled walker (a copy), reporting accepted and reject reason. */
+void or_trial_moves(const or_pop* pop, const or_pot* pot, const or_run* run,
+                    const or_move* moves, int n_moves, int move_idx,
+                    const double* trial_step, const double* emax,
+                    int n_samples, const int32_t* sample_idx,
+                    const double* normals, const double* uniforms,
+                    int32_t* accepted, int32_t* reason, int n_threads) {
+  int R = pop->R, K = pop->K, Nm = pop->Nmax, S = pop->S;
+  int total = R * n_samples;
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+  (void)n_threads;
+#endif
+#pragma omp parallel
+  {
+    double* pos = (double*)malloc(sizeof(double) * 3 * (size_t)Nm);
+    double* dir = (double*)malloc(sizeof(double) * 3 * (size_t)Nm);
+    int32_t* ty = (int32_t*)malloc(sizeof(int32_t) * (size_t)Nm);
+    double* ss = (double*)malloc(sizeof(double) * (size_t)n_moves);
+#pragma omp for schedule(dynamic, 1)
+    for (int q = 0; q < total; ++q) {
+      int r = q / n_samples;
+      int slot = r * K + sample_idx[q];
+      double cell[9];
+      memcpy(pos, pop->pos + (size_t)slot * Nm * 3, sizeof(double) * 3 * (size_t)Nm);
+      memcpy(dir, pop->dir + (size_t)slot * Nm * 3, sizeof(double) * 3 * (size_t)Nm);
+      memcpy(ty, pop->types + (size_t)slot * Nm, sizeof(int32_t) * (size_t)Nm);
+      memcpy(cell, pop->cell + 9 * (size_t)slot, sizeof(cell));
+      double E = pop->energy[slot];
+      for (int m = 0; m < n_moves; ++m) ss[m] = trial_step[r];
+      int32_t ctr[OR_NCOUNT * 16];
+      memset(ctr, 0, sizeof(ctr));
+      int32_t sched = move_idx;
+      or_walk_chain(pop->n_atoms[slot], Nm, pos, ty, cell, &E, dir, pot,
+                    run->pressure[r], run->mu + (size_t)r * S, emax[r], ss,
+                    moves, n_moves, &sched, 1,
+                    normals + (size_t)q * 3 * Nm, uniforms + (size_t)q * 4,
+                    ctr);
+      int32_t* c = ctr + OR_NCOUNT * move_idx;
+      int rs = c[1] ? 0 : (c[2] ? 1 : (c[3] ? 2 : 3));
+      accepted[q] = rs == 0;
+      reason[q] = rs;
+    }
+    free(pos);
+    free(dir);
+    free(ty);
+    free(ss);
+  }
+}

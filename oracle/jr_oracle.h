@@ -1,0 +1,132 @@
+/*
+ * CPU oracle for the nested-sampling inner walk: an FP64 C restatement of the
+ * reference algorithm (nunglert/jaxrens), used ONLY as test infrastructure
+ * (parity checks and the CPU baseline in bench.py).  Nothing in the product
+ * path links or calls this library.
+ *
+ * Every function fixes one canonical floating-point evaluation order (listed
+ * in jr_oracle.c next to each step) so that a device implementation following
+ * the same order can be compared bit-for-bit.  The reference itself leaves
+ * that order to XLA; the per-function comments cite the reference lines whose
+ * semantics are restated.
+ */
+#ifndef JR_ORACLE_H
+#define JR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_MOVE_GMC = 0, OR_MOVE_VOLUME = 1, OR_MOVE_HMC = 2, OR_MOVE_MORPH = 3 };
+
+/* counters per (replica, move): proposed, rr0 (=accepted), rr1 energy,
+ * rr2 cell, rr3 prior, n_evaluations, n_grad_evaluations */
+enum { OR_NCOUNT = 7 };
+
+typedef struct {
+  int n_species;        /* S */
+  double rc2;           /* squared cutoff; +inf for no cutoff */
+  const double* eps4;   /* [S*S] 4*eps_ab */
+  const double* eps24;  /* [S*S] 24*eps_ab */
+  const double* sig2;   /* [S*S] sigma_ab^2 */
+} or_pot;
+
+typedef struct {
+  int type;
+  int n_reflect;
+  int n_leapfrog;
+  int flat_v_prior;
+  int n_species;
+  int pad_;
+  double perturb_angle;
+  double max_vpa, min_vpa, min_aspect;
+} or_move;
+
+typedef struct {
+  int R, K, Nmax, S;
+  double* pos;     /* [R*K][Nmax][3] */
+  int32_t* types;  /* [R*K][Nmax] */
+  int32_t* n_atoms;/* [R*K] */
+  double* cell;    /* [R*K][9] rows = lattice vectors */
+  double* energy;  /* [R*K] full Hamiltonian U + P V - mu.N */
+  double* dir;     /* [R*K][Nmax][3] GMC persistent direction */
+} or_pop;
+
+typedef struct {
+  const double* pressure;   /* [R] */
+  const double* mu;         /* [R*S] */
+  const double* step_sizes; /* [R*n_moves] */
+} or_run;
+
+typedef struct {
+  const double* sel_u;     /* [R*K] worst-walker tie-break noise */
+  const double* clone_u;   /* [R] */
+  const double* extra_u;   /* [R*K] (unused when n_extra == 0) */
+  const int32_t* schedule; /* [n_steps] shared move indices */
+  const double* normals;   /* [R][n_walk][n_steps][3][Nmax] */
+  const double* uniforms;  /* [R][n_walk][n_steps][4] */
+  const double* re_u;      /* [n_cycles][2][max_pairs][2] (NULL: no RE) */
+} or_draws;
+
+typedef struct {
+  double* emax;         /* [R] */
+  int32_t* worst;       /* [R] */
+  int32_t* clone;       /* [R] */
+  int32_t* walk;        /* [R*n_walk] */
+  double* dead_energy;  /* [R] */
+  double* dead_volume;  /* [R] */
+  int32_t* counters;    /* [R*n_moves*7] */
+  int32_t* re_acc;      /* [R-1] */
+  int32_t* re_att;      /* [R-1] */
+} or_iter_out;
+
+double or_det3(const double* c);
+double or_cbrt(double x);
+double or_ipow(double x, int n);
+int or_check_cell_shape(const double* cell, int n_atoms, double max_vpa,
+                        double min_vpa, double min_aspect);
+double or_ensemble_energy(int N, const int32_t* types, const double* cell,
+                          double pressure, const double* mu, int S);
+
+/* U (model), H = U + ensemble term, F = -dU/dx (forces of the ensemble term
+ * are exactly zero).  F may be NULL for an energy-only evaluation. */
+void or_energy_forces(int N, const double* pos, const int32_t* types,
+                      const double* cell, const or_pot* pot, double pressure,
+                      const double* mu, double* U, double* H, double* F);
+
+/* Walk one chain for n_steps MWG sub-steps (state updated in place). */
+void or_walk_chain(int N, int Nmax, double* pos, int32_t* types, double* cell,
+                   double* energy, double* dir, const or_pot* pot,
+                   double pressure, const double* mu, double emax,
+                   const double* step_sizes, const or_move* moves,
+                   int n_moves, const int32_t* schedule, int n_steps,
+                   const double* normals, const double* uniforms,
+                   int32_t* counters);
+
+/* One NS iteration (n_cull = 1) over all R replicas: select, clone, walk,
+ * optional replica exchange. */
+int or_ns_iteration(or_pop* pop, const or_pot* pot, const or_run* run,
+                    const or_move* moves, int n_moves, int n_steps,
+                    int n_extra, int n_swap_cycles, const int32_t* pair_valid,
+                    const or_draws* d, or_iter_out* out, int n_threads);
+
+/* Replica exchange pass (EnsembleSwap). */
+void or_replica_exchange(or_pop* pop, const or_run* run, const double* emax,
+                         int n_swap_cycles, const int32_t* pair_valid,
+                         const double* re_u, int32_t* acc, int32_t* att);
+
+/* One trial move per sampled walker, population untouched (step-size
+ * adaptation).  accepted/reason: [R*n_samples]. */
+void or_trial_moves(const or_pop* pop, const or_pot* pot, const or_run* run,
+                    const or_move* moves, int n_moves, int move_idx,
+                    const double* trial_step, const double* emax,
+                    int n_samples, const int32_t* sample_idx,
+                    const double* normals, const double* uniforms,
+                    int32_t* accepted, int32_t* reason, int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

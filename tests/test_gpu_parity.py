@@ -1,0 +1,253 @@
+"""HIP path vs the CPU oracle on identical inputs (bit-for-bit: compared
+with ``==`` on every float64 output; the two follow one canonical
+floating-point order).  Needs an MI355X."""
+
+import math
+
+import numpy as np
+import pytest
+
+from jaxrens_amd import _lib
+from jaxrens_amd.backends import EnsembleTerm, Hamiltonian, LJBackend
+from jaxrens_amd.draws import make_tape
+from jaxrens_amd.engine import NSEngine, RunConfig
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_cfgs(rng, B, N, triclinic, n_species=1, dense=0.6):
+    L = (N / dense) ** (1 / 3)
+    pos = rng.random((B, N, 3)) * L
+    pos += rng.integers(-2, 3, size=(B, N, 3)) * L
+    cell = np.tile(np.eye(3) * L, (B, 1, 1))
+    if triclinic:
+        cell = cell + rng.uniform(-0.15, 0.15, size=(B, 3, 3)) * L
+    ty = rng.integers(0, n_species, size=(B, N))
+    return pos, ty, cell
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 7, 8, 13, 32, 33, 63, 64, 65, 96, 127, 128, 129, 200, 256])
+@pytest.mark.parametrize("triclinic", [False, True])
+def test_eval_single_species(oracle, N, triclinic):
+    print(f"eval N={N} triclinic={triclinic}", flush=True)
+    rng = np.random.default_rng(N * 7 + triclinic)
+    lj = LJBackend(1.0, 1.0, 2.5)
+    pos, ty, cell = _random_cfgs(rng, 6, N, triclinic)
+    U, H, F = lj._eval(pos, ty, cell, pressure=np.linspace(0.01, 1.0, 6), forces=True)
+    pot = oracle.Pot.from_table(lj.table)
+    for b in range(6):
+        u, h, f = oracle.energy_forces(pot, pos[b], ty[b], cell[b],
+                                       pressure=np.linspace(0.01, 1.0, 6)[b])
+        assert U[b] == u and H[b] == h, (b, U[b], u)
+        assert np.array_equal(F[b], f), (b, np.abs(F[b] - f).max())
+
+
+@pytest.mark.parametrize("N", [8, 48, 64, 96, 128])
+def test_eval_pair_table_and_mu(oracle, N):
+    print(f"eval pair-table N={N}", flush=True)
+    rng = np.random.default_rng(N)
+    eps = np.array([[1.0, 1.5], [1.5, 0.5]])
+    sig = np.array([[1.0, 0.8], [0.8, 0.88]])
+    lj = LJBackend(pair_epsilon=eps, pair_sigma=sig, cutoff=2.5)
+    pos, ty, cell = _random_cfgs(rng, 5, N, True, n_species=2, dense=0.8)
+    mu = rng.normal(size=(5, 2))
+    U, H, F = lj._eval(pos, ty, cell, pressure=1.0, mu=mu, forces=True)
+    pot = oracle.Pot.from_table(lj.table)
+    for b in range(5):
+        u, h, f = oracle.energy_forces(pot, pos[b], ty[b], cell[b], pressure=1.0, mu=mu[b])
+        assert U[b] == u and H[b] == h
+        assert np.array_equal(F[b], f)
+
+
+def test_hamiltonian_api_matches_terms():
+    rng = np.random.default_rng(1)
+    lj = LJBackend(1.0, 1.0, 2.5)
+    H = Hamiltonian(lj, [EnsembleTerm(pressure=0.3)])
+    pos, ty, cell = _random_cfgs(rng, 1, 20, False)
+    r = H.energy_and_forces(pos[0], ty[0], cell[0])
+    u = lj.energy_and_forces(pos[0], ty[0], cell[0])
+    V = abs(np.linalg.det(cell[0]))
+    assert r.energy == pytest.approx(u.energy + 0.3 * V, rel=1e-14)
+    assert np.array_equal(r.forces, u.forces)
+
+
+def _engine(lj, moves, R, K, N, n_extra, n_steps, seed=3, cell_spec=None,
+            pressures=None, mu=None, re_interval=1, species_p=None,
+            v_range=(0.125, 64.0)):
+    run = RunConfig(n_replicas=R, n_walkers=K, n_atoms=N, n_mcmc_steps=n_steps,
+                    n_extra=n_extra, seed=seed, pressures=pressures,
+                    chemical_potentials=mu, re_interval=re_interval)
+    eng = NSEngine(lj, moves, run, cell=cell_spec)
+    rng = np.random.default_rng(seed)
+    eng.init_walkers(rng, v_range[0], v_range[1], True, species_p=species_p)
+    return eng
+
+
+GMC_VOL = [{"type": "galilean", "n_reflect": 10, "step_size": 0.1, "weight": 4},
+           {"type": "volume", "step_size": 0.3, "weight": 1}]
+CELL = {"min_volume_per_atom": 0.125, "max_volume_per_atom": 64.0, "flat_V_prior": True}
+
+
+def _compare_population(eng, opop):
+    d = eng.get_population()
+    assert np.array_equal(d["energy"], opop.energy)
+    assert np.array_equal(d["positions"], opop.pos)
+    assert np.array_equal(d["direction"], opop.dir)
+    assert np.array_equal(d["types"], opop.types)
+    assert np.array_equal(d["cell"].reshape(opop.cell.shape), opop.cell)
+    assert np.array_equal(d["n_atoms"], opop.n_atoms)
+
+
+def _run_parity(oracle, eng, n_iter, tape, pair_valid=None):
+    from oracle.oracle import Population
+    opop = Population.from_engine_dict(eng.get_population())
+    pot = oracle.Pot.from_table(eng.model.table)
+    moves = eng.mwg.as_oracle_moves()
+    for it in range(n_iter):
+        eng_out = eng.ns_iterations(1, tape=tape.iteration(it))
+        fires = eng.run.re_interval > 0 and (eng.iteration - 1) > 0 and \
+            (eng.iteration - 1) % eng.run.re_interval == 0
+        o = oracle.ns_iteration(opop, pot, eng.pressure, eng.mu, eng.step_sizes, moves,
+                                eng.run.n_mcmc_steps, eng.run.n_extra, tape, it=it,
+                                n_swap_cycles=eng.run.n_swap_cycles,
+                                pair_valid=pair_valid, do_re=fires)
+        assert np.array_equal(eng_out["emax"][0], o["emax"]), it
+        assert np.array_equal(eng_out["worst"][0], o["worst"])
+        assert np.array_equal(eng_out["clone"][0], o["clone"])
+        assert np.array_equal(eng_out["walk"][0], o["walk"])
+        assert np.array_equal(eng_out["dead_volume"][0], o["dead_volume"])
+        assert np.array_equal(eng_out["counters"][0], o["counters"]), it
+        if eng.n_replicas > 1:
+            assert np.array_equal(eng_out["re_acc"][0], o["re_acc"])
+            assert np.array_equal(eng_out["re_att"][0], o["re_att"])
+        _compare_population(eng, opop)
+
+
+@pytest.mark.parametrize("n_extra", [0, 14])
+def test_ns_iterations_lj8(oracle, n_extra):
+    print(f"ns lj8 n_extra={n_extra}", flush=True)
+    lj = Hamiltonian(LJBackend(1.0, 1.0, 2.5), [EnsembleTerm(0.1)])
+    eng = _engine(lj, GMC_VOL, 1, 16, 8, n_extra, 20, cell_spec=CELL)
+    tape = make_tape(np.random.default_rng(9), 6, 1, 16, 8, 20, n_extra,
+                     eng.mwg.move_probs, ties=True)
+    _run_parity(oracle, eng, 6, tape)
+
+
+def test_ns_iterations_lj64_replicas_with_re(oracle):
+    print("ns lj64 R=4 with RE", flush=True)
+    lj = LJBackend(1.0, 1.0, 2.5)
+    P = list(10 ** (-2 + 3 * np.arange(4) / 3))
+    eng = _engine(lj, GMC_VOL, 4, 8, 64, 6, 6, cell_spec=CELL, pressures=P)
+    tape = make_tape(np.random.default_rng(10), 4, 4, 8, 64, 6, 6, eng.mwg.move_probs)
+    _run_parity(oracle, eng, 4, tape)
+
+
+def test_ns_iterations_hmc_gmc_n128(oracle):
+    print("ns hmc+gmc N=128", flush=True)
+    lj = LJBackend(1.0, 1.0, 2.5)
+    moves = [{"type": "hmc", "n_leapfrog": 4, "step_size": 0.005, "weight": 1},
+             {"type": "gmc", "n_reflect": 3, "step_size": 0.05, "weight": 1}]
+    eng = _engine(lj, moves, 2, 4, 128, 2, 4, pressures=[0.05, 0.5],
+                  cell_spec={"min_volume_per_atom": 0.5, "max_volume_per_atom": 4.0},
+                  v_range=(1.2, 3.0))
+    tape = make_tape(np.random.default_rng(11), 3, 2, 4, 128, 4, 2, eng.mwg.move_probs)
+    _run_parity(oracle, eng, 3, tape)
+
+
+def test_ns_iterations_semigrand_mixed_n(oracle):
+    print("ns semi-grand mixed N", flush=True)
+    eps = np.array([[1.0, 1.5], [1.5, 0.5]])
+    sig = np.array([[1.0, 0.8], [0.8, 0.88]])
+    lj = LJBackend(pair_epsilon=eps, pair_sigma=sig, cutoff=2.5)
+    moves = [{"type": "gmc", "n_reflect": 4, "step_size": 0.1, "weight": 4},
+             {"type": "volume", "step_size": 0.3, "weight": 1},
+             {"type": "alchemical_morph", "n_species": 2, "weight": 1}]
+    R = 6
+    n_atoms = [32, 32, 48, 48, 96, 96]
+    mu = np.stack([np.zeros(R), np.linspace(-2, 2, R)], axis=1)
+    run = RunConfig(n_replicas=R, n_walkers=6, n_atoms=n_atoms, n_mcmc_steps=6,
+                    n_extra=3, seed=5, pressures=[1.0] * R, chemical_potentials=mu,
+                    re_interval=1)
+    eng = NSEngine(lj, moves, run, cell=CELL)
+    eng.init_walkers(np.random.default_rng(5), 0.5, 8.0, True, species_p=[0.5, 0.5])
+    tape = make_tape(np.random.default_rng(12), 4, R, 6, 96, 6, 3, eng.mwg.move_probs)
+    pv = np.array([n_atoms[i] == n_atoms[i + 1] for i in range(R - 1)], np.int32)
+    _run_parity(oracle, eng, 4, tape, pair_valid=pv)
+
+
+def test_device_draws_replay(oracle):
+    """Device Philox draws, read back and replayed on the oracle."""
+    print("device draws replay", flush=True)
+    from oracle.oracle import Population
+    lj = LJBackend(1.0, 1.0, 2.5)
+    eng = _engine(lj, GMC_VOL, 3, 10, 20, 8, 5, cell_spec=CELL, pressures=[0.1, 0.5, 1.0])
+    opop = Population.from_engine_dict(eng.get_population())
+    pot = oracle.Pot.from_table(lj.table)
+    for it in range(3):
+        out = eng.ns_iterations(1)
+        tape = eng.last_tape()
+        fires = (eng.iteration - 1) > 0
+        o = oracle.ns_iteration(opop, pot, eng.pressure, eng.mu, eng.step_sizes,
+                                eng.mwg.as_oracle_moves(), 5, 8, tape, it=0,
+                                do_re=fires)
+        assert np.array_equal(out["emax"][0], o["emax"])
+        assert np.array_equal(out["counters"][0], o["counters"])
+        _compare_population(eng, opop)
+    # draws look like uniforms / standard normals
+    assert 0.0 <= tape.uniforms.min() and tape.uniforms.max() < 1.0
+    z = tape.normals.ravel()
+    assert abs(z.mean()) < 0.05 and abs(z.std() - 1.0) < 0.05
+
+
+def test_graph_replay_equals_stream_path():
+    print("graph replay", flush=True)
+    lj = LJBackend(1.0, 1.0, 2.5)
+    a = _engine(lj, GMC_VOL, 2, 12, 16, 10, 5, cell_spec=CELL, pressures=[0.1, 1.0], seed=21)
+    b = _engine(lj, GMC_VOL, 2, 12, 16, 10, 5, cell_spec=CELL, pressures=[0.1, 1.0], seed=21)
+    oa = a.ns_iterations(4)
+    ob = b.ns_iterations(4, graph=True)
+    for k in ("emax", "worst", "clone", "counters", "re_acc", "re_att", "walk"):
+        assert np.array_equal(oa[k], ob[k]), k
+    pa, pb = a.get_population(), b.get_population()
+    for k in pa:
+        assert np.array_equal(pa[k], pb[k]), k
+
+
+def test_trial_moves_match_oracle(oracle):
+    print("trial moves", flush=True)
+    from oracle.oracle import Population
+    lj = LJBackend(1.0, 1.0, 2.5)
+    eng = _engine(lj, GMC_VOL, 2, 16, 24, 4, 4, cell_spec=CELL, pressures=[0.1, 1.0])
+    eng.ns_iterations(2)
+    rng = np.random.default_rng(2)
+    idx = rng.integers(0, 16, size=(2, 30)).astype(np.int32)
+    nrm = rng.standard_normal((2, 30, 3, 24))
+    uni = rng.random((2, 30, 4))
+    emax = eng.emax()
+    pop = Population.from_engine_dict(eng.get_population())
+    pot = oracle.Pot.from_table(lj.table)
+    for m, ss in [(0, 0.07), (1, 0.5)]:
+        acc, rs = eng.trial_moves(m, [ss, ss * 2], idx, normals=nrm, uniforms=uni)
+        oacc, ors = oracle.trial_moves(pop, pot, eng.pressure, eng.mu, eng.step_sizes,
+                                       eng.mwg.as_oracle_moves(), m, [ss, ss * 2], emax,
+                                       idx, nrm, uni)
+        assert np.array_equal(acc, oacc) and np.array_equal(rs, ors), m
+    after = eng.get_population()
+    assert np.array_equal(after["positions"], pop.pos)  # population untouched
+
+
+def test_replica_exchange_standalone(oracle):
+    print("replica exchange", flush=True)
+    from oracle.oracle import Population
+    lj = LJBackend(1.0, 1.0, 2.5)
+    eng = _engine(lj, GMC_VOL, 5, 8, 12, 6, 3, cell_spec=CELL,
+                  pressures=[0.01, 0.05, 0.1, 0.5, 1.0])
+    eng.ns_iterations(3)
+    pop = Population.from_engine_dict(eng.get_population())
+    emax = eng.emax()
+    re_u = np.random.default_rng(4).random((1, 2, 2, 2))
+    st = eng.replica_exchange(re_u)
+    acc, att = oracle.replica_exchange(pop, eng.pressure, eng.mu, emax, re_u, 1)
+    assert np.array_equal(st["n_accepted_per_pair"], acc[:4])
+    assert np.array_equal(st["n_attempted_per_pair"], att[:4])
+    _compare_population(eng, pop)
