@@ -1,0 +1,193 @@
+"""Throughput of the nested-sampling inner walk on one MI355X.
+
+Workload (the headline configuration): LJ-64 NPT replica stack, 64 identical
+LJ atoms (eps = sigma = 1, strict unshifted cutoff 2.5, cubic PBC), P = 16
+pressures log-spaced over 0.01-10, K = 1024 walkers each, 20 MCMC sub-steps
+per iteration of galilean (n_reflect 10, step 0.1) and volume (step 0.3) moves
+at weights 4:1, pressure replica exchange every iteration, FP64.  Initial
+volumes uniform in [8, 4096] with uniform positions.  The whole population is
+walked each iteration (n_extra = K - 2); a "step" is one NS iteration.
+
+Prints one JSON line.  ``value`` = walker-steps/s = sum over replicas of
+n_walk * n_mcmc_steps per iteration / wall time, inputs resident in HBM.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector, dense
+METRIC = "walker-steps/s, LJ-64 NPT P×K=16×1024; surrogate vs plain at matched log Z"
+
+# algorithmic FP64 FLOPs per unordered pair (cubic MIC): energy+forces, energy
+FLOP_EF, FLOP_E = 38, 25
+
+
+def build_engine(n_extra: int, R=16, K=1024, N=64, n_steps=20, seed=42):
+    from jaxrens_amd.backends import LJBackend
+    from jaxrens_amd.engine import NSEngine, RunConfig
+    lj = LJBackend(1.0, 1.0, 2.5)
+    moves = [{"type": "galilean", "n_reflect": 10, "step_size": 0.1, "weight": 4},
+             {"type": "volume", "step_size": 0.3, "weight": 1}]
+    cell = {"min_volume_per_atom": 0.125, "max_volume_per_atom": 64.0,
+            "flat_V_prior": True}
+    P = [10 ** (-2 + 3 * i / (R - 1)) for i in range(R)]
+    run = RunConfig(n_replicas=R, n_walkers=K, n_atoms=N, n_mcmc_steps=n_steps,
+                    n_extra=n_extra, seed=seed, pressures=P, re_interval=1,
+                    n_swap_cycles=1)
+    eng = NSEngine(lj, moves, run, cell=cell)
+    eng.init_walkers(np.random.default_rng(seed), 8.0 / N, 4096.0 / N, True)
+    return eng
+
+
+def launch_flops(counters: np.ndarray, eng) -> float:
+    """Algorithmic FLOPs of the walk launches behind ``counters``
+    (iterations, R, n_moves, 7): n_reflect E+F evaluations per galilean
+    step, one energy evaluation per volume step, N(N-1)/2 pairs each."""
+    N = int(eng.n_atoms_rep[0])
+    pairs = N * (N - 1) / 2
+    total = 0.0
+    for m, spec in enumerate(eng.mwg.moves):
+        steps = counters[..., m, 0].sum()
+        if spec.type == "gmc":
+            total += steps * spec.n_reflect * FLOP_EF * pairs
+        elif spec.type == "volume":
+            total += steps * FLOP_E * pairs
+        elif spec.type == "hmc":
+            total += steps * ((spec.n_leapfrog + 1) * FLOP_EF) * pairs
+        else:
+            total += steps * FLOP_E * pairs
+    return float(total)
+
+
+def cpu_baseline(eng, n_replicas_sample: int, threads: int) -> dict:
+    """The CPU oracle (FP64 C restatement, OpenMP) on one iteration of a
+    replica subset of the same workload, from the current population."""
+    from oracle import oracle as O
+    from jaxrens_amd.draws import make_tape
+    O.build()
+    d = eng.get_population()
+    Rs = n_replicas_sample
+    sub = {k: np.ascontiguousarray(v[:Rs]) for k, v in d.items()}
+    pop = O.Population.from_engine_dict(sub)
+    pot = O.Pot.from_table(eng.model.table)
+    K, N, T = eng.n_walkers, eng.Nmax, eng.run.n_mcmc_steps
+    tape = make_tape(np.random.default_rng(7), 1, Rs, K, N, T, eng.run.n_extra,
+                     eng.mwg.move_probs)
+    t0 = time.perf_counter()
+    out = O.ns_iteration(pop, pot, eng.pressure[:Rs], eng.mu[:Rs], eng.step_sizes[:Rs],
+                         eng.mwg.as_oracle_moves(), T, eng.run.n_extra, tape,
+                         do_re=False, n_threads=threads)
+    dt = time.perf_counter() - t0
+    ws = Rs * eng.n_walk * T
+    return {"value": ws / dt, "unit": "walker-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"1 NS iteration of {Rs} of 16 replicas x {eng.n_walk} walked "
+                       f"chains x {T} sub-steps = {ws} walker-steps, {dt:.1f} s, "
+                       f"FP64 C oracle, OpenMP {threads} threads"),
+            "flops_per_s": launch_flops(out["counters"][None], eng) / dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-replicas", type=int, default=2)
+    ap.add_argument("--skip-secondary", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); "
+              "the multi-GPU replica sharding is a design, not a measured path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    eng = build_engine(n_extra=1022)
+    R, n_walk, T = eng.n_replicas, eng.n_walk, eng.run.n_mcmc_steps
+    print(f"[bench] engine ready: R={R} K={eng.n_walkers} N={eng.Nmax} "
+          f"n_walk={n_walk} launch={eng.launch_info()}", file=sys.stderr, flush=True)
+    eng.ns_iterations(args.warmup)
+    eng.walk_timing(reset=True)
+    t0 = time.perf_counter()
+    out = eng.ns_iterations(args.steps)  # one host sync at the end
+    dt = time.perf_counter() - t0
+    walk_ms, launches = eng.walk_timing(reset=True)
+    ws_per_iter = R * n_walk * T
+    value = ws_per_iter * args.steps / dt
+    flops = launch_flops(out["counters"], eng)
+    avg_launch_ms = walk_ms / max(launches, 1)
+    achieved = flops / launches / (avg_launch_ms * 1e-3) / 1e12
+    print(f"[bench] timed {args.steps} iterations in {dt:.3f} s; walk kernel "
+          f"avg {avg_launch_ms:.3f} ms", file=sys.stderr, flush=True)
+
+    secondary = None
+    if not args.skip_secondary:
+        e0 = build_engine(n_extra=0)
+        e0.ns_iterations(5, graph=True)
+        n0 = 200
+        t1 = time.perf_counter()
+        e0.ns_iterations(n0, graph=True)
+        d0 = time.perf_counter() - t1
+        secondary = {"n_extra": 0, "walker_steps_per_s": R * 1 * T * n0 / d0,
+                     "ns_iterations_per_s": n0 / d0, "iterations": n0,
+                     "mode": "hipGraph replay of one iteration"}
+        e0.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(eng, args.cpu_replicas, threads)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (random initial walkers, device Philox draws)",
+        "config": {
+            "workload": "LJ-64 NPT replica stack: N=64, P=16 pressures 0.01-10, "
+                        "K=1024, 20 MCMC sub-steps gmc(n_reflect=10):volume 4:1, "
+                        "pressure RE every iteration, n_extra=K-2 (whole population)",
+            "n_replicas": R, "n_walkers": eng.n_walkers, "n_atoms": eng.Nmax,
+            "n_mcmc_steps": T, "n_extra": eng.run.n_extra,
+            "walker_steps_per_iteration": ws_per_iter,
+            "ns_iterations_per_s": args.steps / dt,
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "walk_kernel (persistent, FP64)",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "flops_per_launch": flops / launches,
+            "avg_launch_ms": avg_launch_ms,
+            "launches": launches,
+        },
+        "cpu_baseline": cpu,
+        "secondary": secondary,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -42,11 +42,25 @@ __device__ __forceinline__ double uniform_d(double v) {
 
 // dst[lane] = src[(lane + 1) % 64]  (DPP wave_rol:1, a VALU move).
 __device__ __forceinline__ double wave_rol1(double v) {
-  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x134, 0xF, 0xF,
-                                       false);
-  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x134, 0xF, 0xF,
-                                       false);
+  int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x134, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x134, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
+}
+
+// Ring rotation when N == 64*A (every ring position occupied): DPP rotate
+// each slot, lane 63 takes lane 0 of the next slot.
+template <int A>
+__device__ __forceinline__ void ring_rot1_full(double (&v)[A], int lane) {
+  double r[A];
+#pragma unroll
+  for (int s = 0; s < A; ++s) r[s] = wave_rol1(v[s]);
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    if (A > 1)
+      v[s] = (lane == 63) ? r[(s + 1) % A] : r[s];
+    else
+      v[s] = r[s];
+  }
 }
 
 // Ring rotation over positions p < N: new[p] = old[(p + 1) % N].

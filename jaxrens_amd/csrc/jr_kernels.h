@@ -55,9 +55,7 @@ struct WalkArgs {
 
 __host__ __device__ inline size_t wave_lds_bytes(int Nmax, int n_moves) {
   (void)n_moves;
-  size_t b = (size_t)Nmax * (6 * sizeof(double) + sizeof(int)) +
-             (size_t)JR_MAX_MOVES * JR_NCOUNT * sizeof(int);
-  return (b + 15) & ~(size_t)15;
+  return wave_lds_bytes_a(Nmax <= 64 ? 1 : (Nmax <= 128 ? 2 : 4));
 }
 
 template <int A, bool MULTI>
@@ -72,14 +70,8 @@ __global__ void __launch_bounds__(256)
   double* tabs = reinterpret_cast<double*>(
       reinterpret_cast<char*>(lds_raw) + per_wave * nwave_blk);
   WaveLds L;
-  L.x = reinterpret_cast<double*>(base);
-  L.y = L.x + a.Nmax;
-  L.z = L.y + a.Nmax;
-  L.hx = L.z + a.Nmax;
-  L.hy = L.hx + a.Nmax;
-  L.hz = L.hy + a.Nmax;
-  L.t = reinterpret_cast<int*>(L.hz + a.Nmax);
-  int* lctr = L.t + a.Nmax;
+  L.base = reinterpret_cast<double*>(base);
+  int* lctr = wave_counters(L, A);
 
   // pair table in LDS (block-shared)
   const int SS = a.S * a.S;
@@ -256,13 +248,7 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs a) {
   double* tabs = reinterpret_cast<double*>(
       reinterpret_cast<char*>(lds_raw) + per_wave * nwave_blk);
   WaveLds L;
-  L.x = reinterpret_cast<double*>(base);
-  L.y = L.x + a.Nmax;
-  L.z = L.y + a.Nmax;
-  L.hx = L.z + a.Nmax;
-  L.hy = L.hx + a.Nmax;
-  L.hz = L.hy + a.Nmax;
-  L.t = reinterpret_cast<int*>(L.hz + a.Nmax);
+  L.base = reinterpret_cast<double*>(base);
   const int SS = a.S * a.S;
   for (int i = threadIdx.x; i < SS; i += blockDim.x) {
     tabs[i] = a.eps4[i];
@@ -301,12 +287,12 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs a) {
   for (int i = 0; i < 9; ++i) cell[i] = a.cell[(size_t)b * 9 + i];
   Geom g;
   make_geom(cell, g);
-  double el, fx[A], fy[A], fz[A];
+  double fx[A], fy[A], fz[A];
+  double U;
   if (a.F)
-    wave_eval<A, true, MULTI>(L, N, g, tab, x, y, z, t, el, fx, fy, fz);
+    U = wave_eval<A, true, MULTI>(L, N, g, tab, x, y, z, t, fx, fy, fz);
   else
-    wave_eval<A, false, MULTI>(L, N, g, tab, x, y, z, t, el, fx, fy, fz);
-  double U = wave_sum((lane < N) ? el : 0.0);
+    U = wave_eval<A, false, MULTI>(L, N, g, tab, x, y, z, t, fx, fy, fz);
   int counts[JR_MAX_SPECIES];
   species_counts<A>(N, a.S, t, counts);
   const double pr = a.pressure ? a.pressure[b] : 0.0;

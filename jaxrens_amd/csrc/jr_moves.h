@@ -83,10 +83,8 @@ __device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
       W.y[s] = W.y[s] + step * W.dy[s];
       W.z[s] = W.z[s] + step * W.dz[s];
     }
-    double el, fx[A], fy[A], fz[A];
-    wave_eval<A, true, MULTI>(C.L, N, g, C.tab, W.x, W.y, W.z, W.t, el, fx,
-                              fy, fz);
-    double U = wave_sum((lane < N) ? el : 0.0);
+    double fx[A], fy[A], fz[A];
+    double U = wave_eval<A, true, MULTI>(C.L, N, g, C.tab, W.x, W.y, W.z, W.t, fx, fy, fz);
     double Hn = U + ens;
     if (__builtin_isnan(Hn)) Hn = __builtin_inf();
     double fsq = wave_dot<A>(N, fx, fy, fz, fx, fy, fz);
@@ -143,10 +141,8 @@ __device__ int volume_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
   }
   Geom g;
   make_geom(nc, g);
-  double el, fx[A], fy[A], fz[A];
-  wave_eval<A, false, MULTI>(C.L, N, g, C.tab, nx, ny, nz, W.t, el, fx, fy,
-                             fz);
-  double U = wave_sum((lane < N) ? el : 0.0);
+  double fx[A], fy[A], fz[A];
+  double U = wave_eval<A, false, MULTI>(C.L, N, g, C.tab, nx, ny, nz, W.t, fx, fy, fz);
   double Hn = U + ensemble_term(nc, C.pressure, C.mu, C.S, W.counts);
   bool cell_ok =
       check_cell_shape(nc, N, mv.max_vpa, mv.min_vpa, mv.min_aspect);
@@ -194,9 +190,9 @@ __device__ bool hmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
   make_geom(W.cell, g);
   const double ens = ensemble_term(W.cell, C.pressure, C.mu, C.S, W.counts);
   const double h = 0.5 * step;
-  double el, fx[A], fy[A], fz[A];
-  wave_eval<A, true, MULTI>(C.L, N, g, C.tab, qx, qy, qz, W.t, el, fx, fy,
-                            fz);
+  double fx[A], fy[A], fz[A];
+  double U = wave_eval<A, true, MULTI>(C.L, N, g, C.tab, qx, qy, qz, W.t, fx,
+                                       fy, fz);
   for (int l = 0; l < mv.n_leapfrog; ++l) {
 #pragma unroll
     for (int s = 0; s < A; ++s) {
@@ -207,8 +203,8 @@ __device__ bool hmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
       qy[s] = qy[s] + step * py[s];
       qz[s] = qz[s] + step * pz[s];
     }
-    wave_eval<A, true, MULTI>(C.L, N, g, C.tab, qx, qy, qz, W.t, el, fx, fy,
-                              fz);
+    U = wave_eval<A, true, MULTI>(C.L, N, g, C.tab, qx, qy, qz, W.t, fx, fy,
+                                  fz);
 #pragma unroll
     for (int s = 0; s < A; ++s) {
       px[s] = px[s] + h * fx[s];
@@ -216,7 +212,6 @@ __device__ bool hmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
       pz[s] = pz[s] + h * fz[s];
     }
   }
-  double U = wave_sum((lane < N) ? el : 0.0);
   double Hn = U + ens;
   double K1 = 0.5 * wave_dot<A>(N, px, py, pz, px, py, pz);
   double dH = (Hn + K1) - (W.E + K0);
@@ -259,10 +254,8 @@ __device__ bool morph_move(const WalkCtx& C, WalkerRegs<A>& W,
     W.counts[sp] += (sp == nt ? 1 : 0) - (sp == cur ? 1 : 0);
   Geom g;
   make_geom(W.cell, g);
-  double el, fx[A], fy[A], fz[A];
-  wave_eval<A, false, MULTI>(C.L, N, g, C.tab, W.x, W.y, W.z, W.t, el, fx,
-                             fy, fz);
-  double U = wave_sum((lane < N) ? el : 0.0);
+  double fx[A], fy[A], fz[A];
+  double U = wave_eval<A, false, MULTI>(C.L, N, g, C.tab, W.x, W.y, W.z, W.t, fx, fy, fz);
   double Hn = U + ensemble_term(W.cell, C.pressure, C.mu, C.S, W.counts);
   bool accepted = Hn < C.emax;
   if (accepted) {

@@ -21,16 +21,36 @@ struct PairTab {
 };
 
 // Per-wave LDS view: positions/types of all atoms plus a scratch row used to
-// return travelling reaction forces home.
+// return travelling reaction forces home.  Arrays are 64*A entries apart so
+// every access is a base register plus an immediate offset.
 struct WaveLds {
-  double* x;
-  double* y;
-  double* z;
-  int* t;
-  double* hx;
-  double* hy;
-  double* hz;
+  double* base;
 };
+
+template <int A>
+struct LdsView {
+  double* b;
+  __device__ __forceinline__ double* x() const { return b; }
+  __device__ __forceinline__ double* y() const { return b + 64 * A; }
+  __device__ __forceinline__ double* z() const { return b + 128 * A; }
+  __device__ __forceinline__ double* hx() const { return b + 192 * A; }
+  __device__ __forceinline__ double* hy() const { return b + 256 * A; }
+  __device__ __forceinline__ double* hz() const { return b + 320 * A; }
+  __device__ __forceinline__ int* t() const {
+    return reinterpret_cast<int*>(b + 384 * A);
+  }
+};
+
+// bytes of one wave's LDS region (arrays + JR_MAX_MOVES*JR_NCOUNT counters)
+__host__ __device__ inline size_t wave_lds_bytes_a(int A) {
+  size_t b = (size_t)384 * A * sizeof(double) + (size_t)64 * A * sizeof(int) +
+             (size_t)JR_MAX_MOVES * JR_NCOUNT * sizeof(int);
+  return (b + 15) & ~(size_t)15;
+}
+
+__device__ __forceinline__ int* wave_counters(const WaveLds& L, int A) {
+  return reinterpret_cast<int*>(L.base + 384 * A) + 64 * A;
+}
 
 struct Geom {
   double c[9];
@@ -50,20 +70,22 @@ __device__ __forceinline__ void make_geom(const double* cell, Geom& g) {
     for (int i = 0; i < 9; ++i) g.c[i] = cell[i];
   }
   inv3(g.c, det, g.inv);
+  // wave-uniform: keep in scalar registers
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    g.c[i] = uniform_d(g.c[i]);
+    g.inv[i] = uniform_d(g.inv[i]);
+  }
   g.diag = (g.c[1] == 0.0) && (g.c[2] == 0.0) && (g.c[3] == 0.0) &&
            (g.c[5] == 0.0) && (g.c[6] == 0.0) && (g.c[7] == 0.0);
 }
 
-// One pair, dr = MIC(x_j - x_i) (backends/lj.py:141-197).  inv2 is forced
-// to zero outside the cutoff or for an inactive ring position, which zeroes
-// both the energy and the force scalar.
-template <bool FORCES, bool DIAG>
-__device__ __forceinline__ void lj_pair(double xi, double yi, double zi,
+// dr = MIC(x_j - x_i) and r^2 (backends/lj.py:141-162).
+template <bool DIAG>
+__device__ __forceinline__ void pair_dr(double xi, double yi, double zi,
                                         double xj, double yj, double zj,
-                                        const Geom& g, double eps4,
-                                        double eps24, double sig2, double rc2,
-                                        bool active, double& e, double& fs,
-                                        double& dx, double& dy, double& dz) {
+                                        const Geom& g, double& dx, double& dy,
+                                        double& dz, double& r2) {
   dx = xj - xi;
   dy = yj - yi;
   dz = zj - zi;
@@ -86,25 +108,82 @@ __device__ __forceinline__ void lj_pair(double xi, double yi, double zi,
     dy = ey;
     dz = ez;
   }
-  double r2 = dot3(dx, dy, dz, dx, dy, dz);
-  double inv2 = 1.0 / r2;
-  inv2 = (active && (r2 < rc2)) ? inv2 : 0.0;
-  double s2 = sig2 * inv2;
-  double s6 = (s2 * s2) * s2;
-  double s12 = s6 * s6;
-  e = eps4 * (s12 - s6);
-  if (FORCES) fs = (eps24 * fma(2.0, s12, -s6)) * inv2;
+  r2 = dot3(dx, dy, dz, dx, dy, dz);
 }
 
-// Energy partial (per lane) and forces of the configuration held in (x,y,z,t)
-// registers; writes the configuration into the wave's LDS first.
+// Pair terms from q, where q = sigma^2/r^2 (one species: epsilon and
+// 24 epsilon/sigma^2 are applied once per evaluation) or q = 1/r^2 (species
+// table).  q == 0 outside the cutoff / for idle ring positions.
+template <bool FORCES, bool MULTI>
+__device__ __forceinline__ void pair_lj(double q, double eps4, double eps24,
+                                        double sig2, double& e, double& fs) {
+  double s2 = MULTI ? sig2 * q : q;
+  double s6 = (s2 * s2) * s2;
+  double s12 = s6 * s6;
+  if (MULTI) {
+    e = eps4 * (s12 - s6);
+    if (FORCES) fs = (eps24 * fma(2.0, s12, -s6)) * q;
+  } else {
+    e = s12 - s6;
+    if (FORCES) fs = fma(2.0, s12, -s6) * q;
+  }
+}
+
+struct PairIn {
+  double dx, dy, dz, r2;
+  bool ok;  // active and r2 < rc2
+  double eps4, eps24, sig2;
+};
+
+template <int A, bool MULTI, bool DIAG, bool FULL>
+__device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int N, int s,
+                                            int k, const Geom& g,
+                                            const PairTab& tab,
+                                            const double (&x)[A],
+                                            const double (&y)[A],
+                                            const double (&z)[A],
+                                            const int (&t)[A]) {
+  const int a = lane_id() + 64 * s;
+  bool active = true;
+  int j;
+  if (FULL) {
+    j = (a + k) & (64 * A - 1);
+  } else {
+    active = a < N;
+    j = a + k;
+    if (j >= N) j -= N;
+    if (!active) j = 0;
+  }
+  PairIn P;
+  pair_dr<DIAG>(x[s], y[s], z[s], L.x()[j], L.y()[j], L.z()[j], g, P.dx, P.dy,
+                P.dz, P.r2);
+  P.ok = active && (P.r2 < tab.rc2);
+  P.eps4 = tab.eps4[0];
+  P.eps24 = tab.eps24[0];
+  P.sig2 = tab.sig2[0];
+  if (MULTI) {
+    const int ab = t[s] * tab.S + L.t()[j];
+    P.eps4 = tab.eps4[ab];
+    P.eps24 = tab.eps24[ab];
+    P.sig2 = tab.sig2[ab];
+  }
+  return P;
+}
+
+// Energy partial (per lane) and forces of the configuration whose positions
+// and types are in the wave's LDS (and in the x,y,z,t registers).
 // Pair schedule: for k = 1 .. (N-1)/2 every ring position p pairs with
 // (p + k) mod N, keeping its own force and adding the reaction to a
 // travelling accumulator that rotates one position per k; for even N the
 // k = N/2 pairs are evaluated from both ends with half the energy each.
-template <int A, bool FORCES, bool MULTI, bool DIAG>
+// Steps k and k+1 (k odd) share one IEEE division:
+//   d = num / (r2_k * r2_{k+1}),  q_k = r2_{k+1} * d,  q_{k+1} = r2_k * d,
+// with num = sigma^2 (one species) or 1; a lone last step and the k = N/2
+// step use q = num / r2.
+// FULL: N == 64*A, so no ring position is idle and (p + k) mod N is a mask.
+template <int A, bool FORCES, bool MULTI, bool DIAG, bool FULL>
 __device__ __forceinline__ void wave_eval_impl(
-    const WaveLds& L, int N, const Geom& g, const PairTab& tab,
+    const LdsView<A>& L, int N, const Geom& g, const PairTab& tab,
     const double (&x)[A], const double (&y)[A], const double (&z)[A],
     const int (&t)[A], double& e_lane, double (&fx)[A], double (&fy)[A],
     double (&fz)[A]) {
@@ -116,97 +195,110 @@ __device__ __forceinline__ void wave_eval_impl(
     ox[s] = oy[s] = oz[s] = 0.0;
     tx[s] = ty[s] = tz[s] = 0.0;
   }
-  double eps4 = tab.eps4[0], eps24 = tab.eps24[0], sig2 = tab.sig2[0];
-  const int kmax = (N - 1) >> 1;
-  for (int k = 1; k <= kmax; ++k) {
-    if (FORCES && k > 1) {
+  const double num = MULTI ? 1.0 : tab.sig2[0];
+  auto rotate = [&]() {
+    if (FULL) {
+      ring_rot1_full<A>(tx, lane);
+      ring_rot1_full<A>(ty, lane);
+      ring_rot1_full<A>(tz, lane);
+    } else {
       ring_rot1<A>(tx, lane, N);
       ring_rot1<A>(ty, lane, N);
       ring_rot1<A>(tz, lane, N);
     }
+  };
+  auto accumulate = [&](int s, const PairIn& P, double q, bool reaction,
+                        double escale) {
+    double ep, fs = 0.0;
+    pair_lj<FORCES, MULTI>(q, P.eps4, P.eps24, P.sig2, ep, fs);
+    e[s] = e[s] + (escale == 1.0 ? ep : escale * ep);
+    if (FORCES) {
+      ox[s] = fma(-fs, P.dx, ox[s]);
+      oy[s] = fma(-fs, P.dy, oy[s]);
+      oz[s] = fma(-fs, P.dz, oz[s]);
+      if (reaction) {
+        tx[s] = fma(fs, P.dx, tx[s]);
+        ty[s] = fma(fs, P.dy, ty[s]);
+        tz[s] = fma(fs, P.dz, tz[s]);
+      }
+    }
+  };
+  const int kmax = (N - 1) >> 1;
+  int k = 1;
+  for (; k + 1 <= kmax; k += 2) {
+    PairIn P0[A], P1[A];
+    double q0[A], q1[A];
 #pragma unroll
     for (int s = 0; s < A; ++s) {
-      const int a = lane + 64 * s;
-      const bool active = a < N;
-      int j = a + k;
-      if (j >= N) j -= N;
-      if (!active) j = 0;
-      double xj = L.x[j], yj = L.y[j], zj = L.z[j];
-      if (MULTI) {
-        int ab = t[s] * tab.S + L.t[j];
-        eps4 = tab.eps4[ab];
-        eps24 = tab.eps24[ab];
-        sig2 = tab.sig2[ab];
-      }
-      double ep, fs = 0.0, dx, dy, dz;
-      lj_pair<FORCES, DIAG>(x[s], y[s], z[s], xj, yj, zj, g, eps4, eps24,
-                            sig2, tab.rc2, active, ep, fs, dx, dy, dz);
-      e[s] = e[s] + ep;
-      if (FORCES) {
-        ox[s] = fma(-fs, dx, ox[s]);
-        oy[s] = fma(-fs, dy, oy[s]);
-        oz[s] = fma(-fs, dz, oz[s]);
-        tx[s] = fma(fs, dx, tx[s]);
-        ty[s] = fma(fs, dy, ty[s]);
-        tz[s] = fma(fs, dz, tz[s]);
-      }
+      P0[s] = load_pair<A, MULTI, DIAG, FULL>(L, N, s, k, g, tab, x, y, z, t);
+      P1[s] = load_pair<A, MULTI, DIAG, FULL>(L, N, s, k + 1, g, tab, x, y, z,
+                                              t);
+      double d = num / (P0[s].r2 * P1[s].r2);
+      q0[s] = P0[s].ok ? P1[s].r2 * d : 0.0;
+      q1[s] = P1[s].ok ? P0[s].r2 * d : 0.0;
+    }
+    if (FORCES && k > 1) rotate();
+#pragma unroll
+    for (int s = 0; s < A; ++s) accumulate(s, P0[s], q0[s], true, 1.0);
+    if (FORCES) rotate();
+#pragma unroll
+    for (int s = 0; s < A; ++s) accumulate(s, P1[s], q1[s], true, 1.0);
+  }
+  if (k <= kmax) {  // lone last step
+    if (FORCES && k > 1) rotate();
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      PairIn P = load_pair<A, MULTI, DIAG, FULL>(L, N, s, k, g, tab, x, y, z, t);
+      double q = P.ok ? num / P.r2 : 0.0;
+      accumulate(s, P, q, true, 1.0);
     }
   }
   if (N >= 2 && !(N & 1)) {
     const int h = N >> 1;
 #pragma unroll
     for (int s = 0; s < A; ++s) {
-      const int a = lane + 64 * s;
-      const bool active = a < N;
-      int j = a + h;
-      if (j >= N) j -= N;
-      if (!active) j = 0;
-      double xj = L.x[j], yj = L.y[j], zj = L.z[j];
-      if (MULTI) {
-        int ab = t[s] * tab.S + L.t[j];
-        eps4 = tab.eps4[ab];
-        eps24 = tab.eps24[ab];
-        sig2 = tab.sig2[ab];
-      }
-      double ep, fs = 0.0, dx, dy, dz;
-      lj_pair<FORCES, DIAG>(x[s], y[s], z[s], xj, yj, zj, g, eps4, eps24,
-                            sig2, tab.rc2, active, ep, fs, dx, dy, dz);
-      e[s] = e[s] + 0.5 * ep;
-      if (FORCES) {
-        ox[s] = fma(-fs, dx, ox[s]);
-        oy[s] = fma(-fs, dy, oy[s]);
-        oz[s] = fma(-fs, dz, oz[s]);
-      }
+      PairIn P = load_pair<A, MULTI, DIAG, FULL>(L, N, s, h, g, tab, x, y, z, t);
+      double q = P.ok ? num / P.r2 : 0.0;
+      accumulate(s, P, q, false, 0.5);
     }
   }
   double el = e[0];
 #pragma unroll
   for (int s = 1; s < A; ++s) {
     const int a = lane + 64 * s;
-    if (a < N) el = el + e[s];
+    if (FULL || a < N) el = el + e[s];
   }
   e_lane = el;
   if (FORCES) {
     // position p holds the reaction of atom (p + kmax) mod N
+    double* HX = L.hx();
+    double* HY = L.hy();
+    double* HZ = L.hz();
 #pragma unroll
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
-      if (a < N) {
+      if (FULL || a < N) {
         int dst = a + kmax;
         if (dst >= N) dst -= N;
-        L.hx[dst] = tx[s];
-        L.hy[dst] = ty[s];
-        L.hz[dst] = tz[s];
+        HX[dst] = tx[s];
+        HY[dst] = ty[s];
+        HZ[dst] = tz[s];
       }
     }
     wave_lds_sync();
+    const double c24 = MULTI ? 1.0 : tab.eps24[0] / tab.sig2[0];
 #pragma unroll
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
-      if (a < N) {
-        fx[s] = ox[s] + L.hx[a];
-        fy[s] = oy[s] + L.hy[a];
-        fz[s] = oz[s] + L.hz[a];
+      if (FULL || a < N) {
+        fx[s] = ox[s] + HX[a];
+        fy[s] = oy[s] + HY[a];
+        fz[s] = oz[s] + HZ[a];
+        if (!MULTI) {
+          fx[s] = c24 * fx[s];
+          fy[s] = c24 * fy[s];
+          fz[s] = c24 * fz[s];
+        }
       } else {
         fx[s] = fy[s] = fz[s] = 0.0;
       }
@@ -215,34 +307,51 @@ __device__ __forceinline__ void wave_eval_impl(
   }
 }
 
+// Model energy U (wave-uniform) and forces of the configuration in
+// registers; writes the configuration into the wave's LDS first.
 template <int A, bool FORCES, bool MULTI>
-__device__ __forceinline__ void wave_eval(const WaveLds& L, int N,
-                                          const Geom& g, const PairTab& tab,
-                                          const double (&x)[A],
-                                          const double (&y)[A],
-                                          const double (&z)[A],
-                                          const int (&t)[A], double& e_lane,
-                                          double (&fx)[A], double (&fy)[A],
-                                          double (&fz)[A]) {
+__device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
+                                            const Geom& g, const PairTab& tab,
+                                            const double (&x)[A],
+                                            const double (&y)[A],
+                                            const double (&z)[A],
+                                            const int (&t)[A],
+                                            double (&fx)[A], double (&fy)[A],
+                                            double (&fz)[A]) {
   const int lane = lane_id();
+  const LdsView<A> L{W.base};
 #pragma unroll
   for (int s = 0; s < A; ++s) {
     const int a = lane + 64 * s;
     if (a < N) {
-      L.x[a] = x[s];
-      L.y[a] = y[s];
-      L.z[a] = z[s];
-      if (MULTI) L.t[a] = t[s];
+      L.x()[a] = x[s];
+      L.y()[a] = y[s];
+      L.z()[a] = z[s];
+      if (MULTI) L.t()[a] = t[s];
     }
   }
   wave_lds_sync();
-  if (g.diag)
-    wave_eval_impl<A, FORCES, MULTI, true>(L, N, g, tab, x, y, z, t, e_lane,
-                                           fx, fy, fz);
-  else
-    wave_eval_impl<A, FORCES, MULTI, false>(L, N, g, tab, x, y, z, t, e_lane,
-                                            fx, fy, fz);
+  double el;
+  const bool full = (N == 64 * A);
+  if (g.diag) {
+    if (full)
+      wave_eval_impl<A, FORCES, MULTI, true, true>(L, N, g, tab, x, y, z, t,
+                                                   el, fx, fy, fz);
+    else
+      wave_eval_impl<A, FORCES, MULTI, true, false>(L, N, g, tab, x, y, z, t,
+                                                    el, fx, fy, fz);
+  } else {
+    if (full)
+      wave_eval_impl<A, FORCES, MULTI, false, true>(L, N, g, tab, x, y, z, t,
+                                                    el, fx, fy, fz);
+    else
+      wave_eval_impl<A, FORCES, MULTI, false, false>(L, N, g, tab, x, y, z, t,
+                                                     el, fx, fy, fz);
+  }
   wave_lds_sync();
+  double U = wave_sum((lane < N) ? el : 0.0);
+  if (!MULTI) U = tab.eps4[0] * U;
+  return U;
 }
 
 // Lane partial of dot3 over active slots, then the wave butterfly.

@@ -1,0 +1,161 @@
+"""Host outer loop of nested sampling over the device engine.
+
+Per iteration, in the reference's order (``sampling/run_loop.py:486-625``):
+adaptation when ``i % adjust_interval == 0`` (including ``i = 0`` with
+``emax = +inf``), the NS step (select, clone, walk; replica exchange inside
+the same device iteration when ``i > 0 and i % re_interval == 0``),
+``.energies`` rows (``io/energy_log.py:57-67``), callbacks, then termination
+on ``min(log Z)`` / ``max(emax)`` over replicas
+(``batch_descriptor.py:415-426``) with ``check_any``.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+from .adaptation import AdaptationConfig, adjust_step_size
+from .termination import PriorMassTermination, check_any
+
+
+class EnergyLogger:
+    """``.energies`` file: header ``K n_cull n_dof 0.0 N``, rows
+    ``iter emax volume`` (``io/energy_log.py:24-104``)."""
+
+    def __init__(self, path, n_walkers: int, n_cull: int = 1, n_dof: int = 0,
+                 n_atoms: int = 0):
+        self.path = Path(path)
+        self.path.parent.mkdir(parents=True, exist_ok=True)
+        self._f = open(self.path, "w")
+        self._f.write(f"{n_walkers} {n_cull} {n_dof} 0.0 {n_atoms}\n")
+
+    def write_entry(self, iteration: int, energy: float, volume: float = 0.0):
+        self._f.write(f"{iteration} {energy:.10e} {volume:.10e}\n")
+
+    def flush(self):
+        self._f.flush()
+
+    def close(self):
+        if self._f is not None:
+            self._f.close()
+            self._f = None
+
+    @staticmethod
+    def read(path):
+        with open(path) as f:
+            hdr = f.readline().split()
+            rows = np.loadtxt(f, ndmin=2)
+        return {"n_walkers": int(hdr[0]), "n_cull": int(hdr[1]),
+                "n_dof": int(hdr[2]), "n_atoms": int(hdr[4]) if len(hdr) > 4 else 0,
+                "iterations": rows[:, 0].astype(int), "energies": rows[:, 1],
+                "volumes": rows[:, 2]}
+
+
+@dataclass
+class RunResult:
+    iterations: int
+    reason: str
+    log_evidence: np.ndarray          # (R,) including the live walkers' share
+    emax: np.ndarray                  # (iterations, R)
+    dead_volume: np.ndarray           # (iterations, R)
+    step_sizes: np.ndarray            # (R, n_moves) at the end
+    counters: np.ndarray              # (R, n_moves, 7) summed over iterations
+    re_acc: np.ndarray
+    re_att: np.ndarray
+    adaptation: list = field(default_factory=list)
+
+
+def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
+           energies_prefix: str | None = None, callbacks=(), seed: int = 0,
+           block: int = 1) -> RunResult:
+    """Drive ``engine`` until a termination criterion fires.
+
+    ``block`` > 1 runs that many iterations per device call when no
+    adaptation falls inside the block and the criteria can only fire at the
+    block's last iteration; with the default 1 every iteration syncs, which
+    reproduces the reference's stopping iteration exactly.
+    """
+    rng = np.random.default_rng(seed)
+    R = engine.n_replicas
+    loggers = []
+    if energies_prefix is not None:
+        for r in range(R):
+            suffix = "" if R == 1 else f".{r}"
+            loggers.append(EnergyLogger(f"{energies_prefix}{suffix}.energies",
+                                        engine.n_walkers, engine.run.n_cull, 0,
+                                        int(engine.n_atoms_rep[r])))
+    adaptable = [m for m, spec in enumerate(engine.mwg.moves)
+                 if spec.type != "alchemical_morph"]
+    emax_hist, vol_hist, adapt_log = [], [], []
+    counters = np.zeros((R, engine.n_moves, 7), np.int64)
+    re_acc = np.zeros(max(R - 1, 1), np.int64)
+    re_att = np.zeros(max(R - 1, 1), np.int64)
+    i = 0
+    reason = ""
+    try:
+        while True:
+            if adaptation is not None and i % adaptation.adjust_interval == 0:
+                emax = engine.emax()
+                ss = engine.step_sizes.copy()
+                rec = {"iteration": i}
+                for m in adaptable:
+                    new, info = adjust_step_size(engine, m, ss[:, m], emax, rng,
+                                                 adaptation)
+                    ss[:, m] = new
+                    rec[engine.mwg.names[m]] = info
+                engine.set_run_params(step_sizes=ss)
+                adapt_log.append(rec)
+            n = 1
+            if block > 1:
+                n = block
+                if adaptation is not None:
+                    n = min(n, adaptation.adjust_interval
+                            - (i % adaptation.adjust_interval))
+                n = max(1, min(n, _iterations_before_check(termination, i)))
+            out = engine.ns_iterations(n)
+            for j in range(n):
+                it = i + j
+                emax_hist.append(out["emax"][j])
+                vol_hist.append(out["dead_volume"][j])
+                for r, lg in enumerate(loggers):
+                    lg.write_entry(it, float(out["emax"][j, r]),
+                                   float(out["dead_volume"][j, r]))
+            counters += out["counters"].sum(0)
+            re_acc += out["re_acc"].sum(0)
+            re_att += out["re_att"].sum(0)
+            for cb in callbacks:
+                cb(i + n - 1, engine, out)
+            i += n
+            log_z = float(np.min(engine.log_evidence))
+            hmax = float(np.max(out["emax"][-1]))
+            for c in termination:
+                if isinstance(c, PriorMassTermination):
+                    c.update_evidence(log_z)
+            stop, reason = check_any(termination, i - 1, hmax)
+            if stop:
+                break
+    finally:
+        for lg in loggers:
+            lg.close()
+    return RunResult(
+        iterations=i, reason=reason, log_evidence=engine.final_log_evidence(),
+        emax=np.array(emax_hist), dead_volume=np.array(vol_hist),
+        step_sizes=engine.step_sizes.copy(), counters=counters,
+        re_acc=re_acc[: R - 1], re_att=re_att[: R - 1], adaptation=adapt_log)
+
+
+def _iterations_before_check(termination, i: int) -> int:
+    """Largest block that cannot skip a stopping iteration: iteration limits
+    are exact; other criteria depend on the running evidence, so blocks are
+    limited to one iteration for them."""
+    from .termination import IterationTermination
+    n = math.inf
+    for c in termination:
+        if isinstance(c, IterationTermination):
+            n = min(n, c.max_iterations - i)
+        else:
+            return 1
+    return int(n) if n != math.inf else 1

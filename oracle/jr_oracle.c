@@ -140,41 +140,76 @@ double or_ensemble_energy(int N, const int32_t* types, const double* cell,
   return e;
 }
 
-/* One pair (i, j): dr = MIC(x_j - x_i); backends/lj.py:141-197. */
+/* One pair, dr = MIC(x_j - x_i) and r^2; backends/lj.py:141-162. */
 typedef struct {
-  double e, fs, dx, dy, dz;
+  double dx, dy, dz, r2;
+  double eps4, eps24, sig2;
+  int ok; /* r2 < rc2 */
 } pair_t;
 
-static inline pair_t lj_pair(const double* xi, const double* xj,
-                             const double* cell, const double* inv, int ti,
-                             int tj, const or_pot* pot) {
+static inline pair_t pair_geom(const double* xi, const double* xj,
+                               const double* cell, const double* inv, int ti,
+                               int tj, const or_pot* pot) {
   pair_t r;
   double dx = xj[0] - xi[0], dy = xj[1] - xi[1], dz = xj[2] - xi[2];
   double f0 = fma(dz, inv[6], fma(dy, inv[3], dx * inv[0]));
   double f1 = fma(dz, inv[7], fma(dy, inv[4], dx * inv[1]));
   double f2 = fma(dz, inv[8], fma(dy, inv[5], dx * inv[2]));
   double n0 = nearbyint(f0), n1 = nearbyint(f1), n2 = nearbyint(f2);
-  dx = fma(-n2, cell[6], fma(-n1, cell[3], fma(-n0, cell[0], dx)));
-  dy = fma(-n2, cell[7], fma(-n1, cell[4], fma(-n0, cell[1], dy)));
-  dz = fma(-n2, cell[8], fma(-n1, cell[5], fma(-n0, cell[2], dz)));
-  double r2 = dot3(dx, dy, dz, dx, dy, dz);
-  int S = pot->n_species;
-  int ab = ti * S + tj;
-  double inv2 = 1.0 / r2;
-  if (!(r2 < pot->rc2)) inv2 = 0.0;
-  double s2 = pot->sig2[ab] * inv2;
-  double s6 = (s2 * s2) * s2;
-  double s12 = s6 * s6;
-  r.e = pot->eps4[ab] * (s12 - s6);
-  r.fs = (pot->eps24[ab] * fma(2.0, s12, -s6)) * inv2;
-  r.dx = dx;
-  r.dy = dy;
-  r.dz = dz;
+  r.dx = fma(-n2, cell[6], fma(-n1, cell[3], fma(-n0, cell[0], dx)));
+  r.dy = fma(-n2, cell[7], fma(-n1, cell[4], fma(-n0, cell[1], dy)));
+  r.dz = fma(-n2, cell[8], fma(-n1, cell[5], fma(-n0, cell[2], dz)));
+  r.r2 = dot3(r.dx, r.dy, r.dz, r.dx, r.dy, r.dz);
+  r.ok = r.r2 < pot->rc2;
+  int ab = ti * pot->n_species + tj;
+  r.eps4 = pot->eps4[ab];
+  r.eps24 = pot->eps24[ab];
+  r.sig2 = pot->sig2[ab];
   return r;
 }
 
+/* LJ terms from q = sigma^2/r^2 (one species; eps applied per evaluation)
+ * or q = 1/r^2 (species table); backends/lj.py:175-197. */
+static inline void pair_lj(const pair_t* p, double q, int multi, double* e,
+                           double* fs) {
+  double s2 = multi ? p->sig2 * q : q;
+  double s6 = (s2 * s2) * s2;
+  double s12 = s6 * s6;
+  if (multi) {
+    *e = p->eps4 * (s12 - s6);
+    *fs = (p->eps24 * fma(2.0, s12, -s6)) * q;
+  } else {
+    *e = s12 - s6;
+    *fs = fma(2.0, s12, -s6) * q;
+  }
+}
+
+static inline void acc_pair(int a, int j, const pair_t* p, double q, int multi,
+                            int reaction, double escale, double* e,
+                            double* own, double* trav) {
+  double ep, fs;
+  pair_lj(p, q, multi, &ep, &fs);
+  e[a] = e[a] + (escale == 1.0 ? ep : escale * ep);
+  own[3 * a + 0] = fma(-fs, p->dx, own[3 * a + 0]);
+  own[3 * a + 1] = fma(-fs, p->dy, own[3 * a + 1]);
+  own[3 * a + 2] = fma(-fs, p->dz, own[3 * a + 2]);
+  if (reaction) {
+    trav[3 * j + 0] = fma(fs, p->dx, trav[3 * j + 0]);
+    trav[3 * j + 1] = fma(fs, p->dy, trav[3 * j + 1]);
+    trav[3 * j + 2] = fma(fs, p->dz, trav[3 * j + 2]);
+  }
+}
+
 /* Model energy U (and forces F = -dU/dx when F != NULL), then
- * H = U + ensemble term; backends/hamiltonian.py:112-153. */
+ * H = U + ensemble term; backends/hamiltonian.py:112-153.
+ * Pair schedule: for k = 1..(N-1)/2 atom a pairs with (a+k) mod N; the
+ * force on a is accumulated in k order into own[a] and the reaction on
+ * j = a+k into trav[j] (also in k order); for even N the k = N/2 pairs are
+ * taken from both ends with half the energy and no reaction.  Steps k and
+ * k+1 (k odd) share one division d = num/(r2_k r2_{k+1}), q_k = r2_{k+1} d,
+ * q_{k+1} = r2_k d (num = sigma^2 for one species, else 1); a lone last step
+ * and the k = N/2 step use q = num/r2.  One species: U = 4 eps * sum and
+ * F = (24 eps / sigma^2) * (own + trav). */
 void or_energy_forces(int N, const double* pos, const int32_t* types,
                       const double* cell_in, const or_pot* pot,
                       double pressure, const double* mu, double* U_out,
@@ -190,47 +225,72 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
   }
   double inv[9];
   inv3(cell, det, inv);
+  const int multi = pot->n_species > 1;
+  const double num = multi ? 1.0 : pot->sig2[0];
 
-  double* e = (double*)calloc((size_t)(N > 0 ? N : 1), sizeof(double));
-  double* own = (double*)calloc((size_t)(3 * (N > 0 ? N : 1)), sizeof(double));
-  double* trav = (double*)calloc((size_t)(3 * (N > 0 ? N : 1)), sizeof(double));
+  size_t nn = (size_t)(N > 0 ? N : 1);
+  double* e = (double*)calloc(nn, sizeof(double));
+  double* own = (double*)calloc(3 * nn, sizeof(double));
+  double* trav = (double*)calloc(3 * nn, sizeof(double));
+  pair_t* p0 = (pair_t*)malloc(nn * sizeof(pair_t));
+  pair_t* p1 = (pair_t*)malloc(nn * sizeof(pair_t));
+  double* q0 = (double*)malloc(nn * sizeof(double));
+  double* q1 = (double*)malloc(nn * sizeof(double));
   int kmax = (N - 1) / 2;
   int half = (N >= 2) && (N % 2 == 0);
-  for (int k = 1; k <= kmax; ++k) {
+  int k = 1;
+  for (; k + 1 <= kmax; k += 2) {
     for (int a = 0; a < N; ++a) {
-      int j = a + k;
-      if (j >= N) j -= N;
-      pair_t p = lj_pair(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
-                         types[j], pot);
-      e[a] = e[a] + p.e;
-      own[3 * a + 0] = fma(-p.fs, p.dx, own[3 * a + 0]);
-      own[3 * a + 1] = fma(-p.fs, p.dy, own[3 * a + 1]);
-      own[3 * a + 2] = fma(-p.fs, p.dz, own[3 * a + 2]);
-      trav[3 * j + 0] = fma(p.fs, p.dx, trav[3 * j + 0]);
-      trav[3 * j + 1] = fma(p.fs, p.dy, trav[3 * j + 1]);
-      trav[3 * j + 2] = fma(p.fs, p.dz, trav[3 * j + 2]);
+      int j0 = (a + k) % N, j1 = (a + k + 1) % N;
+      p0[a] = pair_geom(pos + 3 * a, pos + 3 * j0, cell, inv, types[a],
+                        types[j0], pot);
+      p1[a] = pair_geom(pos + 3 * a, pos + 3 * j1, cell, inv, types[a],
+                        types[j1], pot);
+      double d = num / (p0[a].r2 * p1[a].r2);
+      q0[a] = p0[a].ok ? p1[a].r2 * d : 0.0;
+      q1[a] = p1[a].ok ? p0[a].r2 * d : 0.0;
+    }
+    for (int a = 0; a < N; ++a)
+      acc_pair(a, (a + k) % N, &p0[a], q0[a], multi, 1, 1.0, e, own, trav);
+    for (int a = 0; a < N; ++a)
+      acc_pair(a, (a + k + 1) % N, &p1[a], q1[a], multi, 1, 1.0, e, own,
+               trav);
+  }
+  if (k <= kmax) {
+    for (int a = 0; a < N; ++a) {
+      int j = (a + k) % N;
+      pair_t p = pair_geom(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
+                           types[j], pot);
+      double q = p.ok ? num / p.r2 : 0.0;
+      acc_pair(a, j, &p, q, multi, 1, 1.0, e, own, trav);
     }
   }
   if (half) {
     int h = N / 2;
     for (int a = 0; a < N; ++a) {
-      int j = a + h;
-      if (j >= N) j -= N;
-      pair_t p = lj_pair(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
-                         types[j], pot);
-      e[a] = e[a] + 0.5 * p.e;
-      own[3 * a + 0] = fma(-p.fs, p.dx, own[3 * a + 0]);
-      own[3 * a + 1] = fma(-p.fs, p.dy, own[3 * a + 1]);
-      own[3 * a + 2] = fma(-p.fs, p.dz, own[3 * a + 2]);
+      int j = (a + h) % N;
+      pair_t p = pair_geom(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
+                           types[j], pot);
+      double q = p.ok ? num / p.r2 : 0.0;
+      acc_pair(a, j, &p, q, multi, 0, 0.5, e, own, trav);
     }
   }
   double U = wave_sum_atoms(e, N);
+  if (!multi) U = pot->eps4[0] * U;
   if (F != NULL) {
-    for (int a = 0; a < 3 * N; ++a) F[a] = own[a] + trav[a];
+    double c24 = multi ? 1.0 : pot->eps24[0] / pot->sig2[0];
+    for (int a = 0; a < 3 * N; ++a) {
+      F[a] = own[a] + trav[a];
+      if (!multi) F[a] = c24 * F[a];
+    }
   }
   free(e);
   free(own);
   free(trav);
+  free(p0);
+  free(p1);
+  free(q0);
+  free(q1);
   *U_out = U;
   if (H_out != NULL)
     *H_out = U + or_ensemble_energy(N, types, cell_in, pressure, mu,
