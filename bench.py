@@ -27,16 +27,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector, dense
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed issue)
 METRIC = "walker-steps/s, LJ-64 NPT P×K=16×1024; surrogate vs plain at matched log Z"
 
 # algorithmic FP64 FLOPs per unordered pair (cubic MIC): energy+forces, energy
 FLOP_EF, FLOP_E = 38, 25
 
 
-def build_engine(n_extra: int, R=16, K=1024, N=64, n_steps=20, seed=42):
+def build_engine(n_extra: int, R=16, K=1024, N=64, n_steps=20, seed=42,
+                 precision="fp64"):
     from jaxrens_amd.backends import LJBackend
     from jaxrens_amd.engine import NSEngine, RunConfig
-    lj = LJBackend(1.0, 1.0, 2.5)
+    lj = LJBackend(1.0, 1.0, 2.5, precision=precision)
     moves = [{"type": "galilean", "n_reflect": 10, "step_size": 0.1, "weight": 4},
              {"type": "volume", "step_size": 0.3, "weight": 1}]
     cell = {"min_volume_per_atom": 0.125, "max_volume_per_atom": 64.0,
@@ -70,9 +72,9 @@ def launch_flops(counters: np.ndarray, eng) -> float:
     return float(total)
 
 
-def cpu_baseline(eng, n_replicas_sample: int, threads: int) -> dict:
-    """The CPU oracle (FP64 C restatement, OpenMP) on one iteration of a
-    replica subset of the same workload, from the current population."""
+def cpu_baseline(eng, n_replicas_sample: int, threads: int, n_iter: int = 1) -> dict:
+    """The CPU oracle (FP64 C restatement, OpenMP) on ``n_iter`` iterations
+    of a replica subset of the same workload, from the current population."""
     from oracle import oracle as O
     from jaxrens_amd.draws import make_tape
     O.build()
@@ -82,20 +84,27 @@ def cpu_baseline(eng, n_replicas_sample: int, threads: int) -> dict:
     pop = O.Population.from_engine_dict(sub)
     pot = O.Pot.from_table(eng.model.table)
     K, N, T = eng.n_walkers, eng.Nmax, eng.run.n_mcmc_steps
-    tape = make_tape(np.random.default_rng(7), 1, Rs, K, N, T, eng.run.n_extra,
+    tape = make_tape(np.random.default_rng(7), n_iter, Rs, K, N, T, eng.run.n_extra,
                      eng.mwg.move_probs)
-    t0 = time.perf_counter()
-    out = O.ns_iteration(pop, pot, eng.pressure[:Rs], eng.mu[:Rs], eng.step_sizes[:Rs],
-                         eng.mwg.as_oracle_moves(), T, eng.run.n_extra, tape,
-                         do_re=False, n_threads=threads)
-    dt = time.perf_counter() - t0
-    ws = Rs * eng.n_walk * T
+    dt, flops = 0.0, 0.0
+    for it in range(n_iter):
+        t0 = time.perf_counter()
+        out = O.ns_iteration(pop, pot, eng.pressure[:Rs], eng.mu[:Rs],
+                             eng.step_sizes[:Rs], eng.mwg.as_oracle_moves(), T,
+                             eng.run.n_extra, tape, it=it, do_re=Rs > 1,
+                             n_threads=threads)
+        dt += time.perf_counter() - t0
+        flops += launch_flops(out["counters"][None], eng)
+        print(f"[bench] cpu baseline iteration {it + 1}/{n_iter}: {dt:.1f} s",
+              file=sys.stderr, flush=True)
+    ws = Rs * eng.n_walk * T * n_iter
     return {"value": ws / dt, "unit": "walker-steps/s", "cores": threads,
             "kind": "port",
-            "sample": (f"1 NS iteration of {Rs} of 16 replicas x {eng.n_walk} walked "
-                       f"chains x {T} sub-steps = {ws} walker-steps, {dt:.1f} s, "
-                       f"FP64 C oracle, OpenMP {threads} threads"),
-            "flops_per_s": launch_flops(out["counters"][None], eng) / dt}
+            "sample": (f"{n_iter} NS iteration(s) of {Rs} of {eng.n_replicas} replicas "
+                       f"x {eng.n_walk} walked chains x {T} sub-steps = {ws} "
+                       f"walker-steps in {dt:.1f} s; FP64 C oracle, OpenMP "
+                       f"{threads} threads"),
+            "flops_per_s": flops / dt}
 
 
 def main():
@@ -104,8 +113,10 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-replicas", type=int, default=2)
+    ap.add_argument("--cpu-replicas", type=int, default=16)
+    ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--skip-secondary", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); "
@@ -115,8 +126,10 @@ def main():
 
     eng = build_engine(n_extra=1022)
     R, n_walk, T = eng.n_replicas, eng.n_walk, eng.run.n_mcmc_steps
+    tune = eng.autotune(reps=2) if not args.no_tune else None
     print(f"[bench] engine ready: R={R} K={eng.n_walkers} N={eng.Nmax} "
-          f"n_walk={n_walk} launch={eng.launch_info()}", file=sys.stderr, flush=True)
+          f"n_walk={n_walk} tune={tune} launch={eng.launch_info()}",
+          file=sys.stderr, flush=True)
     eng.ns_iterations(args.warmup)
     eng.walk_timing(reset=True)
     t0 = time.perf_counter()
@@ -139,15 +152,33 @@ def main():
         t1 = time.perf_counter()
         e0.ns_iterations(n0, graph=True)
         d0 = time.perf_counter() - t1
-        secondary = {"n_extra": 0, "walker_steps_per_s": R * 1 * T * n0 / d0,
-                     "ns_iterations_per_s": n0 / d0, "iterations": n0,
-                     "mode": "hipGraph replay of one iteration"}
+        secondary = {"n_extra_0": {
+            "walker_steps_per_s": R * 1 * T * n0 / d0,
+            "ns_iterations_per_s": n0 / d0, "iterations": n0,
+            "mode": "hipGraph replay of one iteration"}}
         e0.close()
+        e32 = build_engine(n_extra=1022, precision="fp32_pairs")
+        t32 = e32.autotune(reps=2) if not args.no_tune else None
+        e32.ns_iterations(2)
+        e32.walk_timing(reset=True)
+        n32 = max(5, args.steps // 2)
+        t2 = time.perf_counter()
+        o32 = e32.ns_iterations(n32)
+        d32 = time.perf_counter() - t2
+        ms32, l32 = e32.walk_timing(reset=True)
+        f32 = launch_flops(o32["counters"], e32) / l32 / (ms32 / l32 * 1e-3) / 1e12
+        secondary["fp32_pairs"] = {
+            "walker_steps_per_s": ws_per_iter * n32 / d32, "iterations": n32,
+            "walk_avg_ms": ms32 / l32, "achieved_tflops": f32,
+            "frac_of_fp32_peak": f32 / FP32_PEAK_TFLOPS,
+            "frac_of_fp64_peak": f32 / FP64_PEAK_TFLOPS, "launch": t32,
+            "note": "float32 pair terms, float64 minimum image and accumulation"}
+        e32.close()
 
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(eng, args.cpu_replicas, threads)
+        cpu = cpu_baseline(eng, args.cpu_replicas, threads, args.cpu_iters)
 
     line = {
         "metric": METRIC,
@@ -170,6 +201,7 @@ def main():
             "n_mcmc_steps": T, "n_extra": eng.run.n_extra,
             "walker_steps_per_iteration": ws_per_iter,
             "ns_iterations_per_s": args.steps / dt,
+            "launch": {**eng.launch_info(), **(tune or {})},
         },
         "roofline": {
             "bound": "valu",

@@ -191,11 +191,20 @@ int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
                    const double* uniforms, int32_t* accepted,
                    int32_t* reason);
 
+/* Autotuner (no reference counterpart; the reference leaves launch shapes to
+ * XLA): times `reps` walk launches per candidate (waves per block 4/2/1 x
+ * occupancy variant 0/3 for N <= 64) from one post-selection state and keeps
+ * the fastest.  Population, contour and draw counters are left unchanged. */
+int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
+                int32_t* best_occ, double* best_ms);
+/* Set the walk launch shape directly (waves per block 1..4, occupancy 0|3). */
+int jr_set_launch(jr_handle* h, int32_t waves_per_block, int32_t occupancy);
+
 /* Walk-kernel timing accumulated since the last reset (HIP events on the
  * handle's stream): total milliseconds and launch count. */
 int jr_walk_timing(jr_handle* h, double* total_ms, int64_t* launches,
                    int32_t reset);
-/* Number of FP64 pair evaluations the walk kernel performed since reset. */
+/* Launch geometry in use: CU count, persistent blocks, waves per block. */
 int jr_get_info(jr_handle* h, int32_t* n_cu, int32_t* blocks,
                 int32_t* waves_per_block);
 

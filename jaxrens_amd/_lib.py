@@ -131,6 +131,10 @@ SIGNATURES = [
     ("jr_replica_exchange", C.c_int, [C.c_void_p, _dp, _ip, _ip]),
     ("jr_trial_moves", C.c_int,
      [C.c_void_p, C.c_int32, _dp, _dp, C.c_int32, _ip, _dp, _dp, _ip, _ip]),
+    ("jr_autotune", C.c_int,
+     [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+      C.POINTER(C.c_double)]),
+    ("jr_set_launch", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     ("jr_walk_timing", C.c_int,
      [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]),
     ("jr_get_info", C.c_int,

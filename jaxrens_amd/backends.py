@@ -88,6 +88,7 @@ class LJBackend:
         pair_epsilon=None,
         pair_sigma=None,
         device: int = 0,
+        precision: str = "fp64",
     ):
         sc = tuple(int(x) for x in supercell_trafo)
         if min(sc) < 1:
@@ -100,6 +101,11 @@ class LJBackend:
         self.cutoff = cutoff
         self.r_cutoff = cutoff if cutoff is not None else 0.0
         self.device = int(device)
+        if precision not in ("fp64", "fp32_pairs"):
+            raise ValueError("precision must be 'fp64' or 'fp32_pairs'")
+        self.precision = precision
+        self._prec_code = (_lib.JR_PREC_FP64 if precision == "fp64"
+                           else _lib.JR_PREC_FP32_PAIRS)
         rc2 = math.inf if cutoff is None else float(cutoff) ** 2
 
         if pair_epsilon is not None or pair_sigma is not None:
@@ -187,7 +193,7 @@ class LJBackend:
         H = np.empty(B)
         F = np.empty((B, Nmax, 3)) if forces else None
         lib = _lib.load_library()
-        pot, keep = self.table.as_struct()
+        pot, keep = self.table.as_struct(self._prec_code)
         rc = lib.jr_eval_energy_forces(
             pot, self.device, B, Nmax, _lib.dptr(p), _lib.iptr(t),
             _lib.iptr(n_atoms), _lib.dptr(c), _lib.dptr(pr), _lib.dptr(m),

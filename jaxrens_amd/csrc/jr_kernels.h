@@ -58,9 +58,14 @@ __host__ __device__ inline size_t wave_lds_bytes(int Nmax, int n_moves) {
   return wave_lds_bytes_a(Nmax <= 64 ? 1 : (Nmax <= 128 ? 2 : 4));
 }
 
-template <int A, bool MULTI>
+// OCC > 0 asks the register allocator for at least OCC waves per SIMD.
+template <int A, int CFG, int OCC>
 __global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8)))
     walk_kernel(WalkArgs a) {
+  constexpr bool MULTI = (CFG & 1) != 0;
+  constexpr bool F32 = (CFG & 2) != 0;
+  (void)F32;
   extern __shared__ double lds_raw[];
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
@@ -154,19 +159,19 @@ __global__ void __launch_bounds__(256)
       int reason = 0, n_eval = 1, n_grad = 0;
       switch (mv.type) {
         case JR_MOVE_GMC:
-          reason = gmc_move<A, MULTI>(C, W, step, mv, xi) ? 0 : 1;
+          reason = gmc_move<A, CFG>(C, W, step, mv, xi) ? 0 : 1;
           n_eval = n_grad = mv.n_reflect;
           break;
         case JR_MOVE_VOLUME:
-          reason = volume_move<A, MULTI>(C, W, step, mv, xi[0], u[0]);
+          reason = volume_move<A, CFG>(C, W, step, mv, xi[0], u[0]);
           break;
         case JR_MOVE_HMC:
-          reason = hmc_move<A, MULTI>(C, W, step, mv, xi) ? 0 : 1;
+          reason = hmc_move<A, CFG>(C, W, step, mv, xi) ? 0 : 1;
           n_eval = 2 * mv.n_leapfrog + 1;
           n_grad = 2 * mv.n_leapfrog;
           break;
         case JR_MOVE_MORPH:
-          reason = morph_move<A, MULTI>(C, W, mv, u[0], u[1]) ? 0 : 1;
+          reason = morph_move<A, CFG>(C, W, mv, u[0], u[1]) ? 0 : 1;
           break;
         default:
           break;
@@ -237,8 +242,11 @@ struct EvalArgs {
   const double* sig2;
 };
 
-template <int A, bool MULTI>
+template <int A, int CFG>
 __global__ void __launch_bounds__(256) eval_kernel(EvalArgs a) {
+  constexpr bool MULTI = (CFG & 1) != 0;
+  constexpr bool F32 = (CFG & 2) != 0;
+  (void)F32;
   extern __shared__ double lds_raw[];
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
@@ -290,9 +298,9 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs a) {
   double fx[A], fy[A], fz[A];
   double U;
   if (a.F)
-    U = wave_eval<A, true, MULTI>(L, N, g, tab, x, y, z, t, fx, fy, fz);
+    U = wave_eval<A, true, CFG>(L, N, g, tab, x, y, z, t, fx, fy, fz);
   else
-    U = wave_eval<A, false, MULTI>(L, N, g, tab, x, y, z, t, fx, fy, fz);
+    U = wave_eval<A, false, CFG>(L, N, g, tab, x, y, z, t, fx, fy, fz);
   int counts[JR_MAX_SPECIES];
   species_counts<A>(N, a.S, t, counts);
   const double pr = a.pressure ? a.pressure[b] : 0.0;
@@ -514,18 +522,15 @@ struct ReArgs {
   int* o_att;
 };
 
-__device__ double walker_ensemble(const PopPtrs& p, size_t slot, int Nmax,
-                                  int S, double P, const double* mu) {
-  int counts[JR_MAX_SPECIES];
-#pragma unroll
-  for (int s = 0; s < JR_MAX_SPECIES; ++s) counts[s] = 0;
+// Species counts of one walker, counted by the whole block (N <= blockDim).
+__device__ __forceinline__ void block_counts(const PopPtrs& p, size_t slot,
+                                             int Nmax, int S, int* counts) {
+  const int tid = threadIdx.x;
   const int N = p.n_atoms[slot];
-  for (int i = 0; i < N; ++i) {
-    int ty = p.types[slot * Nmax + i];
+  const int ty = (tid < N) ? p.types[slot * Nmax + tid] : -1;
 #pragma unroll
-    for (int s = 0; s < JR_MAX_SPECIES; ++s) counts[s] += (ty == s) ? 1 : 0;
-  }
-  return ensemble_term(p.cell + slot * 9, P, mu, S, counts);
+  for (int s = 0; s < JR_MAX_SPECIES; ++s)
+    counts[s] = (s < S) ? __syncthreads_count(ty == s) : 0;
 }
 
 // sampling/moves/replica_exchange.py:318-436 with EnsembleSwap (:120-164).
@@ -568,25 +573,30 @@ __global__ void __launch_bounds__(256) re_kernel(ReArgs a) {
           if (wi > a.K - 1) wi = a.K - 1;
           int wj = (int)(u[1] * (double)a.K);
           if (wj > a.K - 1) wj = a.K - 1;
-          const size_t si = (size_t)i * a.K + wi, sj = (size_t)j * a.K + wj;
+          s_si = (int)((size_t)i * a.K + wi);
+          s_sj = (int)((size_t)j * a.K + wj);
+        }
+        __syncthreads();
+        const size_t si = (size_t)s_si, sj = (size_t)s_sj;
+        int ci[JR_MAX_SPECIES], cj[JR_MAX_SPECIES];
+        block_counts(a.pop, si, a.Nmax, a.S, ci);
+        block_counts(a.pop, sj, a.Nmax, a.S, cj);
+        if (tid == 0) {
           const double* mui = a.mu + (size_t)i * a.S;
           const double* muj = a.mu + (size_t)j * a.S;
-          double ui = a.pop.energy[sj] - walker_ensemble(a.pop, sj, a.Nmax,
-                                                         a.S, a.pressure[j],
-                                                         muj);
-          double into_i = ui + walker_ensemble(a.pop, sj, a.Nmax, a.S,
-                                               a.pressure[i], mui);
-          double uj = a.pop.energy[si] - walker_ensemble(a.pop, si, a.Nmax,
-                                                         a.S, a.pressure[i],
-                                                         mui);
-          double into_j = uj + walker_ensemble(a.pop, si, a.Nmax, a.S,
-                                               a.pressure[j], muj);
+          const double* celli = a.pop.cell + si * 9;
+          const double* cellj = a.pop.cell + sj * 9;
+          // rebase_ensemble_energy (backends/ensemble.py:62-87)
+          double ui = a.pop.energy[sj] -
+                      ensemble_term(cellj, a.pressure[j], muj, a.S, cj);
+          double into_i = ui + ensemble_term(cellj, a.pressure[i], mui, a.S, cj);
+          double uj = a.pop.energy[si] -
+                      ensemble_term(celli, a.pressure[i], mui, a.S, ci);
+          double into_j = uj + ensemble_term(celli, a.pressure[j], muj, a.S, ci);
           int ok = (into_i < a.emax[i]) && (into_j < a.emax[j]);
           att[i] += 1;
           acc[i] += ok;
           s_ok = ok;
-          s_si = (int)si;
-          s_sj = (int)sj;
           s_ei = into_i;
           s_ej = into_j;
         }

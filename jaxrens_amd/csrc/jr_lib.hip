@@ -56,7 +56,8 @@ struct jr_handle {
   int n_walk = 1;
   int max_pairs = 0;
   int A = 1;
-  bool multi = false;
+  int cfgbits = 0;
+  int occ = 0;  // walk-kernel occupancy variant (0 or 3)
   hipStream_t stream = nullptr;
   // population
   DBuf<double> pos, dir, cell, energy;
@@ -128,53 +129,84 @@ namespace {
 
 int atoms_per_lane(int nmax) { return nmax <= 64 ? 1 : (nmax <= 128 ? 2 : 4); }
 
-template <int A, bool M>
+template <int A, int CFG, int OCC>
 void launch_walk_t(const WalkArgs& a, int blocks, int wpb, size_t lds,
                    hipStream_t s) {
-  hipLaunchKernelGGL((walk_kernel<A, M>), dim3(blocks), dim3(64 * wpb), lds,
-                     s, a);
+  hipLaunchKernelGGL((walk_kernel<A, CFG, OCC>), dim3(blocks), dim3(64 * wpb),
+                     lds, s, a);
 }
 
-void launch_walk(int A, bool multi, const WalkArgs& a, int blocks, int wpb,
-                 size_t lds, hipStream_t s) {
+// occupancy variants are built for one atom per lane (N <= 64) only
+template <int CFG>
+void launch_walk_c(int A, int occ, const WalkArgs& a, int blocks, int wpb,
+                   size_t lds, hipStream_t s) {
   if (A == 1) {
-    if (multi) launch_walk_t<1, true>(a, blocks, wpb, lds, s);
-    else launch_walk_t<1, false>(a, blocks, wpb, lds, s);
+    if (occ == 3) launch_walk_t<1, CFG, 3>(a, blocks, wpb, lds, s);
+    else launch_walk_t<1, CFG, 0>(a, blocks, wpb, lds, s);
   } else if (A == 2) {
-    if (multi) launch_walk_t<2, true>(a, blocks, wpb, lds, s);
-    else launch_walk_t<2, false>(a, blocks, wpb, lds, s);
+    launch_walk_t<2, CFG, 0>(a, blocks, wpb, lds, s);
   } else {
-    if (multi) launch_walk_t<4, true>(a, blocks, wpb, lds, s);
-    else launch_walk_t<4, false>(a, blocks, wpb, lds, s);
+    launch_walk_t<4, CFG, 0>(a, blocks, wpb, lds, s);
   }
 }
 
-const void* walk_fn(int A, bool multi) {
-  if (A == 1) return multi ? (const void*)walk_kernel<1, true> : (const void*)walk_kernel<1, false>;
-  if (A == 2) return multi ? (const void*)walk_kernel<2, true> : (const void*)walk_kernel<2, false>;
-  return multi ? (const void*)walk_kernel<4, true> : (const void*)walk_kernel<4, false>;
+// cfg bit 0: species table (S > 1); bit 1: FP32 pair terms
+void launch_walk(int A, int cfg, int occ, const WalkArgs& a, int blocks,
+                 int wpb, size_t lds, hipStream_t s) {
+  switch (cfg) {
+    case 0: launch_walk_c<0>(A, occ, a, blocks, wpb, lds, s); break;
+    case 1: launch_walk_c<1>(A, occ, a, blocks, wpb, lds, s); break;
+    case 2: launch_walk_c<2>(A, occ, a, blocks, wpb, lds, s); break;
+    default: launch_walk_c<3>(A, occ, a, blocks, wpb, lds, s); break;
+  }
 }
 
-template <int A, bool M>
+template <int CFG>
+const void* walk_fn_c(int A, int occ) {
+  if (A == 1)
+    return occ == 3 ? (const void*)walk_kernel<1, CFG, 3>
+                    : (const void*)walk_kernel<1, CFG, 0>;
+  if (A == 2) return (const void*)walk_kernel<2, CFG, 0>;
+  return (const void*)walk_kernel<4, CFG, 0>;
+}
+
+const void* walk_fn(int A, int cfg, int occ) {
+  switch (cfg) {
+    case 0: return walk_fn_c<0>(A, occ);
+    case 1: return walk_fn_c<1>(A, occ);
+    case 2: return walk_fn_c<2>(A, occ);
+    default: return walk_fn_c<3>(A, occ);
+  }
+}
+
+template <int A, int CFG>
 void launch_eval_t(const EvalArgs& a, hipStream_t s) {
   const int wpb = 4;
   size_t lds = wave_lds_bytes(a.Nmax, 0) * wpb + 3 * sizeof(double) * a.S * a.S;
   int blocks = (a.B + wpb - 1) / wpb;
-  hipLaunchKernelGGL((eval_kernel<A, M>), dim3(blocks), dim3(64 * wpb), lds, s,
+  hipLaunchKernelGGL((eval_kernel<A, CFG>), dim3(blocks), dim3(64 * wpb), lds, s,
                      a);
 }
 
-void launch_eval(int A, bool multi, const EvalArgs& a, hipStream_t s) {
-  if (A == 1) {
-    if (multi) launch_eval_t<1, true>(a, s);
-    else launch_eval_t<1, false>(a, s);
-  } else if (A == 2) {
-    if (multi) launch_eval_t<2, true>(a, s);
-    else launch_eval_t<2, false>(a, s);
-  } else {
-    if (multi) launch_eval_t<4, true>(a, s);
-    else launch_eval_t<4, false>(a, s);
+template <int CFG>
+void launch_eval_c(int A, const EvalArgs& a, hipStream_t s) {
+  if (A == 1) launch_eval_t<1, CFG>(a, s);
+  else if (A == 2) launch_eval_t<2, CFG>(a, s);
+  else launch_eval_t<4, CFG>(a, s);
+}
+
+void launch_eval(int A, int cfg, const EvalArgs& a, hipStream_t s) {
+  switch (cfg) {
+    case 0: launch_eval_c<0>(A, a, s); break;
+    case 1: launch_eval_c<1>(A, a, s); break;
+    case 2: launch_eval_c<2>(A, a, s); break;
+    default: launch_eval_c<3>(A, a, s); break;
   }
+}
+
+int pot_cfg(const jr_pot* pot) {
+  return (pot->n_species > 1 ? 1 : 0) |
+         (pot->precision == JR_PREC_FP32_PAIRS ? 2 : 0);
 }
 
 int validate_pot(const jr_pot* pot) {
@@ -184,9 +216,8 @@ int validate_pot(const jr_pot* pot) {
                           JR_MAX_SPECIES, pot->n_species);
   if (!pot->eps4 || !pot->eps24 || !pot->sig2)
     return set_global_err(JR_E_ARG, "pair tables must not be NULL");
-  if (pot->precision != JR_PREC_FP64)
-    return set_global_err(JR_E_UNSUPPORTED, "precision %d not built",
-                          pot->precision);
+  if (pot->precision != JR_PREC_FP64 && pot->precision != JR_PREC_FP32_PAIRS)
+    return set_global_err(JR_E_ARG, "unknown precision %d", pot->precision);
   return JR_OK;
 }
 
@@ -275,7 +306,7 @@ int jr_eval_energy_forces(const jr_pot* pot, int32_t device, int32_t B,
   a.eps4 = d_e4.p;
   a.eps24 = d_e24.p;
   a.sig2 = d_s2.p;
-  launch_eval(atoms_per_lane(Nmax), S > 1, a, s);
+  launch_eval(atoms_per_lane(Nmax), pot_cfg(pot), a, s);
   HIPCHK_G(hipGetLastError());
   if (U) HIPCHK_G(hipMemcpyAsync(U, d_U.p, sizeof(double) * B, hipMemcpyDeviceToHost, s));
   if (H) HIPCHK_G(hipMemcpyAsync(H, d_H.p, sizeof(double) * B, hipMemcpyDeviceToHost, s));
@@ -380,7 +411,7 @@ int recompute_energies(jr_handle* h) {
   a.eps4 = h->eps4.p;
   a.eps24 = h->eps24.p;
   a.sig2 = h->sig2.p;
-  launch_eval(h->A, h->multi, a, h->stream);
+  launch_eval(h->A, h->cfgbits, a, h->stream);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return JR_OK;
@@ -524,7 +555,7 @@ void enqueue_iteration_core(jr_handle* h, bool timed, int ev_idx) {
   hipLaunchKernelGGL(select_kernel, dim3(c.n_replicas), dim3(1024), sel_lds, h->stream, sa);
   WalkArgs wa = walk_args(h);
   if (timed) (void)hipEventRecord(h->ev[2 * ev_idx], h->stream);
-  launch_walk(h->A, h->multi, wa, h->blocks, h->wpb, h->lds_walk, h->stream);
+  launch_walk(h->A, h->cfgbits, h->occ, wa, h->blocks, h->wpb, h->lds_walk, h->stream);
   if (timed) (void)hipEventRecord(h->ev[2 * ev_idx + 1], h->stream);
   if (c.n_replicas > 1) {
     ReArgs ra = re_args(h, h->st.p);
@@ -549,6 +580,38 @@ static int snapshot_iter(jr_handle* h, int it) {
                            nw * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
   return JR_OK;
 }
+
+namespace {
+
+// Persistent walk grid: waves per block, occupancy variant, and as many
+// blocks as are resident (never more than the chains of one iteration).
+int set_geometry(jr_handle* h, int wpb, int occ) {
+  const auto& c = h->cfg;
+  if (h->A != 1) occ = 0;
+  h->wpb = wpb;
+  h->occ = occ;
+  h->lds_walk = wave_lds_bytes(c.n_atoms_max, c.n_moves) * wpb +
+                3 * sizeof(double) * c.pot.n_species * c.pot.n_species;
+  int per_cu = 0;
+  HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, walk_fn(h->A, h->cfgbits, h->occ), 64 * wpb,
+                h->lds_walk));
+  if (per_cu < 1) per_cu = 1;
+  const int total = c.n_replicas * h->n_walk;
+  const int need = (total + wpb - 1) / wpb;
+  h->blocks = std::max(1, std::min(need, per_cu * h->n_cu));
+  if (h->graph_exec) {
+    (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+  }
+  if (h->graph) {
+    (void)hipGraphDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return JR_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -594,7 +657,7 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   h->n_walk = 1 + c.n_extra;
   h->max_pairs = c.n_replicas / 2;
   h->A = atoms_per_lane(c.n_atoms_max);
-  h->multi = c.pot.n_species > 1;
+  h->cfgbits = pot_cfg(&c.pot);
   const int R = c.n_replicas, K = c.n_walkers, Nm = c.n_atoms_max, S = c.pot.n_species;
   auto fail = [&](int code) {
     g_err = h->err;
@@ -680,15 +743,10 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   hipDeviceProp_t prop;
   TRY(hipGetDeviceProperties(&prop, dev));
   h->n_cu = prop.multiProcessorCount;
-  h->wpb = c.waves_per_block > 0 ? std::min(c.waves_per_block, 4) : 4;
-  h->lds_walk = wave_lds_bytes(Nm, c.n_moves) * h->wpb + 3 * sizeof(double) * S * S;
-  int occ = 0;
-  TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, walk_fn(h->A, h->multi),
-                                                    64 * h->wpb, h->lds_walk));
-  if (occ < 1) occ = 1;
-  const int total = R * h->n_walk;
-  const int need = (total + h->wpb - 1) / h->wpb;
-  h->blocks = std::max(1, std::min(need, occ * h->n_cu));
+  {
+    int g = set_geometry(h, c.waves_per_block > 0 ? std::min(c.waves_per_block, 4) : 4, 0);
+    if (g) return fail(g);
+  }
   int ret = ensure_out(h, 1);
   if (ret) return fail(ret);
   *out = h;
@@ -762,6 +820,12 @@ int jr_set_run_params(jr_handle* h, const double* pressure, const double* mu,
   const auto& c = h->cfg;
   const int R = c.n_replicas, S = c.pot.n_species;
   HIPCHK(h, hipSetDevice(c.device));
+  // stored energies carry P V - mu N: re-evaluate only when those change
+  bool ens_changed = false;
+  if (pressure && !std::equal(pressure, pressure + R, h->h_pressure.begin()))
+    ens_changed = true;
+  if (mu && !std::equal(mu, mu + (size_t)R * S, h->h_mu.begin()))
+    ens_changed = true;
   if (pressure) h->h_pressure.assign(pressure, pressure + R);
   if (mu) h->h_mu.assign(mu, mu + (size_t)R * S);
   if (step_sizes) h->h_step.assign(step_sizes, step_sizes + (size_t)R * c.n_moves);
@@ -769,7 +833,7 @@ int jr_set_run_params(jr_handle* h, const double* pressure, const double* mu,
   HIPCHK(h, hipMemcpyAsync(h->mu.p, h->h_mu.data(), sizeof(double) * R * S, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->step_sizes.p, h->h_step.data(), sizeof(double) * R * c.n_moves, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  if (h->have_pop && (pressure || mu)) return recompute_energies(h);
+  if (h->have_pop && ens_changed) return recompute_energies(h);
   return JR_OK;
 }
 
@@ -1049,7 +1113,7 @@ int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
   a.trial_reason = h->t_reason.p;
   a.counters = nullptr;
   int blocks = std::max(1, std::min((int)((T + h->wpb - 1) / h->wpb), h->blocks));
-  launch_walk(h->A, h->multi, a, blocks, h->wpb, h->lds_walk, s);
+  launch_walk(h->A, h->cfgbits, h->occ, a, blocks, h->wpb, h->lds_walk, s);
   HIPCHK(h, hipGetLastError());
   std::vector<int> rs(T);
   HIPCHK(h, hipMemcpyAsync(rs.data(), h->t_reason.p, sizeof(int) * T, hipMemcpyDeviceToHost, s));
@@ -1059,6 +1123,104 @@ int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
     if (reason) reason[q] = rs[q];
   }
   return JR_OK;
+}
+
+int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
+                int32_t* best_occ, double* best_ms) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  if (!h->have_pop) return h->fail(JR_E_STATE, "population not set");
+  if (reps < 1) return h->fail(JR_E_ARG, "reps must be >= 1");
+  const auto& c = h->cfg;
+  HIPCHK(h, hipSetDevice(c.device));
+  const size_t W = (size_t)c.n_replicas * c.n_walkers, Nm = c.n_atoms_max;
+  hipStream_t s = h->stream;
+  // save the population and contour; tuning leaves both unchanged
+  DBuf<double> s_pos, s_dir, s_cell, s_e, s_emax, t_pos, t_dir, t_cell, t_e;
+  DBuf<int> s_t, s_n, t_t, t_n;
+  HIPCHK(h, s_pos.alloc(W * Nm * 3)); HIPCHK(h, s_dir.alloc(W * Nm * 3));
+  HIPCHK(h, s_cell.alloc(W * 9)); HIPCHK(h, s_e.alloc(W)); HIPCHK(h, s_emax.alloc(c.n_replicas));
+  HIPCHK(h, s_t.alloc(W * Nm)); HIPCHK(h, s_n.alloc(W));
+  HIPCHK(h, t_pos.alloc(W * Nm * 3)); HIPCHK(h, t_dir.alloc(W * Nm * 3));
+  HIPCHK(h, t_cell.alloc(W * 9)); HIPCHK(h, t_e.alloc(W));
+  HIPCHK(h, t_t.alloc(W * Nm)); HIPCHK(h, t_n.alloc(W));
+  auto copy_pop = [&](double* dp, double* dd, double* dc, double* de, int* dt, int* dn,
+                      const double* sp, const double* sd, const double* sc,
+                      const double* se, const int* st, const int* sn) -> hipError_t {
+    hipError_t e;
+    if ((e = hipMemcpyAsync(dp, sp, W * Nm * 3 * 8, hipMemcpyDeviceToDevice, s))) return e;
+    if ((e = hipMemcpyAsync(dd, sd, W * Nm * 3 * 8, hipMemcpyDeviceToDevice, s))) return e;
+    if ((e = hipMemcpyAsync(dc, sc, W * 9 * 8, hipMemcpyDeviceToDevice, s))) return e;
+    if ((e = hipMemcpyAsync(de, se, W * 8, hipMemcpyDeviceToDevice, s))) return e;
+    if ((e = hipMemcpyAsync(dt, st, W * Nm * 4, hipMemcpyDeviceToDevice, s))) return e;
+    return hipMemcpyAsync(dn, sn, W * 4, hipMemcpyDeviceToDevice, s);
+  };
+  HIPCHK(h, copy_pop(s_pos.p, s_dir.p, s_cell.p, s_e.p, s_t.p, s_n.p, h->pos.p, h->dir.p,
+                     h->cell.p, h->energy.p, h->types.p, h->n_atoms.p));
+  HIPCHK(h, hipMemcpyAsync(s_emax.p, h->emax.p, c.n_replicas * 8, hipMemcpyDeviceToDevice, s));
+  // one iteration's draws and selection, then the post-selection state
+  int rc = begin_block(h, 1);
+  if (rc) return rc;
+  enqueue_draws(h);
+  SelArgs sa = sel_args(h);
+  size_t sel_lds = sa.Kpad ? (size_t)sa.Kpad * (sizeof(double) + sizeof(int)) : 0;
+  hipLaunchKernelGGL(select_kernel, dim3(c.n_replicas), dim3(1024), sel_lds, s, sa);
+  HIPCHK(h, copy_pop(t_pos.p, t_dir.p, t_cell.p, t_e.p, t_t.p, t_n.p, h->pos.p, h->dir.p,
+                     h->cell.p, h->energy.p, h->types.p, h->n_atoms.p));
+  hipEvent_t e0, e1;
+  HIPCHK(h, hipEventCreate(&e0));
+  HIPCHK(h, hipEventCreate(&e1));
+  const int wpbs[3] = {4, 2, 1};
+  const int occs[2] = {0, 3};
+  int bw = h->wpb, bo = h->occ;
+  double bms = 1e300;
+  for (int oi = 0; oi < (h->A == 1 ? 2 : 1); ++oi) {
+    for (int wi = 0; wi < 3; ++wi) {
+      rc = set_geometry(h, wpbs[wi], occs[oi]);
+      if (rc) return rc;
+      double tot = 0.0;
+      for (int r = 0; r <= reps; ++r) {  // r == 0: warm-up launch
+        HIPCHK(h, copy_pop(h->pos.p, h->dir.p, h->cell.p, h->energy.p, h->types.p,
+                           h->n_atoms.p, t_pos.p, t_dir.p, t_cell.p, t_e.p, t_t.p, t_n.p));
+        HIPCHK(h, hipMemsetAsync(h->queue.p, 0, sizeof(int), s));
+        WalkArgs wa = walk_args(h);
+        HIPCHK(h, hipEventRecord(e0, s));
+        launch_walk(h->A, h->cfgbits, h->occ, wa, h->blocks, h->wpb, h->lds_walk, s);
+        HIPCHK(h, hipEventRecord(e1, s));
+        HIPCHK(h, hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+        if (r > 0) tot += ms;
+      }
+      const double avg = tot / reps;
+      if (avg < bms) {
+        bms = avg;
+        bw = h->wpb;
+        bo = h->occ;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  rc = set_geometry(h, bw, bo);
+  if (rc) return rc;
+  HIPCHK(h, copy_pop(h->pos.p, h->dir.p, h->cell.p, h->energy.p, h->types.p, h->n_atoms.p,
+                     s_pos.p, s_dir.p, s_cell.p, s_e.p, s_t.p, s_n.p));
+  HIPCHK(h, hipMemcpyAsync(h->emax.p, s_emax.p, c.n_replicas * 8, hipMemcpyDeviceToDevice, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  if (best_wpb) *best_wpb = bw;
+  if (best_occ) *best_occ = bo;
+  if (best_ms) *best_ms = bms;
+  return JR_OK;
+}
+
+int jr_set_launch(jr_handle* h, int32_t waves_per_block, int32_t occupancy) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  if (waves_per_block < 1 || waves_per_block > 4)
+    return h->fail(JR_E_ARG, "waves_per_block must be in [1, 4]");
+  if (occupancy != 0 && occupancy != 3)
+    return h->fail(JR_E_ARG, "occupancy variant must be 0 or 3");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  return set_geometry(h, waves_per_block, occupancy);
 }
 
 int jr_walk_timing(jr_handle* h, double* total_ms, int64_t* launches,

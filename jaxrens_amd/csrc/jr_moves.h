@@ -30,9 +30,12 @@ __device__ __forceinline__ double tape_xi(const double* xi, int Nmax, int c,
 }
 
 // sampling/moves/galilean.py:142-300 (use_forces=True, unscoped).
-template <int A, bool MULTI>
+template <int A, int CFG>
 __device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
                          const DevMove& mv, const double* xi) {
+  constexpr bool MULTI = (CFG & 1) != 0;
+  constexpr bool F32 = (CFG & 2) != 0;
+  (void)F32;
   const int lane = lane_id();
   const int N = C.N;
   double dn = sqrt(wave_dot<A>(N, W.dx, W.dy, W.dz, W.dx, W.dy, W.dz));
@@ -84,7 +87,7 @@ __device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
       W.z[s] = W.z[s] + step * W.dz[s];
     }
     double fx[A], fy[A], fz[A];
-    double U = wave_eval<A, true, MULTI>(C.L, N, g, C.tab, W.x, W.y, W.z, W.t, fx, fy, fz);
+    double U = wave_eval<A, true, CFG>(C.L, N, g, C.tab, W.x, W.y, W.z, W.t, fx, fy, fz);
     double Hn = U + ens;
     if (__builtin_isnan(Hn)) Hn = __builtin_inf();
     double fsq = wave_dot<A>(N, fx, fy, fz, fx, fy, fz);
@@ -119,9 +122,12 @@ __device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
 }
 
 // sampling/moves/volume.py:49-109 + _common.py:16-93; returns reject reason.
-template <int A, bool MULTI>
+template <int A, int CFG>
 __device__ int volume_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
                            const DevMove& mv, double xi0, double u0) {
+  constexpr bool MULTI = (CFG & 1) != 0;
+  constexpr bool F32 = (CFG & 2) != 0;
+  (void)F32;
   const int lane = lane_id();
   const int N = C.N;
   double oldV = fabs(det3(W.cell));
@@ -142,7 +148,7 @@ __device__ int volume_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
   Geom g;
   make_geom(nc, g);
   double fx[A], fy[A], fz[A];
-  double U = wave_eval<A, false, MULTI>(C.L, N, g, C.tab, nx, ny, nz, W.t, fx, fy, fz);
+  double U = wave_eval<A, false, CFG>(C.L, N, g, C.tab, nx, ny, nz, W.t, fx, fy, fz);
   double Hn = U + ensemble_term(nc, C.pressure, C.mu, C.S, W.counts);
   bool cell_ok =
       check_cell_shape(nc, N, mv.max_vpa, mv.min_vpa, mv.min_aspect);
@@ -166,9 +172,12 @@ __device__ int volume_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
 
 // sampling/moves/hmc.py:51-152; forces at a position are evaluated once and
 // reused by the following half kick (identical values to re-evaluating).
-template <int A, bool MULTI>
+template <int A, int CFG>
 __device__ bool hmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
                          const DevMove& mv, const double* xi) {
+  constexpr bool MULTI = (CFG & 1) != 0;
+  constexpr bool F32 = (CFG & 2) != 0;
+  (void)F32;
   const int lane = lane_id();
   const int N = C.N;
   double px[A], py[A], pz[A], qx[A], qy[A], qz[A];
@@ -191,7 +200,7 @@ __device__ bool hmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
   const double ens = ensemble_term(W.cell, C.pressure, C.mu, C.S, W.counts);
   const double h = 0.5 * step;
   double fx[A], fy[A], fz[A];
-  double U = wave_eval<A, true, MULTI>(C.L, N, g, C.tab, qx, qy, qz, W.t, fx,
+  double U = wave_eval<A, true, CFG>(C.L, N, g, C.tab, qx, qy, qz, W.t, fx,
                                        fy, fz);
   for (int l = 0; l < mv.n_leapfrog; ++l) {
 #pragma unroll
@@ -203,7 +212,7 @@ __device__ bool hmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
       qy[s] = qy[s] + step * py[s];
       qz[s] = qz[s] + step * pz[s];
     }
-    U = wave_eval<A, true, MULTI>(C.L, N, g, C.tab, qx, qy, qz, W.t, fx, fy,
+    U = wave_eval<A, true, CFG>(C.L, N, g, C.tab, qx, qy, qz, W.t, fx, fy,
                                   fz);
 #pragma unroll
     for (int s = 0; s < A; ++s) {
@@ -229,9 +238,12 @@ __device__ bool hmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
 }
 
 // sampling/moves/alchemical.py:40-79 (build_morph_kernel).
-template <int A, bool MULTI>
+template <int A, int CFG>
 __device__ bool morph_move(const WalkCtx& C, WalkerRegs<A>& W,
                            const DevMove& mv, double u0, double u1) {
+  constexpr bool MULTI = (CFG & 1) != 0;
+  constexpr bool F32 = (CFG & 2) != 0;
+  (void)F32;
   const int lane = lane_id();
   const int N = C.N;
   const int S = mv.n_species;
@@ -255,7 +267,7 @@ __device__ bool morph_move(const WalkCtx& C, WalkerRegs<A>& W,
   Geom g;
   make_geom(W.cell, g);
   double fx[A], fy[A], fz[A];
-  double U = wave_eval<A, false, MULTI>(C.L, N, g, C.tab, W.x, W.y, W.z, W.t, fx, fy, fz);
+  double U = wave_eval<A, false, CFG>(C.L, N, g, C.tab, W.x, W.y, W.z, W.t, fx, fy, fz);
   double Hn = U + ensemble_term(W.cell, C.pressure, C.mu, C.S, W.counts);
   bool accepted = Hn < C.emax;
   if (accepted) {

@@ -2,6 +2,8 @@
 // alchemical morph) for one walker held by one wavefront.
 #pragma once
 
+#include <type_traits>
+
 #include "jr_device.h"
 #include "../../include/jaxrens_amd.h"
 
@@ -129,13 +131,29 @@ __device__ __forceinline__ void pair_lj(double q, double eps4, double eps24,
   }
 }
 
+// Same terms in float (the FP32-pair variant); accumulation stays FP64.
+template <bool FORCES, bool MULTI>
+__device__ __forceinline__ void pair_lj_f32(float q, float eps4, float eps24,
+                                            float sig2, float& e, float& fs) {
+  float s2 = MULTI ? sig2 * q : q;
+  float s6 = (s2 * s2) * s2;
+  float s12 = s6 * s6;
+  if (MULTI) {
+    e = eps4 * (s12 - s6);
+    if (FORCES) fs = (eps24 * fmaf(2.0f, s12, -s6)) * q;
+  } else {
+    e = s12 - s6;
+    if (FORCES) fs = fmaf(2.0f, s12, -s6) * q;
+  }
+}
+
 struct PairIn {
   double dx, dy, dz, r2;
   bool ok;  // active and r2 < rc2
   double eps4, eps24, sig2;
 };
 
-template <int A, bool MULTI, bool DIAG, bool FULL>
+template <int A, int CFG, bool DIAG, bool FULL>
 __device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int N, int s,
                                             int k, const Geom& g,
                                             const PairTab& tab,
@@ -143,6 +161,9 @@ __device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int N, int s,
                                             const double (&y)[A],
                                             const double (&z)[A],
                                             const int (&t)[A]) {
+  constexpr bool MULTI = (CFG & 1) != 0;
+  constexpr bool F32 = (CFG & 2) != 0;
+  (void)F32;
   const int a = lane_id() + 64 * s;
   bool active = true;
   int j;
@@ -181,12 +202,15 @@ __device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int N, int s,
 // with num = sigma^2 (one species) or 1; a lone last step and the k = N/2
 // step use q = num / r2.
 // FULL: N == 64*A, so no ring position is idle and (p + k) mod N is a mask.
-template <int A, bool FORCES, bool MULTI, bool DIAG, bool FULL>
+template <int A, bool FORCES, int CFG, bool DIAG, bool FULL>
 __device__ __forceinline__ void wave_eval_impl(
     const LdsView<A>& L, int N, const Geom& g, const PairTab& tab,
     const double (&x)[A], const double (&y)[A], const double (&z)[A],
     const int (&t)[A], double& e_lane, double (&fx)[A], double (&fy)[A],
     double (&fz)[A]) {
+  constexpr bool MULTI = (CFG & 1) != 0;
+  constexpr bool F32 = (CFG & 2) != 0;
+  (void)F32;
   const int lane = lane_id();
   double e[A], ox[A], oy[A], oz[A], tx[A], ty[A], tz[A];
 #pragma unroll
@@ -207,10 +231,26 @@ __device__ __forceinline__ void wave_eval_impl(
       ring_rot1<A>(tz, lane, N);
     }
   };
-  auto accumulate = [&](int s, const PairIn& P, double q, bool reaction,
+  using QT = typename std::conditional<F32, float, double>::type;
+  // q = num / r^2 for one pair (F32: hardware reciprocal, 1 ulp)
+  auto q_single = [&](const PairIn& P) -> QT {
+    if constexpr (F32)
+      return P.ok ? (float)num * __builtin_amdgcn_rcpf((float)P.r2) : 0.0f;
+    else
+      return P.ok ? num / P.r2 : 0.0;
+  };
+  auto accumulate = [&](int s, const PairIn& P, QT q, bool reaction,
                         double escale) {
     double ep, fs = 0.0;
-    pair_lj<FORCES, MULTI>(q, P.eps4, P.eps24, P.sig2, ep, fs);
+    if constexpr (F32) {
+      float epf, fsf = 0.0f;
+      pair_lj_f32<FORCES, MULTI>(q, (float)P.eps4, (float)P.eps24,
+                                 (float)P.sig2, epf, fsf);
+      ep = (double)epf;
+      fs = (double)fsf;
+    } else {
+      pair_lj<FORCES, MULTI>(q, P.eps4, P.eps24, P.sig2, ep, fs);
+    }
     e[s] = e[s] + (escale == 1.0 ? ep : escale * ep);
     if (FORCES) {
       ox[s] = fma(-fs, P.dx, ox[s]);
@@ -227,15 +267,20 @@ __device__ __forceinline__ void wave_eval_impl(
   int k = 1;
   for (; k + 1 <= kmax; k += 2) {
     PairIn P0[A], P1[A];
-    double q0[A], q1[A];
+    QT q0[A], q1[A];
 #pragma unroll
     for (int s = 0; s < A; ++s) {
-      P0[s] = load_pair<A, MULTI, DIAG, FULL>(L, N, s, k, g, tab, x, y, z, t);
-      P1[s] = load_pair<A, MULTI, DIAG, FULL>(L, N, s, k + 1, g, tab, x, y, z,
+      P0[s] = load_pair<A, CFG, DIAG, FULL>(L, N, s, k, g, tab, x, y, z, t);
+      P1[s] = load_pair<A, CFG, DIAG, FULL>(L, N, s, k + 1, g, tab, x, y, z,
                                               t);
-      double d = num / (P0[s].r2 * P1[s].r2);
-      q0[s] = P0[s].ok ? P1[s].r2 * d : 0.0;
-      q1[s] = P1[s].ok ? P0[s].r2 * d : 0.0;
+      if constexpr (F32) {
+        q0[s] = q_single(P0[s]);
+        q1[s] = q_single(P1[s]);
+      } else {
+        double d = num / (P0[s].r2 * P1[s].r2);
+        q0[s] = P0[s].ok ? P1[s].r2 * d : 0.0;
+        q1[s] = P1[s].ok ? P0[s].r2 * d : 0.0;
+      }
     }
     if (FORCES && k > 1) rotate();
 #pragma unroll
@@ -248,18 +293,16 @@ __device__ __forceinline__ void wave_eval_impl(
     if (FORCES && k > 1) rotate();
 #pragma unroll
     for (int s = 0; s < A; ++s) {
-      PairIn P = load_pair<A, MULTI, DIAG, FULL>(L, N, s, k, g, tab, x, y, z, t);
-      double q = P.ok ? num / P.r2 : 0.0;
-      accumulate(s, P, q, true, 1.0);
+      PairIn P = load_pair<A, CFG, DIAG, FULL>(L, N, s, k, g, tab, x, y, z, t);
+      accumulate(s, P, q_single(P), true, 1.0);
     }
   }
   if (N >= 2 && !(N & 1)) {
     const int h = N >> 1;
 #pragma unroll
     for (int s = 0; s < A; ++s) {
-      PairIn P = load_pair<A, MULTI, DIAG, FULL>(L, N, s, h, g, tab, x, y, z, t);
-      double q = P.ok ? num / P.r2 : 0.0;
-      accumulate(s, P, q, false, 0.5);
+      PairIn P = load_pair<A, CFG, DIAG, FULL>(L, N, s, h, g, tab, x, y, z, t);
+      accumulate(s, P, q_single(P), false, 0.5);
     }
   }
   double el = e[0];
@@ -309,7 +352,7 @@ __device__ __forceinline__ void wave_eval_impl(
 
 // Model energy U (wave-uniform) and forces of the configuration in
 // registers; writes the configuration into the wave's LDS first.
-template <int A, bool FORCES, bool MULTI>
+template <int A, bool FORCES, int CFG>
 __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
                                             const Geom& g, const PairTab& tab,
                                             const double (&x)[A],
@@ -318,6 +361,9 @@ __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
                                             const int (&t)[A],
                                             double (&fx)[A], double (&fy)[A],
                                             double (&fz)[A]) {
+  constexpr bool MULTI = (CFG & 1) != 0;
+  constexpr bool F32 = (CFG & 2) != 0;
+  (void)F32;
   const int lane = lane_id();
   const LdsView<A> L{W.base};
 #pragma unroll
@@ -335,17 +381,17 @@ __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
   const bool full = (N == 64 * A);
   if (g.diag) {
     if (full)
-      wave_eval_impl<A, FORCES, MULTI, true, true>(L, N, g, tab, x, y, z, t,
+      wave_eval_impl<A, FORCES, CFG, true, true>(L, N, g, tab, x, y, z, t,
                                                    el, fx, fy, fz);
     else
-      wave_eval_impl<A, FORCES, MULTI, true, false>(L, N, g, tab, x, y, z, t,
+      wave_eval_impl<A, FORCES, CFG, true, false>(L, N, g, tab, x, y, z, t,
                                                     el, fx, fy, fz);
   } else {
     if (full)
-      wave_eval_impl<A, FORCES, MULTI, false, true>(L, N, g, tab, x, y, z, t,
+      wave_eval_impl<A, FORCES, CFG, false, true>(L, N, g, tab, x, y, z, t,
                                                     el, fx, fy, fz);
     else
-      wave_eval_impl<A, FORCES, MULTI, false, false>(L, N, g, tab, x, y, z, t,
+      wave_eval_impl<A, FORCES, CFG, false, false>(L, N, g, tab, x, y, z, t,
                                                      el, fx, fy, fz);
   }
   wave_lds_sync();

@@ -72,7 +72,7 @@ class NSEngine:
         self.n_replicas, self.n_walkers = R, K
         self.n_walk = run.n_cull + run.n_extra
         self.n_moves = self.mwg.n_moves
-        pot, self._pot_keep = self.model.table.as_struct()
+        pot, self._pot_keep = self.model.table.as_struct(self.model._prec_code)
         self._moves_arr = self.mwg.device_table()
         self._weights = _lib.f64(self.mwg.weights)
         cfg = _lib.JrConfig()
@@ -346,6 +346,17 @@ class NSEngine:
             self.h, int(move_idx), _lib.dptr(ts), _lib.dptr(em), n, _lib.iptr(idx),
             _lib.dptr(nn), _lib.dptr(uu), _lib.iptr(acc), _lib.iptr(rs)))
         return acc.astype(bool), rs
+
+    def autotune(self, reps: int = 2) -> dict:
+        """Pick the walk launch shape by timing candidates on this workload."""
+        w, o, ms = C.c_int32(), C.c_int32(), C.c_double()
+        self._check(self.lib.jr_autotune(self.h, int(reps), C.byref(w), C.byref(o),
+                                         C.byref(ms)))
+        return {"waves_per_block": w.value, "occupancy": o.value,
+                "walk_ms": ms.value}
+
+    def set_launch(self, waves_per_block: int, occupancy: int = 0):
+        self._check(self.lib.jr_set_launch(self.h, int(waves_per_block), int(occupancy)))
 
     def walk_timing(self, reset: bool = False) -> tuple[float, int]:
         ms = C.c_double()

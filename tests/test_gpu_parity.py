@@ -251,3 +251,26 @@ def test_replica_exchange_standalone(oracle):
     assert np.array_equal(st["n_accepted_per_pair"], acc[:4])
     assert np.array_equal(st["n_attempted_per_pair"], att[:4])
     _compare_population(eng, pop)
+
+
+def test_autotune_keeps_state_and_results():
+    """Launch shape does not change results; tuning leaves the state alone."""
+    print("autotune", flush=True)
+    lj = LJBackend(1.0, 1.0, 2.5)
+    a = _engine(lj, GMC_VOL, 3, 32, 64, 30, 4, cell_spec=CELL, pressures=[0.1, 1.0, 5.0], seed=8)
+    b = _engine(lj, GMC_VOL, 3, 32, 64, 30, 4, cell_spec=CELL, pressures=[0.1, 1.0, 5.0], seed=8)
+    before = a.get_population()
+    t = a.autotune(reps=1)
+    assert t["waves_per_block"] in (1, 2, 4) and t["occupancy"] in (0, 3)
+    after = a.get_population()
+    for k in before:
+        assert np.array_equal(before[k], after[k]), k
+    assert np.array_equal(a.emax(), b.emax())
+    b.set_launch(1, 3)
+    oa = a.ns_iterations(3)
+    ob = b.ns_iterations(3)
+    for k in ("emax", "counters", "walk"):
+        assert np.array_equal(oa[k], ob[k]), k
+    pa, pb = a.get_population(), b.get_population()
+    for k in pa:
+        assert np.array_equal(pa[k], pb[k]), k
