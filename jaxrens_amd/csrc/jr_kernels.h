@@ -99,8 +99,13 @@ __global__ void __launch_bounds__(256)
   C.S = a.S;
 
   for (;;) {
-    int q = 0;
-    if (lane == 0) q = atomicAdd(a.queue, 1);
+    // Every lane issues the queue atomic (lane 0 adds 1, the others 0) and
+    // the first lane's result is the wave's work item.  No lane-dependent
+    // branch may guard the fetch: with `if (lane == 0) q = atomicAdd(..)`
+    // the compiler threads lanes 1..63 from an earlier `if (lane == 0)`
+    // straight back to the loop head with q = 0 and no atomic, and those
+    // lanes then repeat item 0 forever.
+    int q = atomicAdd(a.queue, lane == 0 ? 1 : 0);
     q = __builtin_amdgcn_readfirstlane(q);
     if (q >= a.total) break;
     int r, slot, tq;

@@ -184,32 +184,40 @@ static inline void pair_lj(const pair_t* p, double q, int multi, double* e,
   }
 }
 
-static inline void acc_pair(int a, int j, const pair_t* p, double q, int multi,
-                            int reaction, double escale, double* e,
-                            double* own, double* trav) {
+static inline void acc_pair(int a, const pair_t* p, double q, int multi,
+                            double escale, double* e, double* own,
+                            double* react, int jr) {
   double ep, fs;
   pair_lj(p, q, multi, &ep, &fs);
   e[a] = e[a] + (escale == 1.0 ? ep : escale * ep);
   own[3 * a + 0] = fma(-fs, p->dx, own[3 * a + 0]);
   own[3 * a + 1] = fma(-fs, p->dy, own[3 * a + 1]);
   own[3 * a + 2] = fma(-fs, p->dz, own[3 * a + 2]);
-  if (reaction) {
-    trav[3 * j + 0] = fma(fs, p->dx, trav[3 * j + 0]);
-    trav[3 * j + 1] = fma(fs, p->dy, trav[3 * j + 1]);
-    trav[3 * j + 2] = fma(fs, p->dz, trav[3 * j + 2]);
+  if (react != NULL) {
+    react[3 * jr + 0] = react[3 * jr + 0] + fs * p->dx;
+    react[3 * jr + 1] = react[3 * jr + 1] + fs * p->dy;
+    react[3 * jr + 2] = react[3 * jr + 2] + fs * p->dz;
   }
 }
 
 /* Model energy U (and forces F = -dU/dx when F != NULL), then
  * H = U + ensemble term; backends/hamiltonian.py:112-153.
  * Pair schedule: for k = 1..(N-1)/2 atom a pairs with (a+k) mod N; the
- * force on a is accumulated in k order into own[a] and the reaction on
- * j = a+k into trav[j] (also in k order); for even N the k = N/2 pairs are
- * taken from both ends with half the energy and no reaction.  Steps k and
- * k+1 (k odd) share one division d = num/(r2_k r2_{k+1}), q_k = r2_{k+1} d,
- * q_{k+1} = r2_k d (num = sigma^2 for one species, else 1); a lone last step
- * and the k = N/2 step use q = num/r2.  One species: U = 4 eps * sum and
- * F = (24 eps / sigma^2) * (own + trav). */
+ * force on a is accumulated in k order into own[a] (fma(-fs, dr, own)) and
+ * the reaction t = fs*dr into entry a+k of a 2N-entry array R, atom j's
+ * reaction being R[j] + R[j+N]; for even N the k = N/2 pairs are taken from
+ * both ends with half the energy and no reaction.  Steps come in groups of
+ * G = 4 while they last, then one group of 2, then a lone step; a group
+ * shares one division:
+ *   G = 4: p01 = r2_0 r2_1, p23 = r2_2 r2_3, d = num/(p01 p23),
+ *          d01 = p23 d, d23 = p01 d, q0 = r2_1 d01, q1 = r2_0 d01,
+ *          q2 = r2_3 d23, q3 = r2_2 d23;
+ *   G = 2: d = num/(r2_0 r2_1), q0 = r2_1 d, q1 = r2_0 d;
+ *   G = 1 and k = N/2: q = num/r2
+ * (num = sigma^2 for one species, else 1).  Within a group the reactions
+ * are added slot by slot (atoms 64s..64s+63), steps ascending in a slot.
+ * One species: U = 4 eps * sum and F = (24 eps / sigma^2) *
+ * (own + (R[a] + R[a+N])). */
 void or_energy_forces(int N, const double* pos, const int32_t* types,
                       const double* cell_in, const or_pot* pot,
                       double pressure, const double* mu, double* U_out,
@@ -231,39 +239,46 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
   size_t nn = (size_t)(N > 0 ? N : 1);
   double* e = (double*)calloc(nn, sizeof(double));
   double* own = (double*)calloc(3 * nn, sizeof(double));
-  double* trav = (double*)calloc(3 * nn, sizeof(double));
-  pair_t* p0 = (pair_t*)malloc(nn * sizeof(pair_t));
-  pair_t* p1 = (pair_t*)malloc(nn * sizeof(pair_t));
-  double* q0 = (double*)malloc(nn * sizeof(double));
-  double* q1 = (double*)malloc(nn * sizeof(double));
+  double* react = (double*)calloc(6 * nn, sizeof(double)); /* [2N][3] */
   int kmax = (N - 1) / 2;
   int half = (N >= 2) && (N % 2 == 0);
   int k = 1;
-  for (; k + 1 <= kmax; k += 2) {
-    for (int a = 0; a < N; ++a) {
-      int j0 = (a + k) % N, j1 = (a + k + 1) % N;
-      p0[a] = pair_geom(pos + 3 * a, pos + 3 * j0, cell, inv, types[a],
-                        types[j0], pot);
-      p1[a] = pair_geom(pos + 3 * a, pos + 3 * j1, cell, inv, types[a],
-                        types[j1], pot);
-      double d = num / (p0[a].r2 * p1[a].r2);
-      q0[a] = p0[a].ok ? p1[a].r2 * d : 0.0;
-      q1[a] = p1[a].ok ? p0[a].r2 * d : 0.0;
+  while (k <= kmax) {
+    const int G = (k + 3 <= kmax) ? 4 : ((k + 1 <= kmax) ? 2 : 1);
+    for (int s0 = 0; s0 < N; s0 += WAVE) {
+      const int s1 = (s0 + WAVE < N) ? s0 + WAVE : N;
+      pair_t P[4][WAVE];
+      double q[4][WAVE];
+      for (int a = s0; a < s1; ++a) {
+        const int l = a - s0;
+        for (int i = 0; i < G; ++i) {
+          const int j = (a + k + i) % N;
+          P[i][l] = pair_geom(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
+                              types[j], pot);
+        }
+        if (G == 4) {
+          const double p01 = P[0][l].r2 * P[1][l].r2;
+          const double p23 = P[2][l].r2 * P[3][l].r2;
+          const double d = num / (p01 * p23);
+          const double d01 = p23 * d, d23 = p01 * d;
+          q[0][l] = P[0][l].ok ? P[1][l].r2 * d01 : 0.0;
+          q[1][l] = P[1][l].ok ? P[0][l].r2 * d01 : 0.0;
+          q[2][l] = P[2][l].ok ? P[3][l].r2 * d23 : 0.0;
+          q[3][l] = P[3][l].ok ? P[2][l].r2 * d23 : 0.0;
+        } else if (G == 2) {
+          const double d = num / (P[0][l].r2 * P[1][l].r2);
+          q[0][l] = P[0][l].ok ? P[1][l].r2 * d : 0.0;
+          q[1][l] = P[1][l].ok ? P[0][l].r2 * d : 0.0;
+        } else {
+          q[0][l] = P[0][l].ok ? num / P[0][l].r2 : 0.0;
+        }
+      }
+      for (int i = 0; i < G; ++i)
+        for (int a = s0; a < s1; ++a)
+          acc_pair(a, &P[i][a - s0], q[i][a - s0], multi, 1.0, e, own, react,
+                   a + k + i);
     }
-    for (int a = 0; a < N; ++a)
-      acc_pair(a, (a + k) % N, &p0[a], q0[a], multi, 1, 1.0, e, own, trav);
-    for (int a = 0; a < N; ++a)
-      acc_pair(a, (a + k + 1) % N, &p1[a], q1[a], multi, 1, 1.0, e, own,
-               trav);
-  }
-  if (k <= kmax) {
-    for (int a = 0; a < N; ++a) {
-      int j = (a + k) % N;
-      pair_t p = pair_geom(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
-                           types[j], pot);
-      double q = p.ok ? num / p.r2 : 0.0;
-      acc_pair(a, j, &p, q, multi, 1, 1.0, e, own, trav);
-    }
+    k += G;
   }
   if (half) {
     int h = N / 2;
@@ -272,25 +287,23 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
       pair_t p = pair_geom(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
                            types[j], pot);
       double q = p.ok ? num / p.r2 : 0.0;
-      acc_pair(a, j, &p, q, multi, 0, 0.5, e, own, trav);
+      acc_pair(a, &p, q, multi, 0.5, e, own, NULL, 0);
     }
   }
   double U = wave_sum_atoms(e, N);
   if (!multi) U = pot->eps4[0] * U;
   if (F != NULL) {
     double c24 = multi ? 1.0 : pot->eps24[0] / pot->sig2[0];
-    for (int a = 0; a < 3 * N; ++a) {
-      F[a] = own[a] + trav[a];
-      if (!multi) F[a] = c24 * F[a];
-    }
+    for (int a = 0; a < N; ++a)
+      for (int c = 0; c < 3; ++c) {
+        double f = own[3 * a + c] +
+                   (react[3 * a + c] + react[3 * (a + N) + c]);
+        F[3 * a + c] = multi ? f : c24 * f;
+      }
   }
   free(e);
   free(own);
-  free(trav);
-  free(p0);
-  free(p1);
-  free(q0);
-  free(q1);
+  free(react);
   *U_out = U;
   if (H_out != NULL)
     *H_out = U + or_ensemble_energy(N, types, cell_in, pressure, mu,
