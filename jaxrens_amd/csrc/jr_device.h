@@ -21,17 +21,35 @@ __device__ __forceinline__ double dot3(double a0, double a1, double a2,
   return fma(a2, b2, fma(a1, b1, a0 * b0));
 }
 
-// Butterfly sum over the wave; every lane ends with the same value.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = 1; m < JR_WAVE; m <<= 1) v = v + __shfl_xor(v, m, JR_WAVE);
-  return v;
-}
-
 __device__ __forceinline__ double readlane_d(double v, int l) {
   int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
   int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
   return __hiloint2double(hi, lo);
+}
+
+// DPP move of both 32-bit halves (CTRL: quad_perm, row_mirror, ...).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Butterfly sum over the wave, v[l] += v[l ^ m] for m = 1, 2, .., 32; every
+// lane ends with the value of lane 0.  IEEE addition is commutative, so after
+// level m the value is constant on aligned blocks of 2m lanes and any lane of
+// the partner block can stand in for lane l ^ m: quad_perm for m = 1, 2,
+// row_half_mirror (7 - i within 8) for m = 4, row_mirror (15 - i within 16)
+// for m = 8, and the four row sums r0..r3 read out for m = 16, 32, giving
+// (r0 + r1) + (r2 + r3).  Needs the whole wave active.
+__device__ __forceinline__ double wave_sum(double v) {
+  v = v + dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = v + dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = v + dpp_d<0x141>(v);  // row_half_mirror
+  v = v + dpp_d<0x140>(v);  // row_mirror
+  const double r0 = readlane_d(v, 0), r1 = readlane_d(v, 16);
+  const double r2 = readlane_d(v, 32), r3 = readlane_d(v, 48);
+  return (r0 + r1) + (r2 + r3);
 }
 
 __device__ __forceinline__ double uniform_d(double v) {

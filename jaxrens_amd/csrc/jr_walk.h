@@ -29,8 +29,8 @@ struct WaveLds {
 // Per-wave LDS view, M = 64*A ring positions.  Positions (and types) are
 // stored twice, entries j and j + N both holding atom j, so the partner of
 // ring position p at step k is entry p + k with no wrap and every access is a
-// base register plus an immediate offset.  Reaction forces are added into
-// 2M-entry arrays the same way; atom j's reaction is entry j + entry j + N.
+// base register plus an immediate offset.  hx/hy/hz return the travelling
+// reaction forces home once per evaluation.
 template <int A>
 struct LdsView {
   static constexpr int M = 64 * A;
@@ -38,33 +38,24 @@ struct LdsView {
   __device__ __forceinline__ double* x() const { return b; }
   __device__ __forceinline__ double* y() const { return b + 2 * M; }
   __device__ __forceinline__ double* z() const { return b + 4 * M; }
-  __device__ __forceinline__ double* rx() const { return b + 6 * M; }
-  __device__ __forceinline__ double* ry() const { return b + 8 * M; }
-  __device__ __forceinline__ double* rz() const { return b + 10 * M; }
+  __device__ __forceinline__ double* hx() const { return b + 6 * M; }
+  __device__ __forceinline__ double* hy() const { return b + 7 * M; }
+  __device__ __forceinline__ double* hz() const { return b + 8 * M; }
   __device__ __forceinline__ int* t() const {
-    return reinterpret_cast<int*>(b + 12 * M);
+    return reinterpret_cast<int*>(b + 9 * M);
   }
 };
 
 // bytes of one wave's LDS region (arrays + JR_MAX_MOVES*JR_NCOUNT counters)
 __host__ __device__ inline size_t wave_lds_bytes_a(int A) {
-  size_t b = (size_t)12 * 64 * A * sizeof(double) +
+  size_t b = (size_t)9 * 64 * A * sizeof(double) +
              (size_t)2 * 64 * A * sizeof(int) +
              (size_t)JR_MAX_MOVES * JR_NCOUNT * sizeof(int);
   return (b + 15) & ~(size_t)15;
 }
 
 __device__ __forceinline__ int* wave_counters(const WaveLds& L, int A) {
-  return reinterpret_cast<int*>(L.base + 12 * 64 * A) + 2 * 64 * A;
-}
-
-// Relaxed wave-scope FP64 add into LDS (ds_add_f64, no return).  A wave's
-// LDS operations execute in issue order and the compiler keeps atomics in
-// program order, so the additions into one entry happen in program order.
-typedef __attribute__((address_space(3))) double lds_double;
-__device__ __forceinline__ void lds_add(double* p, double v) {
-  __hip_atomic_fetch_add((lds_double*)p, v, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_WAVEFRONT);
+  return reinterpret_cast<int*>(L.base + 9 * 64 * A) + 2 * 64 * A;
 }
 
 struct Geom {
@@ -160,6 +151,14 @@ __device__ __forceinline__ void pair_lj_f32(float q, float eps4, float eps24,
   }
 }
 
+// q for an idle or cut-off pair: only the high dword is cleared, leaving a
+// denormal of at most 2^-1042 whose square underflows to +0, so s6, s12, the
+// pair energy and fs all come out +0 exactly as for q = 0 (one v_cndmask
+// instead of two).
+__device__ __forceinline__ double mask_q(bool ok, double v) {
+  return __hiloint2double(ok ? __double2hiint(v) : 0, __double2loint(v));
+}
+
 struct PairIn {
   double dx, dy, dz, r2;
   bool ok;  // active and r2 < rc2
@@ -192,19 +191,21 @@ __device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int j,
 // Energy partial (per lane) and forces of the configuration whose positions
 // and types are in the wave's LDS (and in the x,y,z,t registers).
 // Pair schedule: for k = 1 .. (N-1)/2 ring position p pairs with
-// (p + k) mod N, keeping its own force (fma(-fs, dr, own)) and adding the
-// reaction t = fs*dr into LDS entry p + k; for even N the k = N/2 pairs are
-// evaluated from both ends with half the energy each and no reaction.
-// Steps are taken in groups of G = 4 while they last, then one group of 2,
-// then a lone step.  A group shares one IEEE division:
+// (p + k) mod N, keeping its own force (own = fma(-fs, dr, own)) and adding
+// the reaction to a travelling accumulator (trav = fma(fs, dr, trav)) that
+// rotates one ring position per step, so every atom's reaction is summed in
+// step order; for even N the k = N/2 pairs are evaluated from both ends with
+// half the energy each and no reaction.  Steps are taken in groups of G = 4
+// while they last, then one group of 2, then a lone step.  A group shares one
+// IEEE division:
 //   G = 4: p01 = r2_0 r2_1, p23 = r2_2 r2_3, d = num / (p01 p23),
 //          d01 = p23 d, d23 = p01 d, q0 = r2_1 d01, q1 = r2_0 d01,
 //          q2 = r2_3 d23, q3 = r2_2 d23;
 //   G = 2: d = num / (r2_0 r2_1), q0 = r2_1 d, q1 = r2_0 d;
 //   G = 1 and k = N/2: q = num / r2,
-// with num = sigma^2 (one species) or 1.  Within a group the reactions are
-// added slot by slot (atoms 64s .. 64s+63), steps ascending within a slot.
-// FULL: N == 64*A, so no ring position is idle.
+// with num = sigma^2 (one species) or 1.
+// FULL: N == 64*A, so no ring position is idle and the rotation is one DPP
+// wave_rol per 32-bit half.
 template <int A, bool FORCES, int CFG, bool DIAG, bool FULL>
 __device__ __forceinline__ void wave_eval_impl(
     const LdsView<A>& L, int N, const Geom& g, const PairTab& tab,
@@ -214,26 +215,35 @@ __device__ __forceinline__ void wave_eval_impl(
   constexpr bool MULTI = (CFG & 1) != 0;
   constexpr bool F32 = (CFG & 2) != 0;
   const int lane = lane_id();
-  double e[A], ox[A], oy[A], oz[A];
+  double e[A], ox[A], oy[A], oz[A], tx[A], ty[A], tz[A];
 #pragma unroll
   for (int s = 0; s < A; ++s) {
     e[s] = 0.0;
     ox[s] = oy[s] = oz[s] = 0.0;
+    tx[s] = ty[s] = tz[s] = 0.0;
   }
   const double num = MULTI ? 1.0 : tab.sig2[0];
-  double* RX = L.rx();
-  double* RY = L.ry();
-  double* RZ = L.rz();
+  auto rotate = [&]() {
+    if (FULL) {
+      ring_rot1_full<A>(tx, lane);
+      ring_rot1_full<A>(ty, lane);
+      ring_rot1_full<A>(tz, lane);
+    } else {
+      ring_rot1<A>(tx, lane, N);
+      ring_rot1<A>(ty, lane, N);
+      ring_rot1<A>(tz, lane, N);
+    }
+  };
   using QT = typename std::conditional<F32, float, double>::type;
   // q = num / r^2 for one pair (F32: hardware reciprocal, 1 ulp)
   auto q_single = [&](const PairIn& P) -> QT {
     if constexpr (F32)
       return P.ok ? (float)num * __builtin_amdgcn_rcpf((float)P.r2) : 0.0f;
     else
-      return P.ok ? num / P.r2 : 0.0;
+      return mask_q(P.ok, num / P.r2);
   };
-  auto accumulate = [&](int s, int j, bool active, const PairIn& P, QT q,
-                        bool reaction, double escale) {
+  auto accumulate = [&](int s, const PairIn& P, QT q, bool reaction,
+                        double escale) {
     double ep, fs = 0.0;
     if constexpr (F32) {
       float epf, fsf = 0.0f;
@@ -250,12 +260,9 @@ __device__ __forceinline__ void wave_eval_impl(
       oy[s] = fma(-fs, P.dy, oy[s]);
       oz[s] = fma(-fs, P.dz, oz[s]);
       if (reaction) {
-        const double tx = fs * P.dx, ty = fs * P.dy, tz = fs * P.dz;
-        if (FULL || active) {
-          lds_add(RX + j, tx);
-          lds_add(RY + j, ty);
-          lds_add(RZ + j, tz);
-        }
+        tx[s] = fma(fs, P.dx, tx[s]);
+        ty[s] = fma(fs, P.dy, ty[s]);
+        tz[s] = fma(fs, P.dz, tz[s]);
       }
     }
   };
@@ -265,67 +272,74 @@ __device__ __forceinline__ void wave_eval_impl(
   // register pressure and the code size of every call site
 #pragma unroll 1
   for (; k + 3 <= kmax; k += 4) {
+    PairIn P[4][A];
+    QT q[4][A];
 #pragma unroll
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
       const bool active = FULL || a < N;
-      const int j = a + k;
-      PairIn P[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        P[i] = load_pair<A, CFG, DIAG>(L, j + i, active, g, tab, x[s], y[s],
-                                       z[s], t[s]);
-      QT q[4];
+        P[i][s] = load_pair<A, CFG, DIAG>(L, a + k + i, active, g, tab, x[s],
+                                          y[s], z[s], t[s]);
       if constexpr (F32) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) q[i] = q_single(P[i]);
+        for (int i = 0; i < 4; ++i) q[i][s] = q_single(P[i][s]);
       } else {
-        const double p01 = P[0].r2 * P[1].r2;
-        const double p23 = P[2].r2 * P[3].r2;
+        const double p01 = P[0][s].r2 * P[1][s].r2;
+        const double p23 = P[2][s].r2 * P[3][s].r2;
         const double d = num / (p01 * p23);
         const double d01 = p23 * d, d23 = p01 * d;
-        q[0] = P[0].ok ? P[1].r2 * d01 : 0.0;
-        q[1] = P[1].ok ? P[0].r2 * d01 : 0.0;
-        q[2] = P[2].ok ? P[3].r2 * d23 : 0.0;
-        q[3] = P[3].ok ? P[2].r2 * d23 : 0.0;
+        q[0][s] = mask_q(P[0][s].ok, P[1][s].r2 * d01);
+        q[1][s] = mask_q(P[1][s].ok, P[0][s].r2 * d01);
+        q[2][s] = mask_q(P[2][s].ok, P[3][s].r2 * d23);
+        q[3][s] = mask_q(P[3][s].ok, P[2][s].r2 * d23);
       }
+    }
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        accumulate(s, j + i, active, P[i], q[i], true, 1.0);
+    for (int i = 0; i < 4; ++i) {
+      if (FORCES && k + i > 1) rotate();
+#pragma unroll
+      for (int s = 0; s < A; ++s) accumulate(s, P[i][s], q[i][s], true, 1.0);
     }
   }
   if (k + 1 <= kmax) {
+    PairIn P0[A], P1[A];
+    QT q0[A], q1[A];
 #pragma unroll
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
       const bool active = FULL || a < N;
-      const int j = a + k;
-      PairIn P0 = load_pair<A, CFG, DIAG>(L, j, active, g, tab, x[s], y[s],
-                                          z[s], t[s]);
-      PairIn P1 = load_pair<A, CFG, DIAG>(L, j + 1, active, g, tab, x[s],
-                                          y[s], z[s], t[s]);
-      QT q0, q1;
+      P0[s] = load_pair<A, CFG, DIAG>(L, a + k, active, g, tab, x[s], y[s],
+                                      z[s], t[s]);
+      P1[s] = load_pair<A, CFG, DIAG>(L, a + k + 1, active, g, tab, x[s],
+                                      y[s], z[s], t[s]);
       if constexpr (F32) {
-        q0 = q_single(P0);
-        q1 = q_single(P1);
+        q0[s] = q_single(P0[s]);
+        q1[s] = q_single(P1[s]);
       } else {
-        const double d = num / (P0.r2 * P1.r2);
-        q0 = P0.ok ? P1.r2 * d : 0.0;
-        q1 = P1.ok ? P0.r2 * d : 0.0;
+        const double d = num / (P0[s].r2 * P1[s].r2);
+        q0[s] = mask_q(P0[s].ok, P1[s].r2 * d);
+        q1[s] = mask_q(P1[s].ok, P0[s].r2 * d);
       }
-      accumulate(s, j, active, P0, q0, true, 1.0);
-      accumulate(s, j + 1, active, P1, q1, true, 1.0);
     }
+    if (FORCES && k > 1) rotate();
+#pragma unroll
+    for (int s = 0; s < A; ++s) accumulate(s, P0[s], q0[s], true, 1.0);
+    if (FORCES) rotate();
+#pragma unroll
+    for (int s = 0; s < A; ++s) accumulate(s, P1[s], q1[s], true, 1.0);
     k += 2;
   }
   if (k <= kmax) {  // lone last step
+    if (FORCES && k > 1) rotate();
 #pragma unroll
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
       const bool active = FULL || a < N;
       PairIn P = load_pair<A, CFG, DIAG>(L, a + k, active, g, tab, x[s], y[s],
                                          z[s], t[s]);
-      accumulate(s, a + k, active, P, q_single(P), true, 1.0);
+      accumulate(s, P, q_single(P), true, 1.0);
     }
   }
   if (N >= 2 && !(N & 1)) {
@@ -336,7 +350,7 @@ __device__ __forceinline__ void wave_eval_impl(
       const bool active = FULL || a < N;
       PairIn P = load_pair<A, CFG, DIAG>(L, a + h, active, g, tab, x[s], y[s],
                                          z[s], t[s]);
-      accumulate(s, a + h, active, P, q_single(P), false, 0.5);
+      accumulate(s, P, q_single(P), false, 0.5);
     }
   }
   double el = e[0];
@@ -347,15 +361,30 @@ __device__ __forceinline__ void wave_eval_impl(
   }
   e_lane = el;
   if (FORCES) {
+    // position p holds the reaction of atom (p + kmax) mod N
+    double* HX = L.hx();
+    double* HY = L.hy();
+    double* HZ = L.hz();
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      const int a = lane + 64 * s;
+      if (FULL || a < N) {
+        int dst = a + kmax;
+        if (dst >= N) dst -= N;
+        HX[dst] = tx[s];
+        HY[dst] = ty[s];
+        HZ[dst] = tz[s];
+      }
+    }
     wave_lds_sync();
     const double c24 = MULTI ? 1.0 : tab.eps24[0] / tab.sig2[0];
 #pragma unroll
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
       if (FULL || a < N) {
-        fx[s] = ox[s] + (RX[a] + RX[a + N]);
-        fy[s] = oy[s] + (RY[a] + RY[a + N]);
-        fz[s] = oz[s] + (RZ[a] + RZ[a + N]);
+        fx[s] = ox[s] + HX[a];
+        fy[s] = oy[s] + HY[a];
+        fz[s] = oz[s] + HZ[a];
         if (!MULTI) {
           fx[s] = c24 * fx[s];
           fy[s] = c24 * fy[s];
@@ -369,8 +398,7 @@ __device__ __forceinline__ void wave_eval_impl(
 }
 
 // Model energy U (wave-uniform) and forces of the configuration in
-// registers; writes the configuration (twice) into the wave's LDS and clears
-// the reaction arrays first.
+// registers; writes the configuration (twice) into the wave's LDS first.
 template <int A, bool FORCES, int CFG>
 __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
                                             const Geom& g, const PairTab& tab,
@@ -397,14 +425,6 @@ __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
       if (MULTI) {
         L.t()[a] = t[s];
         L.t()[a + N] = t[s];
-      }
-      if (FORCES) {
-        L.rx()[a] = 0.0;
-        L.ry()[a] = 0.0;
-        L.rz()[a] = 0.0;
-        L.rx()[a + N] = 0.0;
-        L.ry()[a + N] = 0.0;
-        L.rz()[a + N] = 0.0;
       }
     }
   }

@@ -184,40 +184,37 @@ static inline void pair_lj(const pair_t* p, double q, int multi, double* e,
   }
 }
 
-static inline void acc_pair(int a, const pair_t* p, double q, int multi,
-                            double escale, double* e, double* own,
-                            double* react, int jr) {
+static inline void acc_pair(int a, int j, const pair_t* p, double q, int multi,
+                            int reaction, double escale, double* e,
+                            double* own, double* trav) {
   double ep, fs;
   pair_lj(p, q, multi, &ep, &fs);
   e[a] = e[a] + (escale == 1.0 ? ep : escale * ep);
   own[3 * a + 0] = fma(-fs, p->dx, own[3 * a + 0]);
   own[3 * a + 1] = fma(-fs, p->dy, own[3 * a + 1]);
   own[3 * a + 2] = fma(-fs, p->dz, own[3 * a + 2]);
-  if (react != NULL) {
-    react[3 * jr + 0] = react[3 * jr + 0] + fs * p->dx;
-    react[3 * jr + 1] = react[3 * jr + 1] + fs * p->dy;
-    react[3 * jr + 2] = react[3 * jr + 2] + fs * p->dz;
+  if (reaction) {
+    trav[3 * j + 0] = fma(fs, p->dx, trav[3 * j + 0]);
+    trav[3 * j + 1] = fma(fs, p->dy, trav[3 * j + 1]);
+    trav[3 * j + 2] = fma(fs, p->dz, trav[3 * j + 2]);
   }
 }
 
 /* Model energy U (and forces F = -dU/dx when F != NULL), then
  * H = U + ensemble term; backends/hamiltonian.py:112-153.
  * Pair schedule: for k = 1..(N-1)/2 atom a pairs with (a+k) mod N; the
- * force on a is accumulated in k order into own[a] (fma(-fs, dr, own)) and
- * the reaction t = fs*dr into entry a+k of a 2N-entry array R, atom j's
- * reaction being R[j] + R[j+N]; for even N the k = N/2 pairs are taken from
- * both ends with half the energy and no reaction.  Steps come in groups of
- * G = 4 while they last, then one group of 2, then a lone step; a group
- * shares one division:
+ * force on a is accumulated in k order into own[a] and the reaction on
+ * j = a+k into trav[j] (also in k order); for even N the k = N/2 pairs are
+ * taken from both ends with half the energy and no reaction.  Steps come in
+ * groups of G = 4 while they last, then one group of 2, then a lone step; a
+ * group shares one division:
  *   G = 4: p01 = r2_0 r2_1, p23 = r2_2 r2_3, d = num/(p01 p23),
  *          d01 = p23 d, d23 = p01 d, q0 = r2_1 d01, q1 = r2_0 d01,
  *          q2 = r2_3 d23, q3 = r2_2 d23;
  *   G = 2: d = num/(r2_0 r2_1), q0 = r2_1 d, q1 = r2_0 d;
  *   G = 1 and k = N/2: q = num/r2
- * (num = sigma^2 for one species, else 1).  Within a group the reactions
- * are added slot by slot (atoms 64s..64s+63), steps ascending in a slot.
- * One species: U = 4 eps * sum and F = (24 eps / sigma^2) *
- * (own + (R[a] + R[a+N])). */
+ * (num = sigma^2 for one species, else 1).  One species: U = 4 eps * sum
+ * and F = (24 eps / sigma^2) * (own + trav). */
 void or_energy_forces(int N, const double* pos, const int32_t* types,
                       const double* cell_in, const or_pot* pot,
                       double pressure, const double* mu, double* U_out,
@@ -239,45 +236,43 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
   size_t nn = (size_t)(N > 0 ? N : 1);
   double* e = (double*)calloc(nn, sizeof(double));
   double* own = (double*)calloc(3 * nn, sizeof(double));
-  double* react = (double*)calloc(6 * nn, sizeof(double)); /* [2N][3] */
+  double* trav = (double*)calloc(3 * nn, sizeof(double));
+  pair_t* P = (pair_t*)malloc(4 * nn * sizeof(pair_t)); /* [4][N] */
+  double* q = (double*)malloc(4 * nn * sizeof(double));
   int kmax = (N - 1) / 2;
   int half = (N >= 2) && (N % 2 == 0);
   int k = 1;
   while (k <= kmax) {
     const int G = (k + 3 <= kmax) ? 4 : ((k + 1 <= kmax) ? 2 : 1);
-    for (int s0 = 0; s0 < N; s0 += WAVE) {
-      const int s1 = (s0 + WAVE < N) ? s0 + WAVE : N;
-      pair_t P[4][WAVE];
-      double q[4][WAVE];
-      for (int a = s0; a < s1; ++a) {
-        const int l = a - s0;
-        for (int i = 0; i < G; ++i) {
-          const int j = (a + k + i) % N;
-          P[i][l] = pair_geom(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
-                              types[j], pot);
-        }
-        if (G == 4) {
-          const double p01 = P[0][l].r2 * P[1][l].r2;
-          const double p23 = P[2][l].r2 * P[3][l].r2;
-          const double d = num / (p01 * p23);
-          const double d01 = p23 * d, d23 = p01 * d;
-          q[0][l] = P[0][l].ok ? P[1][l].r2 * d01 : 0.0;
-          q[1][l] = P[1][l].ok ? P[0][l].r2 * d01 : 0.0;
-          q[2][l] = P[2][l].ok ? P[3][l].r2 * d23 : 0.0;
-          q[3][l] = P[3][l].ok ? P[2][l].r2 * d23 : 0.0;
-        } else if (G == 2) {
-          const double d = num / (P[0][l].r2 * P[1][l].r2);
-          q[0][l] = P[0][l].ok ? P[1][l].r2 * d : 0.0;
-          q[1][l] = P[1][l].ok ? P[0][l].r2 * d : 0.0;
-        } else {
-          q[0][l] = P[0][l].ok ? num / P[0][l].r2 : 0.0;
-        }
+    for (int a = 0; a < N; ++a) {
+      pair_t* p = P + a; /* p[i * N] is step k + i */
+      double* qa = q + a;
+      for (int i = 0; i < G; ++i) {
+        const int j = (a + k + i) % N;
+        p[i * N] = pair_geom(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
+                             types[j], pot);
       }
-      for (int i = 0; i < G; ++i)
-        for (int a = s0; a < s1; ++a)
-          acc_pair(a, &P[i][a - s0], q[i][a - s0], multi, 1.0, e, own, react,
-                   a + k + i);
+      if (G == 4) {
+        const double p01 = p[0].r2 * p[N].r2;
+        const double p23 = p[2 * N].r2 * p[3 * N].r2;
+        const double d = num / (p01 * p23);
+        const double d01 = p23 * d, d23 = p01 * d;
+        qa[0] = p[0].ok ? p[N].r2 * d01 : 0.0;
+        qa[N] = p[N].ok ? p[0].r2 * d01 : 0.0;
+        qa[2 * N] = p[2 * N].ok ? p[3 * N].r2 * d23 : 0.0;
+        qa[3 * N] = p[3 * N].ok ? p[2 * N].r2 * d23 : 0.0;
+      } else if (G == 2) {
+        const double d = num / (p[0].r2 * p[N].r2);
+        qa[0] = p[0].ok ? p[N].r2 * d : 0.0;
+        qa[N] = p[N].ok ? p[0].r2 * d : 0.0;
+      } else {
+        qa[0] = p[0].ok ? num / p[0].r2 : 0.0;
+      }
     }
+    for (int i = 0; i < G; ++i)
+      for (int a = 0; a < N; ++a)
+        acc_pair(a, (a + k + i) % N, &P[i * N + a], q[i * N + a], multi, 1,
+                 1.0, e, own, trav);
     k += G;
   }
   if (half) {
@@ -286,24 +281,24 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
       int j = (a + h) % N;
       pair_t p = pair_geom(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
                            types[j], pot);
-      double q = p.ok ? num / p.r2 : 0.0;
-      acc_pair(a, &p, q, multi, 0.5, e, own, NULL, 0);
+      double qh = p.ok ? num / p.r2 : 0.0;
+      acc_pair(a, j, &p, qh, multi, 0, 0.5, e, own, trav);
     }
   }
   double U = wave_sum_atoms(e, N);
   if (!multi) U = pot->eps4[0] * U;
   if (F != NULL) {
     double c24 = multi ? 1.0 : pot->eps24[0] / pot->sig2[0];
-    for (int a = 0; a < N; ++a)
-      for (int c = 0; c < 3; ++c) {
-        double f = own[3 * a + c] +
-                   (react[3 * a + c] + react[3 * (a + N) + c]);
-        F[3 * a + c] = multi ? f : c24 * f;
-      }
+    for (int a = 0; a < 3 * N; ++a) {
+      F[a] = own[a] + trav[a];
+      if (!multi) F[a] = c24 * F[a];
+    }
   }
   free(e);
   free(own);
-  free(react);
+  free(trav);
+  free(P);
+  free(q);
   *U_out = U;
   if (H_out != NULL)
     *H_out = U + or_ensemble_energy(N, types, cell_in, pressure, mu,
