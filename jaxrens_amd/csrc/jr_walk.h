@@ -62,7 +62,12 @@ struct Geom {
   double c[9];
   double inv[9];
   bool diag;
+  bool cubic;  // diag with c[0] == c[4] == c[8]
 };
+
+// Pair geometry kinds: minimum image in fractional coordinates for a cubic
+// cell, Cartesian rint per axis for other diagonal cells, full triclinic.
+enum { GEO_CUBIC = 0, GEO_DIAG = 1, GEO_TRI = 2 };
 
 __device__ __forceinline__ void make_geom(const double* cell, Geom& g) {
   double det = det3(cell);
@@ -84,10 +89,14 @@ __device__ __forceinline__ void make_geom(const double* cell, Geom& g) {
   }
   g.diag = (g.c[1] == 0.0) && (g.c[2] == 0.0) && (g.c[3] == 0.0) &&
            (g.c[5] == 0.0) && (g.c[6] == 0.0) && (g.c[7] == 0.0);
+  g.cubic = g.diag && (g.c[0] == g.c[4]) && (g.c[0] == g.c[8]);
 }
 
 // dr = MIC(x_j - x_i) and r^2 (backends/lj.py:141-162).
-template <bool DIAG>
+// GEO_CUBIC works on fractional coordinates u = x * inv[0] (cell length L):
+// du = u_j - u_i, du -= rint(du), r^2 / L^2 = du.du; lengths come back
+// through L^2 folded into num and the cutoff, and L into the force scale.
+template <int GEO>
 __device__ __forceinline__ void pair_dr(double xi, double yi, double zi,
                                         double xj, double yj, double zj,
                                         const Geom& g, double& dx, double& dy,
@@ -95,7 +104,11 @@ __device__ __forceinline__ void pair_dr(double xi, double yi, double zi,
   dx = xj - xi;
   dy = yj - yi;
   dz = zj - zi;
-  if (DIAG) {
+  if (GEO == GEO_CUBIC) {
+    dx = dx - rint(dx);
+    dy = dy - rint(dy);
+    dz = dz - rint(dz);
+  } else if (GEO == GEO_DIAG) {
     double n0 = rint(dx * g.inv[0]);
     double n1 = rint(dy * g.inv[4]);
     double n2 = rint(dz * g.inv[8]);
@@ -165,17 +178,24 @@ struct PairIn {
   double eps4, eps24, sig2;
 };
 
+// Per-evaluation constants: q numerator, cutoff (both divided by L^2 for
+// GEO_CUBIC) and the force scale.
+struct EvalConst {
+  double num, rc2, cf;
+};
+
 // Pair of the atom in slot s (ring position a) with LDS entry j = a + k.
-template <int A, int CFG, bool DIAG>
+template <int A, int CFG, int GEO>
 __device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int j,
                                             bool active, const Geom& g,
-                                            const PairTab& tab, double xs,
+                                            const PairTab& tab,
+                                            const EvalConst& K, double xs,
                                             double ys, double zs, int ts) {
   constexpr bool MULTI = (CFG & 1) != 0;
   PairIn P;
-  pair_dr<DIAG>(xs, ys, zs, L.x()[j], L.y()[j], L.z()[j], g, P.dx, P.dy, P.dz,
-                P.r2);
-  P.ok = active && (P.r2 < tab.rc2);
+  pair_dr<GEO>(xs, ys, zs, L.x()[j], L.y()[j], L.z()[j], g, P.dx, P.dy, P.dz,
+               P.r2);
+  P.ok = active && (P.r2 < K.rc2);
   P.eps4 = tab.eps4[0];
   P.eps24 = tab.eps24[0];
   P.sig2 = tab.sig2[0];
@@ -206,10 +226,10 @@ __device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int j,
 // with num = sigma^2 (one species) or 1.
 // FULL: N == 64*A, so no ring position is idle and the rotation is one DPP
 // wave_rol per 32-bit half.
-template <int A, bool FORCES, int CFG, bool DIAG, bool FULL>
+template <int A, bool FORCES, int CFG, int GEO, bool FULL>
 __device__ __forceinline__ void wave_eval_impl(
     const LdsView<A>& L, int N, const Geom& g, const PairTab& tab,
-    const double (&x)[A], const double (&y)[A], const double (&z)[A],
+    const EvalConst& K, const double (&x)[A], const double (&y)[A], const double (&z)[A],
     const int (&t)[A], double& e_lane, double (&fx)[A], double (&fy)[A],
     double (&fz)[A]) {
   constexpr bool MULTI = (CFG & 1) != 0;
@@ -222,7 +242,7 @@ __device__ __forceinline__ void wave_eval_impl(
     ox[s] = oy[s] = oz[s] = 0.0;
     tx[s] = ty[s] = tz[s] = 0.0;
   }
-  const double num = MULTI ? 1.0 : tab.sig2[0];
+  const double num = K.num;
   auto rotate = [&]() {
     if (FULL) {
       ring_rot1_full<A>(tx, lane);
@@ -280,7 +300,7 @@ __device__ __forceinline__ void wave_eval_impl(
       const bool active = FULL || a < N;
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        P[i][s] = load_pair<A, CFG, DIAG>(L, a + k + i, active, g, tab, x[s],
+        P[i][s] = load_pair<A, CFG, GEO>(L, a + k + i, active, g, tab, K, x[s],
                                           y[s], z[s], t[s]);
       if constexpr (F32) {
 #pragma unroll
@@ -310,9 +330,9 @@ __device__ __forceinline__ void wave_eval_impl(
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
       const bool active = FULL || a < N;
-      P0[s] = load_pair<A, CFG, DIAG>(L, a + k, active, g, tab, x[s], y[s],
+      P0[s] = load_pair<A, CFG, GEO>(L, a + k, active, g, tab, K, x[s], y[s],
                                       z[s], t[s]);
-      P1[s] = load_pair<A, CFG, DIAG>(L, a + k + 1, active, g, tab, x[s],
+      P1[s] = load_pair<A, CFG, GEO>(L, a + k + 1, active, g, tab, K, x[s],
                                       y[s], z[s], t[s]);
       if constexpr (F32) {
         q0[s] = q_single(P0[s]);
@@ -337,7 +357,7 @@ __device__ __forceinline__ void wave_eval_impl(
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
       const bool active = FULL || a < N;
-      PairIn P = load_pair<A, CFG, DIAG>(L, a + k, active, g, tab, x[s], y[s],
+      PairIn P = load_pair<A, CFG, GEO>(L, a + k, active, g, tab, K, x[s], y[s],
                                          z[s], t[s]);
       accumulate(s, P, q_single(P), true, 1.0);
     }
@@ -348,7 +368,7 @@ __device__ __forceinline__ void wave_eval_impl(
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
       const bool active = FULL || a < N;
-      PairIn P = load_pair<A, CFG, DIAG>(L, a + h, active, g, tab, x[s], y[s],
+      PairIn P = load_pair<A, CFG, GEO>(L, a + h, active, g, tab, K, x[s], y[s],
                                          z[s], t[s]);
       accumulate(s, P, q_single(P), false, 0.5);
     }
@@ -377,7 +397,6 @@ __device__ __forceinline__ void wave_eval_impl(
       }
     }
     wave_lds_sync();
-    const double c24 = MULTI ? 1.0 : tab.eps24[0] / tab.sig2[0];
 #pragma unroll
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
@@ -385,10 +404,10 @@ __device__ __forceinline__ void wave_eval_impl(
         fx[s] = ox[s] + HX[a];
         fy[s] = oy[s] + HY[a];
         fz[s] = oz[s] + HZ[a];
-        if (!MULTI) {
-          fx[s] = c24 * fx[s];
-          fy[s] = c24 * fy[s];
-          fz[s] = c24 * fz[s];
+        if (GEO == GEO_CUBIC || !MULTI) {
+          fx[s] = K.cf * fx[s];
+          fy[s] = K.cf * fy[s];
+          fz[s] = K.cf * fz[s];
         }
       } else {
         fx[s] = fy[s] = fz[s] = 0.0;
@@ -399,6 +418,9 @@ __device__ __forceinline__ void wave_eval_impl(
 
 // Model energy U (wave-uniform) and forces of the configuration in
 // registers; writes the configuration (twice) into the wave's LDS first.
+// A cubic cell is evaluated in fractional coordinates u = x * inv[0] with
+//   num = num_0 / L^2, cutoff rc^2 / L^2, force scale 24 eps / sigma^2 * L
+// (species table: L), where num_0 = sigma^2 (one species) or 1.
 template <int A, bool FORCES, int CFG>
 __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
                                             const Geom& g, const PairTab& tab,
@@ -411,17 +433,41 @@ __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
   constexpr bool MULTI = (CFG & 1) != 0;
   const int lane = lane_id();
   const LdsView<A> L{W.base};
+  EvalConst K;
+  K.num = MULTI ? 1.0 : tab.sig2[0];
+  K.rc2 = tab.rc2;
+  K.cf = MULTI ? 1.0 : tab.eps24[0] / tab.sig2[0];
+  double u[A], v[A], w[A];
+  if (g.cubic) {
+    const double len = g.c[0], len2 = len * len;
+    K.num = K.num / len2;
+    K.rc2 = K.rc2 / len2;
+    K.cf = K.cf * len;
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      u[s] = x[s] * g.inv[0];
+      v[s] = y[s] * g.inv[0];
+      w[s] = z[s] * g.inv[0];
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      u[s] = x[s];
+      v[s] = y[s];
+      w[s] = z[s];
+    }
+  }
   wave_lds_sync();  // earlier readers of this wave's LDS are done
 #pragma unroll
   for (int s = 0; s < A; ++s) {
     const int a = lane + 64 * s;
     if (a < N) {
-      L.x()[a] = x[s];
-      L.y()[a] = y[s];
-      L.z()[a] = z[s];
-      L.x()[a + N] = x[s];
-      L.y()[a + N] = y[s];
-      L.z()[a + N] = z[s];
+      L.x()[a] = u[s];
+      L.y()[a] = v[s];
+      L.z()[a] = w[s];
+      L.x()[a + N] = u[s];
+      L.y()[a + N] = v[s];
+      L.z()[a + N] = w[s];
       if (MULTI) {
         L.t()[a] = t[s];
         L.t()[a + N] = t[s];
@@ -431,20 +477,27 @@ __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
   wave_lds_sync();
   double el;
   const bool full = (N == 64 * A);
-  if (g.diag) {
+  if (g.cubic) {
     if (full)
-      wave_eval_impl<A, FORCES, CFG, true, true>(L, N, g, tab, x, y, z, t,
-                                                   el, fx, fy, fz);
+      wave_eval_impl<A, FORCES, CFG, GEO_CUBIC, true>(L, N, g, tab, K, u, v, w,
+                                                      t, el, fx, fy, fz);
     else
-      wave_eval_impl<A, FORCES, CFG, true, false>(L, N, g, tab, x, y, z, t,
-                                                    el, fx, fy, fz);
+      wave_eval_impl<A, FORCES, CFG, GEO_CUBIC, false>(L, N, g, tab, K, u, v,
+                                                       w, t, el, fx, fy, fz);
+  } else if (g.diag) {
+    if (full)
+      wave_eval_impl<A, FORCES, CFG, GEO_DIAG, true>(L, N, g, tab, K, u, v, w,
+                                                     t, el, fx, fy, fz);
+    else
+      wave_eval_impl<A, FORCES, CFG, GEO_DIAG, false>(L, N, g, tab, K, u, v, w,
+                                                      t, el, fx, fy, fz);
   } else {
     if (full)
-      wave_eval_impl<A, FORCES, CFG, false, true>(L, N, g, tab, x, y, z, t,
-                                                    el, fx, fy, fz);
+      wave_eval_impl<A, FORCES, CFG, GEO_TRI, true>(L, N, g, tab, K, u, v, w,
+                                                    t, el, fx, fy, fz);
     else
-      wave_eval_impl<A, FORCES, CFG, false, false>(L, N, g, tab, x, y, z, t,
-                                                     el, fx, fy, fz);
+      wave_eval_impl<A, FORCES, CFG, GEO_TRI, false>(L, N, g, tab, K, u, v, w,
+                                                     t, el, fx, fy, fz);
   }
   wave_lds_sync();
   double U = wave_sum((lane < N) ? el : 0.0);

@@ -140,7 +140,9 @@ double or_ensemble_energy(int N, const int32_t* types, const double* cell,
   return e;
 }
 
-/* One pair, dr = MIC(x_j - x_i) and r^2; backends/lj.py:141-162. */
+/* One pair, dr = MIC(x_j - x_i) and r^2; backends/lj.py:141-162.
+ * cubic != 0: xi, xj are fractional coordinates u = x * inv[0] and
+ * dr = du - rint(du) (r^2 in units of L^2). */
 typedef struct {
   double dx, dy, dz, r2;
   double eps4, eps24, sig2;
@@ -149,18 +151,25 @@ typedef struct {
 
 static inline pair_t pair_geom(const double* xi, const double* xj,
                                const double* cell, const double* inv, int ti,
-                               int tj, const or_pot* pot) {
+                               int tj, const or_pot* pot, double rc2,
+                               int cubic) {
   pair_t r;
   double dx = xj[0] - xi[0], dy = xj[1] - xi[1], dz = xj[2] - xi[2];
-  double f0 = fma(dz, inv[6], fma(dy, inv[3], dx * inv[0]));
-  double f1 = fma(dz, inv[7], fma(dy, inv[4], dx * inv[1]));
-  double f2 = fma(dz, inv[8], fma(dy, inv[5], dx * inv[2]));
-  double n0 = nearbyint(f0), n1 = nearbyint(f1), n2 = nearbyint(f2);
-  r.dx = fma(-n2, cell[6], fma(-n1, cell[3], fma(-n0, cell[0], dx)));
-  r.dy = fma(-n2, cell[7], fma(-n1, cell[4], fma(-n0, cell[1], dy)));
-  r.dz = fma(-n2, cell[8], fma(-n1, cell[5], fma(-n0, cell[2], dz)));
+  if (cubic) {
+    r.dx = dx - nearbyint(dx);
+    r.dy = dy - nearbyint(dy);
+    r.dz = dz - nearbyint(dz);
+  } else {
+    double f0 = fma(dz, inv[6], fma(dy, inv[3], dx * inv[0]));
+    double f1 = fma(dz, inv[7], fma(dy, inv[4], dx * inv[1]));
+    double f2 = fma(dz, inv[8], fma(dy, inv[5], dx * inv[2]));
+    double n0 = nearbyint(f0), n1 = nearbyint(f1), n2 = nearbyint(f2);
+    r.dx = fma(-n2, cell[6], fma(-n1, cell[3], fma(-n0, cell[0], dx)));
+    r.dy = fma(-n2, cell[7], fma(-n1, cell[4], fma(-n0, cell[1], dy)));
+    r.dz = fma(-n2, cell[8], fma(-n1, cell[5], fma(-n0, cell[2], dz)));
+  }
   r.r2 = dot3(r.dx, r.dy, r.dz, r.dx, r.dy, r.dz);
-  r.ok = r.r2 < pot->rc2;
+  r.ok = r.r2 < rc2;
   int ab = ti * pot->n_species + tj;
   r.eps4 = pot->eps4[ab];
   r.eps24 = pot->eps24[ab];
@@ -214,7 +223,10 @@ static inline void acc_pair(int a, int j, const pair_t* p, double q, int multi,
  *   G = 2: d = num/(r2_0 r2_1), q0 = r2_1 d, q1 = r2_0 d;
  *   G = 1 and k = N/2: q = num/r2
  * (num = sigma^2 for one species, else 1).  One species: U = 4 eps * sum
- * and F = (24 eps / sigma^2) * (own + trav). */
+ * and F = (24 eps / sigma^2) * (own + trav).  A cubic cell (diagonal, equal
+ * lengths L) is evaluated in fractional coordinates u = x * inv[0]:
+ * dr = du - rint(du), num and rc^2 divided by L^2, and F = cf * (own + trav)
+ * with cf = 24 eps / sigma^2 * L (species table: cf = L). */
 void or_energy_forces(int N, const double* pos, const int32_t* types,
                       const double* cell_in, const or_pot* pot,
                       double pressure, const double* mu, double* U_out,
@@ -231,12 +243,27 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
   double inv[9];
   inv3(cell, det, inv);
   const int multi = pot->n_species > 1;
-  const double num = multi ? 1.0 : pot->sig2[0];
+  const int diag = cell[1] == 0.0 && cell[2] == 0.0 && cell[3] == 0.0 &&
+                   cell[5] == 0.0 && cell[6] == 0.0 && cell[7] == 0.0;
+  const int cubic = diag && cell[0] == cell[4] && cell[0] == cell[8];
+  double num = multi ? 1.0 : pot->sig2[0];
+  double rc2 = pot->rc2;
+  double cf = multi ? 1.0 : pot->eps24[0] / pot->sig2[0];
 
   size_t nn = (size_t)(N > 0 ? N : 1);
   double* e = (double*)calloc(nn, sizeof(double));
   double* own = (double*)calloc(3 * nn, sizeof(double));
   double* trav = (double*)calloc(3 * nn, sizeof(double));
+  double* frac = NULL;
+  if (cubic) { /* fractional coordinates; lengths folded into the constants */
+    const double len = cell[0], len2 = len * len;
+    num = num / len2;
+    rc2 = rc2 / len2;
+    cf = cf * len;
+    frac = (double*)malloc(3 * nn * sizeof(double));
+    for (int i = 0; i < 3 * N; ++i) frac[i] = pos[i] * inv[0];
+    pos = frac;
+  }
   pair_t* P = (pair_t*)malloc(4 * nn * sizeof(pair_t)); /* [4][N] */
   double* q = (double*)malloc(4 * nn * sizeof(double));
   int kmax = (N - 1) / 2;
@@ -250,7 +277,7 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
       for (int i = 0; i < G; ++i) {
         const int j = (a + k + i) % N;
         p[i * N] = pair_geom(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
-                             types[j], pot);
+                             types[j], pot, rc2, cubic);
       }
       if (G == 4) {
         const double p01 = p[0].r2 * p[N].r2;
@@ -280,7 +307,7 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
     for (int a = 0; a < N; ++a) {
       int j = (a + h) % N;
       pair_t p = pair_geom(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
-                           types[j], pot);
+                           types[j], pot, rc2, cubic);
       double qh = p.ok ? num / p.r2 : 0.0;
       acc_pair(a, j, &p, qh, multi, 0, 0.5, e, own, trav);
     }
@@ -288,12 +315,12 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
   double U = wave_sum_atoms(e, N);
   if (!multi) U = pot->eps4[0] * U;
   if (F != NULL) {
-    double c24 = multi ? 1.0 : pot->eps24[0] / pot->sig2[0];
     for (int a = 0; a < 3 * N; ++a) {
       F[a] = own[a] + trav[a];
-      if (!multi) F[a] = c24 * F[a];
+      if (cubic || !multi) F[a] = cf * F[a];
     }
   }
+  free(frac);
   free(e);
   free(own);
   free(trav);
