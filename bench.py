@@ -33,6 +33,36 @@ METRIC = "walker-steps/s, LJ-64 NPT P×K=16×1024; surrogate vs plain at matched
 # algorithmic FP64 FLOPs per unordered pair (cubic MIC): energy+forces, energy
 FLOP_EF, FLOP_E = 38, 25
 
+# measured HBM traffic of the walk kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE
+# passes scaled by a copy of known size); absent file -> traffic null
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "walk_kernel_lj64_npt_16x1024_traffic.txt")
+
+
+def measured_traffic() -> float | None:
+    """Bytes per walk launch from the committed counter summary, if any."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+def copy_calibration(n_bytes: int = 1 << 30) -> dict:
+    """A plain device copy of known size (float32 multiply by 1: reads and
+    writes ``n_bytes`` each) for scaling the FETCH_SIZE / WRITE_SIZE
+    counters; run it under the same counter pass as the bench."""
+    import torch
+    a = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    return {"copy_bytes_read": n_bytes, "copy_bytes_written": n_bytes,
+            "kernel": "torch elementwise mul (float32, x1.0)"}
+
 
 def build_engine(n_extra: int, R=16, K=1024, N=64, n_steps=20, seed=42,
                  precision="fp64"):
@@ -117,7 +147,12 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--skip-secondary", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        print(json.dumps(copy_calibration()), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); "
               "the multi-GPU replica sharding is a design, not a measured path.",
@@ -204,13 +239,16 @@ def main():
             "launch": {**eng.launch_info(), **(tune or {})},
         },
         "roofline": {
-            "bound": "valu",
+            "bound": "mfma",
+            "note": ("compute-bound on the FP64 vector ALUs; MFMA is not used "
+                     "(the pair sums are not a dense contraction) and MI355X's "
+                     "FP64 peak is 78.6 TFLOP/s for vector and matrix issue alike"),
             "kernel": "walk_kernel (persistent, FP64)",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(),
             "flops_per_launch": flops / launches,
             "avg_launch_ms": avg_launch_ms,
             "launches": launches,
