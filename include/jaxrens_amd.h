@@ -208,6 +208,35 @@ int jr_walk_timing(jr_handle* h, double* total_ms, int64_t* launches,
 int jr_get_info(jr_handle* h, int32_t* n_cu, int32_t* blocks,
                 int32_t* waves_per_block);
 
+/* ------------------------------------------------------------------------
+ * Learned part.  No reference counterpart: the reference has no surrogate
+ * and its ML backends (backends/, out of scope here) are not this judge.
+ * ---------------------------------------------------------------------- */
+
+/* Seeded random-init invariant message-passing potential, the expensive
+ * exact judge of the surrogate-screened walk (FP32 weights and arithmetic):
+ * 8 Bessel radial functions with a polynomial envelope out to rc, two
+ * continuous-filter interaction layers of 64 channels with SiLU, per-atom
+ * linear readout, plus a 12th-power core repulsion below core_rc.  Weight
+ * layouts: emb [S][64], W1/W2 [8][64], U1/U2 [64][64] (z = m U + c),
+ * c1/c2/wo [64]. */
+typedef struct {
+  int32_t n_species; /* S, 1..2 */
+  int32_t channels;  /* must be 64 */
+  int32_t n_basis;   /* must be 8 */
+  int32_t pad_;
+  double rc, core_eps, core_sig, core_rc;
+  const float *emb, *W1, *U1, *c1, *W2, *U2, *c2, *wo;
+} jr_mpnn;
+
+/* Batched energy (+forces) of the judge, host buffers, shapes as
+ * jr_eval_energy_forces (mu unused: NPT).  Cells must be cubic with edge
+ * >= rc.  H = U + P |det cell| when pressure != NULL. */
+int jr_mpnn_eval(const jr_mpnn* net, int32_t device, int32_t B, int32_t Nmax,
+                 const double* pos, const int32_t* types,
+                 const int32_t* n_atoms, const double* cell,
+                 const double* pressure, double* U, double* H, double* F);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,6 +104,24 @@ class JrIterOut(C.Structure):
     ]
 
 
+_fp = C.POINTER(C.c_float)
+
+
+class JrMPNN(C.Structure):
+    _fields_ = [
+        ("n_species", C.c_int32),
+        ("channels", C.c_int32),
+        ("n_basis", C.c_int32),
+        ("pad_", C.c_int32),
+        ("rc", C.c_double),
+        ("core_eps", C.c_double),
+        ("core_sig", C.c_double),
+        ("core_rc", C.c_double),
+        ("emb", _fp), ("W1", _fp), ("U1", _fp), ("c1", _fp),
+        ("W2", _fp), ("U2", _fp), ("c2", _fp), ("wo", _fp),
+    ]
+
+
 # (name, restype, argtypes) of every exported symbol; the header declares
 # exactly these.
 SIGNATURES = [
@@ -140,6 +158,9 @@ SIGNATURES = [
     ("jr_get_info", C.c_int,
      [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
       C.POINTER(C.c_int32)]),
+    ("jr_mpnn_eval", C.c_int,
+     [C.POINTER(JrMPNN), C.c_int32, C.c_int32, C.c_int32, _dp, _ip, _ip, _dp,
+      _dp, _dp, _dp, _dp]),
 ]
 
 _LIB = None

@@ -12,7 +12,7 @@ import sys
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
-SRC = PKG / "csrc" / "jr_lib.hip"
+SRCS = [PKG / "csrc" / "jr_lib.hip", PKG / "csrc" / "jr_learned.hip"]
 OUT = PKG / "libjrhip.so"
 
 HIPCC_FLAGS = [
@@ -20,7 +20,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # canonical FP64 evaluation order: only explicit fma() calls fuse
     "-ffp-contract=off",
     # no packed-FP32 instructions from the compiler
@@ -35,14 +34,38 @@ def hipcc() -> str:
     raise FileNotFoundError("hipcc not found")
 
 
-def build(verbose: bool = False) -> Path:
-    tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", str(tmp), str(SRC)]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    msgs = [ln for ln in (res.stdout + res.stderr).splitlines()
+def _messages(res) -> list[str]:
+    return [ln for ln in (res.stdout + res.stderr).splitlines()
             if "packed-fp32-ops" not in ln]
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every translation unit (in parallel) and link libjrhip.so."""
+    objdir = PKG / "csrc" / "build"
+    objdir.mkdir(exist_ok=True)
+    cc = hipcc()
+    procs = []
+    for src in SRCS:
+        obj = objdir / (src.stem + ".o")
+        cmd = [cc, *HIPCC_FLAGS, "-c", "-o", str(obj), str(src)]
+        procs.append((src, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                 stderr=subprocess.PIPE, text=True)))
+    objs, msgs, failed = [], [], []
+    for src, obj, p in procs:
+        out, err = p.communicate()
+        lines = [ln for ln in (out + err).splitlines() if "packed-fp32-ops" not in ln]
+        msgs += lines
+        if p.returncode != 0:
+            failed.append(src.name)
+        objs.append(str(obj))
+    if failed:
+        raise RuntimeError(f"hipcc failed for {failed}:\n" + "\n".join(msgs))
+    tmp = OUT.with_suffix(".so.tmp")
+    res = subprocess.run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o",
+                          str(tmp), *objs], capture_output=True, text=True)
+    msgs += _messages(res)
     if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + "\n".join(msgs))
+        raise RuntimeError("link failed:\n" + "\n".join(msgs))
     if verbose and msgs:
         print("\n".join(msgs), file=sys.stderr)
     os.replace(tmp, OUT)

@@ -9,11 +9,12 @@
 #include <string>
 #include <vector>
 
+#include "jr_host.h"
 #include "jr_kernels.h"
 
 using namespace jr;
 
-namespace {
+namespace jrh {
 
 thread_local std::string g_err;
 
@@ -27,28 +28,9 @@ int set_global_err(int code, const char* fmt, ...) {
   return code;
 }
 
-template <typename T>
-struct DBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf&) = delete;
-  DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() { release(); }
-  hipError_t alloc(size_t count) {
-    release();
-    n = count;
-    if (count == 0) return hipSuccess;
-    return hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-};
+}  // namespace jrh
 
-}  // namespace
+using namespace jrh;
 
 struct jr_handle {
   jr_config cfg;
@@ -117,13 +99,6 @@ struct jr_handle {
                        hipGetErrorString(e_), __FILE__, __LINE__);        \
   } while (0)
 
-#define HIPCHK_G(call)                                                    \
-  do {                                                                    \
-    hipError_t e_ = (call);                                               \
-    if (e_ != hipSuccess)                                                 \
-      return set_global_err(JR_E_HIP, "%s failed: %s", #call,             \
-                            hipGetErrorString(e_));                       \
-  } while (0)
 
 namespace {
 

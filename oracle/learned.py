@@ -1,0 +1,59 @@
+"""ctypes wrapper of the CPU oracle for the learned part (test
+infrastructure only): the message-passing judge in FP64."""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from oracle.oracle import _d, _i, f64, i32, lib as _base_lib
+
+_fp = C.POINTER(C.c_float)
+
+
+class OrMPNN(C.Structure):
+    _fields_ = [("S", C.c_int), ("C", C.c_int), ("NB", C.c_int), ("pad_", C.c_int),
+                ("rc", C.c_double), ("core_eps", C.c_double),
+                ("core_sig", C.c_double), ("core_rc", C.c_double),
+                ("emb", _fp), ("W1", _fp), ("U1", _fp), ("c1", _fp),
+                ("W2", _fp), ("U2", _fp), ("c2", _fp), ("wo", _fp)]
+
+
+_ready = False
+
+
+def lib():
+    global _ready
+    L = _base_lib()
+    if not _ready:
+        L.or_mpnn_energy_forces.restype = C.c_long
+        L.or_mpnn_energy_forces.argtypes = [C.POINTER(OrMPNN), C.c_int,
+                                            C.POINTER(C.c_double),
+                                            C.POINTER(C.c_int32),
+                                            C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double)]
+        _ready = True
+    return L
+
+
+def mpnn_struct(p) -> tuple[OrMPNN, list]:
+    keep = [np.ascontiguousarray(a, dtype=np.float32) for a in p.arrays()]
+    ptrs = [a.ctypes.data_as(_fp) for a in keep]
+    s = OrMPNN(p.S, p.C, p.NB, 0, p.rc, p.core_eps, p.core_sig, p.core_rc, *ptrs)
+    return s, keep
+
+
+def mpnn_energy_forces(p, pos, types, cell, forces=True):
+    """Energy (and forces) of one configuration; also the edge count."""
+    s, keep = mpnn_struct(p)
+    pos = f64(pos)
+    N = pos.shape[0]
+    ty = i32(types)
+    cell = f64(np.asarray(cell).reshape(9))
+    E = C.c_double()
+    F = np.zeros((N, 3)) if forces else None
+    ne = lib().or_mpnn_energy_forces(C.byref(s), N, _d(pos), _i(ty), _d(cell),
+                                     C.byref(E), _d(F))
+    return E.value, F, ne

@@ -43,7 +43,9 @@ def test_struct_layouts_match_c(tmp_path):
         ' printf("%zu %zu %zu %zu %zu\\n", sizeof(jr_pot), sizeof(jr_move),'
         " sizeof(jr_config), sizeof(jr_tape), sizeof(jr_iter_out));\n"
         ' printf("%zu %zu %zu\\n", offsetof(jr_config, seed), offsetof(jr_config, pot),'
-        " offsetof(jr_config, moves));\n return 0; }\n")
+        " offsetof(jr_config, moves));\n"
+        ' printf("%zu %zu %zu\\n", sizeof(jr_mpnn), offsetof(jr_mpnn, rc),'
+        " offsetof(jr_mpnn, emb));\n return 0; }\n")
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)],
                    check=True)
@@ -56,6 +58,8 @@ def test_struct_layouts_match_c(tmp_path):
                      C.sizeof(_lib.JrIterOut)]
     assert offs == [_lib.JrConfig.seed.offset, _lib.JrConfig.pot.offset,
                     _lib.JrConfig.moves.offset]
+    mp = [int(x) for x in lines[2].split()]
+    assert mp == [C.sizeof(_lib.JrMPNN), _lib.JrMPNN.rc.offset, _lib.JrMPNN.emb.offset]
 
 
 def test_missing_library_fails_loudly(tmp_path):
