@@ -237,6 +237,34 @@ int jr_mpnn_eval(const jr_mpnn* net, int32_t device, int32_t B, int32_t Nmax,
                  const int32_t* n_atoms, const double* cell,
                  const double* pressure, double* U, double* H, double* F);
 
+/* Radial-descriptor MLP surrogate (BF16 matrix cores, FP32 accumulate):
+ * per atom 32 Gaussians (centres 0.8..2.5, width 0.1, smooth cutoff 3.0) per
+ * neighbour species scaled by 1/4, plus the species one-hot, padded to 80
+ * inputs; 80 -> 128 -> 128 -> 1 with SiLU; E = sum of per-atom energies.
+ * FP32 parameter vector of JR_SUR_NPARAMS values: W1 [80][128], b1 [128],
+ * W2 [128][128], b2 [128], w3 [128], b3. */
+#define JR_SUR_INPUTS 80
+#define JR_SUR_NPARAMS 27009
+
+/* Surrogate energy (+forces) of B cubic configurations (edge >= 3.0),
+ * host buffers as jr_eval_energy_forces; x_out [B][Nmax][80] receives the
+ * BF16 descriptor rows (bit patterns) when not NULL. */
+int jr_sur_eval(const float* params, int32_t device, int32_t B, int32_t Nmax,
+                const double* pos, const int32_t* types, const int32_t* n_atoms,
+                const double* cell, int32_t n_species, double* U, double* F,
+                uint16_t* x_out);
+
+/* Rows of descriptors x [n_rows][80] (BF16 bits): per-row energies e_out
+ * (incl. b3) and, when grad_out != NULL, the gradient of sum_r c[r] e_r with
+ * respect to the FP32 parameters (deterministic order). */
+int jr_sur_grad(const float* params, int32_t device, int32_t n_rows,
+                const uint16_t* x, const float* c, float* e_out, float* grad_out);
+
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, step t >= 1) in place on
+ * host arrays of JR_SUR_NPARAMS values, computed on the device. */
+int jr_sur_adam(int32_t device, float* params, float* m, float* v,
+                const float* grad, double lr, int32_t t);
+
 #ifdef __cplusplus
 }
 #endif

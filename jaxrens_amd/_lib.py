@@ -161,6 +161,13 @@ SIGNATURES = [
     ("jr_mpnn_eval", C.c_int,
      [C.POINTER(JrMPNN), C.c_int32, C.c_int32, C.c_int32, _dp, _ip, _ip, _dp,
       _dp, _dp, _dp, _dp]),
+    ("jr_sur_eval", C.c_int,
+     [_fp, C.c_int32, C.c_int32, C.c_int32, _dp, _ip, _ip, _dp, C.c_int32, _dp, _dp,
+      C.POINTER(C.c_uint16)]),
+    ("jr_sur_grad", C.c_int,
+     [_fp, C.c_int32, C.c_int32, C.POINTER(C.c_uint16), _fp, _fp, _fp]),
+    ("jr_sur_adam", C.c_int,
+     [C.c_int32, _fp, _fp, _fp, _fp, C.c_double, C.c_int32]),
 ]
 
 _LIB = None
