@@ -77,6 +77,62 @@ typedef struct {
   double min_aspect;
 } jr_move;
 
+/* Seeded random-init invariant message-passing potential, the expensive
+ * exact judge of the surrogate-screened walk (FP32 weights and arithmetic):
+ * 8 Bessel radial functions with a polynomial envelope out to rc, two
+ * continuous-filter interaction layers of 64 channels with SiLU, per-atom
+ * linear readout, plus a 12th-power core repulsion below core_rc.  Weight
+ * layouts: emb [S][64], W1/W2 [8][64], U1/U2 [64][64] (z = m U + c),
+ * c1/c2/wo [64]. */
+typedef struct {
+  int32_t n_species; /* S, 1..2 */
+  int32_t channels;  /* must be 64 */
+  int32_t n_basis;   /* must be 8 */
+  int32_t pad_;
+  double rc, core_eps, core_sig, core_rc;
+  const float *emb, *W1, *U1, *c1, *W2, *U2, *c2, *wo;
+} jr_mpnn;
+
+/* Surrogate-screened walk of the learned model (used when jr_config.mpnn is
+ * set).  Modes:
+ *   JR_LEARNED_PLAIN     exact judge forces steer every GMC trajectory and
+ *                        the judge decides every move (no surrogate);
+ *   JR_LEARNED_AUTO      two-stage delayed acceptance while the surrogate's
+ *                        held-out per-atom RMSE is <= rmse_bound, the plain
+ *                        walk otherwise; the surrogate trains online;
+ *   JR_LEARNED_SCREENED  delayed acceptance in every iteration (the bound is
+ *                        ignored); the surrogate trains online.
+ * The surrogate's enthalpy carries an error model refitted after every
+ * iteration from fresh judge evaluations (before they are trained on):
+ * per-atom error e = a + c V/N by least squares over all replicas, then per
+ * replica the residual mean b_r and spread s_r.  With H_s = U_s - (a N + c V)
+ * + P V, GMC trajectories reflect where H_s >= Emax + N b_r, and stage 1
+ * compares the verdicts [H_s < Emax + N (b_r + delta_k s_r)] at the start
+ * and end of a proposal, rejecting when they differ; stage 2 accepts iff the
+ * judge's H < Emax.  Both verdicts are symmetric in the two states and every
+ * constant is fixed during an iteration, so the constrained prior stays
+ * exact.  rmse_bound applies to the residual per-atom RMSE. */
+#define JR_LEARNED_PLAIN 0
+#define JR_LEARNED_AUTO 1
+#define JR_LEARNED_SCREENED 2
+
+typedef struct {
+  int32_t mode;            /* JR_LEARNED_* */
+  int32_t ring_capacity;   /* exactly evaluated configurations kept */
+  int32_t train_batch;     /* configurations per Adam step */
+  int32_t train_steps;     /* Adam steps per NS iteration */
+  int32_t min_train;       /* stored configurations before training starts */
+  int32_t min_heldout;     /* fresh evaluations before the RMSE estimate counts */
+  double lr;               /* Adam step size */
+  double rmse_bound;       /* per-atom RMSE bound of JR_LEARNED_AUTO */
+  double delta_k;          /* screening slack in residual spreads */
+  double train_max_epa;    /* configurations with U/N above this are not stored */
+  /* pair baseline of the surrogate, eps ((sig/r)^12 - (sig/rc)^12) for
+   * r < rc <= 3 (eps = 0: none); the network learns the judge minus it */
+  double base_eps, base_sig, base_rc;
+  const float* sur_params; /* initial surrogate [JR_SUR_NPARAMS] */
+} jr_learned;
+
 typedef struct {
   int32_t n_replicas;    /* R */
   int32_t n_walkers;     /* K (n_live) */
@@ -94,6 +150,11 @@ typedef struct {
   jr_pot pot;
   const jr_move* moves;      /* [n_moves] */
   const double* move_weights;/* [n_moves] relative weights */
+  /* Learned model: when mpnn != NULL the message-passing judge replaces
+   * pot (NPT only; gmc and volume moves; cubic cells with edge >= rc) and
+   * learned configures the surrogate (NULL = JR_LEARNED_PLAIN). */
+  const jr_mpnn* mpnn;
+  const jr_learned* learned;
 } jr_config;
 
 /* Host-side draw tape for n_iter iterations (parity / replay).  Layouts:
@@ -213,21 +274,6 @@ int jr_get_info(jr_handle* h, int32_t* n_cu, int32_t* blocks,
  * and its ML backends (backends/, out of scope here) are not this judge.
  * ---------------------------------------------------------------------- */
 
-/* Seeded random-init invariant message-passing potential, the expensive
- * exact judge of the surrogate-screened walk (FP32 weights and arithmetic):
- * 8 Bessel radial functions with a polynomial envelope out to rc, two
- * continuous-filter interaction layers of 64 channels with SiLU, per-atom
- * linear readout, plus a 12th-power core repulsion below core_rc.  Weight
- * layouts: emb [S][64], W1/W2 [8][64], U1/U2 [64][64] (z = m U + c),
- * c1/c2/wo [64]. */
-typedef struct {
-  int32_t n_species; /* S, 1..2 */
-  int32_t channels;  /* must be 64 */
-  int32_t n_basis;   /* must be 8 */
-  int32_t pad_;
-  double rc, core_eps, core_sig, core_rc;
-  const float *emb, *W1, *U1, *c1, *W2, *U2, *c2, *wo;
-} jr_mpnn;
 
 /* Batched energy (+forces) of the judge, host buffers, shapes as
  * jr_eval_energy_forces (mu unused: NPT).  Cells must be cubic with edge
@@ -248,11 +294,13 @@ int jr_mpnn_eval(const jr_mpnn* net, int32_t device, int32_t B, int32_t Nmax,
 
 /* Surrogate energy (+forces) of B cubic configurations (edge >= 3.0),
  * host buffers as jr_eval_energy_forces; x_out [B][Nmax][80] receives the
- * BF16 descriptor rows (bit patterns) when not NULL. */
-int jr_sur_eval(const float* params, int32_t device, int32_t B, int32_t Nmax,
-                const double* pos, const int32_t* types, const int32_t* n_atoms,
-                const double* cell, int32_t n_species, double* U, double* F,
-                uint16_t* x_out);
+ * BF16 descriptor rows (bit patterns) when not NULL.  base = {eps, sig, rc}
+ * adds the pair baseline eps ((sig/r)^12 - (sig/rc)^12), r < rc <= 3, to
+ * the network's energy (NULL: none). */
+int jr_sur_eval(const float* params, const double* base, int32_t device, int32_t B,
+                int32_t Nmax, const double* pos, const int32_t* types,
+                const int32_t* n_atoms, const double* cell, int32_t n_species,
+                double* U, double* F, uint16_t* x_out);
 
 /* Rows of descriptors x [n_rows][80] (BF16 bits): per-row energies e_out
  * (incl. b3) and, when grad_out != NULL, the gradient of sum_r c[r] e_r with
@@ -264,6 +312,33 @@ int jr_sur_grad(const float* params, int32_t device, int32_t n_rows,
  * host arrays of JR_SUR_NPARAMS values, computed on the device. */
 int jr_sur_adam(int32_t device, float* params, float* m, float* v,
                 const float* grad, double lr, int32_t t);
+
+/* Counters of the learned-model walk, accumulated since the last reset. */
+typedef struct {
+  int64_t iterations_screened; /* iterations walked with delayed acceptance */
+  int64_t iterations_plain;    /* iterations walked with exact forces */
+  int64_t proposals;           /* chain moves proposed */
+  int64_t stage1_pass;         /* screened proposals passing stage 1 */
+  int64_t accepted;            /* accepted moves (both walks) */
+  int64_t exact_force_evals;   /* judge energy+force evaluations */
+  int64_t exact_energy_evals;  /* judge energy-only evaluations */
+  int64_t surrogate_evals;     /* surrogate evaluations */
+  int64_t train_steps;         /* Adam steps */
+  int64_t stored;              /* configurations in the training ring */
+  double heldout_rmse;         /* latest held-out per-atom RMSE after the
+                                  error model (inf: none) */
+  double train_loss;           /* latest minibatch loss (per-atom MSE) */
+  double walk_ms;              /* device time of the walks (HIP events) */
+  double train_ms;             /* device time of training */
+} jr_learned_stats;
+
+int jr_learned_get_stats(jr_handle* h, jr_learned_stats* out, int32_t reset);
+/* Current FP32 surrogate parameters [JR_SUR_NPARAMS]. */
+int jr_learned_get_params(jr_handle* h, float* params);
+/* Replace the surrogate (reset_adam != 0 also clears the Adam moments). */
+int jr_learned_set_params(jr_handle* h, const float* params, int32_t reset_adam);
+/* Switch JR_LEARNED_* mode between iterations. */
+int jr_learned_set_mode(jr_handle* h, int32_t mode);
 
 #ifdef __cplusplus
 }

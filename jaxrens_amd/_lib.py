@@ -55,6 +55,69 @@ class JrMove(C.Structure):
     ]
 
 
+_fp = C.POINTER(C.c_float)
+
+JR_LEARNED_PLAIN = 0
+JR_LEARNED_AUTO = 1
+JR_LEARNED_SCREENED = 2
+
+
+class JrMPNN(C.Structure):
+    _fields_ = [
+        ("n_species", C.c_int32),
+        ("channels", C.c_int32),
+        ("n_basis", C.c_int32),
+        ("pad_", C.c_int32),
+        ("rc", C.c_double),
+        ("core_eps", C.c_double),
+        ("core_sig", C.c_double),
+        ("core_rc", C.c_double),
+        ("emb", _fp), ("W1", _fp), ("U1", _fp), ("c1", _fp),
+        ("W2", _fp), ("U2", _fp), ("c2", _fp), ("wo", _fp),
+    ]
+
+
+class JrLearned(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("ring_capacity", C.c_int32),
+        ("train_batch", C.c_int32),
+        ("train_steps", C.c_int32),
+        ("min_train", C.c_int32),
+        ("min_heldout", C.c_int32),
+        ("lr", C.c_double),
+        ("rmse_bound", C.c_double),
+        ("delta_k", C.c_double),
+        ("train_max_epa", C.c_double),
+        ("base_eps", C.c_double),
+        ("base_sig", C.c_double),
+        ("base_rc", C.c_double),
+        ("sur_params", _fp),
+    ]
+
+
+class JrLearnedStats(C.Structure):
+    _fields_ = [
+        ("iterations_screened", C.c_int64),
+        ("iterations_plain", C.c_int64),
+        ("proposals", C.c_int64),
+        ("stage1_pass", C.c_int64),
+        ("accepted", C.c_int64),
+        ("exact_force_evals", C.c_int64),
+        ("exact_energy_evals", C.c_int64),
+        ("surrogate_evals", C.c_int64),
+        ("train_steps", C.c_int64),
+        ("stored", C.c_int64),
+        ("heldout_rmse", C.c_double),
+        ("train_loss", C.c_double),
+        ("walk_ms", C.c_double),
+        ("train_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class JrConfig(C.Structure):
     _fields_ = [
         ("n_replicas", C.c_int32),
@@ -73,6 +136,8 @@ class JrConfig(C.Structure):
         ("pot", JrPot),
         ("moves", C.POINTER(JrMove)),
         ("move_weights", _dp),
+        ("mpnn", C.POINTER(JrMPNN)),
+        ("learned", C.POINTER(JrLearned)),
     ]
 
 
@@ -103,23 +168,6 @@ class JrIterOut(C.Structure):
         ("walk", _ip),
     ]
 
-
-_fp = C.POINTER(C.c_float)
-
-
-class JrMPNN(C.Structure):
-    _fields_ = [
-        ("n_species", C.c_int32),
-        ("channels", C.c_int32),
-        ("n_basis", C.c_int32),
-        ("pad_", C.c_int32),
-        ("rc", C.c_double),
-        ("core_eps", C.c_double),
-        ("core_sig", C.c_double),
-        ("core_rc", C.c_double),
-        ("emb", _fp), ("W1", _fp), ("U1", _fp), ("c1", _fp),
-        ("W2", _fp), ("U2", _fp), ("c2", _fp), ("wo", _fp),
-    ]
 
 
 # (name, restype, argtypes) of every exported symbol; the header declares
@@ -162,12 +210,17 @@ SIGNATURES = [
      [C.POINTER(JrMPNN), C.c_int32, C.c_int32, C.c_int32, _dp, _ip, _ip, _dp,
       _dp, _dp, _dp, _dp]),
     ("jr_sur_eval", C.c_int,
-     [_fp, C.c_int32, C.c_int32, C.c_int32, _dp, _ip, _ip, _dp, C.c_int32, _dp, _dp,
+     [_fp, _dp, C.c_int32, C.c_int32, C.c_int32, _dp, _ip, _ip, _dp, C.c_int32, _dp, _dp,
       C.POINTER(C.c_uint16)]),
     ("jr_sur_grad", C.c_int,
      [_fp, C.c_int32, C.c_int32, C.POINTER(C.c_uint16), _fp, _fp, _fp]),
     ("jr_sur_adam", C.c_int,
      [C.c_int32, _fp, _fp, _fp, _fp, C.c_double, C.c_int32]),
+    ("jr_learned_get_stats", C.c_int,
+     [C.c_void_p, C.POINTER(JrLearnedStats), C.c_int32]),
+    ("jr_learned_get_params", C.c_int, [C.c_void_p, _fp]),
+    ("jr_learned_set_params", C.c_int, [C.c_void_p, _fp, C.c_int32]),
+    ("jr_learned_set_mode", C.c_int, [C.c_void_p, C.c_int32]),
 ]
 
 _LIB = None

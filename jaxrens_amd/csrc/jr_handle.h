@@ -1,0 +1,103 @@
+// The library handle (one population, one stream) shared by the
+// translation units of libjrhip: the pair-potential walk in jr_lib.hip and
+// the learned-model walk in jr_learned.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "jr_host.h"
+#include "jr_types.h"
+
+namespace jrl {
+struct LearnedState;  // defined in jr_learned.hip
+}
+
+struct jr_handle {
+  jr_config cfg;
+  jr_mpnn mpnn_cfg{};
+  jr_learned learned_cfg{};
+  std::string err;
+  int n_walk = 1;
+  int max_pairs = 0;
+  int A = 1;
+  int cfgbits = 0;
+  int occ = 0;  // walk-kernel occupancy variant (0 or 3)
+  hipStream_t stream = nullptr;
+  // population
+  jrh::DBuf<double> pos, dir, cell, energy;
+  jrh::DBuf<int> types, n_atoms;
+  bool have_pop = false;
+  // run params
+  jrh::DBuf<double> pressure, mu, step_sizes, emax;
+  std::vector<double> h_pressure, h_mu, h_step;
+  // potential
+  jrh::DBuf<double> eps4, eps24, sig2;
+  jrh::DBuf<jr::DevMove> moves;
+  jrh::DBuf<double> cum;  // cumulative move probabilities
+  // iteration work
+  jrh::DBuf<int> walk, counters, queue, pair_valid, order, sched;
+  jrh::DBuf<double> sel_u, clone_u, extra_u, normals, uniforms, re_u;
+  jrh::DBuf<jr::IterState> st;
+  // per-block outputs
+  int out_cap = 0;
+  jrh::DBuf<double> o_emax, o_dead_volume, o_dead_pos, o_dead_cell;
+  jrh::DBuf<int> o_worst, o_clone, o_dead_types, o_counters, o_re_acc, o_re_att,
+      o_walk;
+  // trial buffers
+  jrh::DBuf<int> t_idx, t_reason;
+  jrh::DBuf<double> t_step, t_emax, t_normals, t_uniforms;
+  long long trial_rng = 0;
+  // timing
+  std::vector<hipEvent_t> ev;
+  double walk_ms = 0.0;
+  long long walk_launches = 0;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  // launch geometry
+  int n_cu = 0;
+  int wpb = 4;
+  int blocks = 0;
+  size_t lds_walk = 0;
+  long long iter_i = 0;
+  long long rng_iter = 0;
+  // learned model (message-passing judge + surrogate); NULL for pair potentials
+  jrl::LearnedState* learned = nullptr;
+
+  int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+};
+
+#define HIPCHK(h, call)                                                   \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess)                                                 \
+      return (h)->fail(JR_E_HIP, "%s failed: %s (%s:%d)", #call,          \
+                       hipGetErrorString(e_), __FILE__, __LINE__);        \
+  } while (0)
+
+// Learned-model hooks, implemented in jr_learned.hip.
+namespace jrl {
+// Validate cfg->mpnn / cfg->learned and build the model state.
+int learned_create(jr_handle* h, const jr_config* cfg);
+void learned_destroy(jr_handle* h);
+// Checks a population for the judge (cubic cells, edge >= rc, species).
+int learned_check_population(jr_handle* h, const int32_t* types, const int32_t* n_atoms,
+                             const double* cell);
+// Exact energies H = U + P V of the whole population into h->energy.
+int learned_recompute_energies(jr_handle* h);
+// The walk of one iteration (after select, before RE), on h->stream.
+int learned_walk(jr_handle* h);
+}  // namespace jrl

@@ -6,24 +6,6 @@
 
 namespace jr {
 
-// Device-resident iteration state, read by every kernel of an iteration so a
-// captured graph can be replayed without parameter updates.
-struct IterState {
-  long long i;         // outer-loop iteration index (RE firing rule)
-  long long rng_iter;  // monotone counter feeding the Philox counter
-  int block_pos;       // slot in the per-block output buffers
-  int re_interval;
-};
-
-struct PopPtrs {
-  double* pos;
-  int* types;
-  int* n_atoms;
-  double* cell;
-  double* energy;
-  double* dir;
-};
-
 struct WalkArgs {
   PopPtrs pop;
   const double* pressure;

@@ -1,12 +1,14 @@
 // Host side of the learned part of libjrhip: the message-passing judge
-// (batched energy and forces).
+// (batched energy and forces), the surrogate (evaluation, gradient, Adam)
+// and the surrogate-screened NS walk of the learned model.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
-#include "jr_host.h"
+#include "jr_handle.h"
 #include "jr_mpnn.h"
 
 using namespace jr;
@@ -160,6 +162,7 @@ extern "C" int jr_mpnn_eval(const jr_mpnn* net, int32_t device, int32_t B, int32
   a.n_atoms = d_n.p;
   a.cell = d_cell.p;
   a.cfg_index = nullptr;
+  a.count = nullptr;
   a.E = d_U.p;
   a.F = F ? d_F.p : nullptr;
   a.nbl = d_nbl.p;
@@ -212,6 +215,20 @@ struct SurParams {
   SurDev dev() const { return SurDev{P.p, W1t.p, W1r.p, W2t.p, W2r.p}; }
 };
 
+// Baseline pair term (eps, sig, rc) into the kernel's parameters; base ==
+// NULL or eps == 0 switches it off.
+int set_baseline(SurEvalArgs& a, const double* base) {
+  a.b_eps = a.b_sig2 = a.b_rc2 = a.b_shift = 0.0f;
+  if (!base || base[0] == 0.0) return JR_OK;
+  if (!(base[1] > 0.0) || !(base[2] > 0.0) || base[2] > (double)SG_RD)
+    return set_global_err(JR_E_ARG, "baseline needs sig > 0 and 0 < rc <= %g", (double)SG_RD);
+  a.b_eps = (float)base[0];
+  a.b_sig2 = (float)(base[1] * base[1]);
+  a.b_rc2 = (float)(base[2] * base[2]);
+  a.b_shift = (float)std::pow(base[1] / base[2], 12.0);
+  return JR_OK;
+}
+
 hipError_t launch_sur_eval(const SurEvalArgs& a, int grid, hipStream_t s) {
   hipError_t e = hipFuncSetAttribute((const void*)sur_eval_kernel,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSgLds);
@@ -229,10 +246,10 @@ struct StreamHolder {
 
 }  // namespace
 
-extern "C" int jr_sur_eval(const float* params, int32_t device, int32_t B, int32_t Nmax,
-                           const double* pos, const int32_t* types,
-                           const int32_t* n_atoms, const double* cell, int32_t n_species,
-                           double* U, double* F, uint16_t* x_out) {
+extern "C" int jr_sur_eval(const float* params, const double* base, int32_t device,
+                           int32_t B, int32_t Nmax, const double* pos,
+                           const int32_t* types, const int32_t* n_atoms, const double* cell,
+                           int32_t n_species, double* U, double* F, uint16_t* x_out) {
   if (!params || !pos || !types || !n_atoms || !cell)
     return set_global_err(JR_E_ARG, "params/pos/types/n_atoms/cell must be set");
   if (B < 0 || Nmax < 1 || Nmax > 256)
@@ -286,9 +303,13 @@ extern "C" int jr_sur_eval(const float* params, int32_t device, int32_t B, int32
   a.n_atoms = d_n.p;
   a.cell = d_cell.p;
   a.cfg_index = nullptr;
+  a.count = nullptr;
   a.E = d_U.p;
   a.F = F ? d_F.p : nullptr;
   a.xout = x_out ? d_x.p : nullptr;
+  a.Ebase = nullptr;
+  int brc = set_baseline(a, base);
+  if (brc) return brc;
   HIPCHK_G(launch_sur_eval(a, std::min(B, device_cus(device)), s));
   if (U) HIPCHK_G(hipMemcpyAsync(U, d_U.p, sizeof(double) * B, hipMemcpyDeviceToHost, s));
   if (F)
@@ -371,5 +392,725 @@ extern "C" int jr_sur_adam(int32_t device, float* params, float* m, float* v,
   HIPCHK_G(hipMemcpyAsync(m, d_m.p, sizeof(float) * SG_NP, hipMemcpyDeviceToHost, s));
   HIPCHK_G(hipMemcpyAsync(v, d_v.p, sizeof(float) * SG_NP, hipMemcpyDeviceToHost, s));
   HIPCHK_G(hipStreamSynchronize(s));
+  return JR_OK;
+}
+
+// ------------------------------------------------------- learned-model walk
+#include "jr_lwalk.h"
+
+namespace jrl {
+
+struct LearnedState {
+  jr_learned cfg{};
+  int mode = JR_LEARNED_PLAIN;
+  std::vector<jr_move> moves;
+  MPNNWeights judge;
+  SurParams sur;
+  DBuf<float> adam_m, adam_v;
+  long long adam_t = 0;
+  long long train_counter = 0;
+  int C = 0, Nmax = 0, n_cu = 0;
+  // chain work arrays
+  DBuf<double> cpos, cdir, x0, d0, npos, F, ccell, ncell, cH, cHs, cem, cpres, thr_steer,
+      thr_screen, U, Us, Ub, Hy, Hsy;
+  DBuf<int> ctypes, cn, code, code2, accept, reason, idx, idx2, cnt;
+  DBuf<__bf16> xstage;
+  DBuf<unsigned> nbl;
+  DBuf<int> overflow;
+  // training ring and minibatch
+  DBuf<__bf16> ring;
+  DBuf<double> ringE;
+  DBuf<int> ringN;
+  DBuf<RingState> rs;
+  DBuf<int> tidx, tn;
+  DBuf<__bf16> tX;
+  DBuf<double> tE, tsq;
+  DBuf<float> te, tc, gpart, grad;
+  // statistics
+  DBuf<LWStats> dstats;
+  DBuf<double> hs;            // [R][LW_HS] held-out sums of the running iteration
+  DBuf<double> dbias, dsig;   // [R] residual bias / spread per atom, on the device
+  std::vector<double> bias, sig;
+  double ca = 0.0, cv = 0.0;  // fitted error model e = ca + cv V / N
+  jr_learned_stats acc{};
+  double rmse = INFINITY;     // held-out per-atom RMSE after the error model
+  long long rmse_n = 0;
+  long long stored = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<int> sched;
+
+  ~LearnedState() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+unsigned blocks_for(size_t n, unsigned per = 256) {
+  return (unsigned)std::max<size_t>(1, (n + per - 1) / per);
+}
+
+hipError_t zero_stats(LearnedState* L, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(L->dstats.p, 0, sizeof(LWStats), s);
+  if (e != hipSuccess) return e;
+  e = hipMemsetAsync(L->hs.p, 0, sizeof(double) * L->hs.n, s);
+  if (e != hipSuccess) return e;
+  return hipMemsetAsync(L->overflow.p, 0, sizeof(int), s);
+}
+
+MPNNArgs judge_args(LearnedState* L, const double* pos, const double* cell, int B,
+                    const int* types, const int* n, const int* idx, const int* count,
+                    double* E, double* F) {
+  MPNNArgs a;
+  a.net = L->judge.dev;
+  a.B = B;
+  a.Nmax = L->Nmax;
+  a.pos = pos;
+  a.types = types;
+  a.n_atoms = n;
+  a.cell = cell;
+  a.cfg_index = idx;
+  a.count = count;
+  a.E = E;
+  a.F = F;
+  a.nbl = L->nbl.p;
+  a.overflow = L->overflow.p;
+  return a;
+}
+
+hipError_t run_judge(LearnedState* L, const MPNNArgs& a, hipStream_t s) {
+  return launch_mpnn(a, L->judge.buf.p, std::max(1, std::min(a.B, L->n_cu)), s);
+}
+
+hipError_t run_sur(LearnedState* L, const double* pos, const double* cell, const int* idx,
+                   const int* count, double* E, double* F, __bf16* xout, hipStream_t s) {
+  SurEvalArgs a;
+  const double base[3] = {L->cfg.base_eps, L->cfg.base_sig, L->cfg.base_rc};
+  (void)set_baseline(a, base);  // validated at creation
+  a.Ebase = xout ? L->Ub.p : nullptr;  // training targets exclude the baseline
+  a.net = L->sur.dev();
+  a.B = L->C;
+  a.Nmax = L->Nmax;
+  a.S = L->judge.dev.S;
+  a.pos = pos;
+  a.types = L->ctypes.p;
+  a.n_atoms = L->cn.p;
+  a.cell = cell;
+  a.cfg_index = idx;
+  a.count = count;
+  a.E = E;
+  a.F = F;
+  a.xout = xout;
+  return launch_sur_eval(a, std::max(1, std::min(L->C, L->n_cu)), s);
+}
+
+}  // namespace
+
+int learned_create(jr_handle* h, const jr_config* c) {
+  const jr_mpnn* net = c->mpnn;
+  int rc = validate_mpnn(net);
+  if (rc) {
+    h->err = g_err;
+    return rc;
+  }
+  if (c->n_atoms_max > MP_THREADS)
+    return h->fail(JR_E_UNSUPPORTED, "the judge takes at most %d atoms", MP_THREADS);
+  for (int m = 0; m < c->n_moves; ++m) {
+    const int t = c->moves[m].type;
+    if (t != JR_MOVE_GMC && t != JR_MOVE_VOLUME)
+      return h->fail(JR_E_UNSUPPORTED,
+                     "move %d: the learned-model walk runs gmc and volume moves", m);
+  }
+  jr_learned lc{};
+  lc.mode = JR_LEARNED_PLAIN;
+  if (c->learned) lc = *c->learned;
+  if (lc.mode < JR_LEARNED_PLAIN || lc.mode > JR_LEARNED_SCREENED)
+    return h->fail(JR_E_ARG, "learned.mode %d unknown", lc.mode);
+  if (lc.mode != JR_LEARNED_PLAIN) {
+    if (!lc.sur_params) return h->fail(JR_E_ARG, "learned.sur_params must be set");
+    if (lc.ring_capacity < 1 || lc.train_batch < 1 || lc.train_steps < 0 || lc.min_train < 1 ||
+        lc.min_heldout < 1 || !(lc.lr > 0.0) || !(lc.rmse_bound >= 0.0) ||
+        !(lc.delta_k >= 0.0) || !(lc.train_max_epa > 0.0))
+      return h->fail(JR_E_ARG, "learned: invalid training settings");
+    if (lc.base_eps != 0.0 &&
+        (!(lc.base_sig > 0.0) || !(lc.base_rc > 0.0) || lc.base_rc > (double)SG_RD))
+      return h->fail(JR_E_ARG, "learned: baseline needs sig > 0 and 0 < rc <= %g",
+                     (double)SG_RD);
+  }
+  auto* L = new LearnedState();
+  h->learned = L;
+  L->cfg = lc;
+  L->cfg.sur_params = nullptr;
+  L->mode = lc.mode;
+  L->moves.assign(c->moves, c->moves + c->n_moves);
+  L->Nmax = c->n_atoms_max;
+  L->C = c->n_replicas * (1 + c->n_extra);
+  L->n_cu = device_cus(c->device);
+  L->sched.assign(c->n_mcmc_steps, 0);
+  h->mpnn_cfg = *net;
+  h->mpnn_cfg.emb = h->mpnn_cfg.W1 = h->mpnn_cfg.U1 = h->mpnn_cfg.c1 = nullptr;
+  h->mpnn_cfg.W2 = h->mpnn_cfg.U2 = h->mpnn_cfg.c2 = h->mpnn_cfg.wo = nullptr;
+  h->learned_cfg = L->cfg;
+  hipStream_t s = h->stream;
+  const size_t C = L->C, Nm = L->Nmax, n3 = C * Nm * 3;
+  HIPCHK(h, L->judge.upload(net, s));
+  for (DBuf<double>* b : {&L->cpos, &L->cdir, &L->x0, &L->d0, &L->npos, &L->F})
+    HIPCHK(h, b->alloc(n3));
+  HIPCHK(h, L->ccell.alloc(C * 9));
+  HIPCHK(h, L->ncell.alloc(C * 9));
+  for (DBuf<double>* b : {&L->cH, &L->cHs, &L->cem, &L->cpres, &L->thr_steer, &L->thr_screen,
+                          &L->U, &L->Us, &L->Ub, &L->Hy, &L->Hsy})
+    HIPCHK(h, b->alloc(C));
+  const int R = c->n_replicas;
+  HIPCHK(h, L->hs.alloc((size_t)R * LW_HS));
+  HIPCHK(h, L->dbias.alloc(R));
+  HIPCHK(h, L->dsig.alloc(R));
+  L->bias.assign(R, 0.0);
+  L->sig.assign(R, 0.0);
+  HIPCHK(h, L->ctypes.alloc(C * Nm));
+  for (DBuf<int>* b : {&L->cn, &L->code, &L->code2, &L->accept, &L->reason, &L->idx, &L->idx2})
+    HIPCHK(h, b->alloc(C));
+  HIPCHK(h, L->cnt.alloc(4));
+  // judge launches use at most n_cu persistent blocks, each with its own list
+  HIPCHK(h, L->nbl.alloc((size_t)L->n_cu * MP_MAXNB * MP_THREADS));
+  HIPCHK(h, L->overflow.alloc(1));
+  HIPCHK(h, L->dstats.alloc(1));
+  HIPCHK(h, L->rs.alloc(1));
+  for (auto& e : L->ev) HIPCHK(h, hipEventCreate(&e));
+  if (L->mode != JR_LEARNED_PLAIN) {
+    HIPCHK(h, L->sur.alloc());
+    HIPCHK(h, hipMemcpyAsync(L->sur.P.p, c->learned->sur_params, sizeof(float) * SG_NP,
+                             hipMemcpyHostToDevice, s));
+    L->sur.cast(s);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, L->adam_m.alloc(SG_NP));
+    HIPCHK(h, L->adam_v.alloc(SG_NP));
+    HIPCHK(h, hipMemsetAsync(L->adam_m.p, 0, sizeof(float) * SG_NP, s));
+    HIPCHK(h, hipMemsetAsync(L->adam_v.p, 0, sizeof(float) * SG_NP, s));
+    HIPCHK(h, L->xstage.alloc(C * Nm * SG_D));
+    HIPCHK(h, hipMemsetAsync(L->xstage.p, 0, sizeof(__bf16) * C * Nm * SG_D, s));
+    const size_t cap = (size_t)lc.ring_capacity;
+    HIPCHK(h, L->ring.alloc(cap * Nm * SG_D));
+    HIPCHK(h, L->ringE.alloc(cap));
+    HIPCHK(h, L->ringN.alloc(cap));
+    const size_t Bt = (size_t)lc.train_batch, rows = Bt * Nm;
+    HIPCHK(h, L->tidx.alloc(Bt));
+    HIPCHK(h, L->tn.alloc(Bt));
+    HIPCHK(h, L->tE.alloc(Bt));
+    HIPCHK(h, L->tsq.alloc(Bt));
+    HIPCHK(h, L->tX.alloc(rows * SG_D));
+    HIPCHK(h, L->te.alloc(rows));
+    HIPCHK(h, L->tc.alloc(rows));
+    HIPCHK(h, L->gpart.alloc(((rows + 63) / 64) * SG_NPP));
+    HIPCHK(h, L->grad.alloc(SG_NP));
+  }
+  RingState r0{0, 0, std::max<long long>(1, lc.ring_capacity)};
+  HIPCHK(h, hipMemcpyAsync(L->rs.p, &r0, sizeof(r0), hipMemcpyHostToDevice, s));
+  HIPCHK(h, zero_stats(L, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  return JR_OK;
+}
+
+void learned_destroy(jr_handle* h) {
+  delete h->learned;
+  h->learned = nullptr;
+}
+
+int learned_check_population(jr_handle* h, const int32_t* types, const int32_t* n_atoms,
+                             const double* cell) {
+  const auto& c = h->cfg;
+  const size_t W = (size_t)c.n_replicas * c.n_walkers;
+  const double rc = h->mpnn_cfg.rc;
+  for (size_t w = 0; w < W; ++w) {
+    if (!cubic_ok(cell + w * 9, rc))
+      return h->fail(JR_E_UNSUPPORTED,
+                     "walker %zu: the message-passing judge needs a cubic cell with edge >= "
+                     "rc = %g", w, rc);
+    for (int i = 0; i < n_atoms[w]; ++i)
+      if (types[w * c.n_atoms_max + i] >= h->mpnn_cfg.n_species)
+        return h->fail(JR_E_ARG, "species out of range for the judge");
+    for (const jr_move& mv : h->learned->moves) {
+      if (mv.type != JR_MOVE_VOLUME) continue;
+      // every volume that passes the cell check keeps the edge >= rc, and the
+      // densest allowed cell keeps the judge's neighbour lists in capacity
+      if (mv.min_vpa * n_atoms[w] < rc * rc * rc)
+        return h->fail(JR_E_UNSUPPORTED,
+                       "volume move: min_vpa * N = %g must be >= rc^3 = %g for the judge",
+                       mv.min_vpa * n_atoms[w], rc * rc * rc);
+      if (4.18879020478639 * rc * rc * rc / mv.min_vpa > 0.9 * MP_MAXNB)
+        return h->fail(JR_E_UNSUPPORTED,
+                       "volume move: min_vpa %g allows densities beyond the judge's "
+                       "neighbour capacity (%d)", mv.min_vpa, MP_MAXNB);
+    }
+  }
+  return JR_OK;
+}
+
+int learned_recompute_energies(jr_handle* h) {
+  LearnedState* L = h->learned;
+  const auto& c = h->cfg;
+  const int W = c.n_replicas * c.n_walkers;
+  hipStream_t s = h->stream;
+  std::vector<double> pr(W);
+  for (int w = 0; w < W; ++w) pr[w] = h->h_pressure[w / c.n_walkers];
+  DBuf<double> dU, dP;
+  HIPCHK(h, dU.alloc(W));
+  HIPCHK(h, dP.alloc(W));
+  HIPCHK(h, hipMemcpyAsync(dP.p, pr.data(), sizeof(double) * W, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemsetAsync(L->overflow.p, 0, sizeof(int), s));
+  MPNNArgs a = judge_args(L, h->pos.p, h->cell.p, W, h->types.p, h->n_atoms.p, nullptr,
+                          nullptr, dU.p, nullptr);
+  HIPCHK(h, run_judge(L, a, s));
+  hipLaunchKernelGGL(lw_ham_kernel, dim3(blocks_for(W)), dim3(256), 0, s, W, nullptr, nullptr,
+                     dU.p, h->cell.p, dP.p, nullptr, 0.0, 0.0, h->energy.p);
+  HIPCHK(h, hipGetLastError());
+  int of = 0;
+  HIPCHK(h, hipMemcpyAsync(&of, L->overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  if (of) return h->fail(JR_E_UNSUPPORTED, "judge neighbour list overflow (> %d)", MP_MAXNB);
+  return JR_OK;
+}
+
+namespace {
+
+struct WalkCtx {
+  jr_handle* h;
+  LearnedState* L;
+  hipStream_t s;
+  LWChains ch;
+  int C, Nm, n_steps;
+  bool use_sur;
+  double cap;
+
+  // H of the judge's energies U (all chains, or the listed ones)
+  void ham_exact(const int* idx, const int* cnt, const double* U, const double* cell,
+                 double* H) const {
+    hipLaunchKernelGGL(lw_ham_kernel, dim3(blocks_for(C)), dim3(256), 0, s, C, idx, cnt, U,
+                       cell, L->cpres.p, nullptr, 0.0, 0.0, H);
+  }
+  // corrected H of the surrogate's energies
+  void ham_sur(const int* idx, const int* cnt, const double* Us, const double* cell,
+               double* H) const {
+    hipLaunchKernelGGL(lw_ham_kernel, dim3(blocks_for(C)), dim3(256), 0, s, C, idx, cnt, Us,
+                       cell, L->cpres.p, L->cn.p, L->ca, L->cv, H);
+  }
+};
+
+// Held-out error and ring append over the chains listed in (idx2, cnt[1]);
+// cell holds the evaluated configurations' cells.
+void record_exact(WalkCtx& w, const int* code, const double* cell) {
+  LearnedState* L = w.L;
+  hipStream_t s = w.s;
+  hipLaunchKernelGGL(lw_compact_kernel, dim3(1), dim3(1024), 0, s, w.C, code, 1, 1, L->U.p,
+                     L->cn.p, w.cap, L->idx2.p, L->cnt.p + 1);
+  hipLaunchKernelGGL(lw_heldout_kernel, dim3(w.h->cfg.n_replicas), dim3(256), 0, s, L->idx2.p,
+                     L->cnt.p + 1, w.h->n_walk, L->U.p, L->Us.p, L->cn.p, cell, L->hs.p);
+  hipLaunchKernelGGL(lw_ring_append_kernel, dim3(w.C), dim3(256), 0, s, L->idx2.p,
+                     L->cnt.p + 1, w.Nm, L->xstage.p, L->U.p, L->Ub.p, L->cn.p, L->rs.p, L->ring.p,
+                     L->ringE.p, L->ringN.p);
+  hipLaunchKernelGGL(lw_ring_commit_kernel, dim3(1), dim3(64), 0, s, L->cnt.p + 1, L->rs.p);
+}
+
+void count_step(WalkCtx& w, int m, const int* code, int eval_const, int eval_code1,
+                int grad_const, int screened) {
+  LWCountArgs a;
+  a.C = w.C;
+  a.n_walk = w.h->n_walk;
+  a.n_moves = w.h->cfg.n_moves;
+  a.m = m;
+  a.reason = w.L->reason.p;
+  a.code = code;
+  a.eval_const = eval_const;
+  a.eval_code1 = eval_code1;
+  a.grad_const = grad_const;
+  a.counters = w.h->counters.p;
+  a.stats = w.L->dstats.p;
+  a.screened = screened;
+  hipLaunchKernelGGL(lw_count_kernel, dim3(blocks_for(w.C)), dim3(256), 0, w.s, a);
+}
+
+void stat_add(WalkCtx& w, unsigned long long* dst, const int* count, long long n,
+              long long mult) {
+  hipLaunchKernelGGL(lw_stat_add_kernel, dim3(1), dim3(64), 0, w.s, dst, count, n, mult);
+}
+
+hipError_t gmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
+  LearnedState* L = w.L;
+  jr_handle* h = w.h;
+  hipStream_t s = w.s;
+  const int C = w.C, Nm = w.Nm;
+  const size_t xs = (size_t)w.n_steps * 3 * Nm;
+  hipLaunchKernelGGL(lw_gmc_begin_kernel, dim3(blocks_for(C, 4)), dim3(256), 0, s, w.ch,
+                     L->cdir.p, h->normals.p + (size_t)t * 3 * Nm, xs, mv.perturb_angle,
+                     L->cpos.p, L->x0.p, L->d0.p);
+  const unsigned gc = blocks_for((size_t)C * Nm * 3);
+  const int nr = mv.n_reflect;
+  hipError_t e;
+  for (int k = 0; k < nr; ++k) {
+    const bool last = k == nr - 1;
+    hipLaunchKernelGGL(lw_advance_kernel, dim3(gc), dim3(256), 0, s, w.ch, L->cpos.p,
+                       L->cdir.p, h->step_sizes.p, m);
+    if (da) {
+      e = run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, L->F.p,
+                  last ? L->xstage.p : nullptr, s);
+      if (e != hipSuccess) return e;
+      w.ham_sur(nullptr, nullptr, L->Us.p, L->ccell.p, L->Hsy.p);
+      if (!last)
+        hipLaunchKernelGGL(lw_reflect_kernel, dim3(blocks_for(C, 4)), dim3(256), 0, s, w.ch,
+                           L->cdir.p, L->F.p, L->Hsy.p, L->thr_steer.p);
+    } else {
+      e = run_judge(L, judge_args(L, L->cpos.p, L->ccell.p, C, L->ctypes.p, L->cn.p, nullptr,
+                                  nullptr, L->U.p, L->F.p),
+                    s);
+      if (e != hipSuccess) return e;
+      w.ham_exact(nullptr, nullptr, L->U.p, L->ccell.p, L->Hy.p);
+      if (!last)
+        hipLaunchKernelGGL(lw_reflect_kernel, dim3(blocks_for(C, 4)), dim3(256), 0, s, w.ch,
+                           L->cdir.p, L->F.p, L->Hy.p, L->cem.p);
+    }
+  }
+  const unsigned gb = blocks_for(C);
+  if (da) {
+    // stage 1 on the surrogate's verdicts, stage 2 on the judge's energy
+    hipLaunchKernelGGL(lw_stage1_kernel, dim3(gb), dim3(256), 0, s, C, nullptr, L->cHs.p,
+                       L->Hsy.p, L->thr_screen.p, L->code.p);
+    hipLaunchKernelGGL(lw_compact_kernel, dim3(1), dim3(1024), 0, s, C, L->code.p, 1, 0,
+                       nullptr, nullptr, 0.0, L->idx.p, L->cnt.p);
+    e = run_judge(L, judge_args(L, L->cpos.p, L->ccell.p, C, L->ctypes.p, L->cn.p, L->idx.p,
+                                L->cnt.p, L->U.p, nullptr),
+                  s);
+    if (e != hipSuccess) return e;
+    w.ham_exact(L->idx.p, L->cnt.p, L->U.p, L->ccell.p, L->Hy.p);
+    hipLaunchKernelGGL(lw_accept_kernel, dim3(gb), dim3(256), 0, s, C, L->code.p, L->Hy.p,
+                       L->cem.p, L->accept.p, L->reason.p, L->cH.p, L->cHs.p, L->Hsy.p);
+    record_exact(w, L->code.p, L->ccell.p);
+    count_step(w, m, L->code.p, 0, 1, 0, 1);
+    stat_add(w, &L->dstats.p->exact_e, L->cnt.p, 0, 1);
+    stat_add(w, &L->dstats.p->sur_e, nullptr, C, nr);
+  } else {
+    hipLaunchKernelGGL(lw_accept_kernel, dim3(gb), dim3(256), 0, s, C, nullptr, L->Hy.p,
+                       L->cem.p, L->accept.p, L->reason.p, L->cH.p, nullptr, nullptr);
+    if (w.use_sur) {
+      // the surrogate's prediction at the judged end point: held-out error
+      e = run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, L->xstage.p, s);
+      if (e != hipSuccess) return e;
+      record_exact(w, nullptr, L->ccell.p);
+      stat_add(w, &L->dstats.p->sur_e, nullptr, C, 1);
+    }
+    count_step(w, m, nullptr, nr, 0, nr, 0);
+    stat_add(w, &L->dstats.p->exact_fe, nullptr, C, nr);
+  }
+  hipLaunchKernelGGL(lw_gmc_finish_kernel, dim3(gc), dim3(256), 0, s, w.ch, L->cpos.p,
+                     L->cdir.p, L->x0.p, L->d0.p, L->accept.p);
+  return hipGetLastError();
+}
+
+hipError_t volume_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
+  LearnedState* L = w.L;
+  jr_handle* h = w.h;
+  hipStream_t s = w.s;
+  const int C = w.C, Nm = w.Nm;
+  LWVolArgs va;
+  va.ch = w.ch;
+  va.pos = L->cpos.p;
+  va.cell = L->ccell.p;
+  va.step_sizes = h->step_sizes.p;
+  va.m = m;
+  va.xi = h->normals.p + (size_t)t * 3 * Nm;
+  va.xi_stride = (size_t)w.n_steps * 3 * Nm;
+  va.u = h->uniforms.p + (size_t)t * 4;
+  va.u_stride = (size_t)w.n_steps * 4;
+  va.npos = L->npos.p;
+  va.ncell = L->ncell.p;
+  va.code = L->code.p;
+  va.max_vpa = mv.max_vpa;
+  va.min_vpa = mv.min_vpa;
+  va.min_aspect = mv.min_aspect;
+  va.flat = mv.flat_v_prior;
+  const unsigned gb = blocks_for(C);
+  hipLaunchKernelGGL(lw_volume_propose_kernel, dim3(blocks_for(C, 4)), dim3(256), 0, s, va);
+  hipLaunchKernelGGL(lw_compact_kernel, dim3(1), dim3(1024), 0, s, C, L->code.p, 1, 0, nullptr,
+                     nullptr, 0.0, L->idx.p, L->cnt.p);
+  hipError_t e;
+  const int* fin = L->code.p;  // codes of the final test
+  if (da || w.use_sur) {
+    e = run_sur(L, L->npos.p, L->ncell.p, L->idx.p, L->cnt.p, L->Us.p, nullptr, L->xstage.p, s);
+    if (e != hipSuccess) return e;
+    stat_add(w, &L->dstats.p->sur_e, L->cnt.p, 0, 1);
+  }
+  if (da) {
+    w.ham_sur(L->idx.p, L->cnt.p, L->Us.p, L->ncell.p, L->Hsy.p);
+    hipLaunchKernelGGL(lw_stage1_kernel, dim3(gb), dim3(256), 0, s, C, L->code.p, L->cHs.p,
+                       L->Hsy.p, L->thr_screen.p, L->code2.p);
+    hipLaunchKernelGGL(lw_compact_kernel, dim3(1), dim3(1024), 0, s, C, L->code2.p, 1, 0,
+                       nullptr, nullptr, 0.0, L->idx.p, L->cnt.p);
+    fin = L->code2.p;
+  }
+  e = run_judge(L, judge_args(L, L->npos.p, L->ncell.p, C, L->ctypes.p, L->cn.p, L->idx.p,
+                              L->cnt.p, L->U.p, nullptr),
+                s);
+  if (e != hipSuccess) return e;
+  w.ham_exact(L->idx.p, L->cnt.p, L->U.p, L->ncell.p, L->Hy.p);
+  hipLaunchKernelGGL(lw_accept_kernel, dim3(gb), dim3(256), 0, s, C, fin, L->Hy.p, L->cem.p,
+                     L->accept.p, L->reason.p, L->cH.p, da ? L->cHs.p : nullptr,
+                     da ? L->Hsy.p : nullptr);
+  stat_add(w, &L->dstats.p->exact_e, L->cnt.p, 0, 1);
+  if (w.use_sur) record_exact(w, fin, L->ncell.p);
+  count_step(w, m, fin, 0, 1, 0, da ? 1 : 0);
+  hipLaunchKernelGGL(lw_volume_commit_kernel, dim3(blocks_for((size_t)C * Nm * 3)), dim3(256),
+                     0, s, w.ch, L->cpos.p, L->ccell.p, L->npos.p, L->ncell.p, L->accept.p);
+  return hipGetLastError();
+}
+
+// Adam steps on minibatches drawn from the ring (all on the stream).
+hipError_t train(jr_handle* h, LearnedState* L, hipStream_t s) {
+  const int Bt = L->cfg.train_batch, Nm = L->Nmax;
+  const int rows = Bt * Nm, chunks = (rows + 63) / 64;
+  SurRowsArgs ra;
+  ra.net = L->sur.dev();
+  ra.n_rows = rows;
+  ra.x = L->tX.p;
+  ra.e = L->te.p;
+  ra.c = L->tc.p;
+  ra.gpart = L->gpart.p;
+  for (int k = 0; k < L->cfg.train_steps; ++k) {
+    hipLaunchKernelGGL(lw_train_draw_kernel, dim3(blocks_for(Bt)), dim3(256), 0, s, Bt, L->rs.p,
+                       (unsigned long long)h->cfg.seed, L->train_counter++, L->tidx.p);
+    hipLaunchKernelGGL(lw_train_gather_kernel, dim3(blocks_for((size_t)rows * SG_D)), dim3(256),
+                       0, s, Bt, Nm, L->tidx.p, L->ring.p, L->ringE.p, L->ringN.p, L->tX.p,
+                       L->tE.p, L->tn.p);
+    hipLaunchKernelGGL(sur_rows_forward_kernel, dim3(std::min(chunks, 4 * L->n_cu)), dim3(64), 0,
+                       s, ra);
+    hipLaunchKernelGGL(lw_loss_weights_kernel, dim3(Bt), dim3(256), 0, s, Bt, Nm, L->te.p,
+                       L->tE.p, L->tn.p, L->tc.p, L->tsq.p);
+    hipLaunchKernelGGL(sur_rows_backward_kernel, dim3(chunks), dim3(64), 0, s, ra);
+    hipLaunchKernelGGL(sur_reduce_kernel, dim3((SG_NP + 255) / 256), dim3(256), 0, s, L->gpart.p,
+                       chunks, L->grad.p);
+    L->adam_t += 1;
+    const float c1 = (float)(1.0 - std::pow(0.9, (double)L->adam_t));
+    const float c2 = (float)(1.0 - std::pow(0.999, (double)L->adam_t));
+    hipLaunchKernelGGL(sur_adam_kernel, dim3((SG_NP + 255) / 256), dim3(256), 0, s, L->sur.P.p,
+                       L->adam_m.p, L->adam_v.p, L->grad.p, (float)L->cfg.lr, c1, c2);
+    L->sur.cast(s);
+  }
+  if (L->cfg.train_steps > 0)
+    hipLaunchKernelGGL(lw_loss_sum_kernel, dim3(1), dim3(64), 0, s, Bt, L->tsq.p, L->dstats.p);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// Error model of the surrogate from one iteration's held-out sums (per-atom
+// units e = (U_s - U)/N, v = V/N): e = ca + cv v fitted over all replicas by
+// least squares, then per replica the residuals' mean (bias) and spread.
+// Needs min_heldout evaluations; otherwise the previous model stays.
+static void fit_error_model(LearnedState* L, const std::vector<double>& hs) {
+  const int R = (int)L->bias.size();
+  double t[LW_HS] = {0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < R; ++r)
+    for (int q = 0; q < LW_HS; ++q) t[q] += hs[(size_t)r * LW_HS + q];
+  const double n = t[0];
+  if (n < std::max(1, L->cfg.min_heldout)) return;
+  const double mv = t[3] / n, me = t[1] / n;
+  const double var_v = t[4] / n - mv * mv, cov = t[5] / n - me * mv;
+  const double cv = var_v > 1e-12 * std::max(1.0, mv * mv) ? cov / var_v : 0.0;
+  const double ca = me - cv * mv;
+  auto resid = [&](const double* h, double& s1, double& s2) {
+    s1 = h[1] - ca * h[0] - cv * h[3];
+    s2 = h[2] + ca * ca * h[0] + cv * cv * h[4] - 2.0 * ca * h[1] - 2.0 * cv * h[5] +
+         2.0 * ca * cv * h[3];
+  };
+  double g1, g2;
+  resid(t, g1, g2);
+  const double gb = g1 / n, gs = std::sqrt(std::max(g2 / n - gb * gb, 0.0));
+  for (int r = 0; r < R; ++r) {
+    const double* h = &hs[(size_t)r * LW_HS];
+    if (h[0] >= 2.0) {
+      double s1, s2;
+      resid(h, s1, s2);
+      L->bias[r] = s1 / h[0];
+      L->sig[r] = std::sqrt(std::max(s2 / h[0] - L->bias[r] * L->bias[r], 0.0));
+    } else {
+      L->bias[r] = gb;
+      L->sig[r] = gs;
+    }
+  }
+  L->ca = ca;
+  L->cv = cv;
+  L->rmse = std::sqrt(std::max(g2, 0.0) / n);
+  L->rmse_n = (long long)n;
+}
+
+int learned_walk(jr_handle* h) {
+  LearnedState* L = h->learned;
+  const auto& c = h->cfg;
+  hipStream_t s = h->stream;
+  WalkCtx w;
+  w.h = h;
+  w.L = L;
+  w.s = s;
+  w.C = L->C;
+  w.Nm = L->Nmax;
+  w.n_steps = c.n_mcmc_steps;
+  w.use_sur = L->mode != JR_LEARNED_PLAIN;
+  w.cap = L->cfg.train_max_epa;
+  w.ch = LWChains{L->C, h->n_walk, L->Nmax, c.n_moves, L->cn.p};
+  // the shared move schedule decides which launches follow
+  HIPCHK(h, hipMemcpyAsync(L->sched.data(), h->sched.p, sizeof(int) * c.n_mcmc_steps,
+                           hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  const bool da = L->mode == JR_LEARNED_SCREENED ||
+                  (L->mode == JR_LEARNED_AUTO && L->rmse <= L->cfg.rmse_bound &&
+                   L->rmse_n >= L->cfg.min_heldout);
+  HIPCHK(h, hipEventRecord(L->ev[0], s));
+  LWGatherArgs ga;
+  ga.C = L->C;
+  ga.n_walk = h->n_walk;
+  ga.K = c.n_walkers;
+  ga.Nmax = L->Nmax;
+  ga.walk = h->walk.p;
+  ga.pop = PopPtrs{h->pos.p, h->types.p, h->n_atoms.p, h->cell.p, h->energy.p, h->dir.p};
+  ga.emax = h->emax.p;
+  ga.pressure = h->pressure.p;
+  ga.cpos = L->cpos.p;
+  ga.cdir = L->cdir.p;
+  ga.ccell = L->ccell.p;
+  ga.cH = L->cH.p;
+  ga.cem = L->cem.p;
+  ga.cpres = L->cpres.p;
+  ga.ctypes = L->ctypes.p;
+  ga.cn = L->cn.p;
+  hipLaunchKernelGGL(lw_gather_kernel, dim3(L->C), dim3(256), 0, s, ga);
+  if (da) {
+    // contours of this iteration from the error model of the last one
+    HIPCHK(h, hipMemcpyAsync(L->dbias.p, L->bias.data(), sizeof(double) * L->bias.size(),
+                             hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(L->dsig.p, L->sig.data(), sizeof(double) * L->sig.size(),
+                             hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(lw_thr_kernel, dim3(blocks_for(L->C)), dim3(256), 0, s, L->C, h->n_walk,
+                       L->cem.p, L->cn.p, L->dbias.p, L->dsig.p, L->cfg.delta_k,
+                       L->thr_steer.p, L->thr_screen.p);
+    // the surrogate's verdict on every chain's current state, with the
+    // parameters this iteration uses throughout
+    HIPCHK(h, run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, nullptr, s));
+    w.ham_sur(nullptr, nullptr, L->Us.p, L->ccell.p, L->cHs.p);
+    hipLaunchKernelGGL(lw_stat_add_kernel, dim3(1), dim3(64), 0, s, &L->dstats.p->sur_e,
+                       nullptr, (long long)L->C, 1LL);
+  }
+  for (int t = 0; t < c.n_mcmc_steps; ++t) {
+    const int m = L->sched[t];
+    const jr_move& mv = L->moves[m];
+    hipError_t e = mv.type == JR_MOVE_GMC ? gmc_step(w, t, m, mv, da) : volume_step(w, t, m, mv, da);
+    if (e != hipSuccess)
+      return h->fail(JR_E_HIP, "learned walk step %d: %s", t, hipGetErrorString(e));
+  }
+  hipLaunchKernelGGL(lw_scatter_kernel, dim3(L->C), dim3(256), 0, s, ga);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipEventRecord(L->ev[1], s));
+  const bool do_train = w.use_sur && L->stored >= L->cfg.min_train && L->cfg.train_steps > 0;
+  HIPCHK(h, hipEventRecord(L->ev[2], s));
+  if (do_train) HIPCHK(h, train(h, L, s));
+  HIPCHK(h, hipEventRecord(L->ev[3], s));
+  // statistics of this iteration (one sync)
+  LWStats st;
+  RingState rs;
+  int of = 0;
+  std::vector<double> hs(L->hs.n);
+  HIPCHK(h, hipMemcpyAsync(hs.data(), L->hs.p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost,
+                           s));
+  HIPCHK(h, hipMemcpyAsync(&st, L->dstats.p, sizeof(st), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipMemcpyAsync(&rs, L->rs.p, sizeof(rs), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipMemcpyAsync(&of, L->overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, zero_stats(L, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  if (of) return h->fail(JR_E_UNSUPPORTED, "judge neighbour list overflow (> %d)", MP_MAXNB);
+  float ms_walk = 0.f, ms_train = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms_walk, L->ev[0], L->ev[1]));
+  HIPCHK(h, hipEventElapsedTime(&ms_train, L->ev[2], L->ev[3]));
+  jr_learned_stats& a = L->acc;
+  (da ? a.iterations_screened : a.iterations_plain) += 1;
+  a.proposals += (int64_t)st.proposals;
+  a.stage1_pass += (int64_t)st.stage1_pass;
+  a.accepted += (int64_t)st.accepted;
+  a.exact_force_evals += (int64_t)st.exact_fe;
+  a.exact_energy_evals += (int64_t)st.exact_e;
+  a.surrogate_evals += (int64_t)st.sur_e;
+  if (do_train) {
+    a.train_steps += L->cfg.train_steps;
+    a.train_loss = st.loss;
+  }
+  a.walk_ms += ms_walk;
+  a.train_ms += ms_train;
+  L->stored = rs.size;
+  a.stored = rs.size;
+  fit_error_model(L, hs);
+  a.heldout_rmse = L->rmse;
+  return JR_OK;
+}
+
+}  // namespace jrl
+
+using namespace jrl;
+
+extern "C" int jr_learned_get_stats(jr_handle* h, jr_learned_stats* out, int32_t reset) {
+  if (!h || !out) return set_global_err(JR_E_ARG, "NULL argument");
+  if (!h->learned) return h->fail(JR_E_STATE, "the handle has no learned model");
+  *out = h->learned->acc;
+  out->heldout_rmse = h->learned->rmse;
+  out->stored = h->learned->stored;
+  if (reset) {
+    const double r = h->learned->acc.heldout_rmse, l = h->learned->acc.train_loss;
+    h->learned->acc = jr_learned_stats{};
+    h->learned->acc.heldout_rmse = r;
+    h->learned->acc.train_loss = l;
+  }
+  return JR_OK;
+}
+
+extern "C" int jr_learned_get_params(jr_handle* h, float* params) {
+  if (!h || !params) return set_global_err(JR_E_ARG, "NULL argument");
+  LearnedState* L = h->learned;
+  if (!L || !L->sur.P.p) return h->fail(JR_E_STATE, "the handle has no surrogate");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  HIPCHK(h, hipMemcpyAsync(params, L->sur.P.p, sizeof(float) * SG_NP, hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return JR_OK;
+}
+
+extern "C" int jr_learned_set_params(jr_handle* h, const float* params, int32_t reset_adam) {
+  if (!h || !params) return set_global_err(JR_E_ARG, "NULL argument");
+  LearnedState* L = h->learned;
+  if (!L || !L->sur.P.p) return h->fail(JR_E_STATE, "the handle has no surrogate");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  hipStream_t s = h->stream;
+  HIPCHK(h, hipMemcpyAsync(L->sur.P.p, params, sizeof(float) * SG_NP, hipMemcpyHostToDevice, s));
+  L->sur.cast(s);
+  HIPCHK(h, hipGetLastError());
+  if (reset_adam) {
+    HIPCHK(h, hipMemsetAsync(L->adam_m.p, 0, sizeof(float) * SG_NP, s));
+    HIPCHK(h, hipMemsetAsync(L->adam_v.p, 0, sizeof(float) * SG_NP, s));
+    L->adam_t = 0;
+  }
+  // a new surrogate has no held-out record yet
+  L->rmse = INFINITY;
+  L->rmse_n = 0;
+  L->ca = L->cv = 0.0;
+  std::fill(L->bias.begin(), L->bias.end(), 0.0);
+  std::fill(L->sig.begin(), L->sig.end(), 0.0);
+  HIPCHK(h, hipStreamSynchronize(s));
+  return JR_OK;
+}
+
+extern "C" int jr_learned_set_mode(jr_handle* h, int32_t mode) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  LearnedState* L = h->learned;
+  if (!L) return h->fail(JR_E_STATE, "the handle has no learned model");
+  if (mode < JR_LEARNED_PLAIN || mode > JR_LEARNED_SCREENED)
+    return h->fail(JR_E_ARG, "mode %d unknown", mode);
+  if (mode != JR_LEARNED_PLAIN && !L->sur.P.p)
+    return h->fail(JR_E_STATE, "the handle was created without a surrogate");
+  L->mode = mode;
   return JR_OK;
 }

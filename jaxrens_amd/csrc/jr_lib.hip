@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "jr_host.h"
+#include "jr_handle.h"
 #include "jr_kernels.h"
 
 using namespace jr;
@@ -32,72 +32,6 @@ int set_global_err(int code, const char* fmt, ...) {
 
 using namespace jrh;
 
-struct jr_handle {
-  jr_config cfg;
-  std::string err;
-  int n_walk = 1;
-  int max_pairs = 0;
-  int A = 1;
-  int cfgbits = 0;
-  int occ = 0;  // walk-kernel occupancy variant (0 or 3)
-  hipStream_t stream = nullptr;
-  // population
-  DBuf<double> pos, dir, cell, energy;
-  DBuf<int> types, n_atoms;
-  bool have_pop = false;
-  // run params
-  DBuf<double> pressure, mu, step_sizes, emax;
-  std::vector<double> h_pressure, h_mu, h_step;
-  // potential
-  DBuf<double> eps4, eps24, sig2;
-  DBuf<DevMove> moves;
-  DBuf<double> cum;  // cumulative move probabilities
-  // iteration work
-  DBuf<int> walk, counters, queue, pair_valid, order, sched;
-  DBuf<double> sel_u, clone_u, extra_u, normals, uniforms, re_u;
-  DBuf<IterState> st;
-  // per-block outputs
-  int out_cap = 0;
-  DBuf<double> o_emax, o_dead_volume, o_dead_pos, o_dead_cell;
-  DBuf<int> o_worst, o_clone, o_dead_types, o_counters, o_re_acc, o_re_att,
-      o_walk;
-  // trial buffers
-  DBuf<int> t_idx, t_reason;
-  DBuf<double> t_step, t_emax, t_normals, t_uniforms;
-  long long trial_rng = 0;
-  // timing
-  std::vector<hipEvent_t> ev;
-  double walk_ms = 0.0;
-  long long walk_launches = 0;
-  // graph
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  // launch geometry
-  int n_cu = 0;
-  int wpb = 4;
-  int blocks = 0;
-  size_t lds_walk = 0;
-  long long iter_i = 0;
-  long long rng_iter = 0;
-
-  int fail(int code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    err = buf;
-    return code;
-  }
-};
-
-#define HIPCHK(h, call)                                                   \
-  do {                                                                    \
-    hipError_t e_ = (call);                                               \
-    if (e_ != hipSuccess)                                                 \
-      return (h)->fail(JR_E_HIP, "%s failed: %s (%s:%d)", #call,          \
-                       hipGetErrorString(e_), __FILE__, __LINE__);        \
-  } while (0)
 
 
 namespace {
@@ -297,6 +231,7 @@ void jr_destroy(jr_handle* h) {
   if (!h) return;
   (void)hipSetDevice(h->cfg.device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  jrl::learned_destroy(h);
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
   for (auto e : h->ev) (void)hipEventDestroy(e);
@@ -354,6 +289,7 @@ int ensure_out(jr_handle* h, int n_iter) {
 
 // population energies on the device (needs run params)
 int recompute_energies(jr_handle* h) {
+  if (h->learned) return jrl::learned_recompute_energies(h);
   const auto& c = h->cfg;
   const int R = c.n_replicas, K = c.n_walkers;
   const int S = c.pot.n_species;
@@ -522,22 +458,29 @@ void enqueue_draws(jr_handle* h) {
                        h->re_u.p, (long long)n_re_u(h), seed, st, 0LL, 7u);
 }
 
-// select -> walk -> RE -> advance, all on the handle's stream.
-void enqueue_iteration_core(jr_handle* h, bool timed, int ev_idx) {
+// select -> walk -> RE -> advance, all on the handle's stream.  The learned
+// model's walk is sequenced from the host (one sync for the move schedule,
+// one for its statistics); the pair-potential walk is one launch.
+int enqueue_iteration_core(jr_handle* h, bool timed, int ev_idx) {
   const auto& c = h->cfg;
   SelArgs sa = sel_args(h);
   size_t sel_lds = sa.Kpad ? (size_t)sa.Kpad * (sizeof(double) + sizeof(int)) : 0;
   hipLaunchKernelGGL(select_kernel, dim3(c.n_replicas), dim3(1024), sel_lds, h->stream, sa);
-  WalkArgs wa = walk_args(h);
   if (timed) (void)hipEventRecord(h->ev[2 * ev_idx], h->stream);
-  launch_walk(h->A, h->cfgbits, h->occ, wa, h->blocks, h->wpb, h->lds_walk, h->stream);
+  if (h->learned) {
+    int rc = jrl::learned_walk(h);
+    if (rc) return rc;
+  } else {
+    WalkArgs wa = walk_args(h);
+    launch_walk(h->A, h->cfgbits, h->occ, wa, h->blocks, h->wpb, h->lds_walk, h->stream);
+  }
   if (timed) (void)hipEventRecord(h->ev[2 * ev_idx + 1], h->stream);
   if (c.n_replicas > 1) {
     ReArgs ra = re_args(h, h->st.p);
     hipLaunchKernelGGL(re_kernel, dim3(1), dim3(256), 0, h->stream, ra);
   }
-  // copy this iteration's walk list + counters into the block outputs
   hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, h->stream, h->st.p);
+  return JR_OK;
 }
 
 }  // namespace
@@ -593,9 +536,16 @@ extern "C" {
 int jr_create(const jr_config* cfg, jr_handle** out) {
   if (!cfg || !out) return set_global_err(JR_E_ARG, "cfg/out must not be NULL");
   *out = nullptr;
-  int rc = validate_pot(&cfg->pot);
-  if (rc) return rc;
-  const jr_config& c = *cfg;
+  jr_config c = *cfg;
+  if (c.mpnn) {
+    // the learned model replaces the pair potential
+    if (c.mpnn->n_species < 1 || c.mpnn->n_species > JR_MAX_SPECIES)
+      return set_global_err(JR_E_ARG, "mpnn n_species out of range");
+    c.pot = jr_pot{c.mpnn->n_species, JR_PREC_FP64, INFINITY, nullptr, nullptr, nullptr};
+  } else {
+    int rc = validate_pot(&c.pot);
+    if (rc) return rc;
+  }
   if (c.n_replicas < 1) return set_global_err(JR_E_ARG, "n_replicas must be >= 1");
   if (c.n_walkers < 2 || c.n_walkers > 4096)
     return set_global_err(JR_E_ARG, "n_walkers must be in [2, 4096]");
@@ -679,9 +629,11 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   TRY(h->re_u.alloc(n_re_u(h)));
   TRY(h->st.alloc(1));
 
-  TRY(hipMemcpy(h->eps4.p, c.pot.eps4, sizeof(double) * S * S, hipMemcpyHostToDevice));
-  TRY(hipMemcpy(h->eps24.p, c.pot.eps24, sizeof(double) * S * S, hipMemcpyHostToDevice));
-  TRY(hipMemcpy(h->sig2.p, c.pot.sig2, sizeof(double) * S * S, hipMemcpyHostToDevice));
+  if (!c.mpnn) {
+    TRY(hipMemcpy(h->eps4.p, c.pot.eps4, sizeof(double) * S * S, hipMemcpyHostToDevice));
+    TRY(hipMemcpy(h->eps24.p, c.pot.eps24, sizeof(double) * S * S, hipMemcpyHostToDevice));
+    TRY(hipMemcpy(h->sig2.p, c.pot.sig2, sizeof(double) * S * S, hipMemcpyHostToDevice));
+  }
   std::vector<DevMove> dm(c.n_moves);
   std::vector<double> cum(c.n_moves);
   double acc = 0.0;
@@ -724,6 +676,12 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   }
   int ret = ensure_out(h, 1);
   if (ret) return fail(ret);
+  if (c.mpnn) {
+    h->cfg.mpnn = nullptr;
+    h->cfg.learned = nullptr;
+    ret = jrl::learned_create(h, &c);
+    if (ret) return fail(ret);
+  }
   *out = h;
   return JR_OK;
 #undef TRY
@@ -745,6 +703,10 @@ int jr_set_population(jr_handle* h, const double* pos, const int32_t* types,
       int t = types[w * Nm + i];
       if (t < 0 || t >= S) return h->fail(JR_E_ARG, "species %d out of range", t);
     }
+  }
+  if (h->learned) {
+    int rc = jrl::learned_check_population(h, types, n_atoms, cell);
+    if (rc) return rc;
   }
   HIPCHK(h, hipSetDevice(c.device));
   HIPCHK(h, hipMemcpyAsync(h->pos.p, pos, sizeof(double) * W * Nm * 3, hipMemcpyHostToDevice, h->stream));
@@ -946,7 +908,8 @@ int jr_ns_iterations(jr_handle* h, int32_t n_iter, const jr_tape* tape,
     } else {
       enqueue_draws(h);
     }
-    enqueue_iteration_core(h, true, it);
+    rc = enqueue_iteration_core(h, true, it);
+    if (rc) return rc;
     rc = snapshot_iter(h, it);
     if (rc) return rc;
   }
@@ -957,6 +920,10 @@ int jr_ns_iterations(jr_handle* h, int32_t n_iter, const jr_tape* tape,
 int jr_ns_iterations_graph(jr_handle* h, int32_t n_iter, jr_iter_out* out) {
   if (!h) return set_global_err(JR_E_ARG, "NULL handle");
   if (!h->have_pop) return h->fail(JR_E_STATE, "population not set");
+  if (h->learned)
+    return h->fail(JR_E_UNSUPPORTED,
+                   "the learned-model walk is host-sequenced (data-dependent launches); "
+                   "use jr_ns_iterations");
   if (n_iter <= 0) return n_iter == 0 ? JR_OK : h->fail(JR_E_ARG, "n_iter < 0");
   HIPCHK(h, hipSetDevice(h->cfg.device));
   int rc = begin_block(h, n_iter);
@@ -964,7 +931,7 @@ int jr_ns_iterations_graph(jr_handle* h, int32_t n_iter, jr_iter_out* out) {
   if (!h->graph_exec) {
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     enqueue_draws(h);
-    enqueue_iteration_core(h, false, 0);
+    (void)enqueue_iteration_core(h, false, 0);
     hipError_t e = hipStreamEndCapture(h->stream, &h->graph);
     if (e != hipSuccess) return h->fail(JR_E_HIP, "graph capture: %s", hipGetErrorString(e));
     HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
@@ -1034,6 +1001,8 @@ int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
                    const double* uniforms, int32_t* accepted,
                    int32_t* reason) {
   if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  if (h->learned)
+    return h->fail(JR_E_UNSUPPORTED, "%s: pair-potential walk only", __func__);
   const auto& c = h->cfg;
   if (!h->have_pop) return h->fail(JR_E_STATE, "population not set");
   if (move_idx < 0 || move_idx >= c.n_moves) return h->fail(JR_E_ARG, "bad move index");
@@ -1103,6 +1072,8 @@ int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
 int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
                 int32_t* best_occ, double* best_ms) {
   if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  if (h->learned)
+    return h->fail(JR_E_UNSUPPORTED, "%s: pair-potential walk only", __func__);
   if (!h->have_pop) return h->fail(JR_E_STATE, "population not set");
   if (reps < 1) return h->fail(JR_E_ARG, "reps must be >= 1");
   const auto& c = h->cfg;
@@ -1190,6 +1161,8 @@ int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
 
 int jr_set_launch(jr_handle* h, int32_t waves_per_block, int32_t occupancy) {
   if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  if (h->learned)
+    return h->fail(JR_E_UNSUPPORTED, "%s: pair-potential walk only", __func__);
   if (waves_per_block < 1 || waves_per_block > 4)
     return h->fail(JR_E_ARG, "waves_per_block must be in [1, 4]");
   if (occupancy != 0 && occupancy != 3)

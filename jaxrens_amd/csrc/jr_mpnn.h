@@ -70,6 +70,7 @@ struct MPNNArgs {
   const int* n_atoms;    // [B]
   const double* cell;    // [B][9] cubic
   const int* cfg_index;  // [B] configuration slot per item, or NULL (= item)
+  const int* count;      // device item count (<= B), or NULL (= B)
   double* E;             // [B] model energy
   double* F;             // [B][Nmax][3] or NULL
   unsigned* nbl;         // scratch [gridDim.x][MP_MAXNB][MP_THREADS]
@@ -146,10 +147,20 @@ __global__ void __launch_bounds__(MP_THREADS)
   // W2 in LDS: read as broadcasts inside the pass-3 edge loop
   for (int k = tid; k < MP_NB * MP_C; k += MP_THREADS) W2S[k] = w[O::W2 + k];
 
-  for (int item = blockIdx.x; item < a.B; item += gridDim.x) {
+  const int B = a.count ? min(*a.count, a.B) : a.B;
+  for (int item = blockIdx.x; item < B; item += gridDim.x) {
     const int cfg = a.cfg_index ? a.cfg_index[item] : item;
     const int N = a.n_atoms[cfg];
     const double L = a.cell[(size_t)cfg * 9];
+    if (!(L >= (double)P.rc)) {
+      // outside the images enumerated below: no energy (the walk rejects)
+      if (tid == 0) a.E[cfg] = __builtin_inf();
+      if (a.F && tid < a.Nmax) {
+        double* Fo = a.F + ((size_t)cfg * a.Nmax + tid) * 3;
+        Fo[0] = Fo[1] = Fo[2] = 0.0;
+      }
+      continue;
+    }
     const float Lf = (float)L, iLf = (float)(1.0 / L);
     const float rcf = P.rc / Lf;  // cutoff in fractional units
     const float rcf2 = rcf * rcf;

@@ -161,9 +161,13 @@ struct SurEvalArgs {
   const int* n_atoms;    // [B]
   const double* cell;    // [B][9] cubic
   const int* cfg_index;  // [B] or NULL
+  const int* count;      // device item count (<= B), or NULL (= B)
   double* E;             // [B]
   double* F;             // [B][Nmax][3] or NULL
   __bf16* xout;          // [B][Nmax][80] descriptor rows, or NULL
+  // pair baseline b_eps ((sig/r)^12 - shift) for r^2 < b_rc2 (b_eps = 0: none)
+  float b_eps, b_sig2, b_rc2, b_shift;
+  double* Ebase;         // [B] baseline energy, or NULL
 };
 
 // Gaussians g_k(r) (and derivatives when D).
@@ -190,9 +194,10 @@ __global__ void __launch_bounds__(256) sur_eval_kernel(SurEvalArgs a) {
   __bf16* X = reinterpret_cast<__bf16*>(lds_sg + 4 * 256);         // [256][80]
   __bf16* Hs = X + 256 * SG_D;                                      // [256][128]
   float* dX = reinterpret_cast<float*>(Hs);                         // [256][64] (reuses Hs)
-  __shared__ double red[4];
+  __shared__ double red[4], redb[4];
   const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6;
-  for (int item = blockIdx.x; item < a.B; item += gridDim.x) {
+  const int B = a.count ? min(*a.count, a.B) : a.B;
+  for (int item = blockIdx.x; item < B; item += gridDim.x) {
     const int cfg = a.cfg_index ? a.cfg_index[item] : item;
     const int N = a.n_atoms[cfg];
     const double L = a.cell[(size_t)cfg * 9];
@@ -216,6 +221,8 @@ __global__ void __launch_bounds__(256) sur_eval_kernel(SurEvalArgs a) {
     float G0[SG_K], G1[SG_K];
 #pragma unroll
     for (int k = 0; k < SG_K; ++k) G0[k] = G1[k] = 0.0f;
+    float eb = 0.0f;  // sum over partners of the baseline pair energy
+    const float brc2 = a.b_rc2 * iLf * iLf;  // baseline cutoff, fractional units
     if (act) {
       for (int j = 0; j < N; ++j) {
         const float4 uj = U[j];
@@ -231,6 +238,10 @@ __global__ void __launch_bounds__(256) sur_eval_kernel(SurEvalArgs a) {
           const float r2 = e0 * e0 + e1 * e1 + e2 * e2;
           if (!(r2 < rdf2) || (j == tid && m == 0)) continue;
           const float r = Lf * sqrtf(r2);
+          if (r2 < brc2) {
+            const float q = a.b_sig2 * iLf * iLf / r2, q3 = q * q * q;
+            eb += a.b_eps * (q3 * q3 - a.b_shift);
+          }
           float g[SG_K], dg[SG_K];
           sg_gauss<false>(r, g, dg);
 #pragma unroll
@@ -279,13 +290,23 @@ __global__ void __launch_bounds__(256) sur_eval_kernel(SurEvalArgs a) {
           if (l == 32 && row < N) es += e[rb][v];
         }
     }
-    double ed = (double)es;
+    double ed = (double)es, edb = act ? 0.5 * (double)eb : 0.0;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) ed += __shfl_xor(ed, off, 64);
-    if (l == 0) red[wv] = ed;
+    for (int off = 32; off > 0; off >>= 1) {
+      ed += __shfl_xor(ed, off, 64);
+      edb += __shfl_xor(edb, off, 64);
+    }
+    if (l == 0) {
+      red[wv] = ed;
+      redb[wv] = edb;
+    }
     __syncthreads();
-    if (tid == 0)
-      a.E[cfg] = red[0] + red[1] + red[2] + red[3] + (double)N * (double)a.net.P[SG_OFF_B3];
+    if (tid == 0) {
+      const double base = redb[0] + redb[1] + redb[2] + redb[3];
+      a.E[cfg] = red[0] + red[1] + red[2] + red[3] + (double)N * (double)a.net.P[SG_OFF_B3] +
+                 base;
+      if (a.Ebase) a.Ebase[cfg] = base;
+    }
     if (!a.F) continue;
     // ---- backward to the inputs
     __bf16* Hw = Hs + wv * 64 * SG_H;
@@ -356,7 +377,12 @@ __global__ void __launch_bounds__(256) sur_eval_kernel(SurEvalArgs a) {
             const float mine = (tj == 0) ? own[k] : own[SG_K + k];
             c += (mine + other[k]) * dg[k];
           }
-          const float s = SG_SCALE * c * id;  // d / r = (e0, e1, e2) * id
+          c *= SG_SCALE;
+          if (r2 < brc2) {  // d phi / dr of the baseline pair
+            const float q = a.b_sig2 * iLf * iLf / r2, q3 = q * q * q;
+            c -= 12.0f * a.b_eps * q3 * q3 / r;
+          }
+          const float s = c * id;  // d / r = (e0, e1, e2) * id
           fx += s * e0;
           fy += s * e1;
           fz += s * e2;

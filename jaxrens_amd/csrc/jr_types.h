@@ -1,0 +1,31 @@
+// Plain data shared by the library's translation units and its kernels:
+// the device move table entry, the population pointers and the
+// device-resident iteration state.
+#pragma once
+
+namespace jr {
+
+struct DevMove {
+  int type, n_reflect, n_leapfrog, flat_v_prior, n_species, pad_;
+  double perturb_angle, max_vpa, min_vpa, min_aspect;
+};
+
+// Device-resident iteration state, read by every kernel of an iteration so a
+// captured graph can be replayed without parameter updates.
+struct IterState {
+  long long i;         // outer-loop iteration index (RE firing rule)
+  long long rng_iter;  // monotone counter feeding the Philox counter
+  int block_pos;       // slot in the per-block output buffers
+  int re_interval;
+};
+
+struct PopPtrs {
+  double* pos;
+  int* types;
+  int* n_atoms;
+  double* cell;
+  double* energy;
+  double* dir;
+};
+
+}  // namespace jr

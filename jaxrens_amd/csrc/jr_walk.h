@@ -5,14 +5,10 @@
 #include <type_traits>
 
 #include "jr_device.h"
+#include "jr_types.h"
 #include "../../include/jaxrens_amd.h"
 
 namespace jr {
-
-struct DevMove {
-  int type, n_reflect, n_leapfrog, flat_v_prior, n_species, pad_;
-  double perturb_angle, max_vpa, min_vpa, min_aspect;
-};
 
 struct PairTab {
   double rc2;

@@ -40,10 +40,13 @@ class RunConfig:
     device: int = 0
 
 
-def _model_of(backend) -> tuple[LJBackend, float | None, np.ndarray | None]:
+def _model_of(backend):
+    """(model, default pressure, default mu) of an LJ or message-passing
+    backend, bare or inside a Hamiltonian with ensemble terms."""
+    from .mpnn import MPNNBackend
     model = unwrap_model(backend)
-    if not isinstance(model, LJBackend):
-        raise TypeError("the device walk needs an LJBackend model")
+    if not isinstance(model, (LJBackend, MPNNBackend)):
+        raise TypeError("the device walk needs an LJBackend or MPNNBackend model")
     pressure, mu = None, None
     if isinstance(backend, Hamiltonian):
         pm = backend.ensemble_params_of(None)
@@ -56,9 +59,17 @@ def _model_of(backend) -> tuple[LJBackend, float | None, np.ndarray | None]:
 class NSEngine:
     """One handle of ``libjrhip``: population, run parameters and draws."""
 
-    def __init__(self, backend, moves, run: RunConfig, cell: CellSpec | dict | None = None):
+    def __init__(self, backend, moves, run: RunConfig, cell: CellSpec | dict | None = None,
+                 learned=None):
+        """``learned``: :class:`~jaxrens_amd.surrogate.LearnedConfig` of the
+        surrogate-screened walk (message-passing backend only; None walks
+        plain)."""
+        from .mpnn import MPNNBackend
         self.lib = _lib.load_library()
         self.model, p_default, mu_default = _model_of(backend)
+        self.is_learned = isinstance(self.model, MPNNBackend)
+        if learned is not None and not self.is_learned:
+            raise TypeError("a learned walk needs the message-passing backend")
         self.mwg = MWG(moves, cell)
         self.run = run
         R, K = run.n_replicas, run.n_walkers
@@ -68,11 +79,16 @@ class NSEngine:
         if self.n_atoms_rep.shape != (R,):
             raise ValueError("n_atoms must be an int or one value per replica")
         self.Nmax = int(self.n_atoms_rep.max())
-        self.S = self.model.n_species
         self.n_replicas, self.n_walkers = R, K
         self.n_walk = run.n_cull + run.n_extra
         self.n_moves = self.mwg.n_moves
-        pot, self._pot_keep = self.model.table.as_struct(self.model._prec_code)
+        self._learned_keep = None
+        if self.is_learned:
+            self.S = self.model.params.S
+            pot, self._pot_keep = _lib.JrPot(), None
+        else:
+            self.S = self.model.n_species
+            pot, self._pot_keep = self.model.table.as_struct(self.model._prec_code)
         self._moves_arr = self.mwg.device_table()
         self._weights = _lib.f64(self.mwg.weights)
         cfg = _lib.JrConfig()
@@ -91,6 +107,11 @@ class NSEngine:
         cfg.pot = pot
         cfg.moves = C.cast(self._moves_arr, C.POINTER(_lib.JrMove))
         cfg.move_weights = _lib.dptr(self._weights)
+        if self.is_learned:
+            cfg.mpnn = C.pointer(self.model.struct)
+            if learned is not None:
+                self._learned_st, self._learned_keep = learned.as_struct(self.S)
+                cfg.learned = C.pointer(self._learned_st)
         h = C.c_void_p()
         _lib.check(self.lib.jr_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -368,3 +389,25 @@ class NSEngine:
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         self._check(self.lib.jr_get_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"n_cu": a.value, "blocks": b.value, "waves_per_block": c.value}
+
+    # --------------------------------------------------- learned model
+    def learned_stats(self, reset: bool = False) -> dict:
+        """Counters of the learned-model walk since the last reset."""
+        st = _lib.JrLearnedStats()
+        self._check(self.lib.jr_learned_get_stats(self.h, C.byref(st), int(reset)))
+        return st.as_dict()
+
+    def surrogate_params(self) -> np.ndarray:
+        from .surrogate import N_PARAMS
+        p = np.empty(N_PARAMS, np.float32)
+        self._check(self.lib.jr_learned_get_params(self.h, p.ctypes.data_as(_lib._fp)))
+        return p
+
+    def set_surrogate_params(self, params, reset_adam: bool = True):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.lib.jr_learned_set_params(self.h, p.ctypes.data_as(_lib._fp),
+                                                   int(reset_adam)))
+
+    def set_learned_mode(self, mode: str):
+        from .surrogate import _MODES
+        self._check(self.lib.jr_learned_set_mode(self.h, _MODES[mode]))

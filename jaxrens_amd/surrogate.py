@@ -16,6 +16,8 @@ accumulation; the FP32 master parameters are trained with Adam.
 
 from __future__ import annotations
 
+from dataclasses import dataclass
+
 import numpy as np
 
 D_IN = 80
@@ -50,9 +52,11 @@ def _lib_mod():
 
 
 def sur_eval(params, pos, types, cell, S: int = 2, forces: bool = True,
-             descriptors: bool = False, device: int = 0):
+             descriptors: bool = False, device: int = 0, base=None):
     """Surrogate energies (B,), forces (B, N, 3) and optionally the BF16
-    descriptor rows (B, N, 80) as uint16 bit patterns, on the GPU."""
+    descriptor rows (B, N, 80) as uint16 bit patterns, on the GPU.
+    ``base`` = (eps, sig, rc) adds the pair baseline
+    eps ((sig/r)^12 - (sig/rc)^12) for r < rc."""
     import ctypes as C
     _lib = _lib_mod()
     lib = _lib.load_library()
@@ -68,7 +72,9 @@ def sur_eval(params, pos, types, cell, S: int = 2, forces: bool = True,
     F = np.zeros((B, N, 3)) if forces else None
     X = np.zeros((B, N, D_IN), np.uint16) if descriptors else None
     xp = X.ctypes.data_as(C.POINTER(C.c_uint16)) if X is not None else None
-    _lib.check(lib.jr_sur_eval(p.ctypes.data_as(_lib._fp), device, B, N, _lib.dptr(pos),
+    b = None if base is None else _lib.f64(base)
+    _lib.check(lib.jr_sur_eval(p.ctypes.data_as(_lib._fp), _lib.dptr(b), device, B, N,
+                               _lib.dptr(pos),
                                _lib.iptr(ty), _lib.iptr(n), _lib.dptr(cell), S,
                                _lib.dptr(U), _lib.dptr(F), xp))
     return U, F, X
@@ -103,3 +109,51 @@ def sur_adam(params, m, v, grad, lr: float, t: int, device: int = 0):
     _lib.check(lib.jr_sur_adam(device, p.ctypes.data_as(fp), m.ctypes.data_as(fp),
                                v.ctypes.data_as(fp), g.ctypes.data_as(fp), lr, t))
     return p, m, v
+
+
+_MODES = {"plain": 0, "auto": 1, "screened": 2}
+
+
+@dataclass
+class LearnedConfig:
+    """Settings of the surrogate-screened walk (``jr_learned``).
+
+    mode: "plain" (the judge's forces steer and it decides every move; no
+    surrogate), "auto" (two-stage delayed acceptance while the held-out
+    per-atom RMSE after the error model is <= rmse_bound, plain otherwise)
+    or "screened" (delayed acceptance in every iteration).  The surrogate
+    trains online in "auto" and "screened" on the last ``ring_capacity``
+    exactly evaluated configurations (those with U/N <= train_max_epa).
+    delta_k: stage-1 slack in residual spreads of the error model.
+    baseline: (eps, sig, rc) of the surrogate's pair term
+    eps ((sig/r)^12 - (sig/rc)^12), r < rc <= 3; the network learns the
+    judge's energy minus it (None: no baseline)."""
+
+    mode: str = "auto"
+    ring_capacity: int = 100_000
+    train_batch: int = 32
+    train_steps: int = 8
+    min_train: int = 64
+    min_heldout: int = 16
+    lr: float = 1e-3
+    rmse_bound: float = 0.05
+    delta_k: float = 2.0
+    train_max_epa: float = 50.0
+    baseline: tuple[float, float, float] | None = None
+    params: np.ndarray | None = None
+    seed: int = 0
+
+    def as_struct(self, n_species: int):
+        from jaxrens_amd import _lib
+        if self.mode not in _MODES:
+            raise ValueError(f"learned mode must be one of {sorted(_MODES)}")
+        p = self.params if self.params is not None else init_surrogate(self.seed, n_species)
+        keep = np.ascontiguousarray(p, dtype=np.float32)
+        if keep.shape != (N_PARAMS,):
+            raise ValueError(f"surrogate parameters must have {N_PARAMS} values")
+        be, bs, br = self.baseline if self.baseline is not None else (0.0, 0.0, 0.0)
+        st = _lib.JrLearned(_MODES[self.mode], self.ring_capacity, self.train_batch,
+                            self.train_steps, self.min_train, self.min_heldout, self.lr,
+                            self.rmse_bound, self.delta_k, self.train_max_epa, be, bs, br,
+                            keep.ctypes.data_as(_lib._fp))
+        return st, keep

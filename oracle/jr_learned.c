@@ -399,7 +399,7 @@ static void sur_backward(const float* P, int bf16, int n, const double* x,
 
 double or_sur_energy_forces(const float* params, int bf16, int N,
                             const double* pos, const int32_t* types,
-                            const double* cell, int S, double* F) {
+                            const double* cell, int S, const double* base, double* F) {
   const int D = OR_SUR_D, H = OR_SUR_H;
   size_t nn = (size_t)(N > 0 ? N : 1);
   double* x = (double*)malloc(sizeof(double) * nn * D);
@@ -411,6 +411,16 @@ double or_sur_energy_forces(const float* params, int bf16, int N,
   sur_forward(params, bf16, N, x, e, z1, h1, z2);
   double E = 0.0;
   for (int i = 0; i < N; ++i) E += e[i];
+  const int has_base = base && base[0] != 0.0;
+  if (has_base) {
+    /* every directed edge carries half of its pair's energy */
+    sedge_t* ed;
+    long ne = sur_edges(N, pos, cell, &ed);
+    const double shift = pow(base[1] / base[2], 12.0);
+    for (long k = 0; k < ne; ++k)
+      if (ed[k].r < base[2]) E += 0.5 * base[0] * (pow(base[1] / ed[k].r, 12.0) - shift);
+    free(ed);
+  }
   if (F) {
     double* one = (double*)calloc(nn, sizeof(double));
     double* dX = (double*)malloc(sizeof(double) * nn * D);
@@ -427,6 +437,8 @@ double or_sur_energy_forces(const float* params, int bf16, int N,
       for (int q = 0; q < OR_SUR_K; ++q)
         ce += dX[(size_t)i * D + OR_SUR_K * types[j] + q] * dg[q];
       ce *= SUR_SCALE;
+      if (has_base && ed[k].r < base[2])  /* half of d phi / dr per edge */
+        ce -= 0.5 * 12.0 * base[0] * pow(base[1] / ed[k].r, 12.0) / ed[k].r;
       for (int q = 0; q < 3; ++q) {
         double gq = ce * ed[k].d[q] / ed[k].r;
         F[3 * j + q] -= gq;

@@ -72,10 +72,12 @@ enum {
 void or_sur_descriptors(int N, const double* pos, const int32_t* types,
                         const double* cell, int S, double* x);
 
-/* Energy (and forces when F != NULL) of one configuration. */
+/* Energy (and forces when F != NULL) of one configuration.  base =
+ * {eps, sig, rc} adds the pair baseline eps ((sig/r)^12 - (sig/rc)^12) for
+ * r < rc <= 3 (NULL: none). */
 double or_sur_energy_forces(const float* params, int bf16, int N,
                             const double* pos, const int32_t* types,
-                            const double* cell, int S, double* F);
+                            const double* cell, int S, const double* base, double* F);
 
 /* Training loss over n configurations given their descriptor rows
  * (x [sum N][80]), atom counts and exact energies:

@@ -70,7 +70,7 @@ def _sur_lib():
         L.or_sur_descriptors.restype = None
         L.or_sur_descriptors.argtypes = [C.c_int, dp, ip, dp, C.c_int, dp]
         L.or_sur_energy_forces.restype = C.c_double
-        L.or_sur_energy_forces.argtypes = [fp, C.c_int, C.c_int, dp, ip, dp, C.c_int, dp]
+        L.or_sur_energy_forces.argtypes = [fp, C.c_int, C.c_int, dp, ip, dp, C.c_int, dp, dp]
         L.or_sur_loss_grad.restype = C.c_double
         L.or_sur_loss_grad.argtypes = [fp, C.c_int, C.c_int, ip, dp, dp, dp]
         L.or_adam_step.restype = None
@@ -92,14 +92,18 @@ def sur_descriptors(pos, types, cell, S=2):
     return x
 
 
-def sur_energy_forces(params, pos, types, cell, S=2, bf16=True, forces=True):
+def sur_energy_forces(params, pos, types, cell, S=2, bf16=True, forces=True, base=None):
+    """Surrogate energy (and forces); ``base`` = (eps, sig, rc) adds the
+    pair baseline eps ((sig/r)^12 - (sig/rc)^12) for r < rc."""
     pos = f64(pos)
     N = pos.shape[0]
     p = _f32(params)
     F = np.zeros((N, 3)) if forces else None
+    b = None if base is None else f64(base)
     E = _sur_lib().or_sur_energy_forces(p.ctypes.data_as(_fp), int(bf16), N, _d(pos),
                                         _i(i32(types)),
-                                        _d(f64(np.asarray(cell).reshape(9))), S, _d(F))
+                                        _d(f64(np.asarray(cell).reshape(9))), S, _d(b),
+                                        _d(F))
     return E, F
 
 

@@ -45,7 +45,12 @@ def test_struct_layouts_match_c(tmp_path):
         ' printf("%zu %zu %zu\\n", offsetof(jr_config, seed), offsetof(jr_config, pot),'
         " offsetof(jr_config, moves));\n"
         ' printf("%zu %zu %zu\\n", sizeof(jr_mpnn), offsetof(jr_mpnn, rc),'
-        " offsetof(jr_mpnn, emb));\n return 0; }\n")
+        " offsetof(jr_mpnn, emb));\n"
+        ' printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(jr_learned),'
+        " offsetof(jr_learned, lr), offsetof(jr_learned, sur_params),"
+        " sizeof(jr_learned_stats), offsetof(jr_learned_stats, heldout_rmse),"
+        " offsetof(jr_config, mpnn), offsetof(jr_config, learned));\n"
+        " return 0; }\n")
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)],
                    check=True)
@@ -60,6 +65,11 @@ def test_struct_layouts_match_c(tmp_path):
                     _lib.JrConfig.moves.offset]
     mp = [int(x) for x in lines[2].split()]
     assert mp == [C.sizeof(_lib.JrMPNN), _lib.JrMPNN.rc.offset, _lib.JrMPNN.emb.offset]
+    lw = [int(x) for x in lines[3].split()]
+    assert lw == [C.sizeof(_lib.JrLearned), _lib.JrLearned.lr.offset,
+                  _lib.JrLearned.sur_params.offset, C.sizeof(_lib.JrLearnedStats),
+                  _lib.JrLearnedStats.heldout_rmse.offset, _lib.JrConfig.mpnn.offset,
+                  _lib.JrConfig.learned.offset]
 
 
 def test_missing_library_fails_loudly(tmp_path):

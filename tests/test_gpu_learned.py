@@ -72,27 +72,32 @@ def _bits_to_f(b):
     return (np.asarray(b, np.uint32) << 16).view(np.float32)
 
 
-@pytest.mark.parametrize("N,rho", [(256, 0.9), (100, 0.7)])
-def test_surrogate_eval_matches_oracle(N, rho):
+@pytest.mark.parametrize("N,rho,base", [(256, 0.9, None), (100, 0.7, None),
+                                        (128, 1.1, (1.0, 0.85, 1.2))])
+def test_surrogate_eval_matches_oracle(N, rho, base):
     """BF16 matrix-core surrogate vs the oracle that rounds to BF16 at the same
     points.  Stated accuracy: descriptors within one BF16 ulp, |dE| <= 2e-3
-    sum_i |e_i| and |dF| <= 2e-2 |F|."""
-    print(f"surrogate eval N={N}", flush=True)
+    (sum_i |e_i| + |E_baseline|) and |dF| <= 2e-2 |F|."""
+    print(f"surrogate eval N={N} baseline={base}", flush=True)
     rng = np.random.default_rng(N + 1)
     p = SG.init_surrogate(7)
     cfgs = [_liquid(rng, N, rho) for _ in range(2)]
     pos = np.stack([c[0] for c in cfgs])
     ty = np.stack([c[1] for c in cfgs])
     cell = np.stack([c[2] for c in cfgs])
-    U, F, X = SG.sur_eval(p, pos, ty, cell, descriptors=True)
+    U, F, X = SG.sur_eval(p, pos, ty, cell, descriptors=True, base=base)
     for b in range(2):
         xo = OL.sur_descriptors(pos[b], ty[b], cell[b])
         xg = _bits_to_f(X[b]).astype(np.float64)
         xr = _bits_to_f(_bf16_bits(xo)).astype(np.float64)
         same = np.mean(xg == xr)
         ulp = np.abs(xg - xr) / np.maximum(np.abs(xr), 1e-30)
-        e, f = OL.sur_energy_forces(p, pos[b], ty[b], cell[b], bf16=True)
+        e, f = OL.sur_energy_forces(p, pos[b], ty[b], cell[b], bf16=True, base=base)
         scale = np.abs(SG_row_energies(p, xo)).sum()
+        if base is not None:
+            eb = OL.sur_energy_forces(np.zeros(SG.N_PARAMS, np.float32), pos[b], ty[b],
+                                      cell[b], forces=False, base=base)[0]
+            scale += abs(eb)
         rel_f = np.linalg.norm(F[b] - f) / np.linalg.norm(f)
         print(f"  cfg {b}: descriptors equal {same:.4f}, max rel {ulp.max():.2e}; "
               f"E gpu {U[b]:.6f} oracle {e:.6f} (sum|e| {scale:.3f}); |dF|/|F| {rel_f:.2e}",

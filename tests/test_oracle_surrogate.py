@@ -94,3 +94,40 @@ def test_adam_first_step_is_signed_lr():
     np.testing.assert_allclose(p1 - p, -1e-3 * np.sign(g), rtol=1e-4, atol=1e-7)
     np.testing.assert_allclose(m1, 0.1 * g, rtol=1e-6)
     np.testing.assert_allclose(v1, 0.001 * g * g, rtol=3e-5)  # 1 - 0.999f in FP32
+
+
+def test_pair_baseline_closed_form_and_forces():
+    """With the network's output zeroed (every parameter 0), the surrogate is
+    its pair baseline alone: eps ((sig/r)^12 - (sig/rc)^12) for r < rc,
+    counted once per pair over all images, with forces -d/dx of it."""
+    base = (1.3, 0.85, 1.2)
+    zero = np.zeros(SG.N_PARAMS, np.float32)
+    L = 6.0
+    for r in (0.7, 1.0, 1.19, 1.25):
+        pos = np.array([[1.0, 1.0, 1.0], [1.0 + r, 1.0, 1.0]])
+        E, F = OL.sur_energy_forces(zero, pos, [0, 1], np.eye(3) * L, base=base)
+        want = base[0] * ((base[1] / r) ** 12 - (base[1] / base[2]) ** 12) if r < base[2] else 0.0
+        dphi = -12 * base[0] * base[1] ** 12 / r ** 13 if r < base[2] else 0.0
+        assert E == pytest.approx(want, rel=1e-13, abs=1e-15)
+        np.testing.assert_allclose(F[1], [-dphi, 0, 0], rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(F[0], -F[1], rtol=1e-12, atol=1e-15)
+    # the baseline adds to the network's energy; forces match differences
+    rng = np.random.default_rng(5)
+    p = SG.init_surrogate(3)
+    cell = np.eye(3) * 6.5
+    pos = rng.random((60, 3)) * 6.5
+    ty = rng.integers(0, 2, 60)
+    E0, _ = OL.sur_energy_forces(p, pos, ty, cell, bf16=False)
+    Eb, Fb = OL.sur_energy_forces(p, pos, ty, cell, bf16=False, base=base)
+    Ez, _ = OL.sur_energy_forces(zero, pos, ty, cell, bf16=False, base=base)
+    assert Ez > 1.0  # close pairs present
+    assert Eb == pytest.approx(E0 + Ez, rel=1e-12)
+    h = 1e-6
+    for a in range(3):
+        for q in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[a, q] += h
+            pm[a, q] -= h
+            fd = -(OL.sur_energy_forces(p, pp, ty, cell, bf16=False, forces=False, base=base)[0]
+                   - OL.sur_energy_forces(p, pm, ty, cell, bf16=False, forces=False, base=base)[0]) / (2 * h)
+            assert Fb[a, q] == pytest.approx(fd, rel=1e-5, abs=1e-6 * np.abs(Fb).max())
