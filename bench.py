@@ -236,6 +236,82 @@ def surrogate_bench(n_plain=1, n_timed=2, R=4, K=1024, N=256, T=20) -> dict:
     return out
 
 
+def lj128_hmc_report(R=8, K=2048, N=128, T=20, n_timed=1) -> dict:
+    """LJ-128 with HMC and surrogate delayed acceptance: the FP64 pair
+    potential is the judge, HMC (10 leapfrog steps, dt 0.005) and galilean
+    (n_reflect 10) at 1:1.  Reports the surrogate's held-out error against
+    the FP64 judge and the stage-1 / stage-2 acceptance; log Z against the
+    plain walk is tested over independent runs at reduced size.  The pair
+    potential is far cheaper than the surrogate, so the screened walk is
+    slower here; the configuration checks the method, not a speed-up."""
+    from jaxrens_amd.backends import EnsembleTerm, Hamiltonian, LJBackend
+    from jaxrens_amd.engine import NSEngine, RunConfig
+    from jaxrens_amd.surrogate import LearnedConfig, init_surrogate
+    P = list(np.geomspace(0.05, 5.0, R))
+    moves = [{"type": "hmc", "n_leapfrog": 10, "step_size": 0.005, "weight": 1},
+             {"type": "galilean", "n_reflect": 10, "step_size": 0.1, "weight": 1}]
+    pos, ty, cell = sur_population(R, K, N, seed=9)
+    ty[:] = 0
+    ws_it = R * (K - 1) * T
+    out = {"workload": f"LJ-{N} (eps = sigma = 1, rc 2.5, FP64 judge), P = {R} pressures "
+                       f"0.05-5, K = {K}, {T} sub-steps hmc(10 x 0.005):galilean(10) 1:1, "
+                       "n_extra = K-2, initial jittered lattice at density 0.9",
+           "baseline": "pair term 4 (1/r)^12 below 2.5 (the r^-12 part of LJ)"}
+
+    def run(mode, max_warm, n):
+        lj = LJBackend(1.0, 1.0, 2.5)
+        run = RunConfig(n_replicas=R, n_walkers=K, n_atoms=N, n_mcmc_steps=T, n_extra=K - 2,
+                        seed=5, pressures=P, re_interval=1)
+        # only configurations with U/N <= 0 are stored and scored: judged end
+        # points with overlapping pairs (-4/r^6 below the Gaussians' reach)
+        # are far outside any contour the walk samples
+        # fallback bound 0.1 per atom: the network carries the whole -4/r^6 part
+        # (about -7 per atom), so its BF16 operands alone leave ~1 % of that
+        lc = LearnedConfig(mode=mode, ring_capacity=100_000, train_batch=64, train_steps=1000,
+                           min_train=256, min_heldout=64, lr=2e-3, rmse_bound=0.1,
+                           delta_k=2.0, train_max_epa=0.0, baseline=(4.0, 1.0, 2.5),
+                           params=init_surrogate(2, S=1))
+        eng = NSEngine(Hamiltonian(lj, [EnsembleTerm(P[0])]), moves, run, cell=None, learned=lc)
+        eng.set_population(pos, ty, cell)
+        warm = 0
+        if mode == "plain" and max_warm:
+            eng.ns_iterations(max_warm)  # the same contour as the screened run
+            warm = max_warm
+        # auto walks plain (training online) until the held-out error meets the bound
+        while mode != "plain" and warm < max_warm:
+            eng.ns_iterations(1)
+            warm += 1
+            st = eng.learned_stats()
+            print(f"[bench] lj128 warm-up {warm}: held-out rmse {st['heldout_rmse']:.4f}, "
+                  f"screened iterations {st['iterations_screened']}", file=sys.stderr,
+                  flush=True)
+            if st["iterations_screened"] > 0:
+                break
+        eng.learned_stats(reset=True)
+        t = time.perf_counter()
+        eng.ns_iterations(n)
+        dt = time.perf_counter() - t
+        st = eng.learned_stats()
+        eng.close()
+        r = {"walker_steps_per_s": ws_it * n / dt, "s_per_iteration": dt / n,
+             "acceptance": st["accepted"] / max(st["proposals"], 1),
+             "screened_iterations": st["iterations_screened"], "warm_up_iterations": warm}
+        if mode != "plain":
+            scr = st["iterations_screened"] > 0
+            r.update({"heldout_rmse_per_atom_vs_fp64": st["heldout_rmse"],
+                      "stage1_acceptance": (st["stage1_pass"] / max(st["proposals"], 1)
+                                            if scr else None),
+                      "stage2_acceptance": (st["accepted"] / max(st["stage1_pass"], 1)
+                                            if scr else None),
+                      "judge_energy_evals": st["exact_energy_evals"],
+                      "surrogate_evals": st["surrogate_evals"]})
+        return r
+
+    out["screened"] = run("auto", 8, n_timed)
+    out["plain"] = run("plain", out["screened"]["warm_up_iterations"], n_timed)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,6 +326,9 @@ def main():
                     help="timed iterations of the screened walk (256-atom config)")
     ap.add_argument("--sur-only", action="store_true",
                     help="run only the 256-atom surrogate comparison")
+    ap.add_argument("--lj128-only", action="store_true",
+                    help="run only the LJ-128 HMC surrogate report")
+    ap.add_argument("--skip-lj128", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (counter calibration)")
@@ -265,6 +344,9 @@ def main():
 
     if args.sur_only:
         print(json.dumps({"surrogate": surrogate_bench(n_timed=args.sur_timed)}), flush=True)
+        return
+    if args.lj128_only:
+        print(json.dumps({"lj128_hmc_surrogate": lj128_hmc_report()}), flush=True)
         return
 
     eng = build_engine(n_extra=1022)
@@ -321,6 +403,9 @@ def main():
     surrogate = None
     if not args.skip_surrogate:
         surrogate = surrogate_bench(n_timed=args.sur_timed)
+    lj128 = None
+    if not args.skip_lj128:
+        lj128 = lj128_hmc_report()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -368,6 +453,7 @@ def main():
         "cpu_baseline": cpu,
         "secondary": secondary,
         "surrogate": surrogate,
+        "lj128_hmc_surrogate": lj128,
     }
     print(json.dumps(line), flush=True)
 

@@ -100,4 +100,10 @@ int learned_check_population(jr_handle* h, const int32_t* types, const int32_t* 
 int learned_recompute_energies(jr_handle* h);
 // The walk of one iteration (after select, before RE), on h->stream.
 int learned_walk(jr_handle* h);
+// Pair-potential judge of the learned walk (jr_lib.hip): model energies U and
+// forces F (NULL: energy only) of the configurations listed in idx (NULL:
+// all B) with the device count (NULL: B); outputs indexed by configuration.
+hipError_t lj_eval(jr_handle* h, const double* pos, const int* types, const int* n_atoms,
+                   const double* cell, const int* idx, const int* count, int B, double* U,
+                   double* F, hipStream_t s);
 }  // namespace jrl

@@ -217,6 +217,8 @@ struct EvalArgs {
   const int* types;
   const int* n_atoms;
   const double* cell;
+  const int* cfg_index;  // [B] configuration per item, or NULL (= item)
+  const int* count;      // device item count (<= B), or NULL (= B)
   const double* pressure;
   const double* mu;
   double* U;
@@ -258,8 +260,10 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs a) {
   tab.sig2 = tabs + 2 * SS;
   tab.S = a.S;
 
-  const int b = blockIdx.x * nwave_blk + wave;
-  if (b >= a.B) return;
+  const int item = blockIdx.x * nwave_blk + wave;
+  const int n_items = a.count ? min(*a.count, a.B) : a.B;
+  if (item >= n_items) return;  // wave-uniform; no block barrier follows
+  const int b = a.cfg_index ? a.cfg_index[item] : item;
   const int N = a.n_atoms[b];
   double x[A], y[A], z[A];
   int t[A];

@@ -403,6 +403,8 @@ namespace jrl {
 struct LearnedState {
   jr_learned cfg{};
   int mode = JR_LEARNED_PLAIN;
+  bool lj = false;  // judge: the handle's pair potential (else the message-passing model)
+  int S = 1;        // species
   std::vector<jr_move> moves;
   MPNNWeights judge;
   SurParams sur;
@@ -411,8 +413,8 @@ struct LearnedState {
   long long train_counter = 0;
   int C = 0, Nmax = 0, n_cu = 0;
   // chain work arrays
-  DBuf<double> cpos, cdir, x0, d0, npos, F, ccell, ncell, cH, cHs, cem, cpres, thr_steer,
-      thr_screen, U, Us, Ub, Hy, Hsy;
+  DBuf<double> cpos, cdir, x0, d0, npos, F, mom, ccell, ncell, cH, cHs, cem, cpres, thr_steer,
+      thr_screen, U, Us, Ub, Hy, Hsy, K0;
   DBuf<int> ctypes, cn, code, code2, accept, reason, idx, idx2, cnt;
   DBuf<__bf16> xstage;
   DBuf<unsigned> nbl;
@@ -483,6 +485,15 @@ hipError_t run_judge(LearnedState* L, const MPNNArgs& a, hipStream_t s) {
   return launch_mpnn(a, L->judge.buf.p, std::max(1, std::min(a.B, L->n_cu)), s);
 }
 
+// The judge's model energies U (and forces F unless NULL) of the listed
+// configurations (idx NULL: all B; count NULL: B), indexed by configuration.
+hipError_t judge_eval(jr_handle* h, LearnedState* L, const double* pos, const double* cell,
+                      const int* types, const int* n, const int* idx, const int* count, int B,
+                      double* U, double* F, hipStream_t s) {
+  if (L->lj) return lj_eval(h, pos, types, n, cell, idx, count, B, U, F, s);
+  return run_judge(L, judge_args(L, pos, cell, B, types, n, idx, count, U, F), s);
+}
+
 hipError_t run_sur(LearnedState* L, const double* pos, const double* cell, const int* idx,
                    const int* count, double* E, double* F, __bf16* xout, hipStream_t s) {
   SurEvalArgs a;
@@ -492,7 +503,7 @@ hipError_t run_sur(LearnedState* L, const double* pos, const double* cell, const
   a.net = L->sur.dev();
   a.B = L->C;
   a.Nmax = L->Nmax;
-  a.S = L->judge.dev.S;
+  a.S = L->S;
   a.pos = pos;
   a.types = L->ctypes.p;
   a.n_atoms = L->cn.p;
@@ -509,18 +520,22 @@ hipError_t run_sur(LearnedState* L, const double* pos, const double* cell, const
 
 int learned_create(jr_handle* h, const jr_config* c) {
   const jr_mpnn* net = c->mpnn;
-  int rc = validate_mpnn(net);
-  if (rc) {
-    h->err = g_err;
-    return rc;
+  if (net) {
+    int rc = validate_mpnn(net);
+    if (rc) {
+      h->err = g_err;
+      return rc;
+    }
   }
   if (c->n_atoms_max > MP_THREADS)
-    return h->fail(JR_E_UNSUPPORTED, "the judge takes at most %d atoms", MP_THREADS);
+    return h->fail(JR_E_UNSUPPORTED, "the learned walk takes at most %d atoms", MP_THREADS);
   for (int m = 0; m < c->n_moves; ++m) {
     const int t = c->moves[m].type;
-    if (t != JR_MOVE_GMC && t != JR_MOVE_VOLUME)
+    if (t != JR_MOVE_GMC && t != JR_MOVE_VOLUME && t != JR_MOVE_HMC)
       return h->fail(JR_E_UNSUPPORTED,
-                     "move %d: the learned-model walk runs gmc and volume moves", m);
+                     "move %d: the learned walk runs gmc, volume and hmc moves", m);
+    if (t == JR_MOVE_HMC && c->moves[m].n_leapfrog < 1)
+      return h->fail(JR_E_ARG, "hmc needs n_leapfrog >= 1");
   }
   jr_learned lc{};
   lc.mode = JR_LEARNED_PLAIN;
@@ -531,15 +546,20 @@ int learned_create(jr_handle* h, const jr_config* c) {
     if (!lc.sur_params) return h->fail(JR_E_ARG, "learned.sur_params must be set");
     if (lc.ring_capacity < 1 || lc.train_batch < 1 || lc.train_steps < 0 || lc.min_train < 1 ||
         lc.min_heldout < 1 || !(lc.lr > 0.0) || !(lc.rmse_bound >= 0.0) ||
-        !(lc.delta_k >= 0.0) || !(lc.train_max_epa > 0.0))
+        !(lc.delta_k >= 0.0) || std::isnan(lc.train_max_epa))
       return h->fail(JR_E_ARG, "learned: invalid training settings");
     if (lc.base_eps != 0.0 &&
         (!(lc.base_sig > 0.0) || !(lc.base_rc > 0.0) || lc.base_rc > (double)SG_RD))
       return h->fail(JR_E_ARG, "learned: baseline needs sig > 0 and 0 < rc <= %g",
                      (double)SG_RD);
   }
+  const int S = net ? net->n_species : c->pot.n_species;
+  if (lc.mode != JR_LEARNED_PLAIN && S > 2)
+    return h->fail(JR_E_UNSUPPORTED, "the surrogate takes 1 or 2 species");
   auto* L = new LearnedState();
   h->learned = L;
+  L->lj = net == nullptr;
+  L->S = S;
   L->cfg = lc;
   L->cfg.sur_params = nullptr;
   L->mode = lc.mode;
@@ -548,19 +568,21 @@ int learned_create(jr_handle* h, const jr_config* c) {
   L->C = c->n_replicas * (1 + c->n_extra);
   L->n_cu = device_cus(c->device);
   L->sched.assign(c->n_mcmc_steps, 0);
-  h->mpnn_cfg = *net;
-  h->mpnn_cfg.emb = h->mpnn_cfg.W1 = h->mpnn_cfg.U1 = h->mpnn_cfg.c1 = nullptr;
-  h->mpnn_cfg.W2 = h->mpnn_cfg.U2 = h->mpnn_cfg.c2 = h->mpnn_cfg.wo = nullptr;
+  if (net) {
+    h->mpnn_cfg = *net;
+    h->mpnn_cfg.emb = h->mpnn_cfg.W1 = h->mpnn_cfg.U1 = h->mpnn_cfg.c1 = nullptr;
+    h->mpnn_cfg.W2 = h->mpnn_cfg.U2 = h->mpnn_cfg.c2 = h->mpnn_cfg.wo = nullptr;
+  }
   h->learned_cfg = L->cfg;
   hipStream_t s = h->stream;
   const size_t C = L->C, Nm = L->Nmax, n3 = C * Nm * 3;
-  HIPCHK(h, L->judge.upload(net, s));
-  for (DBuf<double>* b : {&L->cpos, &L->cdir, &L->x0, &L->d0, &L->npos, &L->F})
+  if (net) HIPCHK(h, L->judge.upload(net, s));
+  for (DBuf<double>* b : {&L->cpos, &L->cdir, &L->x0, &L->d0, &L->npos, &L->F, &L->mom})
     HIPCHK(h, b->alloc(n3));
   HIPCHK(h, L->ccell.alloc(C * 9));
   HIPCHK(h, L->ncell.alloc(C * 9));
   for (DBuf<double>* b : {&L->cH, &L->cHs, &L->cem, &L->cpres, &L->thr_steer, &L->thr_screen,
-                          &L->U, &L->Us, &L->Ub, &L->Hy, &L->Hsy})
+                          &L->U, &L->Us, &L->Ub, &L->Hy, &L->Hsy, &L->K0})
     HIPCHK(h, b->alloc(C));
   const int R = c->n_replicas;
   HIPCHK(h, L->hs.alloc((size_t)R * LW_HS));
@@ -573,7 +595,7 @@ int learned_create(jr_handle* h, const jr_config* c) {
     HIPCHK(h, b->alloc(C));
   HIPCHK(h, L->cnt.alloc(4));
   // judge launches use at most n_cu persistent blocks, each with its own list
-  HIPCHK(h, L->nbl.alloc((size_t)L->n_cu * MP_MAXNB * MP_THREADS));
+  if (net) HIPCHK(h, L->nbl.alloc((size_t)L->n_cu * MP_MAXNB * MP_THREADS));
   HIPCHK(h, L->overflow.alloc(1));
   HIPCHK(h, L->dstats.alloc(1));
   HIPCHK(h, L->rs.alloc(1));
@@ -620,18 +642,27 @@ void learned_destroy(jr_handle* h) {
 int learned_check_population(jr_handle* h, const int32_t* types, const int32_t* n_atoms,
                              const double* cell) {
   const auto& c = h->cfg;
+  LearnedState* L = h->learned;
   const size_t W = (size_t)c.n_replicas * c.n_walkers;
-  const double rc = h->mpnn_cfg.rc;
+  // the message-passing judge needs edge >= rc; the surrogate needs edge >= 3
+  const double rc = L->lj ? (L->mode != JR_LEARNED_PLAIN ? (double)SG_RD : 0.0)
+                          : h->mpnn_cfg.rc;
   for (size_t w = 0; w < W; ++w) {
     if (!cubic_ok(cell + w * 9, rc))
       return h->fail(JR_E_UNSUPPORTED,
-                     "walker %zu: the message-passing judge needs a cubic cell with edge >= "
-                     "rc = %g", w, rc);
+                     "walker %zu: the learned walk needs a cubic cell with edge >= %g", w, rc);
     for (int i = 0; i < n_atoms[w]; ++i)
-      if (types[w * c.n_atoms_max + i] >= h->mpnn_cfg.n_species)
+      if (types[w * c.n_atoms_max + i] >= L->S)
         return h->fail(JR_E_ARG, "species out of range for the judge");
-    for (const jr_move& mv : h->learned->moves) {
+    for (const jr_move& mv : L->moves) {
       if (mv.type != JR_MOVE_VOLUME) continue;
+      if (L->lj) {
+        if (mv.min_vpa * n_atoms[w] < rc * rc * rc)
+          return h->fail(JR_E_UNSUPPORTED,
+                         "volume move: min_vpa * N = %g must be >= %g for the surrogate",
+                         mv.min_vpa * n_atoms[w], rc * rc * rc);
+        continue;
+      }
       // every volume that passes the cell check keeps the edge >= rc, and the
       // densest allowed cell keeps the judge's neighbour lists in capacity
       if (mv.min_vpa * n_atoms[w] < rc * rc * rc)
@@ -659,9 +690,8 @@ int learned_recompute_energies(jr_handle* h) {
   HIPCHK(h, dP.alloc(W));
   HIPCHK(h, hipMemcpyAsync(dP.p, pr.data(), sizeof(double) * W, hipMemcpyHostToDevice, s));
   HIPCHK(h, hipMemsetAsync(L->overflow.p, 0, sizeof(int), s));
-  MPNNArgs a = judge_args(L, h->pos.p, h->cell.p, W, h->types.p, h->n_atoms.p, nullptr,
-                          nullptr, dU.p, nullptr);
-  HIPCHK(h, run_judge(L, a, s));
+  HIPCHK(h, judge_eval(h, L, h->pos.p, h->cell.p, h->types.p, h->n_atoms.p, nullptr, nullptr, W,
+                       dU.p, nullptr, s));
   hipLaunchKernelGGL(lw_ham_kernel, dim3(blocks_for(W)), dim3(256), 0, s, W, nullptr, nullptr,
                      dU.p, h->cell.p, dP.p, nullptr, 0.0, 0.0, h->energy.p);
   HIPCHK(h, hipGetLastError());
@@ -760,9 +790,8 @@ hipError_t gmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
         hipLaunchKernelGGL(lw_reflect_kernel, dim3(blocks_for(C, 4)), dim3(256), 0, s, w.ch,
                            L->cdir.p, L->F.p, L->Hsy.p, L->thr_steer.p);
     } else {
-      e = run_judge(L, judge_args(L, L->cpos.p, L->ccell.p, C, L->ctypes.p, L->cn.p, nullptr,
-                                  nullptr, L->U.p, L->F.p),
-                    s);
+      e = judge_eval(h, L, L->cpos.p, L->ccell.p, L->ctypes.p, L->cn.p, nullptr, nullptr, C,
+                     L->U.p, L->F.p, s);
       if (e != hipSuccess) return e;
       w.ham_exact(nullptr, nullptr, L->U.p, L->ccell.p, L->Hy.p);
       if (!last)
@@ -777,9 +806,8 @@ hipError_t gmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
                        L->Hsy.p, L->thr_screen.p, L->code.p);
     hipLaunchKernelGGL(lw_compact_kernel, dim3(1), dim3(1024), 0, s, C, L->code.p, 1, 0,
                        nullptr, nullptr, 0.0, L->idx.p, L->cnt.p);
-    e = run_judge(L, judge_args(L, L->cpos.p, L->ccell.p, C, L->ctypes.p, L->cn.p, L->idx.p,
-                                L->cnt.p, L->U.p, nullptr),
-                  s);
+    e = judge_eval(h, L, L->cpos.p, L->ccell.p, L->ctypes.p, L->cn.p, L->idx.p, L->cnt.p, C,
+                   L->U.p, nullptr, s);
     if (e != hipSuccess) return e;
     w.ham_exact(L->idx.p, L->cnt.p, L->U.p, L->ccell.p, L->Hy.p);
     hipLaunchKernelGGL(lw_accept_kernel, dim3(gb), dim3(256), 0, s, C, L->code.p, L->Hy.p,
@@ -847,9 +875,8 @@ hipError_t volume_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
                        nullptr, nullptr, 0.0, L->idx.p, L->cnt.p);
     fin = L->code2.p;
   }
-  e = run_judge(L, judge_args(L, L->npos.p, L->ncell.p, C, L->ctypes.p, L->cn.p, L->idx.p,
-                              L->cnt.p, L->U.p, nullptr),
-                s);
+  e = judge_eval(h, L, L->npos.p, L->ncell.p, L->ctypes.p, L->cn.p, L->idx.p, L->cnt.p, C,
+                 L->U.p, nullptr, s);
   if (e != hipSuccess) return e;
   w.ham_exact(L->idx.p, L->cnt.p, L->U.p, L->ncell.p, L->Hy.p);
   hipLaunchKernelGGL(lw_accept_kernel, dim3(gb), dim3(256), 0, s, C, fin, L->Hy.p, L->cem.p,
@@ -860,6 +887,74 @@ hipError_t volume_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
   count_step(w, m, fin, 0, 1, 0, da ? 1 : 0);
   hipLaunchKernelGGL(lw_volume_commit_kernel, dim3(blocks_for((size_t)C * Nm * 3)), dim3(256),
                      0, s, w.ch, L->cpos.p, L->ccell.p, L->npos.p, L->ncell.p, L->accept.p);
+  return hipGetLastError();
+}
+
+// HMC: leapfrog with the judge's forces (plain) or the surrogate's
+// (screened), then the reference's test on the judge's energy, after the
+// surrogate screen when screened.  Forces at the end of a leapfrog step are
+// reused at the start of the next (the counters report the reference's
+// 2 n_leapfrog + 1 evaluations, the statistics the evaluations made).
+hipError_t hmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
+  LearnedState* L = w.L;
+  jr_handle* h = w.h;
+  hipStream_t s = w.s;
+  const int C = w.C, Nm = w.Nm;
+  const size_t xs = (size_t)w.n_steps * 3 * Nm;
+  const unsigned gw = blocks_for(C, 4), gc = blocks_for((size_t)C * Nm * 3);
+  hipLaunchKernelGGL(lw_hmc_begin_kernel, dim3(gw), dim3(256), 0, s, w.ch,
+                     h->normals.p + (size_t)t * 3 * Nm, xs, L->cpos.p, L->x0.p, L->mom.p,
+                     L->K0.p);
+  const int nl = mv.n_leapfrog;
+  hipError_t e;
+  auto eval = [&](bool last) -> hipError_t {
+    if (da)
+      return run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, L->F.p,
+                     last ? L->xstage.p : nullptr, s);
+    return judge_eval(h, L, L->cpos.p, L->ccell.p, L->ctypes.p, L->cn.p, nullptr, nullptr, C,
+                      L->U.p, L->F.p, s);
+  };
+  if ((e = eval(false)) != hipSuccess) return e;
+  for (int l = 0; l < nl; ++l) {
+    hipLaunchKernelGGL(lw_hmc_kick_drift_kernel, dim3(gc), dim3(256), 0, s, w.ch, L->mom.p,
+                       L->cpos.p, L->F.p, h->step_sizes.p, m, l == 0 ? 1 : 2, 1);
+    if ((e = eval(l == nl - 1)) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(lw_hmc_kick_drift_kernel, dim3(gc), dim3(256), 0, s, w.ch, L->mom.p,
+                     L->cpos.p, L->F.p, h->step_sizes.p, m, 1, 0);
+  const unsigned gb = blocks_for(C);
+  if (da) {
+    w.ham_sur(nullptr, nullptr, L->Us.p, L->ccell.p, L->Hsy.p);
+    hipLaunchKernelGGL(lw_stage1_kernel, dim3(gb), dim3(256), 0, s, C, nullptr, L->cHs.p,
+                       L->Hsy.p, L->thr_screen.p, L->code.p);
+    hipLaunchKernelGGL(lw_compact_kernel, dim3(1), dim3(1024), 0, s, C, L->code.p, 1, 0,
+                       nullptr, nullptr, 0.0, L->idx.p, L->cnt.p);
+    e = judge_eval(h, L, L->cpos.p, L->ccell.p, L->ctypes.p, L->cn.p, L->idx.p, L->cnt.p, C,
+                   L->U.p, nullptr, s);
+    if (e != hipSuccess) return e;
+    w.ham_exact(L->idx.p, L->cnt.p, L->U.p, L->ccell.p, L->Hy.p);
+    hipLaunchKernelGGL(lw_hmc_finish_kernel, dim3(gw), dim3(256), 0, s, w.ch, L->code.p,
+                       L->Hy.p, L->cH.p, L->K0.p, L->mom.p, L->cem.p, L->accept.p, L->reason.p,
+                       L->cpos.p, L->x0.p, L->cHs.p, L->Hsy.p);
+    record_exact(w, L->code.p, L->ccell.p);
+    count_step(w, m, L->code.p, 0, 1, 0, 1);
+    stat_add(w, &L->dstats.p->exact_e, L->cnt.p, 0, 1);
+    stat_add(w, &L->dstats.p->sur_e, nullptr, C, nl + 1);
+  } else {
+    w.ham_exact(nullptr, nullptr, L->U.p, L->ccell.p, L->Hy.p);
+    if (w.use_sur) {
+      // the surrogate's prediction at the judged end point: held-out error
+      e = run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, L->xstage.p, s);
+      if (e != hipSuccess) return e;
+      record_exact(w, nullptr, L->ccell.p);
+      stat_add(w, &L->dstats.p->sur_e, nullptr, C, 1);
+    }
+    hipLaunchKernelGGL(lw_hmc_finish_kernel, dim3(gw), dim3(256), 0, s, w.ch, nullptr, L->Hy.p,
+                       L->cH.p, L->K0.p, L->mom.p, L->cem.p, L->accept.p, L->reason.p,
+                       L->cpos.p, L->x0.p, nullptr, nullptr);
+    count_step(w, m, nullptr, 2 * nl + 1, 0, 2 * nl, 0);
+    stat_add(w, &L->dstats.p->exact_fe, nullptr, C, nl + 1);
+  }
   return hipGetLastError();
 }
 
@@ -956,6 +1051,8 @@ int learned_walk(jr_handle* h) {
   w.use_sur = L->mode != JR_LEARNED_PLAIN;
   w.cap = L->cfg.train_max_epa;
   w.ch = LWChains{L->C, h->n_walk, L->Nmax, c.n_moves, L->cn.p};
+  for (double mu : h->h_mu)
+    if (mu != 0.0) return h->fail(JR_E_UNSUPPORTED, "the learned walk is NPT (mu = 0)");
   // the shared move schedule decides which launches follow
   HIPCHK(h, hipMemcpyAsync(L->sched.data(), h->sched.p, sizeof(int) * c.n_mcmc_steps,
                            hipMemcpyDeviceToHost, s));
@@ -1001,7 +1098,9 @@ int learned_walk(jr_handle* h) {
   for (int t = 0; t < c.n_mcmc_steps; ++t) {
     const int m = L->sched[t];
     const jr_move& mv = L->moves[m];
-    hipError_t e = mv.type == JR_MOVE_GMC ? gmc_step(w, t, m, mv, da) : volume_step(w, t, m, mv, da);
+    hipError_t e = mv.type == JR_MOVE_GMC   ? gmc_step(w, t, m, mv, da)
+                   : mv.type == JR_MOVE_HMC ? hmc_step(w, t, m, mv, da)
+                                            : volume_step(w, t, m, mv, da);
     if (e != hipSuccess)
       return h->fail(JR_E_HIP, "learned walk step %d: %s", t, hipGetErrorString(e));
   }

@@ -199,6 +199,7 @@ int jr_eval_energy_forces(const jr_pot* pot, int32_t device, int32_t B,
   if (pressure) HIPCHK_G(h2d(d_pr.p, pressure, sizeof(double) * B));
   if (mu) HIPCHK_G(h2d(d_mu.p, mu, sizeof(double) * B * S));
   EvalArgs a;
+  std::memset(&a, 0, sizeof(a));
   a.pos = d_pos.p;
   a.types = d_t.p;
   a.n_atoms = d_n.p;
@@ -306,6 +307,7 @@ int recompute_energies(jr_handle* h) {
   HIPCHK(h, hipMemcpyAsync(dpr.p, pr.data(), pr.size() * 8, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(dmu.p, mu.data(), mu.size() * 8, hipMemcpyHostToDevice, h->stream));
   EvalArgs a;
+  std::memset(&a, 0, sizeof(a));
   a.pos = h->pos.p;
   a.types = h->types.p;
   a.n_atoms = h->n_atoms.p;
@@ -531,6 +533,34 @@ int set_geometry(jr_handle* h, int wpb, int occ) {
 
 }  // namespace
 
+namespace jrl {
+
+hipError_t lj_eval(jr_handle* h, const double* pos, const int* types, const int* n_atoms,
+                   const double* cell, const int* idx, const int* count, int B, double* U,
+                   double* F, hipStream_t s) {
+  EvalArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.pos = pos;
+  a.types = types;
+  a.n_atoms = n_atoms;
+  a.cell = cell;
+  a.cfg_index = idx;
+  a.count = count;
+  a.U = U;
+  a.F = F;
+  a.B = B;
+  a.Nmax = h->cfg.n_atoms_max;
+  a.S = h->cfg.pot.n_species;
+  a.rc2 = h->cfg.pot.rc2;
+  a.eps4 = h->eps4.p;
+  a.eps24 = h->eps24.p;
+  a.sig2 = h->sig2.p;
+  launch_eval(h->A, h->cfgbits, a, s);
+  return hipGetLastError();
+}
+
+}  // namespace jrl
+
 extern "C" {
 
 int jr_create(const jr_config* cfg, jr_handle** out) {
@@ -676,7 +706,7 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   }
   int ret = ensure_out(h, 1);
   if (ret) return fail(ret);
-  if (c.mpnn) {
+  if (c.mpnn || c.learned) {
     h->cfg.mpnn = nullptr;
     h->cfg.learned = nullptr;
     ret = jrl::learned_create(h, &c);

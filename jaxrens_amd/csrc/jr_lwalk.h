@@ -11,11 +11,17 @@
 // Volume (sampling/moves/volume.py:49-109, moves/_common.py:16-93):
 // V' = |V + s N xi|, cell and positions scaled by (V'/V)^(1/3); prior
 // (V'/V)^N unless flat; volume-per-atom and aspect bounds.
+// HMC (sampling/moves/hmc.py:51-152): p = xi; n_leapfrog x {p += s/2 F;
+// x += s p; F = F(x); p += s/2 F}; accept iff H' < emax and
+// (H' + K') - (H + K) < 1, else x reverts.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "jr_device.h"
+
+// FP64 move arithmetic rounds as written (the CPU restatement does the same)
+#pragma clang fp contract(off)
 #include "jr_types.h"
 #include "../../include/jaxrens_amd.h"
 
@@ -207,6 +213,84 @@ __global__ void lw_gmc_finish_kernel(LWChains ch, double* pos, double* dir, cons
   if ((int)((i % per) / 3) >= ch.n_atoms[c] || accept[c]) return;
   pos[i] = x0[i];
   dir[i] = -d0[i];
+}
+
+// ------------------------------------------------------------------- HMC
+// p = xi (component-major tape slot), x0 = x, K0 = p.p / 2 (one wave per chain).
+__global__ void __launch_bounds__(256) lw_hmc_begin_kernel(LWChains ch, const double* xi,
+                                                           size_t xi_stride, const double* pos,
+                                                           double* x0, double* p, double* K0) {
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= ch.C) return;
+  const int N = ch.n_atoms[c];
+  const size_t o = (size_t)c * ch.Nmax * 3;
+  const double* z = xi + (size_t)c * xi_stride;
+  for (int a = (int)(threadIdx.x & 63); a < N; a += 64)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      p[o + 3 * a + k] = z[(size_t)k * ch.Nmax + a];
+      x0[o + 3 * a + k] = pos[o + 3 * a + k];
+    }
+  const double pp = lw_chain_sum(N, [&](int a) {
+    const double* q = p + o + 3 * a;
+    return dot3(q[0], q[1], q[2], q[0], q[1], q[2]);
+  });
+  if ((threadIdx.x & 63) == 0) K0[c] = 0.5 * pp;
+}
+
+// kicks (one, or two: the closing half kick of the previous leapfrog step and
+// the opening one of this step, rounded separately), then the drift
+__global__ void lw_hmc_kick_drift_kernel(LWChains ch, double* p, double* pos, const double* F,
+                                         const double* step_sizes, int m, int kicks,
+                                         int drift) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t per = (size_t)ch.Nmax * 3;
+  if (i >= (size_t)ch.C * per) return;
+  const int c = (int)(i / per);
+  if ((int)((i % per) / 3) >= ch.n_atoms[c]) return;
+  const double s = step_sizes[(size_t)(c / ch.n_walk) * ch.n_moves + m];
+  const double h = 0.5 * s;
+  double q = p[i];
+  for (int k = 0; k < kicks; ++k) q = q + h * F[i];
+  p[i] = q;
+  if (drift) pos[i] = pos[i] + s * q;
+}
+
+// Final test: accept iff code == 1 (NULL = all), H' < emax and
+// (H' + K') - (H + K0) < 1; rejected chains return to x0.
+__global__ void __launch_bounds__(256) lw_hmc_finish_kernel(LWChains ch, const int* code,
+                                                            const double* Hy, double* Hcur,
+                                                            const double* K0, const double* p,
+                                                            const double* em, int* accept,
+                                                            int* reason, double* pos,
+                                                            const double* x0, double* Hs_cur,
+                                                            const double* Hs_new) {
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= ch.C) return;
+  const int N = ch.n_atoms[c];
+  const size_t o = (size_t)c * ch.Nmax * 3;
+  const double pp = lw_chain_sum(N, [&](int a) {
+    const double* q = p + o + 3 * a;
+    return dot3(q[0], q[1], q[2], q[0], q[1], q[2]);
+  });
+  const int k = code ? code[c] : 1;
+  bool acc = false;
+  if (k == 1) {
+    const double dH = (Hy[c] + 0.5 * pp) - (Hcur[c] + K0[c]);
+    acc = (Hy[c] < em[c]) && (dH < 1.0);
+  }
+  if (!acc)
+    for (int a = (int)(threadIdx.x & 63); a < N; a += 64)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) pos[o + 3 * a + q] = x0[o + 3 * a + q];
+  if ((threadIdx.x & 63) == 0) {
+    accept[c] = acc;
+    reason[c] = acc ? 0 : 1;
+    if (acc) {
+      Hcur[c] = Hy[c];
+      if (Hs_cur) Hs_cur[c] = Hs_new[c];
+    }
+  }
 }
 
 // ---------------------------------------------------------------- volume

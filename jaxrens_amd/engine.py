@@ -62,14 +62,14 @@ class NSEngine:
     def __init__(self, backend, moves, run: RunConfig, cell: CellSpec | dict | None = None,
                  learned=None):
         """``learned``: :class:`~jaxrens_amd.surrogate.LearnedConfig` of the
-        surrogate-screened walk (message-passing backend only; None walks
-        plain)."""
+        surrogate-screened walk.  The judge is the backend's model (the
+        message-passing potential, or the pair potential); with the
+        message-passing backend and ``learned`` None the walk is plain."""
         from .mpnn import MPNNBackend
         self.lib = _lib.load_library()
         self.model, p_default, mu_default = _model_of(backend)
-        self.is_learned = isinstance(self.model, MPNNBackend)
-        if learned is not None and not self.is_learned:
-            raise TypeError("a learned walk needs the message-passing backend")
+        self.is_mpnn = isinstance(self.model, MPNNBackend)
+        self.is_learned = self.is_mpnn or learned is not None
         self.mwg = MWG(moves, cell)
         self.run = run
         R, K = run.n_replicas, run.n_walkers
@@ -83,7 +83,7 @@ class NSEngine:
         self.n_walk = run.n_cull + run.n_extra
         self.n_moves = self.mwg.n_moves
         self._learned_keep = None
-        if self.is_learned:
+        if self.is_mpnn:
             self.S = self.model.params.S
             pot, self._pot_keep = _lib.JrPot(), None
         else:
@@ -107,11 +107,11 @@ class NSEngine:
         cfg.pot = pot
         cfg.moves = C.cast(self._moves_arr, C.POINTER(_lib.JrMove))
         cfg.move_weights = _lib.dptr(self._weights)
-        if self.is_learned:
+        if self.is_mpnn:
             cfg.mpnn = C.pointer(self.model.struct)
-            if learned is not None:
-                self._learned_st, self._learned_keep = learned.as_struct(self.S)
-                cfg.learned = C.pointer(self._learned_st)
+        if learned is not None:
+            self._learned_st, self._learned_keep = learned.as_struct(self.S)
+            cfg.learned = C.pointer(self._learned_st)
         h = C.c_void_p()
         _lib.check(self.lib.jr_create(C.byref(cfg), C.byref(h)))
         self.h = h

@@ -4,6 +4,7 @@ infrastructure only; the product path never imports this module).
 Follows the reference for the parts it has:
   select / clone      sampling/nested_sampling.py:86-133,151-186,486-518
   galilean (GMC)      sampling/moves/galilean.py:65-95,142-300
+  HMC                 sampling/moves/hmc.py:51-152
   volume              sampling/moves/volume.py:49-109, moves/_common.py:16-93,
                       utils/cell.py:105-125
   per-move counters   sampling/nested_sampling.py:533-609
@@ -27,7 +28,7 @@ import numpy as np
 from oracle import learned as OL
 from oracle import oracle as OR
 
-GMC, VOLUME = 0, 1
+GMC, VOLUME, HMC = 0, 1, 2
 
 
 def select(E, sel_u, clone_u, extra_u, n_extra):
@@ -63,6 +64,9 @@ def _ham_s(Us, cell, P, N, corr):
 
 
 class Models:
+    """The judge (message-passing parameters, or an oracle.oracle.Pot pair
+    table) and the surrogate (parameters, species, pair baseline)."""
+
     def __init__(self, judge_params, sur_params, S=2, base=None):
         self.jp = judge_params
         self.sp = sur_params
@@ -70,6 +74,9 @@ class Models:
         self.base = base
 
     def judge(self, pos, ty, cell, forces):
+        if isinstance(self.jp, OR.Pot):
+            u, _, f = OR.energy_forces(self.jp, pos, ty, cell, forces=forces)
+            return u, f
         e, f, _ = OL.mpnn_energy_forces(self.jp, pos, ty, cell, forces=forces)
         return e, f
 
@@ -135,6 +142,59 @@ def gmc(ch, mv, step, xi, models, emax, P, thr, da, stats, records, use_sur, cor
         ch["pos"][:N], ch["dir"][:N] = x0, -d0
     reason = 0 if acc else 1
     return reason, n_eval, n_grad
+
+
+def hmc(ch, mv, step, xi, models, emax, P, thr, da, stats, records, use_sur, corr=(0.0, 0.0)):
+    """One HMC move (sampling/moves/hmc.py:51-152): p = xi; n_leapfrog x
+    {p += s/2 F; x += s p; F = F(x); p += s/2 F}; accept iff H' < emax and
+    (H' + K') - (H + K) < 1.  Screened: the surrogate's forces drive the
+    leapfrog and its verdicts screen before the judge's test."""
+    N = ch["n"]
+    x0 = ch["pos"][:N].copy()
+    ty, cell = ch["types"][:N], ch["cell"]
+    p = xi[:, :N].T.copy()
+    K0 = 0.5 * np.sum(p * p)
+    h = 0.5 * step
+    force = (lambda x: models.sur(x, ty, cell, True)) if da else \
+        (lambda x: models.judge(x, ty, cell, True))
+    x = x0.copy()
+    E, F = force(x)
+    for _ in range(mv["n_leapfrog"]):
+        p = p + h * F
+        x = x + step * p
+        E, F = force(x)
+        p = p + h * F
+    K1 = 0.5 * np.sum(p * p)
+    nl = mv["n_leapfrog"]
+    code = 1
+    if da:
+        stats["sur_e"] += nl + 1
+        Hs_y = _ham_s(E, cell, P, N, corr)
+        code = 1 if (ch["Hs"] < thr[1]) == (Hs_y < thr[1]) else 0
+        acc = False
+        if code == 1:
+            U = models.judge(x, ty, cell, False)[0]
+            Hy = _ham(U, cell, P)
+            acc = (Hy < emax) and ((Hy + K1) - (ch["H"] + K0) < 1.0)
+            stats["exact_e"] += 1
+            stats["stage1_pass"] += 1
+            records.append((U, E, N, _det(cell)))
+        n_eval, n_grad = code, 0
+    else:
+        U = E
+        Hy = _ham(U, cell, P)
+        acc = (Hy < emax) and ((Hy + K1) - (ch["H"] + K0) < 1.0)
+        stats["exact_fe"] += nl + 1
+        if use_sur:
+            records.append((U, models.sur(x, ty, cell, False)[0], N, _det(cell)))
+            stats["sur_e"] += 1
+        n_eval, n_grad = 2 * nl + 1, 2 * nl
+    if acc:
+        ch["pos"][:N] = x
+        ch["H"] = Hy
+        if da:
+            ch["Hs"] = Hs_y
+    return (0 if acc else 1), n_eval, n_grad
 
 
 def volume(ch, mv, step, xi0, u0, models, emax, P, thr, da, stats, records, use_sur,
@@ -232,6 +292,9 @@ def ns_iteration(pop, models, pressure, step_sizes, moves, tape, it, n_extra, mo
             xi = tape.normals[it, r, j, t]
             if mv["type"] == GMC:
                 rs, ne, ng = gmc(ch, mv, step, xi, models, emax, pressure[r], thr, da,
+                                 stats, step_records, use_sur, corr)
+            elif mv["type"] == HMC:
+                rs, ne, ng = hmc(ch, mv, step, xi, models, emax, pressure[r], thr, da,
                                  stats, step_records, use_sur, corr)
             else:
                 rs, ne, ng = volume(ch, mv, step, xi[0, 0], tape.uniforms[it, r, j, t, 0],

@@ -15,13 +15,13 @@
 import numpy as np
 import pytest
 
-from jaxrens_amd.backends import EnsembleTerm, Hamiltonian
+from jaxrens_amd.backends import EnsembleTerm, Hamiltonian, LJBackend
 from jaxrens_amd.draws import make_tape
 from jaxrens_amd.engine import NSEngine, RunConfig
 from jaxrens_amd.mpnn import MPNNBackend, init_mpnn
 from jaxrens_amd.surrogate import LearnedConfig, init_surrogate
 from oracle import learned_walk as OW
-from oracle.oracle import Population
+from oracle.oracle import Pot, Population
 
 pytestmark = pytest.mark.gpu
 
@@ -34,7 +34,7 @@ BASE = (1.0, 0.85, 1.2)
 def _moves_dicts(eng):
     out = []
     for m in eng.mwg.device_table():
-        out.append({"type": m.type, "n_reflect": m.n_reflect,
+        out.append({"type": m.type, "n_reflect": m.n_reflect, "n_leapfrog": m.n_leapfrog,
                     "perturb_angle": m.perturb_angle, "flat_v_prior": m.flat_v_prior,
                     "max_vpa": m.max_vpa, "min_vpa": m.min_vpa,
                     "min_aspect": m.min_aspect})
@@ -108,6 +108,119 @@ def test_learned_walk_matches_oracle(mode):
     np.testing.assert_allclose(pop_gpu["cell"].reshape(R, K, 9), pop0.cell, rtol=1e-12)
     np.testing.assert_allclose(pop_gpu["energy"], pop0.energy,
                                atol=1e-4 * (np.abs(pop0.energy).max() + N))
+
+
+# LJ judge with HMC + GMC (+ volume): the surrogate's pair baseline is the
+# r^-12 part of LJ (eps = sigma = 1, rc = 2.5), the network learns the rest
+LJ_BASE = (4.0, 1.0, 2.5)
+LJ_MOVES = [{"type": "hmc", "n_leapfrog": 5, "step_size": 0.005, "weight": 2},
+            {"type": "galilean", "n_reflect": 4, "step_size": 0.05, "weight": 2},
+            {"type": "volume", "step_size": 0.3, "weight": 1}]
+
+
+def _lj_engine(mode, R, K, N, n_extra, n_steps, seed, pressures, vpa=(1.2, 3.0),
+               moves=LJ_MOVES, learned_kw=None):
+    lj = LJBackend(1.0, 1.0, 2.5)
+    cell = {"min_volume_per_atom": vpa[0], "max_volume_per_atom": vpa[1],
+            "flat_V_prior": True}
+    run = RunConfig(n_replicas=R, n_walkers=K, n_atoms=N, n_mcmc_steps=n_steps,
+                    n_extra=n_extra, seed=seed, pressures=pressures, re_interval=0)
+    kw = dict(mode=mode, ring_capacity=4096, train_batch=16, train_steps=0, min_train=32,
+              min_heldout=8, lr=2e-3, rmse_bound=0.05, delta_k=2.0, baseline=LJ_BASE,
+              params=init_surrogate(6, S=1))
+    kw.update(learned_kw or {})
+    eng = NSEngine(Hamiltonian(lj, [EnsembleTerm(pressures[0])]), moves, run, cell=cell,
+                   learned=LearnedConfig(**kw))
+    eng.init_walkers(np.random.default_rng(seed), vpa[0] * 1.001, vpa[1], True)
+    return eng, lj
+
+
+@pytest.mark.parametrize("mode", ["plain", "auto", "screened"])
+def test_learned_walk_lj_hmc_matches_oracle(mode):
+    """The learned walk with the FP64 pair potential as the judge and HMC,
+    GMC and volume moves: one tape-driven iteration against the oracle (the
+    judge is bit-exact with the oracle's, so decisions and counters must
+    match exactly and positions to rounding)."""
+    R, K, N, n_extra, T = 2, 4, 32, 2, 8
+    print(f"learned walk, LJ judge, mode={mode} R={R} K={K} N={N} steps={T}", flush=True)
+    eng, lj = _lj_engine(mode, R, K, N, n_extra, T, seed=12, pressures=[0.5, 1.0])
+    # relax the random initial walkers with the pair-potential walk first, so
+    # that HMC proposals are accepted and trajectories are well conditioned
+    cell = {"min_volume_per_atom": 1.2, "max_volume_per_atom": 3.0, "flat_V_prior": True}
+    pre = NSEngine(Hamiltonian(lj, [EnsembleTerm(0.5)]), LJ_MOVES,
+                   RunConfig(n_replicas=R, n_walkers=K, n_atoms=N, n_mcmc_steps=T,
+                             n_extra=n_extra, seed=13, pressures=[0.5, 1.0]), cell=cell)
+    pre.set_population(*(eng.get_population()[k] for k in ("positions", "types", "cell")))
+    pre.ns_iterations(40)
+    rel = pre.get_population()
+    eng.set_population(rel["positions"], rel["types"], rel["cell"], direction=rel["direction"])
+    pop0 = Population.from_engine_dict(eng.get_population())
+    tape = make_tape(np.random.default_rng(8), 1, R, K, N, T, n_extra, [2, 2, 1])
+    out = eng.ns_iterations(1, tape=tape)
+    st = eng.learned_stats()
+    pop_gpu = eng.get_population()
+    models = OW.Models(Pot.from_table(lj.table), init_surrogate(6, S=1), S=1, base=LJ_BASE)
+    omode = {"plain": "plain", "auto": "plain+sur", "screened": "screened"}[mode]
+    oout, ost, recs = OW.ns_iteration(pop0, models, eng.pressure, eng.step_sizes,
+                                      _moves_dicts(eng), tape, 0, n_extra, omode,
+                                      train_max_epa=50.0)
+    print("  counters gpu", out["counters"][0].tolist(), flush=True)
+    print("  counters ora", oout["counters"].tolist(), flush=True)
+    assert np.array_equal(out["walk"][0], oout["walk"])
+    assert np.array_equal(out["counters"][0], oout["counters"])
+    hmc_acc = out["counters"][0][:, 0, 1].sum()
+    assert hmc_acc > 0, "no HMC proposal accepted: the accept path is untested"
+    for k in ("proposals", "stage1_pass", "accepted"):
+        assert st[k] == ost[k], k
+    assert st["exact_force_evals"] == ost["exact_fe"]
+    assert st["exact_energy_evals"] == ost["exact_e"]
+    L = pop_gpu["cell"][..., 0, 0].max()
+    # screened: the surrogate's BF16 matrix-core forces (stated accuracy 2e-2 |F|
+    # against the BF16-emulating oracle) steer HMC and GMC, so trajectories
+    # agree to that order; plain/auto: FP64 judge forces, rounding only
+    tol = 1e-9 if mode != "screened" else 1e-3
+    np.testing.assert_allclose(pop_gpu["positions"], pop0.pos, atol=tol * L, rtol=0)
+    np.testing.assert_allclose(pop_gpu["cell"].reshape(R, K, 9), pop0.cell, rtol=1e-12)
+    if mode != "screened":
+        np.testing.assert_allclose(pop_gpu["energy"], pop0.energy, rtol=1e-10, atol=1e-10)
+    else:
+        # positions differ at the surrogate-force level: every stored energy must
+        # be the FP64 judge's enthalpy of the stored configuration itself
+        from oracle.oracle import energy_forces as or_ef
+        pot = Pot.from_table(lj.table)
+        for r in range(R):
+            for k in range(K):
+                c = pop_gpu["cell"][r, k]
+                _, Hr, _ = or_ef(pot, pop_gpu["positions"][r, k], pop_gpu["types"][r, k], c,
+                                 pressure=eng.pressure[r], forces=False)
+                assert pop_gpu["energy"][r, k] == pytest.approx(Hr, rel=1e-12, abs=1e-10)
+
+
+def test_lj_hmc_screened_keeps_log_z():
+    """LJ judge, HMC + GMC: log Z of 16 independent runs per walk, the
+    screened walk with online training against the plain walk (2 sigma)."""
+    R, K, N, n_iter = 16, 32, 8, 320
+    moves = [{"type": "hmc", "n_leapfrog": 5, "step_size": 0.01, "weight": 1},
+             {"type": "galilean", "n_reflect": 4, "step_size": 0.15, "weight": 2},
+             {"type": "volume", "step_size": 1.0, "weight": 1}]
+    res = {}
+    for mode, seed in (("plain", 41), ("auto", 42)):
+        print(f"log Z LJ: {mode}", flush=True)
+        eng, _ = _lj_engine(mode, R, K, N, 0, 12, seed, [1.0] * R, vpa=(4.0, 40.0),
+                            moves=moves,
+                            learned_kw=dict(train_steps=4, min_train=64, rmse_bound=0.2))
+        eng.ns_iterations(n_iter)
+        res[mode] = (eng.final_log_evidence(), eng.learned_stats())
+    se = lambda a: np.std(a, ddof=1) / np.sqrt(len(a))  # noqa: E731
+    lz_p, lz_a = res["plain"][0], res["auto"][0]
+    z = (lz_a.mean() - lz_p.mean()) / np.hypot(se(lz_a), se(lz_p))
+    st = res["auto"][1]
+    print(f"  plain log Z {lz_p.mean():.4f} +- {se(lz_p):.4f}; auto {lz_a.mean():.4f} +- "
+          f"{se(lz_a):.4f}; z = {z:+.2f}; screened iters {st['iterations_screened']} "
+          f"stage-1 pass {st['stage1_pass']} of {st['proposals']} accepted {st['accepted']} "
+          f"rmse {st['heldout_rmse']:.4f}", flush=True)
+    assert abs(z) < 2.0
+    assert st["iterations_screened"] > 0
 
 
 def _log_z_runs(mode, n_iter, seed, learned_kw=None):
