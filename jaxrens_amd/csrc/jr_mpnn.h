@@ -24,6 +24,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "jr_images.h"
+
 // FP32 judge: multiply-adds contract to FMA here (the library's FP64 walk
 // keeps -ffp-contract=off for its canonical order; this file is compared
 // with the FP64 oracle to a stated tolerance, not bit for bit).
@@ -187,8 +189,10 @@ __global__ void __launch_bounds__(MP_THREADS)
         float d[3] = {uj.x - ui.x, uj.y - ui.y, uj.z - ui.z};
 #pragma unroll
         for (int k = 0; k < 3; ++k) d[k] -= rintf(d[k]);
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
+        const int ax = image_axes(d, rcf);
+        // the images of the axes in ax, ascending (subsets of ax)
+#pragma unroll 1
+        for (int m = 0, go = 1; go; m = (m - ax) & ax, go = m != 0) {
           float e0 = (m & 1) ? d[0] - copysignf(1.0f, d[0]) : d[0];
           float e1 = (m & 2) ? d[1] - copysignf(1.0f, d[1]) : d[1];
           float e2 = (m & 4) ? d[2] - copysignf(1.0f, d[2]) : d[2];

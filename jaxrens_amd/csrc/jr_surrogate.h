@@ -13,6 +13,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "jr_images.h"
+
 // FP32 surrogate: multiply-adds contract to FMA (see jr_mpnn.h).
 #pragma clang fp contract(fast)
 
@@ -230,8 +232,10 @@ __global__ void __launch_bounds__(256) sur_eval_kernel(SurEvalArgs a) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) d[k] -= rintf(d[k]);
         const int tj = __float_as_int(uj.w);
+        const int ax = image_axes(d, SG_RD * iLf);
+        // the images of the axes in ax, ascending (subsets of ax)
 #pragma unroll 1
-        for (int m = 0; m < 8; ++m) {
+        for (int m = 0, go = 1; go; m = (m - ax) & ax, go = m != 0) {
           const float e0 = (m & 1) ? d[0] - copysignf(1.0f, d[0]) : d[0];
           const float e1 = (m & 2) ? d[1] - copysignf(1.0f, d[1]) : d[1];
           const float e2 = (m & 4) ? d[2] - copysignf(1.0f, d[2]) : d[2];
@@ -360,8 +364,9 @@ __global__ void __launch_bounds__(256) sur_eval_kernel(SurEvalArgs a) {
         for (int k = 0; k < 3; ++k) d[k] -= rintf(d[k]);
         const int tj = __float_as_int(uj.w);
         const float* other = dX + j * 64 + ti * SG_K;
+        const int ax = image_axes(d, SG_RD * iLf);
 #pragma unroll 1
-        for (int m = 0; m < 8; ++m) {
+        for (int m = 0, go = 1; go; m = (m - ax) & ax, go = m != 0) {
           const float e0 = (m & 1) ? d[0] - copysignf(1.0f, d[0]) : d[0];
           const float e1 = (m & 2) ? d[1] - copysignf(1.0f, d[1]) : d[1];
           const float e2 = (m & 4) ? d[2] - copysignf(1.0f, d[2]) : d[2];
