@@ -194,8 +194,6 @@ extern "C" int jr_mpnn_eval(const jr_mpnn* net, int32_t device, int32_t B, int32
 
 namespace {
 
-constexpr size_t kSgLds = sizeof(float) * 4 * 256 + sizeof(__bf16) * 256 * (SG_D + SG_H);
-
 // Device parameters of the surrogate: FP32 master copy and BF16 operands.
 struct SurParams {
   DBuf<float> P;
@@ -215,9 +213,9 @@ struct SurParams {
   SurDev dev() const { return SurDev{P.p, W1t.p, W1r.p, W2t.p, W2r.p}; }
 };
 
-// Baseline pair term (eps, sig, rc) into the kernel's parameters; base ==
+// Baseline pair term (eps, sig, rc) into the kernels' parameters; base ==
 // NULL or eps == 0 switches it off.
-int set_baseline(SurEvalArgs& a, const double* base) {
+int set_baseline(SurCfgArgs& a, const double* base) {
   a.b_eps = a.b_sig2 = a.b_rc2 = a.b_shift = 0.0f;
   if (!base || base[0] == 0.0) return JR_OK;
   if (!(base[1] > 0.0) || !(base[2] > 0.0) || base[2] > (double)SG_RD)
@@ -229,11 +227,33 @@ int set_baseline(SurEvalArgs& a, const double* base) {
   return JR_OK;
 }
 
-hipError_t launch_sur_eval(const SurEvalArgs& a, int grid, hipStream_t s) {
-  hipError_t e = hipFuncSetAttribute((const void*)sur_eval_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSgLds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(sur_eval_kernel, dim3(grid), dim3(256), kSgLds, s, a);
+// The surrogate's passes over the items of c (c.X, c.Ebase and, with F,
+// c.dX set by the caller): E [cfg] and F [cfg][Nmax][3] (NULL: energy only).
+hipError_t sur_passes(const SurDev& net, SurCfgArgs c, float* erow, double* E, double* F,
+                      hipStream_t s) {
+  if (c.B < 1) return hipSuccess;
+  hipLaunchKernelGGL(sur_desc_kernel, dim3(c.B), dim3(256), 0, s, c);
+  SurMlpArgs m;
+  m.net = net;
+  m.B = c.B;
+  m.Nmax = c.Nmax;
+  m.tiles = (c.Nmax + 31) / 32;
+  m.cfg_index = c.cfg_index;
+  m.count = c.count;
+  m.X = c.X;
+  m.erow = erow;
+  m.dX = F ? const_cast<float*>(c.dX) : nullptr;
+  hipLaunchKernelGGL(sur_mlp_kernel, dim3(c.B * m.tiles), dim3(64), 0, s, m);
+  hipLaunchKernelGGL(sur_energy_kernel, dim3((c.B + 3) / 4), dim3(256), 0, s, c.B, c.Nmax,
+                     c.cfg_index, c.count, c.n_atoms, erow, net.P, c.Ebase, E);
+  if (F) {
+    c.F = F;
+    const size_t lds = sizeof(float) * (4 * 256 + (size_t)c.Nmax * 64);
+    hipError_t e = hipFuncSetAttribute((const void*)sur_force_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sur_force_kernel, dim3(c.B), dim3(256), lds, s, c);
+  }
   return hipGetLastError();
 }
 
@@ -268,6 +288,10 @@ extern "C" int jr_sur_eval(const float* params, const double* base, int32_t devi
       if (t < 0 || t >= n_species) return set_global_err(JR_E_ARG, "species %d out of range", t);
     }
   }
+  SurCfgArgs c;
+  std::memset(&c, 0, sizeof(c));
+  int brc = set_baseline(c, base);
+  if (brc) return brc;
   HIPCHK_G(hipSetDevice(device));
   StreamHolder sh;
   HIPCHK_G(hipStreamCreate(&sh.s));
@@ -277,46 +301,41 @@ extern "C" int jr_sur_eval(const float* params, const double* base, int32_t devi
   HIPCHK_G(hipMemcpyAsync(w.P.p, params, sizeof(float) * SG_NP, hipMemcpyHostToDevice, s));
   w.cast(s);
   HIPCHK_G(hipGetLastError());
-  DBuf<double> d_pos, d_cell, d_U, d_F;
+  const size_t rows = (size_t)B * Nmax;
+  DBuf<double> d_pos, d_cell, d_U, d_F, d_Eb;
   DBuf<int> d_t, d_n;
   DBuf<__bf16> d_x;
-  HIPCHK_G(d_pos.alloc((size_t)B * Nmax * 3));
+  DBuf<float> d_erow, d_dX;
+  HIPCHK_G(d_pos.alloc(rows * 3));
   HIPCHK_G(d_cell.alloc((size_t)B * 9));
-  HIPCHK_G(d_t.alloc((size_t)B * Nmax));
+  HIPCHK_G(d_t.alloc(rows));
   HIPCHK_G(d_n.alloc((size_t)B));
   HIPCHK_G(d_U.alloc((size_t)B));
-  if (F) HIPCHK_G(d_F.alloc((size_t)B * Nmax * 3));
-  if (x_out) HIPCHK_G(d_x.alloc((size_t)B * Nmax * SG_D));
-  HIPCHK_G(hipMemcpyAsync(d_pos.p, pos, sizeof(double) * B * Nmax * 3, hipMemcpyHostToDevice, s));
+  HIPCHK_G(d_Eb.alloc((size_t)B));
+  HIPCHK_G(d_x.alloc(rows * SG_D));
+  HIPCHK_G(d_erow.alloc(rows));
+  if (F) {
+    HIPCHK_G(d_F.alloc(rows * 3));
+    HIPCHK_G(d_dX.alloc(rows * 64));
+  }
+  HIPCHK_G(hipMemcpyAsync(d_pos.p, pos, sizeof(double) * rows * 3, hipMemcpyHostToDevice, s));
   HIPCHK_G(hipMemcpyAsync(d_cell.p, cell, sizeof(double) * B * 9, hipMemcpyHostToDevice, s));
-  HIPCHK_G(hipMemcpyAsync(d_t.p, types, sizeof(int) * B * Nmax, hipMemcpyHostToDevice, s));
+  HIPCHK_G(hipMemcpyAsync(d_t.p, types, sizeof(int) * rows, hipMemcpyHostToDevice, s));
   HIPCHK_G(hipMemcpyAsync(d_n.p, n_atoms, sizeof(int) * B, hipMemcpyHostToDevice, s));
-  if (F) HIPCHK_G(hipMemsetAsync(d_F.p, 0, sizeof(double) * B * Nmax * 3, s));
-  if (x_out) HIPCHK_G(hipMemsetAsync(d_x.p, 0, sizeof(__bf16) * B * Nmax * SG_D, s));
-  SurEvalArgs a;
-  a.net = w.dev();
-  a.B = B;
-  a.Nmax = Nmax;
-  a.S = n_species;
-  a.pos = d_pos.p;
-  a.types = d_t.p;
-  a.n_atoms = d_n.p;
-  a.cell = d_cell.p;
-  a.cfg_index = nullptr;
-  a.count = nullptr;
-  a.E = d_U.p;
-  a.F = F ? d_F.p : nullptr;
-  a.xout = x_out ? d_x.p : nullptr;
-  a.Ebase = nullptr;
-  int brc = set_baseline(a, base);
-  if (brc) return brc;
-  HIPCHK_G(launch_sur_eval(a, std::min(B, device_cus(device)), s));
+  c.B = B;
+  c.Nmax = Nmax;
+  c.pos = d_pos.p;
+  c.types = d_t.p;
+  c.n_atoms = d_n.p;
+  c.cell = d_cell.p;
+  c.X = d_x.p;
+  c.dX = d_dX.p;
+  c.Ebase = d_Eb.p;
+  HIPCHK_G(sur_passes(w.dev(), c, d_erow.p, d_U.p, F ? d_F.p : nullptr, s));
   if (U) HIPCHK_G(hipMemcpyAsync(U, d_U.p, sizeof(double) * B, hipMemcpyDeviceToHost, s));
-  if (F)
-    HIPCHK_G(hipMemcpyAsync(F, d_F.p, sizeof(double) * B * Nmax * 3, hipMemcpyDeviceToHost, s));
+  if (F) HIPCHK_G(hipMemcpyAsync(F, d_F.p, sizeof(double) * rows * 3, hipMemcpyDeviceToHost, s));
   if (x_out)
-    HIPCHK_G(hipMemcpyAsync(x_out, d_x.p, sizeof(__bf16) * B * Nmax * SG_D,
-                            hipMemcpyDeviceToHost, s));
+    HIPCHK_G(hipMemcpyAsync(x_out, d_x.p, sizeof(__bf16) * rows * SG_D, hipMemcpyDeviceToHost, s));
   HIPCHK_G(hipStreamSynchronize(s));
   return JR_OK;
 }
@@ -416,7 +435,8 @@ struct LearnedState {
   DBuf<double> cpos, cdir, x0, d0, npos, F, mom, ccell, ncell, cH, cHs, cem, cpres, thr_steer,
       thr_screen, U, Us, Ub, Hy, Hsy, K0;
   DBuf<int> ctypes, cn, code, code2, accept, reason, idx, idx2, cnt;
-  DBuf<__bf16> xstage;
+  DBuf<__bf16> xstage;     // [C][Nmax][80] descriptor rows of the latest surrogate pass
+  DBuf<float> erow, dXg;    // [C][Nmax] row energies, [C][Nmax][64] dE/dG rows
   DBuf<unsigned> nbl;
   DBuf<int> overflow;
   // training ring and minibatch
@@ -494,26 +514,27 @@ hipError_t judge_eval(jr_handle* h, LearnedState* L, const double* pos, const do
   return run_judge(L, judge_args(L, pos, cell, B, types, n, idx, count, U, F), s);
 }
 
+// The surrogate on chain arrays (items idx/count, or all C): energies E and
+// forces F (NULL: energy only); the descriptor rows land in xstage and the
+// baseline energies in Ub for the training ring.
 hipError_t run_sur(LearnedState* L, const double* pos, const double* cell, const int* idx,
-                   const int* count, double* E, double* F, __bf16* xout, hipStream_t s) {
-  SurEvalArgs a;
+                   const int* count, double* E, double* F, hipStream_t s) {
+  SurCfgArgs c;
+  std::memset(&c, 0, sizeof(c));
   const double base[3] = {L->cfg.base_eps, L->cfg.base_sig, L->cfg.base_rc};
-  (void)set_baseline(a, base);  // validated at creation
-  a.Ebase = xout ? L->Ub.p : nullptr;  // training targets exclude the baseline
-  a.net = L->sur.dev();
-  a.B = L->C;
-  a.Nmax = L->Nmax;
-  a.S = L->S;
-  a.pos = pos;
-  a.types = L->ctypes.p;
-  a.n_atoms = L->cn.p;
-  a.cell = cell;
-  a.cfg_index = idx;
-  a.count = count;
-  a.E = E;
-  a.F = F;
-  a.xout = xout;
-  return launch_sur_eval(a, std::max(1, std::min(L->C, L->n_cu)), s);
+  (void)set_baseline(c, base);  // validated at creation
+  c.B = L->C;
+  c.Nmax = L->Nmax;
+  c.pos = pos;
+  c.types = L->ctypes.p;
+  c.n_atoms = L->cn.p;
+  c.cell = cell;
+  c.cfg_index = idx;
+  c.count = count;
+  c.X = L->xstage.p;
+  c.dX = L->dXg.p;
+  c.Ebase = L->Ub.p;
+  return sur_passes(L->sur.dev(), c, L->erow.p, E, F, s);
 }
 
 }  // namespace
@@ -611,6 +632,8 @@ int learned_create(jr_handle* h, const jr_config* c) {
     HIPCHK(h, hipMemsetAsync(L->adam_m.p, 0, sizeof(float) * SG_NP, s));
     HIPCHK(h, hipMemsetAsync(L->adam_v.p, 0, sizeof(float) * SG_NP, s));
     HIPCHK(h, L->xstage.alloc(C * Nm * SG_D));
+    HIPCHK(h, L->erow.alloc(C * Nm));
+    HIPCHK(h, L->dXg.alloc(C * Nm * 64));
     HIPCHK(h, hipMemsetAsync(L->xstage.p, 0, sizeof(__bf16) * C * Nm * SG_D, s));
     const size_t cap = (size_t)lc.ring_capacity;
     HIPCHK(h, L->ring.alloc(cap * Nm * SG_D));
@@ -782,8 +805,7 @@ hipError_t gmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
     hipLaunchKernelGGL(lw_advance_kernel, dim3(gc), dim3(256), 0, s, w.ch, L->cpos.p,
                        L->cdir.p, h->step_sizes.p, m);
     if (da) {
-      e = run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, L->F.p,
-                  last ? L->xstage.p : nullptr, s);
+      e = run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, L->F.p, s);
       if (e != hipSuccess) return e;
       w.ham_sur(nullptr, nullptr, L->Us.p, L->ccell.p, L->Hsy.p);
       if (!last)
@@ -821,7 +843,7 @@ hipError_t gmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
                        L->cem.p, L->accept.p, L->reason.p, L->cH.p, nullptr, nullptr);
     if (w.use_sur) {
       // the surrogate's prediction at the judged end point: held-out error
-      e = run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, L->xstage.p, s);
+      e = run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s);
       if (e != hipSuccess) return e;
       record_exact(w, nullptr, L->ccell.p);
       stat_add(w, &L->dstats.p->sur_e, nullptr, C, 1);
@@ -863,7 +885,7 @@ hipError_t volume_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
   hipError_t e;
   const int* fin = L->code.p;  // codes of the final test
   if (da || w.use_sur) {
-    e = run_sur(L, L->npos.p, L->ncell.p, L->idx.p, L->cnt.p, L->Us.p, nullptr, L->xstage.p, s);
+    e = run_sur(L, L->npos.p, L->ncell.p, L->idx.p, L->cnt.p, L->Us.p, nullptr, s);
     if (e != hipSuccess) return e;
     stat_add(w, &L->dstats.p->sur_e, L->cnt.p, 0, 1);
   }
@@ -907,18 +929,17 @@ hipError_t hmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
                      L->K0.p);
   const int nl = mv.n_leapfrog;
   hipError_t e;
-  auto eval = [&](bool last) -> hipError_t {
+  auto eval = [&]() -> hipError_t {
     if (da)
-      return run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, L->F.p,
-                     last ? L->xstage.p : nullptr, s);
+      return run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, L->F.p, s);
     return judge_eval(h, L, L->cpos.p, L->ccell.p, L->ctypes.p, L->cn.p, nullptr, nullptr, C,
                       L->U.p, L->F.p, s);
   };
-  if ((e = eval(false)) != hipSuccess) return e;
+  if ((e = eval()) != hipSuccess) return e;
   for (int l = 0; l < nl; ++l) {
     hipLaunchKernelGGL(lw_hmc_kick_drift_kernel, dim3(gc), dim3(256), 0, s, w.ch, L->mom.p,
                        L->cpos.p, L->F.p, h->step_sizes.p, m, l == 0 ? 1 : 2, 1);
-    if ((e = eval(l == nl - 1)) != hipSuccess) return e;
+    if ((e = eval()) != hipSuccess) return e;
   }
   hipLaunchKernelGGL(lw_hmc_kick_drift_kernel, dim3(gc), dim3(256), 0, s, w.ch, L->mom.p,
                      L->cpos.p, L->F.p, h->step_sizes.p, m, 1, 0);
@@ -944,7 +965,7 @@ hipError_t hmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
     w.ham_exact(nullptr, nullptr, L->U.p, L->ccell.p, L->Hy.p);
     if (w.use_sur) {
       // the surrogate's prediction at the judged end point: held-out error
-      e = run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, L->xstage.p, s);
+      e = run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s);
       if (e != hipSuccess) return e;
       record_exact(w, nullptr, L->ccell.p);
       stat_add(w, &L->dstats.p->sur_e, nullptr, C, 1);
@@ -1090,7 +1111,7 @@ int learned_walk(jr_handle* h) {
                        L->thr_steer.p, L->thr_screen.p);
     // the surrogate's verdict on every chain's current state, with the
     // parameters this iteration uses throughout
-    HIPCHK(h, run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, nullptr, s));
+    HIPCHK(h, run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s));
     w.ham_sur(nullptr, nullptr, L->Us.p, L->ccell.p, L->cHs.p);
     hipLaunchKernelGGL(lw_stat_add_kernel, dim3(1), dim3(64), 0, s, &L->dstats.p->sur_e,
                        nullptr, (long long)L->C, 1LL);

@@ -83,10 +83,10 @@ __device__ __forceinline__ void wave_gemm(const __bf16* A, int lda, const __bf16
   }
 }
 
-template <int NB>
-__device__ __forceinline__ void zero_acc(f16v (&c)[2][NB]) {
+template <int RB, int NB>
+__device__ __forceinline__ void zero_acc(f16v (&c)[RB][NB]) {
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
+  for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
@@ -108,7 +108,7 @@ __device__ __forceinline__ void sur_forward_rows(const SurDev& W, const __bf16* 
                                                  __bf16* H, f16v (&z1)[2][4],
                                                  f16v (&z2)[2][4], float (&e)[2][16]) {
   const int l = sg_lane();
-  zero_acc<4>(z1);
+  zero_acc<2, 4>(z1);
   wave_gemm<SG_D, 4>(X, SG_D, W.W1t, SG_D, z1);
 #pragma unroll
   for (int nb = 0; nb < 4; ++nb) {
@@ -123,7 +123,7 @@ __device__ __forceinline__ void sur_forward_rows(const SurDev& W, const __bf16* 
       }
   }
   sg_wave_sync();
-  zero_acc<4>(z2);
+  zero_acc<2, 4>(z2);
   wave_gemm<SG_H, 4>(H, SG_H, W.W2t, SG_H, z2);
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb)
@@ -155,22 +155,36 @@ __device__ __forceinline__ void sur_forward_rows(const SurDev& W, const __bf16* 
 }
 
 // ---------------------------------------------------------------- eval
-struct SurEvalArgs {
-  SurDev net;
-  int B, Nmax, S;
+// ---------------------------------------------------------------- eval
+// Energy (and forces) of a batch of configurations in three passes:
+//   sur_desc_kernel   one block per configuration: descriptor rows (BF16,
+//                     [B][Nmax][80], zero beyond N) and the pair baseline
+//   sur_mlp_kernel    one wave per 32-row tile on the matrix cores: row
+//                     energies and, for forces, dE/dG rows ([B][Nmax][64])
+//   sur_energy_kernel sum of the row energies, N b3 and the baseline
+//   sur_force_kernel  one block per configuration: pair forces from the
+//                     dE/dG rows staged in LDS
+// Splitting keeps each pass's registers and LDS small enough to run several
+// waves per SIMD; the rows travel through HBM (well below a millisecond).
+struct SurCfgArgs {
+  int B, Nmax;
   const double* pos;     // [B][Nmax][3]
   const int* types;      // [B][Nmax]
   const int* n_atoms;    // [B]
   const double* cell;    // [B][9] cubic
   const int* cfg_index;  // [B] or NULL
   const int* count;      // device item count (<= B), or NULL (= B)
-  double* E;             // [B]
-  double* F;             // [B][Nmax][3] or NULL
-  __bf16* xout;          // [B][Nmax][80] descriptor rows, or NULL
+  __bf16* X;             // [B][Nmax][80] descriptor rows
+  const float* dX;       // [B][Nmax][64] dE/dG rows (force pass)
+  double* F;             // [B][Nmax][3] (force pass)
+  double* Ebase;         // [B] baseline energy (descriptor pass)
   // pair baseline b_eps ((sig/r)^12 - shift) for r^2 < b_rc2 (b_eps = 0: none)
   float b_eps, b_sig2, b_rc2, b_shift;
-  double* Ebase;         // [B] baseline energy, or NULL
 };
+
+__device__ __forceinline__ int sur_n_items(const int* count, int B) {
+  return count ? min(*count, B) : B;
+}
 
 // Gaussians g_k(r) (and derivatives when D).
 template <bool D>
@@ -189,215 +203,330 @@ __device__ __forceinline__ void sg_gauss(float r, float (&g)[SG_K], float (&dg)[
   }
 }
 
-// One 256-thread block per configuration (4 waves x 64 rows).
-__global__ void __launch_bounds__(256) sur_eval_kernel(SurEvalArgs a) {
-  extern __shared__ float lds_sg[];
-  float4* U = reinterpret_cast<float4*>(lds_sg);                    // [256]
-  __bf16* X = reinterpret_cast<__bf16*>(lds_sg + 4 * 256);         // [256][80]
-  __bf16* Hs = X + 256 * SG_D;                                      // [256][128]
-  float* dX = reinterpret_cast<float*>(Hs);                         // [256][64] (reuses Hs)
-  __shared__ double red[4], redb[4];
+// Fractional coordinates (wrapped) and species of one configuration into U.
+__device__ __forceinline__ void sur_stage_positions(const SurCfgArgs& a, int cfg, int N,
+                                                    double L, float4* U) {
+  const int tid = threadIdx.x;
+  if (tid < N) {
+    const double* p = a.pos + ((size_t)cfg * a.Nmax + tid) * 3;
+    double f0 = p[0] / L, f1 = p[1] / L, f2 = p[2] / L;
+    f0 -= floor(f0);
+    f1 -= floor(f1);
+    f2 -= floor(f2);
+    U[tid] = make_float4((float)f0, (float)f1, (float)f2,
+                         __int_as_float(a.types[(size_t)cfg * a.Nmax + tid]));
+  }
+}
+
+__global__ void __launch_bounds__(256) sur_desc_kernel(SurCfgArgs a) {
+  __shared__ float4 U[256];
+  __shared__ double red[4];
+  const int item = blockIdx.x;
+  if (item >= sur_n_items(a.count, a.B)) return;  // block-uniform
+  const int cfg = a.cfg_index ? a.cfg_index[item] : item;
   const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6;
-  const int B = a.count ? min(*a.count, a.B) : a.B;
-  for (int item = blockIdx.x; item < B; item += gridDim.x) {
-    const int cfg = a.cfg_index ? a.cfg_index[item] : item;
-    const int N = a.n_atoms[cfg];
-    const double L = a.cell[(size_t)cfg * 9];
-    const float Lf = (float)L, iLf = (float)(1.0 / L);
-    const float rdf2 = (SG_RD * iLf) * (SG_RD * iLf);
-    const bool act = tid < N;
-    __syncthreads();
-    if (act) {
-      const double* p = a.pos + ((size_t)cfg * a.Nmax + tid) * 3;
-      double f0 = p[0] / L, f1 = p[1] / L, f2 = p[2] / L;
-      f0 -= floor(f0);
-      f1 -= floor(f1);
-      f2 -= floor(f2);
-      U[tid] = make_float4((float)f0, (float)f1, (float)f2,
-                           __int_as_float(a.types[(size_t)cfg * a.Nmax + tid]));
-    }
-    __syncthreads();
-    const float4 ui = act ? U[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const int ti = act ? __float_as_int(ui.w) : 0;
-    // ---- descriptor row
-    float G0[SG_K], G1[SG_K];
+  const int N = a.n_atoms[cfg];
+  const double L = a.cell[(size_t)cfg * 9];
+  const float Lf = (float)L, iLf = (float)(1.0 / L);
+  const float rdf2 = (SG_RD * iLf) * (SG_RD * iLf);
+  const float brc2 = a.b_rc2 * iLf * iLf;  // baseline cutoff, fractional units
+  const bool act = tid < N;
+  sur_stage_positions(a, cfg, N, L, U);
+  __syncthreads();
+  const float4 ui = act ? U[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const int ti = act ? __float_as_int(ui.w) : 0;
+  float G0[SG_K], G1[SG_K];
 #pragma unroll
-    for (int k = 0; k < SG_K; ++k) G0[k] = G1[k] = 0.0f;
-    float eb = 0.0f;  // sum over partners of the baseline pair energy
-    const float brc2 = a.b_rc2 * iLf * iLf;  // baseline cutoff, fractional units
-    if (act) {
-      for (int j = 0; j < N; ++j) {
-        const float4 uj = U[j];
-        float d[3] = {uj.x - ui.x, uj.y - ui.y, uj.z - ui.z};
+  for (int k = 0; k < SG_K; ++k) G0[k] = G1[k] = 0.0f;
+  float eb = 0.0f;  // sum over partners of the baseline pair energy
+  if (act) {
+    for (int j = 0; j < N; ++j) {
+      const float4 uj = U[j];
+      float d[3] = {uj.x - ui.x, uj.y - ui.y, uj.z - ui.z};
 #pragma unroll
-        for (int k = 0; k < 3; ++k) d[k] -= rintf(d[k]);
-        const int tj = __float_as_int(uj.w);
-        const int ax = image_axes(d, SG_RD * iLf);
-        // the images of the axes in ax, ascending (subsets of ax)
+      for (int k = 0; k < 3; ++k) d[k] -= rintf(d[k]);
+      const int tj = __float_as_int(uj.w);
+      const int ax = image_axes(d, SG_RD * iLf);
+      // the images of the axes in ax, ascending (subsets of ax)
 #pragma unroll 1
-        for (int m = 0, go = 1; go; m = (m - ax) & ax, go = m != 0) {
-          const float e0 = (m & 1) ? d[0] - copysignf(1.0f, d[0]) : d[0];
-          const float e1 = (m & 2) ? d[1] - copysignf(1.0f, d[1]) : d[1];
-          const float e2 = (m & 4) ? d[2] - copysignf(1.0f, d[2]) : d[2];
-          const float r2 = e0 * e0 + e1 * e1 + e2 * e2;
-          if (!(r2 < rdf2) || (j == tid && m == 0)) continue;
-          const float r = Lf * sqrtf(r2);
-          if (r2 < brc2) {
-            const float q = a.b_sig2 * iLf * iLf / r2, q3 = q * q * q;
-            eb += a.b_eps * (q3 * q3 - a.b_shift);
-          }
-          float g[SG_K], dg[SG_K];
-          sg_gauss<false>(r, g, dg);
+      for (int m = 0, go = 1; go; m = (m - ax) & ax, go = m != 0) {
+        const float e0 = (m & 1) ? d[0] - copysignf(1.0f, d[0]) : d[0];
+        const float e1 = (m & 2) ? d[1] - copysignf(1.0f, d[1]) : d[1];
+        const float e2 = (m & 4) ? d[2] - copysignf(1.0f, d[2]) : d[2];
+        const float r2 = e0 * e0 + e1 * e1 + e2 * e2;
+        if (!(r2 < rdf2) || (j == tid && m == 0)) continue;
+        const float r = Lf * sqrtf(r2);
+        if (r2 < brc2) {
+          const float q = a.b_sig2 * iLf * iLf / r2, q3 = q * q * q;
+          eb += a.b_eps * (q3 * q3 - a.b_shift);
+        }
+        float g[SG_K], dg[SG_K];
+        sg_gauss<false>(r, g, dg);
 #pragma unroll
-          for (int k = 0; k < SG_K; ++k) {
-            if (tj == 0) G0[k] += g[k];
-            else G1[k] += g[k];
-          }
+        for (int k = 0; k < SG_K; ++k) {
+          if (tj == 0) G0[k] += g[k];
+          else G1[k] += g[k];
         }
       }
     }
-    {
-      __bf16* row = X + tid * SG_D;
-      __bf16* orow = (a.xout && act) ? a.xout + ((size_t)cfg * a.Nmax + tid) * SG_D : nullptr;
+  }
+  if (tid < a.Nmax) {
+    __bf16 row[SG_D];
 #pragma unroll
-      for (int k = 0; k < SG_D; ++k) {
-        float v = 0.0f;
-        if (k < SG_K) v = SG_SCALE * G0[k];
-        else if (k < 2 * SG_K) v = SG_SCALE * G1[k - SG_K];
-        else if (k == 2 * SG_K + ti) v = 1.0f;
-        if (!act) v = 0.0f;
-        const __bf16 b = (__bf16)v;
-        row[k] = b;
-        if (orow) orow[k] = b;
-      }
+    for (int k = 0; k < SG_D; ++k) {
+      float v = 0.0f;
+      if (k < SG_K) v = SG_SCALE * G0[k];
+      else if (k < 2 * SG_K) v = SG_SCALE * G1[k - SG_K];
+      else if (k == 2 * SG_K + ti) v = 1.0f;
+      row[k] = (__bf16)(act ? v : 0.0f);
     }
-    __syncthreads();
-    // ---- MLP on this wave's 64 rows
-    f16v z1[2][4], z2[2][4];
-    float e[2][16];
-    sur_forward_rows(a.net, X + wv * 64 * SG_D, Hs + wv * 64 * SG_H, z1, z2, e);
-    float es = 0.0f;
-    if (l < 32) {  // one copy of each row: lanes 0..31 hold the l/32 = 0 rows
+    bf8v* o = reinterpret_cast<bf8v*>(a.X + ((size_t)cfg * a.Nmax + tid) * SG_D);
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
+    for (int q = 0; q < SG_D / 8; ++q) {
+      bf8v v;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const int row = wv * 64 + sg_row(rb, v, l);
-          if (l == 0 && row < N) es += e[rb][v];
-        }
+      for (int e = 0; e < 8; ++e) v[e] = row[8 * q + e];
+      o[q] = v;
+    }
+  }
+  double edb = act ? 0.5 * (double)eb : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) edb += __shfl_xor(edb, off, 64);
+  if (l == 0) red[wv] = edb;
+  __syncthreads();
+  if (tid == 0) a.Ebase[cfg] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// C[32 RB x 32 NB] += A[32 RB x K] B[K x 32 NB]: A rows in LDS (stride lda),
+// B given n-major (Bt[n][k], stride ldb) in global memory.
+template <int K, int NB, int RB>
+__device__ __forceinline__ void wave_gemm_rb(const __bf16* A, int lda, const __bf16* Bt, int ldb,
+                                             f16v (&c)[RB][NB]) {
+  const int l = sg_lane(), r = l & 31, kh = (l >> 5) * 8;
+#pragma unroll 2
+  for (int kb = 0; kb < K / 16; ++kb) {
+    bf8v av[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) av[rb] = ld8(A + (32 * rb + r) * lda + kb * 16 + kh);
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const bf8v b = ld8(Bt + (nb * 32 + r) * ldb + kb * 16 + kh);
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+        c[rb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[rb], b, c[rb][nb], 0, 0, 0);
+    }
+  }
+}
+
+struct SurMlpArgs {
+  SurDev net;
+  int B, Nmax, tiles;    // tiles of 32 rows per configuration
+  const int* cfg_index;  // [B] or NULL
+  const int* count;      // device item count, or NULL
+  const __bf16* X;       // [B][Nmax][80]
+  float* erow;           // [B][Nmax] row energies (without b3)
+  float* dX;             // [B][Nmax][64] dE/dG rows, or NULL (energy only)
+};
+
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+
+// One wave per 32-row tile.  silu'(z1) is kept as BF16 pairs (the operand
+// rounding the next product applies anyway) and z2 is consumed into the row
+// energies and the BF16 tile G2 = w3 silu'(z2), so no pre-activation stays live
+// across a matrix product.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
+    sur_mlp_kernel(SurMlpArgs a) {
+  __shared__ __attribute__((aligned(16))) __bf16 Xs[32 * SG_D];
+  __shared__ __attribute__((aligned(16))) __bf16 Hs[32 * SG_H];
+  const int item = blockIdx.x;
+  if (item >= sur_n_items(a.count, a.B) * a.tiles) return;
+  const int q = item / a.tiles, tile = item - q * a.tiles;
+  const int cfg = a.cfg_index ? a.cfg_index[q] : q;
+  const int row0 = tile * 32;
+  const int l = sg_lane();
+  const size_t base = (size_t)cfg * a.Nmax + row0;
+  for (int i = l; i < 32 * SG_D / 8; i += 64) {
+    const int r = i / (SG_D / 8), part = i - r * (SG_D / 8);
+    bf8v v;
+    if (row0 + r < a.Nmax) {
+      v = ld8(a.X + (base + r) * SG_D + part * 8);
     } else {
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const int row = wv * 64 + sg_row(rb, v, l);
-          if (l == 32 && row < N) es += e[rb][v];
-        }
+      for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.0f;
     }
-    double ed = (double)es, edb = act ? 0.5 * (double)eb : 0.0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      ed += __shfl_xor(ed, off, 64);
-      edb += __shfl_xor(edb, off, 64);
-    }
-    if (l == 0) {
-      red[wv] = ed;
-      redb[wv] = edb;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      const double base = redb[0] + redb[1] + redb[2] + redb[3];
-      a.E[cfg] = red[0] + red[1] + red[2] + red[3] + (double)N * (double)a.net.P[SG_OFF_B3] +
-                 base;
-      if (a.Ebase) a.Ebase[cfg] = base;
-    }
-    if (!a.F) continue;
-    // ---- backward to the inputs
-    __bf16* Hw = Hs + wv * 64 * SG_H;
+    *reinterpret_cast<bf8v*>(Xs + r * SG_D + part * 8) = v;
+  }
+  sg_wave_sync();
+  const bool want_f = a.dX != nullptr;
+  bf2v s1[4][8];
+  float e[16];
+  {
+    f16v z[1][4];
+    zero_acc<1, 4>(z);
+    wave_gemm_rb<SG_D, 4, 1>(Xs, SG_D, a.net.W1t, SG_D, z);
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
-      const float w3 = a.net.P[SG_OFF_W3 + nb * 32 + (l & 31)];
+      const float b = a.net.P[SG_OFF_B1 + nb * 32 + (l & 31)];
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int v = 0; v < 16; ++v)
-          Hw[sg_row(rb, v, l) * SG_H + nb * 32 + (l & 31)] =
-              (__bf16)(w3 * sg_silu_d(z2[rb][nb][v]));
+      for (int v = 0; v < 16; ++v) {
+        const float zz = z[0][nb][v] + b;
+        const float sg = __builtin_amdgcn_rcpf(1.0f + __expf(-zz));
+        Hs[sg_row(0, v, l) * SG_H + nb * 32 + (l & 31)] = (__bf16)(zz * sg);
+        s1[nb][v >> 1][v & 1] = (__bf16)(sg * (1.0f + zz * (1.0f - sg)));
+      }
     }
-    sg_wave_sync();
-    f16v g[2][4];
-    zero_acc<4>(g);
-    wave_gemm<SG_H, 4>(Hw, SG_H, a.net.W2r, SG_H, g);
+  }
+  sg_wave_sync();
+  {
+    f16v z[1][4];
+    zero_acc<1, 4>(z);
+    wave_gemm_rb<SG_H, 4, 1>(Hs, SG_H, a.net.W2t, SG_H, z);
+    sg_wave_sync();  // h1 fully read before G2 replaces it
+#pragma unroll
+    for (int v = 0; v < 16; ++v) e[v] = 0.0f;
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+      const int col = nb * 32 + (l & 31);
+      const float b = a.net.P[SG_OFF_B2 + col], w3 = a.net.P[SG_OFF_W3 + col];
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const float zz = z[0][nb][v] + b;
+        const float sg = __builtin_amdgcn_rcpf(1.0f + __expf(-zz));
+        e[v] += w3 * (zz * sg);
+        if (want_f)
+          Hs[sg_row(0, v, l) * SG_H + col] = (__bf16)(w3 * (sg * (1.0f + zz * (1.0f - sg))));
+      }
+    }
+  }
+  // row energies: sum over the 32 lanes holding the same rows
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    float t = e[v];
+#pragma unroll
+    for (int m = 1; m < 32; m <<= 1) t += __shfl_xor(t, m, 64);
+    e[v] = t;
+  }
+  if ((l & 31) == 0) {
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int r = sg_row(0, v, l);
+      if (row0 + r < a.Nmax) a.erow[base + r] = e[v];
+    }
+  }
+  if (!want_f) return;
+  sg_wave_sync();
+  {
+    f16v g[1][4];
+    zero_acc<1, 4>(g);
+    wave_gemm_rb<SG_H, 4, 1>(Hs, SG_H, a.net.W2r, SG_H, g);
     sg_wave_sync();
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
+      for (int v = 0; v < 16; ++v)
+        Hs[sg_row(0, v, l) * SG_H + nb * 32 + (l & 31)] =
+            (__bf16)(g[0][nb][v] * (float)s1[nb][v >> 1][v & 1]);
+  }
+  sg_wave_sync();
+  f16v gx[1][2];
+  zero_acc<1, 2>(gx);
+  wave_gemm_rb<SG_H, 2, 1>(Hs, SG_H, a.net.W1r, SG_H, gx);
 #pragma unroll
-        for (int v = 0; v < 16; ++v)
-          Hw[sg_row(rb, v, l) * SG_H + nb * 32 + (l & 31)] =
-              (__bf16)(g[rb][nb][v] * sg_silu_d(z1[rb][nb][v]));
-    sg_wave_sync();
-    f16v gx[2][2];
-    zero_acc<2>(gx);
-    wave_gemm<SG_H, 2>(Hw, SG_H, a.net.W1r, SG_H, gx);
-    sg_wave_sync();
-    float* dXw = dX + wv * 64 * 64;
+  for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int v = 0; v < 16; ++v)
-          dXw[sg_row(rb, v, l) * 64 + nb * 32 + (l & 31)] = gx[rb][nb][v];
-    __syncthreads();
-    // ---- pair forces
-    if (act) {
-      float own[2 * SG_K];
-#pragma unroll
-      for (int k = 0; k < 2 * SG_K; ++k) own[k] = dX[tid * 64 + k];
-      float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-      for (int j = 0; j < N; ++j) {
-        const float4 uj = U[j];
-        float d[3] = {uj.x - ui.x, uj.y - ui.y, uj.z - ui.z};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) d[k] -= rintf(d[k]);
-        const int tj = __float_as_int(uj.w);
-        const float* other = dX + j * 64 + ti * SG_K;
-        const int ax = image_axes(d, SG_RD * iLf);
-#pragma unroll 1
-        for (int m = 0, go = 1; go; m = (m - ax) & ax, go = m != 0) {
-          const float e0 = (m & 1) ? d[0] - copysignf(1.0f, d[0]) : d[0];
-          const float e1 = (m & 2) ? d[1] - copysignf(1.0f, d[1]) : d[1];
-          const float e2 = (m & 4) ? d[2] - copysignf(1.0f, d[2]) : d[2];
-          const float r2 = e0 * e0 + e1 * e1 + e2 * e2;
-          if (!(r2 < rdf2) || (j == tid && m == 0)) continue;
-          const float id = __builtin_amdgcn_rsqf(r2);
-          const float r = Lf * r2 * id;
-          float gg[SG_K], dg[SG_K];
-          sg_gauss<true>(r, gg, dg);
-          float c = 0.0f;
-#pragma unroll
-          for (int k = 0; k < SG_K; ++k) {
-            const float mine = (tj == 0) ? own[k] : own[SG_K + k];
-            c += (mine + other[k]) * dg[k];
-          }
-          c *= SG_SCALE;
-          if (r2 < brc2) {  // d phi / dr of the baseline pair
-            const float q = a.b_sig2 * iLf * iLf / r2, q3 = q * q * q;
-            c -= 12.0f * a.b_eps * q3 * q3 / r;
-          }
-          const float s = c * id;  // d / r = (e0, e1, e2) * id
-          fx += s * e0;
-          fy += s * e1;
-          fz += s * e2;
-        }
-      }
-      double* Fo = a.F + ((size_t)cfg * a.Nmax + tid) * 3;
-      Fo[0] = (double)fx;
-      Fo[1] = (double)fy;
-      Fo[2] = (double)fz;
+    for (int v = 0; v < 16; ++v) {
+      const int r = sg_row(0, v, l);
+      if (row0 + r < a.Nmax) a.dX[(base + r) * 64 + nb * 32 + (l & 31)] = gx[0][nb][v];
     }
+}
+
+// E = sum of the first N row energies + N b3 + baseline (one wave per
+// configuration, fixed order).
+__global__ void __launch_bounds__(256) sur_energy_kernel(int B, int Nmax, const int* cfg_index,
+                                                         const int* count, const int* n_atoms,
+                                                         const float* erow, const float* P,
+                                                         const double* Ebase, double* E) {
+  const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (item >= sur_n_items(count, B)) return;
+  const int cfg = cfg_index ? cfg_index[item] : item;
+  const int N = n_atoms[cfg];
+  double s = 0.0;
+  for (int i = (int)(threadIdx.x & 63); i < N; i += 64) s += (double)erow[(size_t)cfg * Nmax + i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) E[cfg] = s + (double)N * (double)P[SG_OFF_B3] + Ebase[cfg];
+}
+
+// Pair forces: F_i = sum_j (dE/dr_ij) (x_j - x_i)/r_ij with dE/dr from both
+// atoms' descriptor gradients (dE/dG rows staged in LDS) plus the baseline.
+__global__ void __launch_bounds__(256) sur_force_kernel(SurCfgArgs a) {
+  extern __shared__ float lds_f[];
+  float4* U = reinterpret_cast<float4*>(lds_f);  // [256]
+  float* dX = lds_f + 4 * 256;                    // [Nmax][64]
+  const int item = blockIdx.x;
+  if (item >= sur_n_items(a.count, a.B)) return;
+  const int cfg = a.cfg_index ? a.cfg_index[item] : item;
+  const int tid = threadIdx.x;
+  const int N = a.n_atoms[cfg];
+  const double L = a.cell[(size_t)cfg * 9];
+  const float Lf = (float)L, iLf = (float)(1.0 / L);
+  const float rdf2 = (SG_RD * iLf) * (SG_RD * iLf);
+  const float brc2 = a.b_rc2 * iLf * iLf;
+  const bool act = tid < N;
+  sur_stage_positions(a, cfg, N, L, U);
+  const float* src = a.dX + (size_t)cfg * a.Nmax * 64;
+  for (int i = tid; i < N * 64; i += 256) dX[i] = src[i];
+  __syncthreads();
+  if (act) {
+    const float4 ui = U[tid];
+    const int ti = __float_as_int(ui.w);
+    float own[2 * SG_K];
+#pragma unroll
+    for (int k = 0; k < 2 * SG_K; ++k) own[k] = dX[tid * 64 + k];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for (int j = 0; j < N; ++j) {
+      const float4 uj = U[j];
+      float d[3] = {uj.x - ui.x, uj.y - ui.y, uj.z - ui.z};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) d[k] -= rintf(d[k]);
+      const int tj = __float_as_int(uj.w);
+      const float* other = dX + j * 64 + ti * SG_K;
+      const int ax = image_axes(d, SG_RD * iLf);
+#pragma unroll 1
+      for (int m = 0, go = 1; go; m = (m - ax) & ax, go = m != 0) {
+        const float e0 = (m & 1) ? d[0] - copysignf(1.0f, d[0]) : d[0];
+        const float e1 = (m & 2) ? d[1] - copysignf(1.0f, d[1]) : d[1];
+        const float e2 = (m & 4) ? d[2] - copysignf(1.0f, d[2]) : d[2];
+        const float r2 = e0 * e0 + e1 * e1 + e2 * e2;
+        if (!(r2 < rdf2) || (j == tid && m == 0)) continue;
+        const float id = __builtin_amdgcn_rsqf(r2);
+        const float r = Lf * r2 * id;
+        float gg[SG_K], dg[SG_K];
+        sg_gauss<true>(r, gg, dg);
+        float c = 0.0f;
+#pragma unroll
+        for (int k = 0; k < SG_K; ++k) {
+          const float mine = (tj == 0) ? own[k] : own[SG_K + k];
+          c += (mine + other[k]) * dg[k];
+        }
+        c *= SG_SCALE;
+        if (r2 < brc2) {  // d phi / dr of the baseline pair
+          const float q = a.b_sig2 * iLf * iLf / r2, q3 = q * q * q;
+          c -= 12.0f * a.b_eps * q3 * q3 / r;
+        }
+        const float s = c * id;  // d / r = (e0, e1, e2) * id
+        fx += s * e0;
+        fy += s * e1;
+        fz += s * e2;
+      }
+    }
+    double* Fo = a.F + ((size_t)cfg * a.Nmax + tid) * 3;
+    Fo[0] = (double)fx;
+    Fo[1] = (double)fy;
+    Fo[2] = (double)fz;
+  } else if (tid < a.Nmax) {
+    double* Fo = a.F + ((size_t)cfg * a.Nmax + tid) * 3;
+    Fo[0] = Fo[1] = Fo[2] = 0.0;
   }
 }
 
@@ -554,7 +683,7 @@ __global__ void __launch_bounds__(64) sur_rows_backward_kernel(SurRowsArgs a) {
       }
     // g1 = (g2 W2^T) * silu'(z1)
     f16v g[2][4];
-    zero_acc<4>(g);
+    zero_acc<2, 4>(g);
     wave_gemm<SG_H, 4>(H, SG_H, a.net.W2r, SG_H, g);
     sg_wave_sync();
 #pragma unroll

@@ -368,7 +368,9 @@ static void sur_backward(const float* P, int bf16, int n, const double* x,
         double w = P[OR_SUR_W2 + h * H + k];
         acc += g2b[k] * (bf16 ? bf16r(w) : w);
       }
-      g1[h] = acc * silu_d(Z1[h]);
+      /* the forces path holds silu'(z1) as a BF16 operand on the device */
+      const double s1 = (bf16 && dX && !grad) ? bf16r(silu_d(Z1[h])) : silu_d(Z1[h]);
+      g1[h] = acc * s1;
       g1b[h] = bf16 ? bf16r(g1[h]) : g1[h];
     }
     if (dX) {
