@@ -16,10 +16,6 @@ using namespace jrh;
 
 namespace {
 
-constexpr size_t kMpLds =
-    sizeof(float) * (4 * MP_THREADS + 2 * MP_THREADS * MP_C +
-                     MP_THREADS * MP_NB * MP_MAXS + MP_NB * MP_C);
-
 int validate_mpnn(const jr_mpnn* net) {
   if (!net) return set_global_err(JR_E_ARG, "mpnn is NULL");
   if (net->n_species < 1 || net->n_species > MP_MAXS)
@@ -76,21 +72,26 @@ struct MPNNWeights {
   }
 };
 
-// Launch the judge on items [0, a.B) with `grid` persistent blocks.
-hipError_t launch_mpnn(const MPNNArgs& a, const float* w, int grid, hipStream_t s) {
-  hipError_t e;
-  if (a.net.S == 1) {
-    e = hipFuncSetAttribute((const void*)mpnn_kernel<1>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMpLds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(mpnn_kernel<1>, dim3(grid), dim3(MP_THREADS), kMpLds, s, a, w);
-  } else {
-    e = hipFuncSetAttribute((const void*)mpnn_kernel<2>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMpLds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(mpnn_kernel<2>, dim3(grid), dim3(MP_THREADS), kMpLds, s, a, w);
-  }
+template <int S, bool FORCES>
+hipError_t launch_mpnn_t(const MPNNArgs& a, const float* w, int grid, hipStream_t s) {
+  const size_t lds = mp_lds_bytes(FORCES);
+  hipError_t e = hipFuncSetAttribute((const void*)mpnn_kernel<S, FORCES>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((mpnn_kernel<S, FORCES>), dim3(grid), dim3(MP_THREADS), lds, s, a, w);
   return hipGetLastError();
+}
+
+// Persistent blocks per CU the judge's scratch lists are sized for: the
+// energy-only kernel fits two blocks per CU, the force kernel one.
+constexpr int kMpBlocksPerCu = 2;
+
+// Launch the judge on items [0, a.B) with `grid` persistent blocks (at most
+// the blocks its neighbour-list scratch holds).
+hipError_t launch_mpnn(const MPNNArgs& a, const float* w, int grid, hipStream_t s) {
+  if (a.net.S == 1)
+    return a.F ? launch_mpnn_t<1, true>(a, w, grid, s) : launch_mpnn_t<1, false>(a, w, grid, s);
+  return a.F ? launch_mpnn_t<2, true>(a, w, grid, s) : launch_mpnn_t<2, false>(a, w, grid, s);
 }
 
 int device_cus(int device) {
@@ -135,7 +136,7 @@ extern "C" int jr_mpnn_eval(const jr_mpnn* net, int32_t device, int32_t B, int32
   } guard{s};
   MPNNWeights w;
   HIPCHK_G(w.upload(net, s));
-  const int grid = std::min(B, device_cus(device));
+  const int grid = std::min(B, (F ? 1 : kMpBlocksPerCu) * device_cus(device));
   DBuf<double> d_pos, d_cell, d_U, d_F;
   DBuf<int> d_t, d_n, d_of;
   DBuf<unsigned> d_nbl;
@@ -502,7 +503,8 @@ MPNNArgs judge_args(LearnedState* L, const double* pos, const double* cell, int 
 }
 
 hipError_t run_judge(LearnedState* L, const MPNNArgs& a, hipStream_t s) {
-  return launch_mpnn(a, L->judge.buf.p, std::max(1, std::min(a.B, L->n_cu)), s);
+  const int per_cu = a.F ? 1 : kMpBlocksPerCu;
+  return launch_mpnn(a, L->judge.buf.p, std::max(1, std::min(a.B, per_cu * L->n_cu)), s);
 }
 
 // The judge's model energies U (and forces F unless NULL) of the listed
@@ -615,8 +617,10 @@ int learned_create(jr_handle* h, const jr_config* c) {
   for (DBuf<int>* b : {&L->cn, &L->code, &L->code2, &L->accept, &L->reason, &L->idx, &L->idx2})
     HIPCHK(h, b->alloc(C));
   HIPCHK(h, L->cnt.alloc(4));
-  // judge launches use at most n_cu persistent blocks, each with its own list
-  if (net) HIPCHK(h, L->nbl.alloc((size_t)L->n_cu * MP_MAXNB * MP_THREADS));
+  // judge launches use at most kMpBlocksPerCu * n_cu persistent blocks,
+  // each with its own neighbour list
+  if (net)
+    HIPCHK(h, L->nbl.alloc((size_t)kMpBlocksPerCu * L->n_cu * MP_MAXNB * MP_THREADS));
   HIPCHK(h, L->overflow.alloc(1));
   HIPCHK(h, L->dstats.alloc(1));
   HIPCHK(h, L->rs.alloc(1));

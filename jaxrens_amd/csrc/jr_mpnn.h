@@ -132,7 +132,17 @@ __device__ __forceinline__ void mp_edge(unsigned e, const float4* U, float4 ui,
   tj = __float_as_int(uj.w);
 }
 
-template <int S>
+// LDS of the judge: positions and h1 for the energy; with forces also M,
+// the layer-1 gradients and W2 (bytes).
+constexpr size_t mp_lds_bytes(bool forces) {
+  return sizeof(float) * (4 * MP_THREADS + MP_THREADS * MP_C +
+                          (forces ? MP_THREADS * MP_C + MP_THREADS * MP_NB * MP_MAXS + MP_NB * MP_C
+                                  : 0));
+}
+
+// FORCES = false is a separate instantiation: without the backward its
+// registers and LDS (68 KB) allow two blocks per CU.
+template <int S, bool FORCES>
 __global__ void __launch_bounds__(MP_THREADS)
     mpnn_kernel(MPNNArgs a, const float* __restrict__ w) {
   using O = MPW<S>;
@@ -140,14 +150,15 @@ __global__ void __launch_bounds__(MP_THREADS)
   const int tid = threadIdx.x;
   float4* U = reinterpret_cast<float4*>(lds_mp);              // [256]
   float* H1 = lds_mp + 4 * MP_THREADS;                         // [256][64]
-  float* M = H1 + MP_THREADS * MP_C;                           // [256][64]
-  float* GS = M + MP_THREADS * MP_C;                           // [256][NB*MAXS]
-  float* W2S = GS + MP_THREADS * MP_NB * MP_MAXS;              // [NB][C]
+  float* M = H1 + MP_THREADS * MP_C;                           // [256][64] (forces)
+  float* GS = M + MP_THREADS * MP_C;                           // [256][NB*MAXS] (forces)
+  float* W2S = GS + MP_THREADS * MP_NB * MP_MAXS;              // [NB][C] (forces)
   __shared__ double red[MP_THREADS / 64];
   unsigned* nbl = a.nbl + (size_t)blockIdx.x * MP_MAXNB * MP_THREADS;
   const MPNNDev& P = a.net;
   // W2 in LDS: read as broadcasts inside the pass-3 edge loop
-  for (int k = tid; k < MP_NB * MP_C; k += MP_THREADS) W2S[k] = w[O::W2 + k];
+  if (FORCES)
+    for (int k = tid; k < MP_NB * MP_C; k += MP_THREADS) W2S[k] = w[O::W2 + k];
 
   const int B = a.count ? min(*a.count, a.B) : a.B;
   for (int item = blockIdx.x; item < B; item += gridDim.x) {
@@ -157,7 +168,7 @@ __global__ void __launch_bounds__(MP_THREADS)
     if (!(L >= (double)P.rc)) {
       // outside the images enumerated below: no energy (the walk rejects)
       if (tid == 0) a.E[cfg] = __builtin_inf();
-      if (a.F && tid < a.Nmax) {
+      if (FORCES && tid < a.Nmax) {
         double* Fo = a.F + ((size_t)cfg * a.Nmax + tid) * 3;
         Fo[0] = Fo[1] = Fo[2] = 0.0;
       }
@@ -335,7 +346,7 @@ __global__ void __launch_bounds__(MP_THREADS)
       for (int w = 0; w < MP_THREADS / 64; ++w) e += red[w];
       a.E[cfg] = e;
     }
-    if (!a.F) continue;
+    if (!FORCES) continue;
 
     // ---- backward: M_i = dE/dm2_i = U2 (wo * silu'(z2))
 #pragma unroll
