@@ -22,6 +22,8 @@
  *                             sampling/inter_re_manager.py:218-270
  *   jr_trial_moves         <- sampling/adaptation/stepsize_handler.py:215-261
  *                             (trial moves of adjust_step_size)
+ *   jr_initial_walk        <- init/burn_in.py:48-148 (_one_walk, one walk of
+ *                             initial_walk :154-407)
  */
 #ifndef JAXRENS_AMD_H
 #define JAXRENS_AMD_H
@@ -39,11 +41,13 @@ extern "C" {
 #define JR_E_UNSUPPORTED (-4)
 
 /* move types (cli/schema/moves.py: gmc|galilean, volume, hmc,
- * alchemical_morph) */
+ * alchemical_morph, shear, stretch) */
 #define JR_MOVE_GMC 0
 #define JR_MOVE_VOLUME 1
 #define JR_MOVE_HMC 2
 #define JR_MOVE_MORPH 3
+#define JR_MOVE_SHEAR 4   /* sampling/moves/shear.py:19-120 */
+#define JR_MOVE_STRETCH 5 /* sampling/moves/stretch.py:18-97 */
 
 #define JR_MAX_MOVES 8
 #define JR_MAX_SPECIES 8
@@ -54,7 +58,14 @@ extern "C" {
 
 typedef struct jr_handle jr_handle;
 
-/* Pair potential: per species pair (a, b) at index a*S+b. */
+/* Pair potential: per species pair (a, b) at index a*S+b.
+ * images[d] = supercell_trafo[d] / 2 (integer division) of the reference's
+ * LJBackend (backends/lj.py:106-113,152-166): every pair is summed over the
+ * minimum-image displacement plus the lattice translations
+ * (i0, i1, i2) . cell with -images[d] <= i_d <= images[d], self pairs only at
+ * non-zero translations; all zero (supercell_trafo (1,1,1)) is plain minimum
+ * image.  Each entry at most JR_MAX_IMAGES; not with JR_PREC_FP32_PAIRS. */
+#define JR_MAX_IMAGES 4
 typedef struct {
   int32_t n_species;   /* S, 1..JR_MAX_SPECIES */
   int32_t precision;   /* JR_PREC_* */
@@ -62,6 +73,8 @@ typedef struct {
   const double* eps4;  /* [S*S] 4*eps_ab  */
   const double* eps24; /* [S*S] 24*eps_ab */
   const double* sig2;  /* [S*S] sigma_ab^2 */
+  int32_t images[3];   /* image half-ranges per lattice vector */
+  int32_t pad_;
 } jr_pot;
 
 typedef struct {
@@ -72,7 +85,7 @@ typedef struct {
   int32_t n_species;   /* alchemical_morph */
   int32_t pad_;
   double perturb_angle;/* gmc */
-  double max_vpa;      /* volume: cell-geometry guard */
+  double max_vpa;      /* volume, shear, stretch: cell-geometry guard */
   double min_vpa;
   double min_aspect;
 } jr_move;
@@ -164,7 +177,11 @@ typedef struct {
  *   schedule  [n_iter][n_steps]            move index per MWG sub-step
  *   normals   [n_iter][R][n_walk][n_steps][3][Nmax]
  *   uniforms  [n_iter][R][n_walk][n_steps][4]
- *   re_u      [n_iter][n_cycles][2][R/2][2] (NULL: RE draws unused)       */
+ *   re_u      [n_iter][n_cycles][2][R/2][2] (NULL: RE draws unused)
+ * One sub-step reads, by move type: gmc and hmc the normals [3][Nmax];
+ * volume normal [0][0] and uniform 0; morph uniforms 0 (atom) and 1
+ * (species); shear uniform 0 (cell vector floor(3u)) and normals [0][0],
+ * [1][0]; stretch uniform 0 (axis pair floor(3u)) and normal [0][0]. */
 typedef struct {
   const double* sel_u;
   const double* clone_u;
@@ -223,6 +240,14 @@ int jr_get_step_sizes(jr_handle* h, double* step_sizes);
 int jr_set_pair_valid(jr_handle* h, const int32_t* pair_valid);
 /* Outer-loop iteration index i used for the RE firing rule. */
 int jr_set_iteration(jr_handle* h, int64_t i);
+/* Counters of the device draw streams (Philox4x32-10 under the handle's
+ * seed): state[0] NS iterations drawn, state[1] trial moves and standalone RE
+ * passes, state[2] burn-in walks.  With the population, run parameters and
+ * iteration index they make a checkpoint resume draw exactly what an
+ * uninterrupted run draws (io/checkpoint.py:187-197 persists the run's PRNG
+ * key for the same purpose). */
+int jr_get_rng_state(jr_handle* h, int64_t* state /* [3] */);
+int jr_set_rng_state(jr_handle* h, const int64_t* state /* [3] */);
 int jr_get_emax(jr_handle* h, double* emax);
 
 /* Run n_iter NS iterations (select -> clone -> walk -> RE).  tape == NULL
@@ -251,6 +276,22 @@ int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
                    const int32_t* sample_idx, const double* normals,
                    const double* uniforms, int32_t* accepted,
                    int32_t* reason);
+
+/* Fixed-contour walk of the whole live population before nested sampling
+ * (init/burn_in.py:48-148, one walk of initial_walk :154-407): every walker
+ * of replica r takes walklength MWG sub-steps at Emax = emax[r], held fixed,
+ * with the handle's move table and step sizes, and one move schedule shared
+ * by all walkers and replicas.  emax [R] (host).  Draws: schedule
+ * [walklength], normals [R][K][walklength][3][Nmax] and uniforms
+ * [R][K][walklength][4] from the host, or all three NULL for device draws.
+ * Walkers go in launches of max_chunk per replica (<= 0: as many as keep the
+ * draws of one launch within 1 GiB).  counters_out [R][n_moves][JR_NCOUNT]
+ * (may be NULL) sums this walk's counters.  The NS contour, iteration state
+ * and per-iteration outputs are left unchanged. */
+int jr_initial_walk(jr_handle* h, int32_t walklength, const double* emax,
+                    const int32_t* schedule, const double* normals,
+                    const double* uniforms, int32_t max_chunk,
+                    int32_t* counters_out);
 
 /* Autotuner (no reference counterpart; the reference leaves launch shapes to
  * XLA): times `reps` walk launches per candidate (waves per block 4/2/1 x

@@ -19,6 +19,9 @@ JR_MOVE_GMC = 0
 JR_MOVE_VOLUME = 1
 JR_MOVE_HMC = 2
 JR_MOVE_MORPH = 3
+JR_MOVE_SHEAR = 4
+JR_MOVE_STRETCH = 5
+JR_MAX_IMAGES = 4
 JR_MAX_MOVES = 8
 JR_MAX_SPECIES = 8
 JR_NCOUNT = 7
@@ -37,6 +40,8 @@ class JrPot(C.Structure):
         ("eps4", _dp),
         ("eps24", _dp),
         ("sig2", _dp),
+        ("images", C.c_int32 * 3),
+        ("pad_", C.c_int32),
     ]
 
 
@@ -187,6 +192,8 @@ SIGNATURES = [
     ("jr_get_step_sizes", C.c_int, [C.c_void_p, _dp]),
     ("jr_set_pair_valid", C.c_int, [C.c_void_p, _ip]),
     ("jr_set_iteration", C.c_int, [C.c_void_p, C.c_int64]),
+    ("jr_get_rng_state", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("jr_set_rng_state", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("jr_get_emax", C.c_int, [C.c_void_p, _dp]),
     ("jr_ns_iterations", C.c_int,
      [C.c_void_p, C.c_int32, C.POINTER(JrTape), C.POINTER(JrIterOut)]),
@@ -197,6 +204,8 @@ SIGNATURES = [
     ("jr_replica_exchange", C.c_int, [C.c_void_p, _dp, _ip, _ip]),
     ("jr_trial_moves", C.c_int,
      [C.c_void_p, C.c_int32, _dp, _dp, C.c_int32, _ip, _dp, _dp, _ip, _ip]),
+    ("jr_initial_walk", C.c_int,
+     [C.c_void_p, C.c_int32, _dp, _ip, _dp, _dp, C.c_int32, _ip]),
     ("jr_autotune", C.c_int,
      [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
       C.POINTER(C.c_double)]),

@@ -52,12 +52,15 @@ class PairTable(NamedTuple):
     eps24: np.ndarray  # (S, S) 24 * eps_ab
     sig2: np.ndarray   # (S, S) sigma_ab ** 2
     rc2: float
+    images: tuple[int, int, int] = (0, 0, 0)  # supercell_trafo // 2
 
     def as_struct(self, precision: int = _lib.JR_PREC_FP64):
         keep = (_lib.f64(self.eps4), _lib.f64(self.eps24), _lib.f64(self.sig2))
         pot = _lib.JrPot(self.n_species, precision, float(self.rc2),
                          _lib.dptr(keep[0]), _lib.dptr(keep[1]),
                          _lib.dptr(keep[2]))
+        for d in range(3):
+            pot.images[d] = int(self.images[d])
         return pot, keep
 
 
@@ -74,8 +77,9 @@ class LJBackend:
     explicit symmetric (S, S) table in float64 instead of mixing (the binary
     semi-grand workload's eps_AB / sigma_AB; the reference has no such
     table).  ``cutoff=None`` means no cutoff; the cutoff is strict
-    ``r^2 < rc^2`` and unshifted.  Only ``supercell_trafo=(1, 1, 1)`` (minimum
-    image) is supported.
+    ``r^2 < rc^2`` and unshifted.  ``supercell_trafo`` adds the lattice
+    translations ``-(s//2) .. s//2`` per cell vector on top of the minimum
+    image, as ``lj.py:106-113,152-166`` does (entries up to 9, FP64 only).
     """
 
     def __init__(
@@ -94,9 +98,12 @@ class LJBackend:
         if min(sc) < 1:
             raise ValueError(
                 f"supercell_trafo entries must be >= 1, got {supercell_trafo}")
-        if sc != (1, 1, 1):
-            raise NotImplementedError(
-                "supercell_trafo > (1, 1, 1) is not built: minimum image only")
+        images = tuple(s // 2 for s in sc)
+        if max(images) > _lib.JR_MAX_IMAGES:
+            raise ValueError(
+                f"supercell_trafo entries must be <= {2 * _lib.JR_MAX_IMAGES + 1}")
+        if max(images) > 0 and precision != "fp64":
+            raise NotImplementedError("supercell images are built for fp64 only")
         self.supercell_trafo = sc
         self.cutoff = cutoff
         self.r_cutoff = cutoff if cutoff is not None else 0.0
@@ -155,7 +162,8 @@ class LJBackend:
                 sig2 = np.array([[self.sigma ** 2]])
         if self.n_species > _lib.JR_MAX_SPECIES:
             raise ValueError(f"at most {_lib.JR_MAX_SPECIES} species")
-        self.table = PairTable(self.n_species, 4.0 * eps, 24.0 * eps, sig2, rc2)
+        self.table = PairTable(self.n_species, 4.0 * eps, 24.0 * eps, sig2, rc2,
+                               images)
 
     # -- evaluation ---------------------------------------------------------
     def _eval(self, positions, species, cell, pressure=None, mu=None,

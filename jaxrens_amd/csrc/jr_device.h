@@ -187,6 +187,89 @@ __device__ __forceinline__ bool check_cell_shape(const double* c, int n_atoms,
   return (vpa <= max_vpa) && (vpa >= min_vpa) && (aspect >= min_aspect);
 }
 
+// exp(x) from IEEE +, *, rint and exponent arithmetic only, so that every
+// build of it rounds alike (Cody and Waite's reduction): x = k ln2 + r with
+// ln2 split into a head of 32 significant bits (k * head is exact) and a
+// tail, |r| <= ln2 / 2; exp(r) by its Taylor polynomial of degree 13 in
+// Horner form (truncation below 5e-18 relative); the result scaled by 2^k in
+// two exact power-of-two factors.
+__device__ __forceinline__ double pow2i_c(int k) {
+  return __longlong_as_double((long long)(k + 1023) << 52);
+}
+
+__device__ __forceinline__ double exp_c(double x_in) {
+  // out-of-range and NaN inputs take the polynomial path on a clamped value
+  // and are replaced by selects at the end (no branches)
+  const double x = fmin(fmax(x_in, -0x1.74910d52d3052p+9), 0x1.62e42fefa39efp+9);
+  const double k = rint(x * 0x1.71547652b82fep+0);
+  const double r = (x - k * 0x1.62e42fee00000p-1) - k * 0x1.a39ef35793c76p-33;
+  double p = 0x1.6124613a86d09p-33;
+  p = p * r + 0x1.1eed8eff8d898p-29;
+  p = p * r + 0x1.ae64567f544e4p-26;
+  p = p * r + 0x1.27e4fb7789f5cp-22;
+  p = p * r + 0x1.71de3a556c734p-19;
+  p = p * r + 0x1.a01a01a01a01ap-16;
+  p = p * r + 0x1.a01a01a01a01ap-13;
+  p = p * r + 0x1.6c16c16c16c17p-10;
+  p = p * r + 0x1.1111111111111p-7;
+  p = p * r + 0x1.5555555555555p-5;
+  p = p * r + 0x1.5555555555555p-3;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  const int ki = (int)k;
+  const int k1 = ki / 2;
+  const double y = (p * pow2i_c(k1)) * pow2i_c(ki - k1);
+  return x_in != x_in ? x_in
+         : x_in > 0x1.62e42fefa39efp+9 ? __builtin_inf()
+         : x_in < -0x1.74910d52d3052p+9 ? 0.0
+         : y;
+}
+
+// sampling/moves/shear.py:19-52: cell vector idx moves by rv1 u + rv2 w,
+// u the unit vector along the first other cell vector and w the unit
+// vector of the second one's part orthogonal to u.  Rows are picked by
+// selects so that the cell stays in registers.
+__device__ __forceinline__ void shear_cell(const double* c, int idx,
+                                           double rv1, double rv2,
+                                           double* nc) {
+  double v1[3], v2[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    v1[k] = idx == 0 ? c[3 + k] : c[k];
+    v2[k] = idx == 2 ? c[3 + k] : c[6 + k];
+  }
+  const double n1 = sqrt(dot3(v1[0], v1[1], v1[2], v1[0], v1[1], v1[2]));
+  const double u0 = v1[0] / n1, u1 = v1[1] / n1, u2 = v1[2] / n1;
+  const double dp = dot3(v2[0], v2[1], v2[2], u0, u1, u2);
+  const double w0 = v2[0] - dp * u0, w1 = v2[1] - dp * u1,
+               w2 = v2[2] - dp * u2;
+  const double n2 = sqrt(dot3(w0, w1, w2, w0, w1, w2));
+  const double uu[3] = {u0, u1, u2};
+  const double ww[3] = {w0 / n2, w1 / n2, w2 / n2};
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double moved = (c[3 * r + k] + rv1 * uu[k]) + rv2 * ww[k];
+      nc[3 * r + k] = r == idx ? moved : c[3 * r + k];
+    }
+}
+
+// utils/cell.py:128-152: positions follow a cell change as x' = x T with
+// T = old^-1 new (rows are lattice vectors).
+__device__ __forceinline__ void cell_transform(const double* oc,
+                                               const double* nc, double* T) {
+  double inv[9];
+  inv3(oc, det3(oc), inv);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      T[3 * i + k] = dot3(inv[3 * i], inv[3 * i + 1], inv[3 * i + 2], nc[k],
+                          nc[3 + k], nc[6 + k]);
+}
+
 // ----------------------------------------------------------- Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
 // SC'11.  Counter-based: (counter, key) -> 4 x 32 bits.

@@ -52,6 +52,12 @@ struct jr_handle {
   jrh::DBuf<int> t_idx, t_reason;
   jrh::DBuf<double> t_step, t_emax, t_normals, t_uniforms;
   long long trial_rng = 0;
+  // fixed-contour walk of the whole population (burn-in)
+  jrh::DBuf<int> b_walk, b_sched, b_ctr;
+  jrh::DBuf<double> b_normals, b_uniforms;
+  int b_chunk = 0;  // walkers per replica the draw buffers hold
+  int b_steps = 0;  // sub-steps they were sized for
+  long long burn_rng = 0;
   // timing
   std::vector<hipEvent_t> ev;
   double walk_ms = 0.0;
@@ -63,6 +69,7 @@ struct jr_handle {
   int n_cu = 0;
   int wpb = 4;
   int blocks = 0;
+  int max_blocks = 0;  // resident blocks of the walk kernel at this geometry
   size_t lds_walk = 0;
   long long iter_i = 0;
   long long rng_iter = 0;

@@ -33,6 +33,9 @@ struct WalkArgs {
   const int* sample_idx;
   const double* trial_step;
   int* trial_reason;
+  // image half-ranges (CFG bit 2); kept last: the register allocation of the
+  // minimum-image walk kernels depends on the offsets of the arguments above
+  int img[3];
 };
 
 __host__ __device__ inline size_t wave_lds_bytes(int Nmax, int n_moves) {
@@ -77,6 +80,9 @@ __global__ void __launch_bounds__(256)
   C.tab.eps24 = tabs + SS;
   C.tab.sig2 = tabs + 2 * SS;
   C.tab.S = a.S;
+  C.tab.img[0] = a.img[0];
+  C.tab.img[1] = a.img[1];
+  C.tab.img[2] = a.img[2];
   C.Nmax = a.Nmax;
   C.S = a.S;
 
@@ -144,13 +150,19 @@ __global__ void __launch_bounds__(256)
           a.normals + ((size_t)tq * n_steps + t) * 3 * (size_t)a.Nmax;
       const double* u = a.uniforms + ((size_t)tq * n_steps + t) * 4;
       int reason = 0, n_eval = 1, n_grad = 0;
-      switch (mv.type) {
+      int type = mv.type;
+      if constexpr ((CFG & 8) != 0)
+        if (type == JR_MOVE_SHEAR || type == JR_MOVE_STRETCH) type = JR_MOVE_VOLUME;
+      switch (type) {
         case JR_MOVE_GMC:
           reason = gmc_move<A, CFG>(C, W, step, mv, xi) ? 0 : 1;
           n_eval = n_grad = mv.n_reflect;
           break;
-        case JR_MOVE_VOLUME:
-          reason = volume_move<A, CFG>(C, W, step, mv, xi[0], u[0]);
+        case JR_MOVE_VOLUME:  // with cfg bit 3: also shear and stretch
+          if constexpr ((CFG & 8) != 0)
+            reason = cell_move<A, CFG>(C, W, step, mv, xi[0], xi[a.Nmax], u[0]);
+          else
+            reason = volume_move<A, CFG>(C, W, step, mv, xi[0], u[0]);
           break;
         case JR_MOVE_HMC:
           reason = hmc_move<A, CFG>(C, W, step, mv, xi) ? 0 : 1;
@@ -160,6 +172,7 @@ __global__ void __launch_bounds__(256)
         case JR_MOVE_MORPH:
           reason = morph_move<A, CFG>(C, W, mv, u[0], u[1]) ? 0 : 1;
           break;
+
         default:
           break;
       }
@@ -229,6 +242,7 @@ struct EvalArgs {
   const double* eps4;
   const double* eps24;
   const double* sig2;
+  int img[3];  // image half-ranges (CFG bit 2)
 };
 
 template <int A, int CFG>
@@ -259,6 +273,9 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs a) {
   tab.eps24 = tabs + SS;
   tab.sig2 = tabs + 2 * SS;
   tab.S = a.S;
+  tab.img[0] = a.img[0];
+  tab.img[1] = a.img[1];
+  tab.img[2] = a.img[2];
 
   const int item = blockIdx.x * nwave_blk + wave;
   const int n_items = a.count ? min(*a.count, a.B) : a.B;
@@ -664,10 +681,11 @@ __global__ void fill_normal_kernel(double* out, long long n,
 // Shared move schedule: m = first index with u < cum[m] (sampling/mwg.py:243).
 __global__ void schedule_kernel(int* sched, int n_steps,
                                 unsigned long long seed, const IterState* st,
+                                long long it_fixed, unsigned stream,
                                 const double* cum, int n_moves) {
-  const long long it = st->rng_iter;
+  const long long it = st ? st->rng_iter : it_fixed;
   for (int t = threadIdx.x; t < n_steps; t += blockDim.x) {
-    uint32_t c[4] = {(uint32_t)t, 0u, (uint32_t)it, 4u};
+    uint32_t c[4] = {(uint32_t)t, 0u, (uint32_t)it, stream};
     philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     double u = u53(c[0], c[1]);
     int m = 0;

@@ -45,12 +45,13 @@ void launch_walk_t(const WalkArgs& a, int blocks, int wpb, size_t lds,
                      lds, s, a);
 }
 
-// occupancy variants are built for one atom per lane (N <= 64) only
+// occupancy variants are built for one atom per lane (N <= 64), without
+// image sums and shape moves only
 template <int CFG>
 void launch_walk_c(int A, int occ, const WalkArgs& a, int blocks, int wpb,
                    size_t lds, hipStream_t s) {
   if (A == 1) {
-    if (occ == 3) launch_walk_t<1, CFG, 3>(a, blocks, wpb, lds, s);
+    if (!(CFG & 12) && occ == 3) launch_walk_t<1, CFG & 3, 3>(a, blocks, wpb, lds, s);
     else launch_walk_t<1, CFG, 0>(a, blocks, wpb, lds, s);
   } else if (A == 2) {
     launch_walk_t<2, CFG, 0>(a, blocks, wpb, lds, s);
@@ -59,22 +60,29 @@ void launch_walk_c(int A, int occ, const WalkArgs& a, int blocks, int wpb,
   }
 }
 
-// cfg bit 0: species table (S > 1); bit 1: FP32 pair terms
+// cfg bit 0: species table (S > 1); bit 1: FP32 pair terms; bit 2: image
+// sums; bit 3: shear / stretch in the move table (bits 2 and 3 FP64 only)
 void launch_walk(int A, int cfg, int occ, const WalkArgs& a, int blocks,
                  int wpb, size_t lds, hipStream_t s) {
   switch (cfg) {
     case 0: launch_walk_c<0>(A, occ, a, blocks, wpb, lds, s); break;
     case 1: launch_walk_c<1>(A, occ, a, blocks, wpb, lds, s); break;
     case 2: launch_walk_c<2>(A, occ, a, blocks, wpb, lds, s); break;
-    default: launch_walk_c<3>(A, occ, a, blocks, wpb, lds, s); break;
+    case 3: launch_walk_c<3>(A, occ, a, blocks, wpb, lds, s); break;
+    case 4: launch_walk_c<4>(A, occ, a, blocks, wpb, lds, s); break;
+    case 5: launch_walk_c<5>(A, occ, a, blocks, wpb, lds, s); break;
+    case 8: launch_walk_c<8>(A, occ, a, blocks, wpb, lds, s); break;
+    case 9: launch_walk_c<9>(A, occ, a, blocks, wpb, lds, s); break;
+    case 12: launch_walk_c<12>(A, occ, a, blocks, wpb, lds, s); break;
+    default: launch_walk_c<13>(A, occ, a, blocks, wpb, lds, s); break;
   }
 }
 
 template <int CFG>
 const void* walk_fn_c(int A, int occ) {
   if (A == 1)
-    return occ == 3 ? (const void*)walk_kernel<1, CFG, 3>
-                    : (const void*)walk_kernel<1, CFG, 0>;
+    return (!(CFG & 12) && occ == 3) ? (const void*)walk_kernel<1, CFG & 3, 3>
+                                     : (const void*)walk_kernel<1, CFG, 0>;
   if (A == 2) return (const void*)walk_kernel<2, CFG, 0>;
   return (const void*)walk_kernel<4, CFG, 0>;
 }
@@ -84,7 +92,13 @@ const void* walk_fn(int A, int cfg, int occ) {
     case 0: return walk_fn_c<0>(A, occ);
     case 1: return walk_fn_c<1>(A, occ);
     case 2: return walk_fn_c<2>(A, occ);
-    default: return walk_fn_c<3>(A, occ);
+    case 3: return walk_fn_c<3>(A, occ);
+    case 4: return walk_fn_c<4>(A, occ);
+    case 5: return walk_fn_c<5>(A, occ);
+    case 8: return walk_fn_c<8>(A, occ);
+    case 9: return walk_fn_c<9>(A, occ);
+    case 12: return walk_fn_c<12>(A, occ);
+    default: return walk_fn_c<13>(A, occ);
   }
 }
 
@@ -109,13 +123,24 @@ void launch_eval(int A, int cfg, const EvalArgs& a, hipStream_t s) {
     case 0: launch_eval_c<0>(A, a, s); break;
     case 1: launch_eval_c<1>(A, a, s); break;
     case 2: launch_eval_c<2>(A, a, s); break;
-    default: launch_eval_c<3>(A, a, s); break;
+    case 3: launch_eval_c<3>(A, a, s); break;
+    case 4: launch_eval_c<4>(A, a, s); break;
+    default: launch_eval_c<5>(A, a, s); break;
   }
+}
+
+bool pot_images(const jr_pot* pot) {
+  return pot->images[0] > 0 || pot->images[1] > 0 || pot->images[2] > 0;
 }
 
 int pot_cfg(const jr_pot* pot) {
   return (pot->n_species > 1 ? 1 : 0) |
-         (pot->precision == JR_PREC_FP32_PAIRS ? 2 : 0);
+         (pot->precision == JR_PREC_FP32_PAIRS ? 2 : 0) |
+         (pot_images(pot) ? 4 : 0);
+}
+
+void set_images(int* dst, const jr_pot& pot) {
+  for (int d = 0; d < 3; ++d) dst[d] = pot.images[d];
 }
 
 int validate_pot(const jr_pot* pot) {
@@ -127,6 +152,12 @@ int validate_pot(const jr_pot* pot) {
     return set_global_err(JR_E_ARG, "pair tables must not be NULL");
   if (pot->precision != JR_PREC_FP64 && pot->precision != JR_PREC_FP32_PAIRS)
     return set_global_err(JR_E_ARG, "unknown precision %d", pot->precision);
+  for (int d = 0; d < 3; ++d)
+    if (pot->images[d] < 0 || pot->images[d] > JR_MAX_IMAGES)
+      return set_global_err(JR_E_ARG, "images[%d] must be in [0, %d], got %d", d,
+                            JR_MAX_IMAGES, pot->images[d]);
+  if (pot_images(pot) && pot->precision != JR_PREC_FP64)
+    return set_global_err(JR_E_UNSUPPORTED, "image sums are built for FP64 pairs only");
   return JR_OK;
 }
 
@@ -216,6 +247,7 @@ int jr_eval_energy_forces(const jr_pot* pot, int32_t device, int32_t B,
   a.eps4 = d_e4.p;
   a.eps24 = d_e24.p;
   a.sig2 = d_s2.p;
+  set_images(a.img, *pot);
   launch_eval(atoms_per_lane(Nmax), pot_cfg(pot), a, s);
   HIPCHK_G(hipGetLastError());
   if (U) HIPCHK_G(hipMemcpyAsync(U, d_U.p, sizeof(double) * B, hipMemcpyDeviceToHost, s));
@@ -324,7 +356,8 @@ int recompute_energies(jr_handle* h) {
   a.eps4 = h->eps4.p;
   a.eps24 = h->eps24.p;
   a.sig2 = h->sig2.p;
-  launch_eval(h->A, h->cfgbits, a, h->stream);
+  set_images(a.img, c.pot);
+  launch_eval(h->A, h->cfgbits & 7, a, h->stream);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return JR_OK;
@@ -358,6 +391,7 @@ WalkArgs walk_args(jr_handle* h) {
   a.eps4 = h->eps4.p;
   a.eps24 = h->eps24.p;
   a.sig2 = h->sig2.p;
+  set_images(a.img, c.pot);
   a.moves = h->moves.p;
   return a;
 }
@@ -450,7 +484,7 @@ void enqueue_draws(jr_handle* h) {
     hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for((size_t)R * K)), dim3(256), 0, s,
                        h->extra_u.p, (long long)R * K, seed, st, 0LL, 3u);
   hipLaunchKernelGGL(schedule_kernel, dim3(1), dim3(64), 0, s, h->sched.p,
-                     c.n_mcmc_steps, seed, st, h->cum.p, c.n_moves);
+                     c.n_mcmc_steps, seed, st, 0LL, 4u, h->cum.p, c.n_moves);
   hipLaunchKernelGGL(fill_normal_kernel, dim3(grid_for(n_normals(h))), dim3(256), 0, s,
                      h->normals.p, (long long)n_normals(h), seed, st, 0LL, 5u);
   hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(n_uniforms(h))), dim3(256), 0, s,
@@ -507,7 +541,7 @@ namespace {
 // blocks as are resident (never more than the chains of one iteration).
 int set_geometry(jr_handle* h, int wpb, int occ) {
   const auto& c = h->cfg;
-  if (h->A != 1) occ = 0;
+  if (h->A != 1 || (h->cfgbits & 12)) occ = 0;
   h->wpb = wpb;
   h->occ = occ;
   h->lds_walk = wave_lds_bytes(c.n_atoms_max, c.n_moves) * wpb +
@@ -519,7 +553,8 @@ int set_geometry(jr_handle* h, int wpb, int occ) {
   if (per_cu < 1) per_cu = 1;
   const int total = c.n_replicas * h->n_walk;
   const int need = (total + wpb - 1) / wpb;
-  h->blocks = std::max(1, std::min(need, per_cu * h->n_cu));
+  h->max_blocks = per_cu * h->n_cu;
+  h->blocks = std::max(1, std::min(need, h->max_blocks));
   if (h->graph_exec) {
     (void)hipGraphExecDestroy(h->graph_exec);
     h->graph_exec = nullptr;
@@ -555,7 +590,8 @@ hipError_t lj_eval(jr_handle* h, const double* pos, const int* types, const int*
   a.eps4 = h->eps4.p;
   a.eps24 = h->eps24.p;
   a.sig2 = h->sig2.p;
-  launch_eval(h->A, h->cfgbits, a, s);
+  set_images(a.img, h->cfg.pot);
+  launch_eval(h->A, h->cfgbits & 7, a, s);
   return hipGetLastError();
 }
 
@@ -575,6 +611,9 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   } else {
     int rc = validate_pot(&c.pot);
     if (rc) return rc;
+    if (c.learned && pot_images(&c.pot))
+      return set_global_err(JR_E_UNSUPPORTED,
+                            "the surrogate-screened walk takes minimum-image pairs only");
   }
   if (c.n_replicas < 1) return set_global_err(JR_E_ARG, "n_replicas must be >= 1");
   if (c.n_walkers < 2 || c.n_walkers > 4096)
@@ -592,12 +631,15 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   double wsum = 0.0;
   for (int m = 0; m < c.n_moves; ++m) {
     const jr_move& mv = c.moves[m];
-    if (mv.type < 0 || mv.type > JR_MOVE_MORPH)
+    if (mv.type < 0 || mv.type > JR_MOVE_STRETCH)
       return set_global_err(JR_E_ARG, "move %d: unknown type %d", m, mv.type);
     if (mv.type == JR_MOVE_MORPH && (mv.n_species < 2 || mv.n_species > c.pot.n_species))
       return set_global_err(JR_E_ARG, "alchemical_morph needs 2 <= n_species <= S");
     if (mv.type == JR_MOVE_GMC && mv.n_reflect < 1)
       return set_global_err(JR_E_ARG, "gmc needs n_reflect >= 1");
+    if ((mv.type == JR_MOVE_SHEAR || mv.type == JR_MOVE_STRETCH) && !c.mpnn &&
+        c.pot.precision != JR_PREC_FP64)
+      return set_global_err(JR_E_UNSUPPORTED, "shear and stretch are built for FP64 pairs only");
     if (c.move_weights[m] < 0.0) return set_global_err(JR_E_ARG, "negative move weight");
     wsum += c.move_weights[m];
   }
@@ -613,6 +655,9 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   h->max_pairs = c.n_replicas / 2;
   h->A = atoms_per_lane(c.n_atoms_max);
   h->cfgbits = pot_cfg(&c.pot);
+  for (int m = 0; m < c.n_moves; ++m)
+    if (c.moves[m].type == JR_MOVE_SHEAR || c.moves[m].type == JR_MOVE_STRETCH)
+      h->cfgbits |= 8;
   const int R = c.n_replicas, K = c.n_walkers, Nm = c.n_atoms_max, S = c.pot.n_species;
   auto fail = [&](int code) {
     g_err = h->err;
@@ -822,6 +867,24 @@ int jr_set_pair_valid(jr_handle* h, const int32_t* pair_valid) {
 int jr_set_iteration(jr_handle* h, int64_t i) {
   if (!h) return set_global_err(JR_E_ARG, "NULL handle");
   h->iter_i = i;
+  return JR_OK;
+}
+
+int jr_get_rng_state(jr_handle* h, int64_t* state) {
+  if (!h || !state) return set_global_err(JR_E_ARG, "NULL argument");
+  state[0] = h->rng_iter;
+  state[1] = h->trial_rng;
+  state[2] = h->burn_rng;
+  return JR_OK;
+}
+
+int jr_set_rng_state(jr_handle* h, const int64_t* state) {
+  if (!h || !state) return set_global_err(JR_E_ARG, "NULL argument");
+  if (state[0] < 0 || state[1] < 0 || state[2] < 0)
+    return h->fail(JR_E_ARG, "draw counters must be >= 0");
+  h->rng_iter = state[0];
+  h->trial_rng = state[1];
+  h->burn_rng = state[2];
   return JR_OK;
 }
 
@@ -1096,6 +1159,107 @@ int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
     if (accepted) accepted[q] = rs[q] == 0;
     if (reason) reason[q] = rs[q];
   }
+  return JR_OK;
+}
+
+int jr_initial_walk(jr_handle* h, int32_t walklength, const double* emax,
+                    const int32_t* schedule, const double* normals,
+                    const double* uniforms, int32_t max_chunk,
+                    int32_t* counters_out) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  if (h->learned)
+    return h->fail(JR_E_UNSUPPORTED, "%s: pair-potential walk only", __func__);
+  const auto& c = h->cfg;
+  if (!h->have_pop) return h->fail(JR_E_STATE, "population not set");
+  if (walklength < 1 || !emax) return h->fail(JR_E_ARG, "need walklength >= 1 and emax");
+  const bool host = schedule != nullptr;
+  if ((normals != nullptr) != host || (uniforms != nullptr) != host)
+    return h->fail(JR_E_ARG, "schedule, normals and uniforms must all be given or all NULL");
+  const int R = c.n_replicas, K = c.n_walkers, Nm = c.n_atoms_max, wl = walklength;
+  if (host)
+    for (int t = 0; t < wl; ++t)
+      if (schedule[t] < 0 || schedule[t] >= c.n_moves)
+        return h->fail(JR_E_ARG, "schedule entry %d out of range", t);
+  HIPCHK(h, hipSetDevice(c.device));
+  hipStream_t s = h->stream;
+  // walkers per replica per launch: all K unless their normals pass 1 GiB
+  const size_t per_walker = (size_t)wl * 3 * Nm;
+  const size_t cap = ((size_t)1 << 30) / sizeof(double);
+  int chunk = (int)std::min<size_t>(
+      (size_t)K, std::max<size_t>(1, cap / ((size_t)R * per_walker)));
+  if (max_chunk > 0) chunk = std::min(chunk, (int)max_chunk);
+  if (h->b_chunk != chunk || h->b_steps != wl) {
+    HIPCHK(h, h->b_normals.alloc((size_t)R * chunk * per_walker));
+    HIPCHK(h, h->b_uniforms.alloc((size_t)R * chunk * wl * 4));
+    HIPCHK(h, h->b_walk.alloc((size_t)R * chunk));
+    HIPCHK(h, h->b_sched.alloc((size_t)wl));
+    h->b_chunk = chunk;
+    h->b_steps = wl;
+  }
+  if (!h->b_ctr.p) HIPCHK(h, h->b_ctr.alloc((size_t)R * c.n_moves * JR_NCOUNT));
+  if (!h->t_emax.p) {
+    HIPCHK(h, h->t_step.alloc(R));
+    HIPCHK(h, h->t_emax.alloc(R));
+  }
+  HIPCHK(h, hipMemcpyAsync(h->t_emax.p, emax, sizeof(double) * R, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemsetAsync(h->b_ctr.p, 0, sizeof(int) * R * c.n_moves * JR_NCOUNT, s));
+  // one move schedule for every walker of every replica (burn_in.py:113-117)
+  if (host) {
+    HIPCHK(h, hipMemcpyAsync(h->b_sched.p, schedule, sizeof(int) * wl, hipMemcpyHostToDevice, s));
+  } else {
+    hipLaunchKernelGGL(schedule_kernel, dim3(1), dim3(64), 0, s, h->b_sched.p, wl,
+                       (unsigned long long)c.seed, (const IterState*)nullptr,
+                       (long long)h->burn_rng, 14u, h->cum.p, c.n_moves);
+  }
+  std::vector<int> widx((size_t)R * chunk);
+  for (int k0 = 0; k0 < K; k0 += chunk) {
+    const int ck = std::min(chunk, K - k0);
+    for (int r = 0; r < R; ++r)
+      for (int j = 0; j < ck; ++j) widx[(size_t)r * ck + j] = k0 + j;
+    HIPCHK(h, hipMemcpyAsync(h->b_walk.p, widx.data(), sizeof(int) * R * ck,
+                             hipMemcpyHostToDevice, s));
+    if (host) {
+      // rows k0 .. k0+ck of [R][K][wl][3][Nmax] and [R][K][wl][4]
+      HIPCHK(h, hipMemcpy2DAsync(h->b_normals.p, sizeof(double) * ck * per_walker,
+                                 normals + (size_t)k0 * per_walker,
+                                 sizeof(double) * K * per_walker,
+                                 sizeof(double) * ck * per_walker, R,
+                                 hipMemcpyHostToDevice, s));
+      HIPCHK(h, hipMemcpy2DAsync(h->b_uniforms.p, sizeof(double) * ck * wl * 4,
+                                 uniforms + (size_t)k0 * wl * 4,
+                                 sizeof(double) * K * wl * 4,
+                                 sizeof(double) * ck * wl * 4, R,
+                                 hipMemcpyHostToDevice, s));
+    } else {
+      const long long it = h->burn_rng;
+      const size_t nn = (size_t)R * ck * per_walker, nu = (size_t)R * ck * wl * 4;
+      hipLaunchKernelGGL(fill_normal_kernel, dim3(grid_for(nn)), dim3(256), 0, s,
+                         h->b_normals.p, (long long)nn, (unsigned long long)c.seed,
+                         (const IterState*)nullptr, it * 4096 + k0 / chunk, 12u);
+      hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(nu)), dim3(256), 0, s,
+                         h->b_uniforms.p, (long long)nu, (unsigned long long)c.seed,
+                         (const IterState*)nullptr, it * 4096 + k0 / chunk, 13u);
+    }
+    HIPCHK(h, hipMemsetAsync(h->queue.p, 0, sizeof(int), s));
+    WalkArgs a = walk_args(h);
+    a.emax = h->t_emax.p;
+    a.walk = h->b_walk.p;
+    a.n_walk = ck;
+    a.n_steps = wl;
+    a.schedule = h->b_sched.p;
+    a.normals = h->b_normals.p;
+    a.uniforms = h->b_uniforms.p;
+    a.counters = h->b_ctr.p;
+    a.total = R * ck;
+    const int blocks = std::max(1, std::min((R * ck + h->wpb - 1) / h->wpb, h->max_blocks));
+    launch_walk(h->A, h->cfgbits, h->occ, a, blocks, h->wpb, h->lds_walk, s);
+    HIPCHK(h, hipGetLastError());
+  }
+  if (!host) h->burn_rng += 1;
+  if (counters_out)
+    HIPCHK(h, hipMemcpyAsync(counters_out, h->b_ctr.p, sizeof(int) * R * c.n_moves * JR_NCOUNT,
+                             hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
   return JR_OK;
 }
 

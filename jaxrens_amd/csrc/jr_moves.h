@@ -121,14 +121,11 @@ __device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
   return accepted;
 }
 
-// sampling/moves/volume.py:49-109 + _common.py:16-93; returns reject reason.
+// sampling/moves/volume.py:49-109 + _common.py:16-93 (kernels without shape
+// moves); returns the reject reason.
 template <int A, int CFG>
 __device__ int volume_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
                            const DevMove& mv, double xi0, double u0) {
-  constexpr bool MULTI = (CFG & 1) != 0;
-  constexpr bool F32 = (CFG & 2) != 0;
-  (void)F32;
-  const int lane = lane_id();
   const int N = C.N;
   double oldV = fabs(det3(W.cell));
   double dV = (step * (double)N) * xi0;
@@ -155,6 +152,86 @@ __device__ int volume_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
   double p_acc = mv.flat_v_prior ? 1.0 : fmin(1.0, ipow_c(ratio, N));
   bool energy_ok = Hn < C.emax;
   bool prior_ok = u0 < p_acc;
+  if (energy_ok && cell_ok && prior_ok) {
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      W.x[s] = nx[s];
+      W.y[s] = ny[s];
+      W.z[s] = nz[s];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) W.cell[i] = nc[i];
+    W.E = Hn;
+    return 0;
+  }
+  return !energy_ok ? 1 : (!cell_ok ? 2 : 3);
+}
+
+// The cell moves of kernels whose move table holds shear or stretch (cfg
+// bit 3): volume (sampling/moves/volume.py:49-109), shear (shear.py:56-120)
+// and stretch (stretch.py:37-97), with the accept tail of _common.py:16-93.
+// Draws: xi0 = normal [0][0], xi1 = normal [1][0], u0 = uniform 0.
+//   volume   V' = |V + s N xi0|, cell' = cell r, x' = x r, r = (V'/V)^(1/3),
+//            prior min(1, (V'/V)^N) unless flat;
+//   shear    cell vector floor(3 u0) moved by (s xi0, s xi1) in the plane of
+//            the other two, x' = x old^-1 new;
+//   stretch  axis pair floor(3 u0) of (0,1), (0,2), (1,2) scaled by exp(rv)
+//            and exp(-rv), rv = s xi0: cell' = cell D, x' = x D.
+// All three proposals are formed (a few hundred flops) and the move's picked
+// by wave-uniform selects, so a single straight-line energy evaluation serves
+// the three.  Returns the reject reason: energy 1 > cell 2 > prior 3.
+template <int A, int CFG>
+__device__ int cell_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
+                         const DevMove& mv, double xi0, double xi1,
+                         double u0) {
+  const int N = C.N;
+  const bool iso = mv.type == JR_MOVE_VOLUME;
+  const bool shear = mv.type == JR_MOVE_SHEAR;
+  const double oldV = fabs(det3(W.cell));
+  const double newV = fabs(oldV + (step * (double)N) * xi0);
+  const double ratio = newV / oldV;
+  const double scale = cbrt_c(ratio);
+  int sel = (int)(u0 * 3.0);
+  if (sel > 2) sel = 2;
+  double ncs[9], Ts[9];
+  shear_cell(W.cell, sel, step * xi0, step * xi1, ncs);
+  cell_transform(W.cell, ncs, Ts);
+  const double up = exp_c(step * xi0), dn = exp_c(-(step * xi0));
+  const double d[3] = {sel < 2 ? up : 1.0,
+                       sel == 0 ? dn : (sel == 2 ? up : 1.0),
+                       sel == 0 ? 1.0 : dn};
+  double nc[9], T[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int i = 3 * r + c;
+      const double cv = W.cell[i] * scale, ct = W.cell[i] * d[c];
+      const double tt = r == c ? d[c] : 0.0;
+      nc[i] = iso ? cv : (shear ? ncs[i] : ct);
+      T[i] = shear ? Ts[i] : tt;
+    }
+  const double p_acc =
+      (iso && !mv.flat_v_prior) ? fmin(1.0, ipow_c(ratio, N)) : 1.0;
+  double nx[A], ny[A], nz[A];
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    const double tx = dot3(W.x[s], W.y[s], W.z[s], T[0], T[3], T[6]);
+    const double ty = dot3(W.x[s], W.y[s], W.z[s], T[1], T[4], T[7]);
+    const double tz = dot3(W.x[s], W.y[s], W.z[s], T[2], T[5], T[8]);
+    nx[s] = iso ? W.x[s] * scale : tx;
+    ny[s] = iso ? W.y[s] * scale : ty;
+    nz[s] = iso ? W.z[s] * scale : tz;
+  }
+  const bool cell_ok =
+      check_cell_shape(nc, N, mv.max_vpa, mv.min_vpa, mv.min_aspect);
+  const bool prior_ok = u0 < p_acc;
+  Geom g;
+  make_geom(nc, g);
+  double fx[A], fy[A], fz[A];
+  double U = wave_eval<A, false, CFG>(C.L, N, g, C.tab, nx, ny, nz, W.t, fx, fy, fz);
+  double Hn = U + ensemble_term(nc, C.pressure, C.mu, C.S, W.counts);
+  const bool energy_ok = Hn < C.emax;
   if (energy_ok && cell_ok && prior_ok) {
 #pragma unroll
     for (int s = 0; s < A; ++s) {

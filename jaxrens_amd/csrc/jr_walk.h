@@ -16,6 +16,7 @@ struct PairTab {
   const double* eps24;
   const double* sig2;
   int S;
+  int img[3];  // image half-ranges (CFG bit 2 kernels only)
 };
 
 struct WaveLds {
@@ -168,6 +169,16 @@ __device__ __forceinline__ double mask_q(bool ok, double v) {
   return __hiloint2double(ok ? __double2hiint(v) : 0, __double2loint(v));
 }
 
+// Lattice translation i0 c0 + i1 c1 + i2 c2 (rows of the cell).
+__device__ __forceinline__ void image_shift(const Geom& g, int i0, int i1,
+                                            int i2, double& tx, double& ty,
+                                            double& tz) {
+  const double a = (double)i0, b = (double)i1, c = (double)i2;
+  tx = fma(c, g.c[6], fma(b, g.c[3], a * g.c[0]));
+  ty = fma(c, g.c[7], fma(b, g.c[4], a * g.c[1]));
+  tz = fma(c, g.c[8], fma(b, g.c[5], a * g.c[2]));
+}
+
 struct PairIn {
   double dx, dy, dz, r2;
   bool ok;  // active and r2 < rc2
@@ -230,6 +241,8 @@ __device__ __forceinline__ void wave_eval_impl(
     double (&fz)[A]) {
   constexpr bool MULTI = (CFG & 1) != 0;
   constexpr bool F32 = (CFG & 2) != 0;
+  constexpr bool IMG = (CFG & 4) != 0;
+  static_assert(!(IMG && F32), "image sums are FP64 only");
   const int lane = lane_id();
   double e[A], ox[A], oy[A], oz[A], tx[A], ty[A], tz[A];
 #pragma unroll
@@ -284,10 +297,73 @@ __device__ __forceinline__ void wave_eval_impl(
   };
   const int kmax = (N - 1) >> 1;
   int k = 1;
+  if constexpr (IMG) {
+    // backends/lj.py:152-197 with supercell images: each ring pair is summed
+    // over the translations T = i0 c0 + i1 c1 + i2 c2 (i_d ascending, i2
+    // fastest) of its minimum-image displacement, one division per term;
+    // then the self pairs at T != 0 with half weight and no force.
+    const int h0 = tab.img[0], h1 = tab.img[1], h2 = tab.img[2];
+    auto image_terms = [&](int s, const PairIn& P0, bool active, bool reaction,
+                           double escale) {
+      for (int i0 = -h0; i0 <= h0; ++i0)
+        for (int i1 = -h1; i1 <= h1; ++i1)
+          for (int i2 = -h2; i2 <= h2; ++i2) {
+            double Tx, Ty, Tz;
+            image_shift(g, i0, i1, i2, Tx, Ty, Tz);
+            PairIn P = P0;
+            P.dx = P0.dx + Tx;
+            P.dy = P0.dy + Ty;
+            P.dz = P0.dz + Tz;
+            P.r2 = dot3(P.dx, P.dy, P.dz, P.dx, P.dy, P.dz);
+            P.ok = active && (P.r2 < K.rc2);
+            accumulate(s, P, mask_q(P.ok, num / P.r2), reaction, escale);
+          }
+    };
+#pragma unroll 1
+    for (; k <= kmax; ++k) {
+      if (FORCES && k > 1) rotate();
+#pragma unroll
+      for (int s = 0; s < A; ++s) {
+        const int a = lane + 64 * s;
+        const bool active = FULL || a < N;
+        image_terms(s, load_pair<A, CFG, GEO>(L, a + k, active, g, tab, K, x[s],
+                                             y[s], z[s], t[s]),
+                    active, true, 1.0);
+      }
+    }
+    if (N >= 2 && !(N & 1)) {
+      const int hN = N >> 1;
+#pragma unroll
+      for (int s = 0; s < A; ++s) {
+        const int a = lane + 64 * s;
+        const bool active = FULL || a < N;
+        image_terms(s, load_pair<A, CFG, GEO>(L, a + hN, active, g, tab, K,
+                                             x[s], y[s], z[s], t[s]),
+                    active, false, 0.5);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      const int ab = MULTI ? t[s] * tab.S + t[s] : 0;
+      for (int i0 = -h0; i0 <= h0; ++i0)
+        for (int i1 = -h1; i1 <= h1; ++i1)
+          for (int i2 = -h2; i2 <= h2; ++i2) {
+            if (i0 == 0 && i1 == 0 && i2 == 0) continue;
+            double Tx, Ty, Tz;
+            image_shift(g, i0, i1, i2, Tx, Ty, Tz);
+            const double r2 = dot3(Tx, Ty, Tz, Tx, Ty, Tz);
+            double ep, fs;
+            pair_lj<false, MULTI>(mask_q(r2 < K.rc2, num / r2), tab.eps4[ab],
+                                  tab.eps24[ab], tab.sig2[ab], ep, fs);
+            e[s] = e[s] + 0.5 * ep;
+          }
+    }
+    k = kmax + 1;
+  }
   // kept rolled: with N known (N == 64*A) a full unroll multiplies the
   // register pressure and the code size of every call site
 #pragma unroll 1
-  for (; k + 3 <= kmax; k += 4) {
+  for (; !IMG && k + 3 <= kmax; k += 4) {
     PairIn P[4][A];
     QT q[4][A];
 #pragma unroll
@@ -319,7 +395,7 @@ __device__ __forceinline__ void wave_eval_impl(
       for (int s = 0; s < A; ++s) accumulate(s, P[i][s], q[i][s], true, 1.0);
     }
   }
-  if (k + 1 <= kmax) {
+  if (!IMG && k + 1 <= kmax) {
     PairIn P0[A], P1[A];
     QT q0[A], q1[A];
 #pragma unroll
@@ -347,7 +423,7 @@ __device__ __forceinline__ void wave_eval_impl(
     for (int s = 0; s < A; ++s) accumulate(s, P1[s], q1[s], true, 1.0);
     k += 2;
   }
-  if (k <= kmax) {  // lone last step
+  if (!IMG && k <= kmax) {  // lone last step
     if (FORCES && k > 1) rotate();
 #pragma unroll
     for (int s = 0; s < A; ++s) {
@@ -358,7 +434,7 @@ __device__ __forceinline__ void wave_eval_impl(
       accumulate(s, P, q_single(P), true, 1.0);
     }
   }
-  if (N >= 2 && !(N & 1)) {
+  if (!IMG && N >= 2 && !(N & 1)) {
     const int h = N >> 1;
 #pragma unroll
     for (int s = 0; s < A; ++s) {
@@ -427,6 +503,7 @@ __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
                                             double (&fx)[A], double (&fy)[A],
                                             double (&fz)[A]) {
   constexpr bool MULTI = (CFG & 1) != 0;
+  constexpr bool IMG = (CFG & 4) != 0;
   const int lane = lane_id();
   const LdsView<A> L{W.base};
   EvalConst K;
@@ -434,7 +511,7 @@ __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
   K.rc2 = tab.rc2;
   K.cf = MULTI ? 1.0 : tab.eps24[0] / tab.sig2[0];
   double u[A], v[A], w[A];
-  if (g.cubic) {
+  if (!IMG && g.cubic) {
     const double len = g.c[0], len2 = len * len;
     K.num = K.num / len2;
     K.rc2 = K.rc2 / len2;
@@ -473,7 +550,10 @@ __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
   wave_lds_sync();
   double el;
   const bool full = (N == 64 * A);
-  if (g.cubic) {
+  if constexpr (IMG) {
+    wave_eval_impl<A, FORCES, CFG, GEO_TRI, false>(L, N, g, tab, K, u, v, w, t,
+                                                   el, fx, fy, fz);
+  } else if (g.cubic) {
     if (full)
       wave_eval_impl<A, FORCES, CFG, GEO_CUBIC, true>(L, N, g, tab, K, u, v, w,
                                                       t, el, fx, fy, fz);

@@ -368,6 +368,25 @@ class NSEngine:
             _lib.dptr(nn), _lib.dptr(uu), _lib.iptr(acc), _lib.iptr(rs)))
         return acc.astype(bool), rs
 
+    def initial_walk(self, walklength: int, emax, schedule=None, normals=None,
+                     uniforms=None, max_chunk: int = 0) -> np.ndarray:
+        """One fixed-contour walk of every live walker (``init/burn_in.py:48-148``):
+        ``walklength`` sub-steps at ``emax`` [R] held fixed.  Draws from the
+        device, or the host arrays ``schedule`` [wl], ``normals``
+        [R, K, wl, 3, Nmax], ``uniforms`` [R, K, wl, 4].  ``max_chunk`` > 0
+        caps the walkers per replica of one launch.  Returns the walk's
+        counters [R, n_moves, 7]."""
+        R = self.n_replicas
+        em = _lib.f64(np.broadcast_to(np.asarray(emax, dtype=np.float64), (R,)))
+        ctr = np.zeros((R, self.n_moves, _lib.JR_NCOUNT), np.int32)
+        sc = None if schedule is None else _lib.i32(schedule)
+        nn = None if normals is None else _lib.f64(normals)
+        uu = None if uniforms is None else _lib.f64(uniforms)
+        self._check(self.lib.jr_initial_walk(
+            self.h, int(walklength), _lib.dptr(em), _lib.iptr(sc), _lib.dptr(nn),
+            _lib.dptr(uu), int(max_chunk), _lib.iptr(ctr)))
+        return ctr
+
     def autotune(self, reps: int = 2) -> dict:
         """Pick the walk launch shape by timing candidates on this workload."""
         w, o, ms = C.c_int32(), C.c_int32(), C.c_double()

@@ -3,8 +3,9 @@
 Field names and defaults follow ``cli/schema/moves.py`` (``BaseMoveSpec``
 :60-120, ``GMCMoveSpec`` :249-406 with the legacy ``galilean`` alias of
 ``cli/schema/root.py:29-40``, ``HMCMoveSpec`` :409-447, ``VolumeMoveSpec``
-:628-682, ``AlchemicalMorphMoveSpec`` :791-828) and ``cli/schema/cell.py``
-for the cell-geometry guard of the volume move.  ``build_mwg`` mirrors
+:628-682, ``ShearMoveSpec`` :685-736, ``StretchMoveSpec`` :739-788,
+``AlchemicalMorphMoveSpec`` :791-828) and ``cli/schema/cell.py`` for the
+cell-geometry guard of the cell moves.  ``build_mwg`` mirrors
 ``sampling/mwg.py:40-259``: weights are normalised to probabilities and each
 move carries its own step size.
 """
@@ -90,6 +91,30 @@ class VolumeMoveSpec(BaseMoveSpec):
                            min_aspect=cell.min_aspect_ratio)
 
 
+class ShearMoveSpec(BaseMoveSpec):
+    """Volume-preserving shear of one cell vector (``sampling/moves/shear.py``)."""
+
+    type: Literal["shear"] = "shear"
+
+    def device_move(self, cell: CellSpec) -> _lib.JrMove:
+        return _lib.JrMove(type=_lib.JR_MOVE_SHEAR,
+                           max_vpa=cell.max_volume_per_atom,
+                           min_vpa=cell.min_volume_per_atom,
+                           min_aspect=cell.min_aspect_ratio)
+
+
+class StretchMoveSpec(BaseMoveSpec):
+    """Volume-preserving stretch of two Cartesian axes (``sampling/moves/stretch.py``)."""
+
+    type: Literal["stretch"] = "stretch"
+
+    def device_move(self, cell: CellSpec) -> _lib.JrMove:
+        return _lib.JrMove(type=_lib.JR_MOVE_STRETCH,
+                           max_vpa=cell.max_volume_per_atom,
+                           min_vpa=cell.min_volume_per_atom,
+                           min_aspect=cell.min_aspect_ratio)
+
+
 class AlchemicalMorphMoveSpec(BaseMoveSpec):
     """Random atom to a random other species (``sampling/moves/alchemical.py``)."""
 
@@ -103,13 +128,13 @@ class AlchemicalMorphMoveSpec(BaseMoveSpec):
 
 
 MoveSpec = Annotated[
-    Union[GMCMoveSpec, HMCMoveSpec, VolumeMoveSpec, AlchemicalMorphMoveSpec],
+    Union[GMCMoveSpec, HMCMoveSpec, VolumeMoveSpec, ShearMoveSpec,
+          StretchMoveSpec, AlchemicalMorphMoveSpec],
     Field(discriminator="type"),
 ]
 _MOVE_ADAPTER = TypeAdapter(MoveSpec)
 
-_UNBUILT = {"random_walk", "single_atom", "single_atom_sweep", "species_swap",
-            "shear", "stretch"}
+_UNBUILT = {"random_walk", "single_atom", "single_atom_sweep", "species_swap"}
 
 
 def parse_move(spec) -> BaseMoveSpec:

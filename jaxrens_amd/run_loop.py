@@ -23,14 +23,26 @@ from .termination import PriorMassTermination, check_any
 
 class EnergyLogger:
     """``.energies`` file: header ``K n_cull n_dof 0.0 N``, rows
-    ``iter emax volume`` (``io/energy_log.py:24-104``)."""
+    ``iter emax volume`` (``io/energy_log.py:24-104``).  A resume at
+    ``restart_iteration`` > 0 drops the rows at or past it and appends
+    (``io/restart_truncate.py:28-50``)."""
 
     def __init__(self, path, n_walkers: int, n_cull: int = 1, n_dof: int = 0,
-                 n_atoms: int = 0):
+                 n_atoms: int = 0, restart_iteration: int = 0):
         self.path = Path(path)
         self.path.parent.mkdir(parents=True, exist_ok=True)
-        self._f = open(self.path, "w")
-        self._f.write(f"{n_walkers} {n_cull} {n_dof} 0.0 {n_atoms}\n")
+        if restart_iteration > 0 and self.path.exists() and self.path.stat().st_size > 0:
+            with open(self.path) as f:
+                lines = f.readlines()
+            kept = [r for r in lines[1:]
+                    if r.strip() and int(r.split()[0]) < restart_iteration]
+            with open(self.path, "w") as f:
+                f.write(lines[0])
+                f.writelines(kept)
+            self._f = open(self.path, "a")
+        else:
+            self._f = open(self.path, "w")
+            self._f.write(f"{n_walkers} {n_cull} {n_dof} 0.0 {n_atoms}\n")
 
     def write_entry(self, iteration: int, energy: float, volume: float = 0.0):
         self._f.write(f"{iteration} {energy:.10e} {volume:.10e}\n")
@@ -70,15 +82,29 @@ class RunResult:
 
 def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
            energies_prefix: str | None = None, callbacks=(), seed: int = 0,
-           block: int = 1) -> RunResult:
+           block: int = 1, checkpoint_path=None, checkpoint_interval: int = 0,
+           resume=None) -> RunResult:
     """Drive ``engine`` until a termination criterion fires.
 
     ``block`` > 1 runs that many iterations per device call when no
     adaptation falls inside the block and the criteria can only fire at the
     block's last iteration; with the default 1 every iteration syncs, which
     reproduces the reference's stopping iteration exactly.
+
+    ``checkpoint_path`` / ``checkpoint_interval``: write the run state each
+    time the count of finished iterations reaches a multiple of the interval
+    (``cli/monitor.py:1225-1260``, ``CheckpointCallback``).  ``resume``: a
+    checkpoint (path or loaded
+    dict) to continue from; iterations keep their global numbers, and the
+    ``.energies`` files are cut back to the checkpoint and appended to.
     """
+    from .checkpoint import load_checkpoint, restore, save_checkpoint
     rng = np.random.default_rng(seed)
+    if resume is not None:
+        state = resume if isinstance(resume, dict) else load_checkpoint(resume)
+        host = restore(engine, state)
+        if host is not None:
+            rng = host
     R = engine.n_replicas
     loggers = []
     if energies_prefix is not None:
@@ -86,14 +112,15 @@ def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
             suffix = "" if R == 1 else f".{r}"
             loggers.append(EnergyLogger(f"{energies_prefix}{suffix}.energies",
                                         engine.n_walkers, engine.run.n_cull, 0,
-                                        int(engine.n_atoms_rep[r])))
+                                        int(engine.n_atoms_rep[r]),
+                                        restart_iteration=engine.iteration))
     adaptable = [m for m, spec in enumerate(engine.mwg.moves)
                  if spec.type != "alchemical_morph"]
     emax_hist, vol_hist, adapt_log = [], [], []
     counters = np.zeros((R, engine.n_moves, 7), np.int64)
     re_acc = np.zeros(max(R - 1, 1), np.int64)
     re_att = np.zeros(max(R - 1, 1), np.int64)
-    i = 0
+    i = engine.iteration
     reason = ""
     try:
         while True:
@@ -129,6 +156,11 @@ def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
             for cb in callbacks:
                 cb(i + n - 1, engine, out)
             i += n
+            if (checkpoint_path is not None and checkpoint_interval > 0
+                    and i // checkpoint_interval > (i - n) // checkpoint_interval):
+                for lg in loggers:
+                    lg.flush()
+                save_checkpoint(checkpoint_path, engine, rng)
             log_z = float(np.min(engine.log_evidence))
             hmax = float(np.max(out["emax"][-1]))
             for c in termination:

@@ -92,6 +92,37 @@ double or_cbrt(double x) {
   return y;
 }
 
+/* exp(x) for the stretch factors exp(+-rv) of sampling/moves/stretch.py:
+ * 58-60, from IEEE +, *, rint and exponent arithmetic only (Cody-Waite
+ * reduction): k = rint(x log2 e), r = (x - k ln2_hi) - k ln2_lo with ln2_hi
+ * the first 32 significant bits of ln 2 (k ln2_hi exact), exp(r) by the
+ * degree-13 Taylor polynomial sum r^n / n! in Horner form, times 2^k as
+ * 2^(k/2) 2^(k - k/2).  Coefficients are 1/n! rounded to double. */
+static double pow2i(int k) {
+  int64_t b = (int64_t)(k + 1023) << 52;
+  double y;
+  memcpy(&y, &b, 8);
+  return y;
+}
+
+double or_exp(double x) {
+  static const double inv_fact[14] = {
+      1.0, 1.0, 0.5, 0x1.5555555555555p-3, 0x1.5555555555555p-5,
+      0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
+      0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22,
+      0x1.ae64567f544e4p-26, 0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33};
+  if (isnan(x)) return x;
+  if (x > 0x1.62e42fefa39efp+9) return INFINITY; /* ln(DBL_MAX) */
+  if (x < -0x1.74910d52d3052p+9) return 0.0;     /* below the least subnormal */
+  double k = nearbyint(x * 0x1.71547652b82fep+0);
+  double r = (x - k * 0x1.62e42fee00000p-1) - k * 0x1.a39ef35793c76p-33;
+  double p = inv_fact[13];
+  for (int n = 12; n >= 0; --n) p = p * r + inv_fact[n];
+  int ki = (int)k;
+  int k1 = ki / 2;
+  return (p * pow2i(k1)) * pow2i(ki - k1);
+}
+
 /* x**n for integer n >= 0 by binary exponentiation (volume.py:92). */
 double or_ipow(double x, int n) {
   double r = 1.0, b = x;
@@ -209,6 +240,72 @@ static inline void acc_pair(int a, int j, const pair_t* p, double q, int multi,
   }
 }
 
+/* Lattice translation i0 c0 + i1 c1 + i2 c2 (cell rows). */
+static void image_shift(const double* cell, int i0, int i1, int i2,
+                        double* t) {
+  double a = (double)i0, b = (double)i1, c = (double)i2;
+  for (int d = 0; d < 3; ++d)
+    t[d] = fma(c, cell[6 + d], fma(b, cell[3 + d], a * cell[d]));
+}
+
+/* The pair sums with supercell images, backends/lj.py:152-197:
+ *   U = 1/2 sum_T sum_{i,j, not (i = j and T = 0)} phi(|MIC(x_j - x_i) + T|)
+ * over the translations T of image_shift with |i_d| <= img[d].  The mirror
+ * term (j, i, -T) equals (i, j, T), so each unordered pair is taken once,
+ * k = 1..(N-1)/2 (atom a with (a+k) mod N, every T per pair, i0, i1, i2
+ * ascending with i2 fastest, one division per term, reaction on the
+ * partner), then for even N the k = N/2 pairs from both ends with half the
+ * energy and no reaction, then the self terms phi(|T|), T != 0, with half
+ * weight and no force (their position derivative is exactly zero). */
+static void image_pairs(int N, const double* pos, const int32_t* types,
+                        const double* cell, const double* inv,
+                        const or_pot* pot, double num, double rc2, int multi,
+                        double* e, double* own, double* trav) {
+  const int* h = pot->img;
+  int kmax = (N - 1) / 2;
+  for (int k = 1; k <= kmax + ((N >= 2 && N % 2 == 0) ? 1 : 0); ++k) {
+    const int lone = k > kmax; /* the k = N/2 pairs */
+    for (int a = 0; a < N; ++a) {
+      const int j = (a + k) % N;
+      pair_t p0 = pair_geom(pos + 3 * a, pos + 3 * j, cell, inv, types[a],
+                            types[j], pot, rc2, 0);
+      for (int i0 = -h[0]; i0 <= h[0]; ++i0)
+        for (int i1 = -h[1]; i1 <= h[1]; ++i1)
+          for (int i2 = -h[2]; i2 <= h[2]; ++i2) {
+            double t[3];
+            image_shift(cell, i0, i1, i2, t);
+            pair_t p = p0;
+            p.dx = p0.dx + t[0];
+            p.dy = p0.dy + t[1];
+            p.dz = p0.dz + t[2];
+            p.r2 = dot3(p.dx, p.dy, p.dz, p.dx, p.dy, p.dz);
+            p.ok = p.r2 < rc2;
+            double qv = p.ok ? num / p.r2 : 0.0;
+            acc_pair(a, j, &p, qv, multi, !lone, lone ? 0.5 : 1.0, e, own,
+                     trav);
+          }
+    }
+  }
+  for (int a = 0; a < N; ++a) {
+    pair_t p;
+    int ab = types[a] * pot->n_species + types[a];
+    p.eps4 = pot->eps4[ab];
+    p.eps24 = pot->eps24[ab];
+    p.sig2 = pot->sig2[ab];
+    for (int i0 = -h[0]; i0 <= h[0]; ++i0)
+      for (int i1 = -h[1]; i1 <= h[1]; ++i1)
+        for (int i2 = -h[2]; i2 <= h[2]; ++i2) {
+          if (i0 == 0 && i1 == 0 && i2 == 0) continue;
+          double t[3];
+          image_shift(cell, i0, i1, i2, t);
+          double r2 = dot3(t[0], t[1], t[2], t[0], t[1], t[2]);
+          double ep, fs;
+          pair_lj(&p, r2 < rc2 ? num / r2 : 0.0, multi, &ep, &fs);
+          e[a] = e[a] + 0.5 * ep;
+        }
+  }
+}
+
 /* Model energy U (and forces F = -dU/dx when F != NULL), then
  * H = U + ensemble term; backends/hamiltonian.py:112-153.
  * Pair schedule: for k = 1..(N-1)/2 atom a pairs with (a+k) mod N; the
@@ -245,7 +342,8 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
   const int multi = pot->n_species > 1;
   const int diag = cell[1] == 0.0 && cell[2] == 0.0 && cell[3] == 0.0 &&
                    cell[5] == 0.0 && cell[6] == 0.0 && cell[7] == 0.0;
-  const int cubic = diag && cell[0] == cell[4] && cell[0] == cell[8];
+  const int images = pot->img[0] > 0 || pot->img[1] > 0 || pot->img[2] > 0;
+  const int cubic = !images && diag && cell[0] == cell[4] && cell[0] == cell[8];
   double num = multi ? 1.0 : pot->sig2[0];
   double rc2 = pot->rc2;
   double cf = multi ? 1.0 : pot->eps24[0] / pot->sig2[0];
@@ -269,6 +367,11 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
   int kmax = (N - 1) / 2;
   int half = (N >= 2) && (N % 2 == 0);
   int k = 1;
+  if (images) {
+    image_pairs(N, pos, types, cell, inv, pot, num, rc2, multi, e, own, trav);
+    k = kmax + 1;
+    half = 0;
+  }
   while (k <= kmax) {
     const int G = (k + 3 <= kmax) ? 4 : ((k + 1 <= kmax) ? 2 : 1);
     for (int a = 0; a < N; ++a) {
@@ -452,6 +555,88 @@ static int volume_move(const chain_ctx* c, double* x, const int32_t* types,
   return reason;
 }
 
+/* _common.py:16-93 without the prior factor, for the volume-preserving
+ * cell moves: positions x' = x T, one energy evaluation, accepted iff
+ * H' < Emax and the cell passes the geometry guard; reason energy > cell. */
+static int cell_shape_move(const chain_ctx* c, double* x,
+                           const int32_t* types, double* cell, double* E,
+                           const or_move* mv, const double* nc,
+                           const double* T) {
+  int N = c->N;
+  double* nx = (double*)malloc(sizeof(double) * 3 * (size_t)(N + 1));
+  for (int a = 0; a < N; ++a)
+    for (int k = 0; k < 3; ++k)
+      nx[3 * a + k] = dot3(x[3 * a], x[3 * a + 1], x[3 * a + 2], T[k],
+                           T[3 + k], T[6 + k]);
+  double U, Hn;
+  or_energy_forces(N, nx, types, nc, c->pot, c->pressure, c->mu, &U, &Hn,
+                   NULL);
+  int cell_ok = or_check_cell_shape(nc, N, mv->max_vpa, mv->min_vpa,
+                                    mv->min_aspect);
+  int energy_ok = Hn < c->emax;
+  int reason = !energy_ok ? 1 : (!cell_ok ? 2 : 0);
+  if (reason == 0) {
+    memcpy(x, nx, sizeof(double) * 3 * (size_t)N);
+    memcpy(cell, nc, 9 * sizeof(double));
+    *E = Hn;
+  }
+  free(nx);
+  return reason;
+}
+
+/* sampling/moves/shear.py:19-120: cell vector idx = floor(3 u) gains
+ * rv1 e1 + rv2 e2 (rv = step * normal), e1 the unit vector of the first
+ * other cell vector v1, e2 that of the second one's part orthogonal to e1;
+ * positions follow by T = old^-1 new (utils/cell.py:128-152). */
+static int shear_move(const chain_ctx* c, double* x, const int32_t* types,
+                      double* cell, double* E, double s, const or_move* mv,
+                      double xi0, double xi1, double u) {
+  int idx = (int)(u * 3.0);
+  if (idx > 2) idx = 2;
+  const double* v1 = cell + 3 * (idx == 0 ? 1 : 0);
+  const double* v2 = cell + 3 * (idx == 2 ? 1 : 2);
+  double n1 = sqrt(dot3(v1[0], v1[1], v1[2], v1[0], v1[1], v1[2]));
+  double e1[3], e2[3], w[3];
+  for (int d = 0; d < 3; ++d) e1[d] = v1[d] / n1;
+  double proj = dot3(v2[0], v2[1], v2[2], e1[0], e1[1], e1[2]);
+  for (int d = 0; d < 3; ++d) w[d] = v2[d] - proj * e1[d];
+  double n2 = sqrt(dot3(w[0], w[1], w[2], w[0], w[1], w[2]));
+  for (int d = 0; d < 3; ++d) e2[d] = w[d] / n2;
+  double rv1 = s * xi0, rv2 = s * xi1;
+  double nc[9], inv[9], T[9];
+  memcpy(nc, cell, sizeof(nc));
+  for (int d = 0; d < 3; ++d)
+    nc[3 * idx + d] = (cell[3 * idx + d] + rv1 * e1[d]) + rv2 * e2[d];
+  inv3(cell, or_det3(cell), inv);
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k)
+      T[3 * i + k] = dot3(inv[3 * i], inv[3 * i + 1], inv[3 * i + 2], nc[k],
+                          nc[3 + k], nc[6 + k]);
+  return cell_shape_move(c, x, types, cell, E, mv, nc, T);
+}
+
+/* sampling/moves/stretch.py:37-97: axis pair floor(3 u) of (0,1), (0,2),
+ * (1,2); D = diag with exp(rv) on the first axis and exp(-rv) on the second
+ * (rv = step * normal); cell' = cell D, x' = x D. */
+static int stretch_move(const chain_ctx* c, double* x, const int32_t* types,
+                        double* cell, double* E, double s, const or_move* mv,
+                        double xi0, double u) {
+  static const int axes[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  int pr = (int)(u * 3.0);
+  if (pr > 2) pr = 2;
+  double rv = s * xi0;
+  double d[3] = {1.0, 1.0, 1.0};
+  d[axes[pr][0]] = or_exp(rv);
+  d[axes[pr][1]] = or_exp(-rv);
+  double nc[9], T[9];
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k) {
+      nc[3 * r + k] = cell[3 * r + k] * d[k];
+      T[3 * r + k] = r == k ? d[k] : 0.0;
+    }
+  return cell_shape_move(c, x, types, cell, E, mv, nc, T);
+}
+
 /* sampling/moves/hmc.py:51-152.  Forces at a position are evaluated once and
  * reused by the next half kick (bit-identical to re-evaluating). */
 static int hmc_move(const chain_ctx* c, double* x, const int32_t* types,
@@ -559,6 +744,18 @@ void or_walk_chain(int N, int Nmax, double* pos, int32_t* types, double* cell,
       case OR_MOVE_MORPH: {
         int acc = morph_move(&c, pos, types, cell, energy, mv, u);
         count(ctr, acc ? 0 : 1, 1, 0);
+        break;
+      }
+      case OR_MOVE_SHEAR: {
+        int reason = shear_move(&c, pos, types, cell, energy, s, mv, xi[0],
+                                xi[Nmax], u[0]);
+        count(ctr, reason, 1, 0);
+        break;
+      }
+      case OR_MOVE_STRETCH: {
+        int reason = stretch_move(&c, pos, types, cell, energy, s, mv, xi[0],
+                                  u[0]);
+        count(ctr, reason, 1, 0);
         break;
       }
       default:
@@ -781,6 +978,47 @@ int or_ns_iteration(or_pop* pop, const or_pot* pot, const or_run* run,
 
 /* sampling/adaptation/stepsize_handler.py:215-261: one trial move on each
  * sampled walker (a copy), reporting accepted and reject reason. */
+/* init/burn_in.py:48-148 (_one_walk): every walker of replica r walks
+ * walklength sub-steps at the fixed contour emax[r] with one schedule for
+ * all; normals [R][K][walklength][3][Nmax], uniforms [R][K][walklength][4];
+ * counters [R][n_moves][7] are overwritten with the walk's sums. */
+void or_initial_walk(or_pop* pop, const or_pot* pot, const or_run* run,
+                     const or_move* moves, int n_moves, const double* emax,
+                     const int32_t* schedule, int walklength,
+                     const double* normals, const double* uniforms,
+                     int32_t* counters, int n_threads) {
+  int R = pop->R, K = pop->K, Nm = pop->Nmax, S = pop->S;
+  memset(counters, 0, sizeof(int32_t) * (size_t)R * n_moves * OR_NCOUNT);
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+  (void)n_threads;
+#endif
+#pragma omp parallel
+  {
+    int32_t* local =
+        (int32_t*)calloc((size_t)R * n_moves * OR_NCOUNT, sizeof(int32_t));
+#pragma omp for schedule(dynamic, 1)
+    for (int q = 0; q < R * K; ++q) {
+      int r = q / K;
+      const double* xi = normals + (size_t)q * walklength * 3 * Nm;
+      const double* u = uniforms + (size_t)q * walklength * 4;
+      or_walk_chain(pop->n_atoms[q], Nm, pop->pos + (size_t)q * Nm * 3,
+                    pop->types + (size_t)q * Nm, pop->cell + 9 * (size_t)q,
+                    pop->energy + q, pop->dir + (size_t)q * Nm * 3, pot,
+                    run->pressure[r], run->mu + (size_t)r * S, emax[r],
+                    run->step_sizes + (size_t)r * n_moves, moves, n_moves,
+                    schedule, walklength, xi, u,
+                    local + (size_t)r * n_moves * OR_NCOUNT);
+    }
+#pragma omp critical
+    {
+      for (int i = 0; i < R * n_moves * OR_NCOUNT; ++i) counters[i] += local[i];
+    }
+    free(local);
+  }
+}
+
 void or_trial_moves(const or_pop* pop, const or_pot* pot, const or_run* run,
                     const or_move* moves, int n_moves, int move_idx,
                     const double* trial_step, const double* emax,

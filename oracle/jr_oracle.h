@@ -19,7 +19,8 @@
 extern "C" {
 #endif
 
-enum { OR_MOVE_GMC = 0, OR_MOVE_VOLUME = 1, OR_MOVE_HMC = 2, OR_MOVE_MORPH = 3 };
+enum { OR_MOVE_GMC = 0, OR_MOVE_VOLUME = 1, OR_MOVE_HMC = 2, OR_MOVE_MORPH = 3,
+       OR_MOVE_SHEAR = 4, OR_MOVE_STRETCH = 5 };
 
 /* counters per (replica, move): proposed, rr0 (=accepted), rr1 energy,
  * rr2 cell, rr3 prior, n_evaluations, n_grad_evaluations */
@@ -31,6 +32,8 @@ typedef struct {
   const double* eps4;   /* [S*S] 4*eps_ab */
   const double* eps24;  /* [S*S] 24*eps_ab */
   const double* sig2;   /* [S*S] sigma_ab^2 */
+  int img[3];           /* image half-ranges supercell_trafo[d] / 2; all 0:
+                           minimum image only (backends/lj.py:106-113) */
 } or_pot;
 
 typedef struct {
@@ -84,6 +87,7 @@ typedef struct {
 
 double or_det3(const double* c);
 double or_cbrt(double x);
+double or_exp(double x);
 double or_ipow(double x, int n);
 int or_check_cell_shape(const double* cell, int n_atoms, double max_vpa,
                         double min_vpa, double min_aspect);
@@ -95,6 +99,13 @@ double or_ensemble_energy(int N, const int32_t* types, const double* cell,
 void or_energy_forces(int N, const double* pos, const int32_t* types,
                       const double* cell, const or_pot* pot, double pressure,
                       const double* mu, double* U, double* H, double* F);
+
+/* Fixed-contour walk of every walker (burn-in), see jr_oracle.c. */
+void or_initial_walk(or_pop* pop, const or_pot* pot, const or_run* run,
+                     const or_move* moves, int n_moves, const double* emax,
+                     const int32_t* schedule, int walklength,
+                     const double* normals, const double* uniforms,
+                     int32_t* counters, int n_threads);
 
 /* Walk one chain for n_steps MWG sub-steps (state updated in place). */
 void or_walk_chain(int N, int Nmax, double* pos, int32_t* types, double* cell,

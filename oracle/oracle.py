@@ -21,7 +21,7 @@ _ip = C.POINTER(C.c_int32)
 
 class OrPot(C.Structure):
     _fields_ = [("n_species", C.c_int), ("rc2", C.c_double), ("eps4", _dp),
-                ("eps24", _dp), ("sig2", _dp)]
+                ("eps24", _dp), ("sig2", _dp), ("img", C.c_int * 3)]
 
 
 class OrMove(C.Structure):
@@ -69,6 +69,8 @@ def lib() -> C.CDLL:
         L.or_det3.argtypes = [_dp]
         L.or_cbrt.restype = C.c_double
         L.or_cbrt.argtypes = [C.c_double]
+        L.or_exp.restype = C.c_double
+        L.or_exp.argtypes = [C.c_double]
         L.or_ipow.restype = C.c_double
         L.or_ipow.argtypes = [C.c_double, C.c_int]
         L.or_check_cell_shape.restype = C.c_int
@@ -91,6 +93,10 @@ def lib() -> C.CDLL:
         L.or_replica_exchange.restype = None
         L.or_replica_exchange.argtypes = [C.POINTER(OrPop), C.POINTER(OrRun), _dp,
                                           C.c_int, _ip, _dp, _ip, _ip]
+        L.or_initial_walk.restype = None
+        L.or_initial_walk.argtypes = [C.POINTER(OrPop), C.POINTER(OrPot),
+                                      C.POINTER(OrRun), C.POINTER(OrMove), C.c_int,
+                                      _dp, _ip, C.c_int, _dp, _dp, _ip, C.c_int]
         L.or_trial_moves.restype = None
         L.or_trial_moves.argtypes = [C.POINTER(OrPop), C.POINTER(OrPot),
                                      C.POINTER(OrRun), C.POINTER(OrMove), C.c_int,
@@ -125,14 +131,16 @@ def i32(a):
 class Pot:
     """Pair table as the oracle takes it (from an LJ backend's table)."""
 
-    def __init__(self, n_species, eps4, eps24, sig2, rc2):
+    def __init__(self, n_species, eps4, eps24, sig2, rc2, images=(0, 0, 0)):
         self.keep = (f64(eps4), f64(eps24), f64(sig2))
         self.s = OrPot(int(n_species), float(rc2), _d(self.keep[0]),
-                       _d(self.keep[1]), _d(self.keep[2]))
+                       _d(self.keep[1]), _d(self.keep[2]),
+                       (C.c_int * 3)(*(int(h) for h in images)))
 
     @classmethod
     def from_table(cls, t):
-        return cls(t.n_species, t.eps4, t.eps24, t.sig2, t.rc2)
+        return cls(t.n_species, t.eps4, t.eps24, t.sig2, t.rc2,
+                   getattr(t, "images", (0, 0, 0)))
 
 
 def energy_forces(pot: Pot, pos, types, cell, pressure=0.0, mu=None, forces=True):
@@ -238,6 +246,24 @@ def trial_moves(pop: Population, pot: Pot, pressure, mu, step_sizes, moves, move
     return acc.astype(bool), rs
 
 
+def initial_walk(pop: Population, pot: Pot, pressure, mu, step_sizes, moves, emax,
+                 schedule, normals, uniforms, n_threads=0):
+    """Fixed-contour walk of every walker of ``pop`` (in place); returns the
+    counters [R, n_moves, 7]."""
+    R, K, Nm = pop.pos.shape[:3]
+    S = pot.s.n_species
+    pr, m, ss = f64(pressure), f64(np.asarray(mu).reshape(R, S)), f64(step_sizes)
+    run = OrRun(_d(pr), _d(m), _d(ss))
+    sc = i32(schedule)
+    em, nn, uu = f64(emax), f64(normals), f64(uniforms)
+    ctr = np.zeros((R, len(moves), 7), np.int32)
+    ps = pop.struct(S)
+    lib().or_initial_walk(C.byref(ps), C.byref(pot.s), C.byref(run), moves_array(moves),
+                          len(moves), _d(em), _i(sc), len(sc), _d(nn), _d(uu), _i(ctr),
+                          n_threads)
+    return ctr
+
+
 def replica_exchange(pop: Population, pressure, mu, emax, re_u, n_swap_cycles=1,
                      pair_valid=None, S=1):
     R = pop.pos.shape[0]
@@ -256,6 +282,10 @@ def replica_exchange(pop: Population, pressure, mu, emax, re_u, n_swap_cycles=1,
 
 def cbrt(x):
     return lib().or_cbrt(float(x))
+
+
+def exp(x):
+    return lib().or_exp(float(x))
 
 
 def det3(cell):
