@@ -83,7 +83,8 @@ class RunResult:
 def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
            energies_prefix: str | None = None, callbacks=(), seed: int = 0,
            block: int = 1, checkpoint_path=None, checkpoint_interval: int = 0,
-           resume=None) -> RunResult:
+           resume=None, traj_prefix: str | None = None, traj_interval: int = 1,
+           symbol_map: dict[int, str] | None = None, traj_wrap: bool = True) -> RunResult:
     """Drive ``engine`` until a termination criterion fires.
 
     ``block`` > 1 runs that many iterations per device call when no
@@ -97,7 +98,15 @@ def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
     checkpoint (path or loaded
     dict) to continue from; iterations keep their global numbers, and the
     ``.energies`` files are cut back to the checkpoint and appended to.
+
+    ``traj_prefix``: write the dead walker of every iteration ``i`` with
+    ``i % traj_interval == 0`` to ``<prefix>[.r].traj.extxyz`` per replica
+    (``cli/monitor.py:1405-1515``, ``io/trajectory.py:84-139``), species
+    named by ``symbol_map`` (default: type code t as ``"X<t>"``), positions
+    wrapped into the cell unless ``traj_wrap`` is False (``output.wrap_atoms``,
+    ``cli/schema/output.py:220-237``).
     """
+    from .trajectory import ExtxyzTrajectoryWriter
     from .checkpoint import load_checkpoint, restore, save_checkpoint
     rng = np.random.default_rng(seed)
     if resume is not None:
@@ -114,6 +123,16 @@ def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
                                         engine.n_walkers, engine.run.n_cull, 0,
                                         int(engine.n_atoms_rep[r]),
                                         restart_iteration=engine.iteration))
+    trajs = []
+    if traj_prefix is not None:
+        if traj_interval < 1:
+            raise ValueError("traj_interval must be >= 1")
+        smap = symbol_map if symbol_map is not None else {t: f"X{t}" for t in range(engine.S)}
+        for r in range(R):
+            suffix = "" if R == 1 else f".{r}"
+            trajs.append(ExtxyzTrajectoryWriter(f"{traj_prefix}{suffix}.traj.extxyz", smap,
+                                                wrap=traj_wrap,
+                                                restart_iteration=engine.iteration))
     adaptable = [m for m, spec in enumerate(engine.mwg.moves)
                  if spec.type != "alchemical_morph"]
     emax_hist, vol_hist, adapt_log = [], [], []
@@ -142,7 +161,7 @@ def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
                     n = min(n, adaptation.adjust_interval
                             - (i % adaptation.adjust_interval))
                 n = max(1, min(n, _iterations_before_check(termination, i)))
-            out = engine.ns_iterations(n)
+            out = engine.ns_iterations(n, dead_records=bool(trajs))
             for j in range(n):
                 it = i + j
                 emax_hist.append(out["emax"][j])
@@ -150,6 +169,12 @@ def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
                 for r, lg in enumerate(loggers):
                     lg.write_entry(it, float(out["emax"][j, r]),
                                    float(out["dead_volume"][j, r]))
+                if trajs and it % traj_interval == 0:
+                    for r, tw in enumerate(trajs):
+                        na = int(engine.n_atoms_rep[r])
+                        tw.write_dead_point(it, out["dead_pos"][j, r, :na],
+                                            out["dead_types"][j, r, :na],
+                                            out["dead_cell"][j, r], float(out["emax"][j, r]))
             counters += out["counters"].sum(0)
             re_acc += out["re_acc"].sum(0)
             re_att += out["re_att"].sum(0)
@@ -158,7 +183,7 @@ def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
             i += n
             if (checkpoint_path is not None and checkpoint_interval > 0
                     and i // checkpoint_interval > (i - n) // checkpoint_interval):
-                for lg in loggers:
+                for lg in loggers + trajs:
                     lg.flush()
                 save_checkpoint(checkpoint_path, engine, rng)
             log_z = float(np.min(engine.log_evidence))
@@ -170,7 +195,7 @@ def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
             if stop:
                 break
     finally:
-        for lg in loggers:
+        for lg in loggers + trajs:
             lg.close()
     return RunResult(
         iterations=i, reason=reason, log_evidence=engine.final_log_evidence(),

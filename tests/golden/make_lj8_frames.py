@@ -4,7 +4,9 @@ Reads (as text only) tests/_assets/data/lj8_npt/output/lj8_npt.traj.extxyz of
 the reference checkout and writes lj8_dead_frames.json next to this script:
 for each frame the cell (rows = lattice vectors), positions, the logged
 ``ns_energy`` (= U_LJ + P V, float32 in the reference run) and the iteration.
-Run once in a container that holds the reference; the JSON is committed.
+The file's text is also copied beside it (``lj8_npt.traj.extxyz``), the
+byte-level golden of the extxyz writer.  Run once in a container that holds
+the reference; both outputs are committed.
 """
 
 import json
@@ -33,6 +35,7 @@ def main():
     OUT.write_text(json.dumps({"source": "lj8_npt.traj.extxyz", "pressure": 0.1,
                                "epsilon": 1.0, "sigma": 1.0, "cutoff": 2.5,
                                "frames": frames}, indent=1))
+    OUT.with_name("lj8_npt.traj.extxyz").write_text(SRC.read_text())
     print(f"wrote {len(frames)} frames to {OUT}")
 
 

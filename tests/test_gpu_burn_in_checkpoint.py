@@ -16,6 +16,7 @@ from jaxrens_amd.checkpoint import load_checkpoint
 from jaxrens_amd.engine import NSEngine, RunConfig
 from jaxrens_amd.run_loop import EnergyLogger, run_ns
 from jaxrens_amd.termination import IterationTermination
+from jaxrens_amd.trajectory import read_extxyz
 
 pytestmark = pytest.mark.gpu
 
@@ -103,7 +104,8 @@ def test_checkpoint_resume_is_exact(tmp_path):
     a = _engine(seed=5)
     ra = run_ns(a, [IterationTermination(14)], adaptation=ad, seed=9,
                 energies_prefix=str(tmp_path / "a"),
-                checkpoint_path=tmp_path / "ck.npz", checkpoint_interval=7)
+                checkpoint_path=tmp_path / "ck.npz", checkpoint_interval=7,
+                traj_prefix=str(tmp_path / "a"), traj_interval=3, symbol_map={0: "Ar"})
     assert ra.iterations == 14
     ck = load_checkpoint(tmp_path / "ck.npz")
     assert ck["iteration"] == 14  # the last write
@@ -111,7 +113,8 @@ def test_checkpoint_resume_is_exact(tmp_path):
     b = _engine(seed=5)
     run_ns(b, [IterationTermination(7)], adaptation=ad, seed=9,
            energies_prefix=str(tmp_path / "b"),
-           checkpoint_path=tmp_path / "mid.npz", checkpoint_interval=7)
+           checkpoint_path=tmp_path / "mid.npz", checkpoint_interval=7,
+           traj_prefix=str(tmp_path / "b"), traj_interval=3, symbol_map={0: "Ar"})
     mid = load_checkpoint(tmp_path / "mid.npz")
     assert mid["iteration"] == 7 and mid["n_dead"] == 7
     # a third, fresh engine resumes at 7 and appends to b's energies
@@ -120,7 +123,8 @@ def test_checkpoint_resume_is_exact(tmp_path):
     with open(tmp_path / "b.0.energies", "a") as f:
         f.write("7 1.0 1.0\n8 1.0 1.0\n")
     rc = run_ns(c, [IterationTermination(14)], adaptation=ad,
-                energies_prefix=str(tmp_path / "b"), resume=tmp_path / "mid.npz")
+                energies_prefix=str(tmp_path / "b"), resume=tmp_path / "mid.npz",
+                traj_prefix=str(tmp_path / "b"), traj_interval=3, symbol_map={0: "Ar"})
     assert rc.iterations == 14
     pa, pc = a.get_population(), c.get_population()
     for k in ("energy", "positions", "cell", "direction", "types"):
@@ -133,3 +137,30 @@ def test_checkpoint_resume_is_exact(tmp_path):
         assert np.array_equal(ea["iterations"], eb["iterations"])
         assert np.array_equal(ea["energies"], eb["energies"])
         assert np.array_equal(ea["volumes"], eb["volumes"])
+        ta = (tmp_path / f"a.{r}.traj.extxyz").read_text()
+        assert ta == (tmp_path / f"b.{r}.traj.extxyz").read_text()
+        assert [f["iter"] for f in read_extxyz(tmp_path / f"a.{r}.traj.extxyz")] == \
+            [0, 3, 6, 9, 12]
+
+
+def test_run_writes_trajectory(tmp_path):
+    """Dead points of every second iteration: the frame energy is the
+    .energies row, and the LJ energy of the written (wrapped, 8-decimal)
+    geometry plus P V reproduces it."""
+    print("trajectory", flush=True)
+    eng = _engine(R=2, K=12, seed=3)
+    run_ns(eng, [IterationTermination(9)], energies_prefix=str(tmp_path / "t"),
+           traj_prefix=str(tmp_path / "t"), traj_interval=2, symbol_map={0: "Ar"})
+    lj = LJBackend(1.0, 1.0, 2.5, supercell_trafo=(4, 4, 4))
+    for r in range(2):
+        fr = read_extxyz(tmp_path / f"t.{r}.traj.extxyz")
+        en = EnergyLogger.read(tmp_path / f"t.{r}.energies")
+        assert [f["iter"] for f in fr] == [0, 2, 4, 6, 8]
+        for f in fr:
+            assert f["ns_energy"] == en["energies"][f["iter"]] or \
+                abs(f["ns_energy"] - en["energies"][f["iter"]]) <= 1e-9 * abs(f["ns_energy"])
+            frac = np.linalg.solve(f["cell"].T, f["positions"].T).T
+            assert frac.min() > -1e-6 and frac.max() < 1 + 1e-6
+            u = lj(f["positions"], np.zeros(8, np.int32), f["cell"]).energy
+            h = u + eng.pressure[r] * abs(np.linalg.det(f["cell"]))
+            assert h == pytest.approx(f["ns_energy"], rel=1e-5, abs=1e-5)

@@ -42,3 +42,58 @@ def test_energy_log_fresh_run_overwrites(tmp_path):
     p.write_text("stale\n1 2 3\n")
     EnergyLogger(p, n_walkers=4).close()
     assert p.read_text() == "4 1 0 0.0 0\n"
+
+
+GOLD = __import__("pathlib").Path(__file__).parent / "golden"
+
+
+def test_extxyz_matches_the_reference_run_output_byte_for_byte():
+    """The four dead frames of the reference's LJ-8 NPT run written again
+    through ours; that run kept absolute Cartesians (``wrap_atoms: false`` in
+    its config snapshot), so the writer runs with ``wrap=False`` here."""
+    from jaxrens_amd.trajectory import ExtxyzTrajectoryWriter, read_extxyz
+    src = GOLD / "lj8_npt.traj.extxyz"
+    frames = read_extxyz(src)
+    assert [f["iter"] for f in frames] == [0, 50, 100, 150]
+    out = src.parent.parent / "__traj_roundtrip.extxyz"
+    try:
+        w = ExtxyzTrajectoryWriter(out, {0: "Ar"}, wrap=False)
+        for f in frames:
+            w.write_dead_point(f["iter"], f["positions"], [0] * len(f["positions"]),
+                               f["cell"], f["ns_energy"])
+        w.close()
+        assert out.read_text() == src.read_text()
+    finally:
+        out.unlink(missing_ok=True)
+
+
+def test_wrap_positions_convention():
+    from jaxrens_amd.trajectory import WRAP_EPS, wrap_positions
+    rng = np.random.default_rng(0)
+    cell = np.array([[3.0, 0.2, -0.1], [0.4, 2.5, 0.3], [-0.2, 0.1, 2.8]])
+    x = rng.normal(size=(50, 3)) * 10
+    w = wrap_positions(x, cell)
+    f = np.linalg.solve(cell.T, w.T).T
+    assert np.all(f >= WRAP_EPS - 1e-12) and np.all(f < 1 + WRAP_EPS + 1e-12)
+    shift = rng.integers(-4, 5, (50, 3)) @ cell
+    np.testing.assert_allclose(wrap_positions(x + shift, cell), w, atol=1e-11)
+    d = np.linalg.solve(cell.T, (w - x).T).T  # whole lattice vectors moved
+    np.testing.assert_allclose(d, np.round(d), atol=1e-9)
+    np.testing.assert_array_equal(wrap_positions(x, np.zeros((3, 3))), x)
+
+
+def test_extxyz_resume_drops_frames_past_the_checkpoint(tmp_path):
+    from jaxrens_amd.trajectory import ExtxyzTrajectoryWriter, read_extxyz
+    p = tmp_path / "t.traj.extxyz"
+    cell = np.eye(3) * 4.0
+    w = ExtxyzTrajectoryWriter(p, {0: "Ar", 1: "Kr"})
+    for i in range(6):
+        w.write_dead_point(i, np.full((2, 3), 0.5 + i * 0.1), [0, 1], cell, -float(i))
+    w.close()
+    w = ExtxyzTrajectoryWriter(p, {0: "Ar", 1: "Kr"}, restart_iteration=4)
+    w.write_dead_point(4, np.full((2, 3), 1.5), [1, 1], cell, 40.0)
+    w.close()
+    fr = read_extxyz(p)
+    assert [f["iter"] for f in fr] == [0, 1, 2, 3, 4]
+    assert fr[4]["ns_energy"] == 40.0 and fr[4]["symbols"] == ["Kr", "Kr"]
+    assert fr[1]["symbols"] == ["Ar", "Kr"] and fr[3]["ns_energy"] == -3.0
