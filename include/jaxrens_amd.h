@@ -41,13 +41,14 @@ extern "C" {
 #define JR_E_UNSUPPORTED (-4)
 
 /* move types (cli/schema/moves.py: gmc|galilean, volume, hmc,
- * alchemical_morph, shear, stretch) */
+ * alchemical_morph, shear, stretch, species_swap) */
 #define JR_MOVE_GMC 0
 #define JR_MOVE_VOLUME 1
 #define JR_MOVE_HMC 2
 #define JR_MOVE_MORPH 3
 #define JR_MOVE_SHEAR 4   /* sampling/moves/shear.py:19-120 */
 #define JR_MOVE_STRETCH 5 /* sampling/moves/stretch.py:18-97 */
+#define JR_MOVE_SWAP 6    /* sampling/moves/swap.py:119-262 (species_swap) */
 
 #define JR_MAX_MOVES 8
 #define JR_MAX_SPECIES 8
@@ -83,7 +84,8 @@ typedef struct {
   int32_t n_leapfrog;  /* hmc */
   int32_t flat_v_prior;/* volume */
   int32_t n_species;   /* alchemical_morph */
-  int32_t pad_;
+  int32_t species_mask;/* species_swap: bit c set = type code c may be
+                          exchanged; 0 = every species */
   double perturb_angle;/* gmc */
   double max_vpa;      /* volume, shear, stretch: cell-geometry guard */
   double min_vpa;
@@ -181,7 +183,8 @@ typedef struct {
  * One sub-step reads, by move type: gmc and hmc the normals [3][Nmax];
  * volume normal [0][0] and uniform 0; morph uniforms 0 (atom) and 1
  * (species); shear uniform 0 (cell vector floor(3u)) and normals [0][0],
- * [1][0]; stretch uniform 0 (axis pair floor(3u)) and normal [0][0]. */
+ * [1][0]; stretch uniform 0 (axis pair floor(3u)) and normal [0][0];
+ * species_swap uniforms 0 (species pair), 1 and 2 (atom of each species). */
 typedef struct {
   const double* sel_u;
   const double* clone_u;

@@ -21,6 +21,7 @@ JR_MOVE_HMC = 2
 JR_MOVE_MORPH = 3
 JR_MOVE_SHEAR = 4
 JR_MOVE_STRETCH = 5
+JR_MOVE_SWAP = 6
 JR_MAX_IMAGES = 4
 JR_MAX_MOVES = 8
 JR_MAX_SPECIES = 8
@@ -52,7 +53,7 @@ class JrMove(C.Structure):
         ("n_leapfrog", C.c_int32),
         ("flat_v_prior", C.c_int32),
         ("n_species", C.c_int32),
-        ("pad_", C.c_int32),
+        ("species_mask", C.c_int32),
         ("perturb_angle", C.c_double),
         ("max_vpa", C.c_double),
         ("min_vpa", C.c_double),

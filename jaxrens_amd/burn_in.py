@@ -16,6 +16,7 @@ from __future__ import annotations
 import numpy as np
 
 from .adaptation import AdaptationConfig, adjust_step_size
+from .moves import STEPLESS
 
 
 def burn_in_emax(engine, emax_offset_per_atom: float) -> np.ndarray:
@@ -47,7 +48,7 @@ def initial_walk(engine, *, n_walks: int, walklength: int, adjust_interval: int,
     adapt = adaptation is not None and adjust_interval > 0
     rng = rng if rng is not None else np.random.default_rng(0)
     adaptable = [m for m, spec in enumerate(engine.mwg.moves)
-                 if spec.type != "alchemical_morph"]
+                 if spec.type not in STEPLESS]
     for i in range(n_walks):
         if adapt and i % adjust_interval == 0:
             ss = engine.step_sizes.copy()

@@ -151,8 +151,10 @@ __global__ void __launch_bounds__(256)
       const double* u = a.uniforms + ((size_t)tq * n_steps + t) * 4;
       int reason = 0, n_eval = 1, n_grad = 0;
       int type = mv.type;
-      if constexpr ((CFG & 8) != 0)
+      if constexpr ((CFG & 8) != 0) {
         if (type == JR_MOVE_SHEAR || type == JR_MOVE_STRETCH) type = JR_MOVE_VOLUME;
+        if (type == JR_MOVE_SWAP) type = JR_MOVE_MORPH;
+      }
       switch (type) {
         case JR_MOVE_GMC:
           reason = gmc_move<A, CFG>(C, W, step, mv, xi) ? 0 : 1;
@@ -169,8 +171,11 @@ __global__ void __launch_bounds__(256)
           n_eval = 2 * mv.n_leapfrog + 1;
           n_grad = 2 * mv.n_leapfrog;
           break;
-        case JR_MOVE_MORPH:
-          reason = morph_move<A, CFG>(C, W, mv, u[0], u[1]) ? 0 : 1;
+        case JR_MOVE_MORPH:  // with cfg bit 3: also species_swap
+          if constexpr ((CFG & 8) != 0)
+            reason = species_move<A, CFG>(C, W, mv, u[0], u[1], u[2]) ? 0 : 1;
+          else
+            reason = morph_move<A, CFG>(C, W, mv, u[0], u[1]) ? 0 : 1;
           break;
 
         default:

@@ -61,7 +61,8 @@ void launch_walk_c(int A, int occ, const WalkArgs& a, int blocks, int wpb,
 }
 
 // cfg bit 0: species table (S > 1); bit 1: FP32 pair terms; bit 2: image
-// sums; bit 3: shear / stretch in the move table (bits 2 and 3 FP64 only)
+// sums; bit 3: shear, stretch or species_swap in the move table (bits 2 and 3
+// FP64 only)
 void launch_walk(int A, int cfg, int occ, const WalkArgs& a, int blocks,
                  int wpb, size_t lds, hipStream_t s) {
   switch (cfg) {
@@ -631,15 +632,24 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   double wsum = 0.0;
   for (int m = 0; m < c.n_moves; ++m) {
     const jr_move& mv = c.moves[m];
-    if (mv.type < 0 || mv.type > JR_MOVE_STRETCH)
+    if (mv.type < 0 || mv.type > JR_MOVE_SWAP)
       return set_global_err(JR_E_ARG, "move %d: unknown type %d", m, mv.type);
     if (mv.type == JR_MOVE_MORPH && (mv.n_species < 2 || mv.n_species > c.pot.n_species))
       return set_global_err(JR_E_ARG, "alchemical_morph needs 2 <= n_species <= S");
     if (mv.type == JR_MOVE_GMC && mv.n_reflect < 1)
       return set_global_err(JR_E_ARG, "gmc needs n_reflect >= 1");
-    if ((mv.type == JR_MOVE_SHEAR || mv.type == JR_MOVE_STRETCH) && !c.mpnn &&
-        c.pot.precision != JR_PREC_FP64)
-      return set_global_err(JR_E_UNSUPPORTED, "shear and stretch are built for FP64 pairs only");
+    if ((mv.type == JR_MOVE_SHEAR || mv.type == JR_MOVE_STRETCH || mv.type == JR_MOVE_SWAP) &&
+        !c.mpnn && c.pot.precision != JR_PREC_FP64)
+      return set_global_err(JR_E_UNSUPPORTED,
+                            "shear, stretch and species_swap are built for FP64 pairs only");
+    if (mv.type == JR_MOVE_SWAP) {
+      const int S = c.pot.n_species;
+      const int full = (1 << S) - 1;
+      const int scope = mv.species_mask ? mv.species_mask : full;
+      if (S < 2 || (scope & ~full) || __builtin_popcount(scope) < 2)
+        return set_global_err(JR_E_ARG,
+                              "species_swap needs >= 2 species in scope, codes below S = %d", S);
+    }
     if (c.move_weights[m] < 0.0) return set_global_err(JR_E_ARG, "negative move weight");
     wsum += c.move_weights[m];
   }
@@ -656,7 +666,8 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   h->A = atoms_per_lane(c.n_atoms_max);
   h->cfgbits = pot_cfg(&c.pot);
   for (int m = 0; m < c.n_moves; ++m)
-    if (c.moves[m].type == JR_MOVE_SHEAR || c.moves[m].type == JR_MOVE_STRETCH)
+    if (c.moves[m].type == JR_MOVE_SHEAR || c.moves[m].type == JR_MOVE_STRETCH ||
+        c.moves[m].type == JR_MOVE_SWAP)
       h->cfgbits |= 8;
   const int R = c.n_replicas, K = c.n_walkers, Nm = c.n_atoms_max, S = c.pot.n_species;
   auto fail = [&](int code) {
@@ -714,7 +725,8 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   double acc = 0.0;
   for (int m = 0; m < c.n_moves; ++m) {
     const jr_move& mv = c.moves[m];
-    dm[m] = DevMove{mv.type, mv.n_reflect, mv.n_leapfrog, mv.flat_v_prior, mv.n_species, 0,
+    dm[m] = DevMove{mv.type, mv.n_reflect, mv.n_leapfrog, mv.flat_v_prior, mv.n_species,
+                    mv.type == JR_MOVE_SWAP ? mv.species_mask : 0,
                     mv.perturb_angle, mv.max_vpa, mv.min_vpa, mv.min_aspect};
     acc += c.move_weights[m] / wsum;
     cum[m] = acc;

@@ -360,4 +360,137 @@ __device__ bool morph_move(const WalkCtx& C, WalkerRegs<A>& W,
   return accepted;
 }
 
+// Index of the k-th atom (0-based, atom order) whose type is code; N - 1
+// when there are at most k of them (swap.py:88-108).
+template <int A>
+__device__ __forceinline__ int nth_of_species(const int (&t)[A], int N, int code,
+                                              int k) {
+  const int lane = lane_id();
+  int idx = N - 1, seen = 0;
+  bool found = false;
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    const bool m = (lane + 64 * s < N) && (t[s] == code);
+    const unsigned long long bal = __ballot(m);
+    const int c = __popcll(bal);
+    if (!found && k < seen + c) {
+      const int r = k - seen;
+      const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+      const unsigned long long pick = __ballot(m && pre == r);
+      idx = 64 * s + __ffsll((long long)pick) - 1;
+      found = true;
+    }
+    seen += c;
+  }
+  return idx;
+}
+
+// Kernels with cfg bit 3: alchemical morph (alchemical.py:40-79) and species
+// swap (swap.py:119-262) with one energy evaluation between them.  Morph as
+// morph_move.  Swap: species pairs (a < b, both in scope, ascending codes)
+// weighted by n_a n_b; the first pair whose cumulative weight exceeds
+// u0 * total, then atom floor(u1 n_a) of species a and floor(u2 n_b) of
+// species b exchange types; accepted iff an unlike pair exists and H < Emax.
+template <int A, int CFG>
+__device__ bool species_move(const WalkCtx& C, WalkerRegs<A>& W,
+                             const DevMove& mv, double u0, double u1,
+                             double u2) {
+  const int lane = lane_id();
+  const int N = C.N;
+  const bool swap = mv.type == JR_MOVE_SWAP;
+  int ia, ib = -1, ta, tb = 0;
+  bool has_pair = true;
+  if (swap) {
+    const int S = C.S;
+    const int scope = mv.species_mask ? mv.species_mask : ((1 << S) - 1);
+    double total = 0.0;
+    for (int a = 0; a < S; ++a)
+      for (int b = a + 1; b < S; ++b)
+        if (((scope >> a) & 1) && ((scope >> b) & 1))
+          total += (double)W.counts[a] * (double)W.counts[b];
+    has_pair = total > 0.0;
+    const double target = u0 * (has_pair ? total : 1.0);
+    int lo = -1, hi = -1, last_lo = 0, last_hi = 1;
+    double cum = 0.0;
+    for (int a = 0; a < S; ++a)
+      for (int b = a + 1; b < S; ++b)
+        if (((scope >> a) & 1) && ((scope >> b) & 1)) {
+          cum += (double)W.counts[a] * (double)W.counts[b];
+          last_lo = a;
+          last_hi = b;
+          if (lo < 0 && target < cum) {
+            lo = a;
+            hi = b;
+          }
+        }
+    if (lo < 0) {
+      lo = last_lo;
+      hi = last_hi;
+    }
+    const int nlo = max(W.counts[lo], 1), nhi = max(W.counts[hi], 1);
+    int klo = (int)(u1 * (double)nlo);
+    if (klo > nlo - 1) klo = nlo - 1;
+    int khi = (int)(u2 * (double)nhi);
+    if (khi > nhi - 1) khi = nhi - 1;
+    ia = nth_of_species<A>(W.t, N, lo, klo);
+    ib = nth_of_species<A>(W.t, N, hi, khi);
+    ta = hi;
+    tb = lo;
+  } else {
+    const int Sm = mv.n_species;
+    ia = (int)(u0 * (double)N);
+    if (ia > N - 1) ia = N - 1;
+    int cand = (int)(u1 * (double)(Sm - 1));
+    if (cand > Sm - 2) cand = Sm - 2;
+    int curv = 0;
+#pragma unroll
+    for (int s = 0; s < A; ++s)
+      if (s == (ia >> 6)) curv = W.t[s];
+    const int cur = __builtin_amdgcn_readlane(curv, ia & 63);
+    ta = cand + (cand >= cur ? 1 : 0);
+  }
+  if (!has_pair) return false;
+  int old_a = 0, old_b = 0;
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    const int a = lane + 64 * s;
+    if (a == ia) {
+      old_a = W.t[s];
+      W.t[s] = ta;
+    }
+    if (a == ib) {
+      old_b = W.t[s];
+      W.t[s] = tb;
+    }
+  }
+  const int ca = __builtin_amdgcn_readlane(old_a, ia & 63);
+  if (!swap) {
+#pragma unroll
+    for (int sp = 0; sp < JR_MAX_SPECIES; ++sp)
+      W.counts[sp] += (sp == ta ? 1 : 0) - (sp == ca ? 1 : 0);
+  }
+  Geom g;
+  make_geom(W.cell, g);
+  double fx[A], fy[A], fz[A];
+  double U = wave_eval<A, false, CFG>(C.L, N, g, C.tab, W.x, W.y, W.z, W.t, fx, fy, fz);
+  double Hn = U + ensemble_term(W.cell, C.pressure, C.mu, C.S, W.counts);
+  const bool accepted = Hn < C.emax;
+  if (accepted) {
+    W.E = Hn;
+  } else {
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      const int a = lane + 64 * s;
+      if (a == ia) W.t[s] = old_a;
+      if (a == ib) W.t[s] = old_b;
+    }
+    if (!swap) {
+#pragma unroll
+      for (int sp = 0; sp < JR_MAX_SPECIES; ++sp)
+        W.counts[sp] += (sp == ca ? 1 : 0) - (sp == ta ? 1 : 0);
+    }
+  }
+  return accepted;
+}
+
 }  // namespace jr

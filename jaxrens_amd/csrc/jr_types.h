@@ -6,7 +6,7 @@
 namespace jr {
 
 struct DevMove {
-  int type, n_reflect, n_leapfrog, flat_v_prior, n_species, pad_;
+  int type, n_reflect, n_leapfrog, flat_v_prior, n_species, species_mask;
   double perturb_angle, max_vpa, min_vpa, min_aspect;
 };
 

@@ -4,7 +4,8 @@ Field names and defaults follow ``cli/schema/moves.py`` (``BaseMoveSpec``
 :60-120, ``GMCMoveSpec`` :249-406 with the legacy ``galilean`` alias of
 ``cli/schema/root.py:29-40``, ``HMCMoveSpec`` :409-447, ``VolumeMoveSpec``
 :628-682, ``ShearMoveSpec`` :685-736, ``StretchMoveSpec`` :739-788,
-``AlchemicalMorphMoveSpec`` :791-828) and ``cli/schema/cell.py`` for the
+``SpeciesSwapMoveSpec`` :514-625, ``AlchemicalMorphMoveSpec`` :791-828) and
+``cli/schema/cell.py`` for the
 cell-geometry guard of the cell moves.  ``build_mwg`` mirrors
 ``sampling/mwg.py:40-259``: weights are normalised to probabilities and each
 move carries its own step size.
@@ -127,14 +128,38 @@ class AlchemicalMorphMoveSpec(BaseMoveSpec):
         return _lib.JrMove(type=_lib.JR_MOVE_MORPH, n_species=self.n_species)
 
 
+class SpeciesSwapMoveSpec(BaseMoveSpec):
+    """Exchange the types of two atoms of different species
+    (``sampling/moves/swap.py``).  ``species`` restricts the exchange to the
+    listed type codes (the reference names elements; here the integer codes
+    of the pair table).  ``step_size`` is not used."""
+
+    type: Literal["species_swap"] = "species_swap"
+    species: tuple[int, ...] | None = None
+
+    def device_move(self, cell: CellSpec) -> _lib.JrMove:
+        mask = 0
+        if self.species is not None:
+            codes = set(int(c) for c in self.species)
+            if len(codes) < 2:
+                raise ValueError("species_swap needs at least two distinct species")
+            if min(codes) < 0 or max(codes) >= _lib.JR_MAX_SPECIES:
+                raise ValueError(f"species codes must be in [0, {_lib.JR_MAX_SPECIES})")
+            mask = sum(1 << c for c in codes)
+        return _lib.JrMove(type=_lib.JR_MOVE_SWAP, species_mask=mask)
+
+
 MoveSpec = Annotated[
     Union[GMCMoveSpec, HMCMoveSpec, VolumeMoveSpec, ShearMoveSpec,
-          StretchMoveSpec, AlchemicalMorphMoveSpec],
+          StretchMoveSpec, SpeciesSwapMoveSpec, AlchemicalMorphMoveSpec],
     Field(discriminator="type"),
 ]
 _MOVE_ADAPTER = TypeAdapter(MoveSpec)
 
-_UNBUILT = {"random_walk", "single_atom", "single_atom_sweep", "species_swap"}
+_UNBUILT = {"random_walk", "single_atom", "single_atom_sweep"}
+
+# moves without a continuous step (adaptation leaves them alone)
+STEPLESS = {"alchemical_morph", "species_swap"}
 
 
 def parse_move(spec) -> BaseMoveSpec:

@@ -18,6 +18,7 @@ from pathlib import Path
 import numpy as np
 
 from .adaptation import AdaptationConfig, adjust_step_size
+from .moves import STEPLESS
 from .termination import PriorMassTermination, check_any
 
 
@@ -134,7 +135,7 @@ def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
                                                 wrap=traj_wrap,
                                                 restart_iteration=engine.iteration))
     adaptable = [m for m, spec in enumerate(engine.mwg.moves)
-                 if spec.type != "alchemical_morph"]
+                 if spec.type not in STEPLESS]
     emax_hist, vol_hist, adapt_log = [], [], []
     counters = np.zeros((R, engine.n_moves, 7), np.int64)
     re_acc = np.zeros(max(R - 1, 1), np.int64)

@@ -700,6 +700,68 @@ static int morph_move(const chain_ctx* c, const double* x, int32_t* types,
   return accepted;
 }
 
+/* sampling/moves/swap.py:119-262 (species_swap).  Species pairs (a < b, both
+ * in scope, ascending codes) weighted by n_a n_b: the first pair whose
+ * cumulative weight exceeds u0 * total (swap.py:210-218); atom floor(u1 n_a)
+ * of species a and floor(u2 n_b) of species b (the k-th atoms of their
+ * species in atom order) exchange types; accepted iff an unlike pair exists
+ * and H' < Emax. */
+static int nth_of_species(const int32_t* types, int N, int code, int k) {
+  int seen = 0;
+  for (int a = 0; a < N; ++a)
+    if (types[a] == code && seen++ == k) return a;
+  return N - 1;
+}
+
+static int swap_move(const chain_ctx* c, const double* x, int32_t* types,
+                     const double* cell, double* E, const or_move* mv,
+                     const double* u) {
+  int N = c->N, S = c->S;
+  int scope = mv->species_mask ? mv->species_mask : (1 << S) - 1;
+  int counts[16] = {0};
+  for (int a = 0; a < N; ++a) counts[types[a]] += 1;
+  double total = 0.0;
+  for (int a = 0; a < S; ++a)
+    for (int b = a + 1; b < S; ++b)
+      if (((scope >> a) & 1) && ((scope >> b) & 1))
+        total += (double)counts[a] * (double)counts[b];
+  if (!(total > 0.0)) return 0;
+  double target = u[0] * total, cum = 0.0;
+  int lo = -1, hi = -1, last_lo = 0, last_hi = 1;
+  for (int a = 0; a < S; ++a)
+    for (int b = a + 1; b < S; ++b)
+      if (((scope >> a) & 1) && ((scope >> b) & 1)) {
+        cum += (double)counts[a] * (double)counts[b];
+        last_lo = a;
+        last_hi = b;
+        if (lo < 0 && target < cum) {
+          lo = a;
+          hi = b;
+        }
+      }
+  if (lo < 0) {
+    lo = last_lo;
+    hi = last_hi;
+  }
+  int nlo = counts[lo] > 1 ? counts[lo] : 1, nhi = counts[hi] > 1 ? counts[hi] : 1;
+  int klo = (int)(u[1] * (double)nlo);
+  if (klo > nlo - 1) klo = nlo - 1;
+  int khi = (int)(u[2] * (double)nhi);
+  if (khi > nhi - 1) khi = nhi - 1;
+  int ia = nth_of_species(types, N, lo, klo), ib = nth_of_species(types, N, hi, khi);
+  types[ia] = hi;
+  types[ib] = lo;
+  double U, Hn;
+  or_energy_forces(N, x, types, cell, c->pot, c->pressure, c->mu, &U, &Hn, NULL);
+  if (Hn < c->emax) {
+    *E = Hn;
+    return 1;
+  }
+  types[ia] = lo;
+  types[ib] = hi;
+  return 0;
+}
+
 static void count(int32_t* ctr, int reason, int n_eval, int n_grad) {
   ctr[0] += 1;
   ctr[1 + reason] += 1;
@@ -743,6 +805,11 @@ void or_walk_chain(int N, int Nmax, double* pos, int32_t* types, double* cell,
       }
       case OR_MOVE_MORPH: {
         int acc = morph_move(&c, pos, types, cell, energy, mv, u);
+        count(ctr, acc ? 0 : 1, 1, 0);
+        break;
+      }
+      case OR_MOVE_SWAP: {
+        int acc = swap_move(&c, pos, types, cell, energy, mv, u);
         count(ctr, acc ? 0 : 1, 1, 0);
         break;
       }

@@ -20,7 +20,7 @@ extern "C" {
 #endif
 
 enum { OR_MOVE_GMC = 0, OR_MOVE_VOLUME = 1, OR_MOVE_HMC = 2, OR_MOVE_MORPH = 3,
-       OR_MOVE_SHEAR = 4, OR_MOVE_STRETCH = 5 };
+       OR_MOVE_SHEAR = 4, OR_MOVE_STRETCH = 5, OR_MOVE_SWAP = 6 };
 
 /* counters per (replica, move): proposed, rr0 (=accepted), rr1 energy,
  * rr2 cell, rr3 prior, n_evaluations, n_grad_evaluations */
@@ -42,7 +42,7 @@ typedef struct {
   int n_leapfrog;
   int flat_v_prior;
   int n_species;
-  int pad_;
+  int species_mask; /* species_swap scope, 0 = all */
   double perturb_angle;
   double max_vpa, min_vpa, min_aspect;
 } or_move;

@@ -26,7 +26,7 @@ class OrPot(C.Structure):
 
 class OrMove(C.Structure):
     _fields_ = [("type", C.c_int), ("n_reflect", C.c_int), ("n_leapfrog", C.c_int),
-                ("flat_v_prior", C.c_int), ("n_species", C.c_int), ("pad_", C.c_int),
+                ("flat_v_prior", C.c_int), ("n_species", C.c_int), ("species_mask", C.c_int),
                 ("perturb_angle", C.c_double), ("max_vpa", C.c_double),
                 ("min_vpa", C.c_double), ("min_aspect", C.c_double)]
 
@@ -162,8 +162,8 @@ def moves_array(jr_moves):
     arr = (OrMove * len(jr_moves))()
     for i, m in enumerate(jr_moves):
         arr[i] = OrMove(m.type, m.n_reflect, m.n_leapfrog, m.flat_v_prior,
-                        m.n_species, 0, m.perturb_angle, m.max_vpa, m.min_vpa,
-                        m.min_aspect)
+                        m.n_species, m.species_mask, m.perturb_angle, m.max_vpa,
+                        m.min_vpa, m.min_aspect)
     return arr
 
 

@@ -145,3 +145,41 @@ def test_trial_moves_shear_stretch(oracle):
                                        idx, nrm, uni)
         assert np.array_equal(acc, oacc) and np.array_equal(rs, ors), m
         assert acc.any()
+
+
+def test_ns_iterations_species_swap_and_morph(oracle):
+    """species_swap next to alchemical_morph in a binary pair-table system
+    with mixed sizes (N = 32 and 96, two atoms per lane), bit-exact."""
+    print("ns species swap + morph", flush=True)
+    from oracle.oracle import Population
+    eps = np.array([[1.0, 1.5], [1.5, 0.5]])
+    sig = np.array([[1.0, 0.8], [0.8, 0.88]])
+    lj = LJBackend(pair_epsilon=eps, pair_sigma=sig, cutoff=2.5)
+    moves = [{"type": "gmc", "n_reflect": 4, "step_size": 0.1, "weight": 3},
+             {"type": "volume", "step_size": 0.3, "weight": 1},
+             {"type": "species_swap", "weight": 2},
+             {"type": "alchemical_morph", "n_species": 2, "weight": 1}]
+    R, K = 4, 6
+    mu = np.stack([np.zeros(R), np.linspace(-1, 1, R)], axis=1)
+    run = RunConfig(n_replicas=R, n_walkers=K, n_atoms=[32, 32, 96, 96], n_mcmc_steps=8,
+                    n_extra=3, seed=9, pressures=[1.0] * R, chemical_potentials=mu,
+                    re_interval=0)
+    eng = NSEngine(lj, moves, run, cell={"min_volume_per_atom": 0.5,
+                                         "max_volume_per_atom": 8.0, "flat_V_prior": True})
+    eng.init_walkers(np.random.default_rng(9), 0.9, 3.0, True, species_p=[0.5, 0.5])
+    tape = make_tape(np.random.default_rng(19), 4, R, K, 96, 8, 3, eng.mwg.move_probs)
+    opop = Population.from_engine_dict(eng.get_population())
+    pot = oracle.Pot.from_table(lj.table)
+    total = 0
+    for it in range(4):
+        out = eng.ns_iterations(1, tape=tape.iteration(it))
+        o = oracle.ns_iteration(opop, pot, eng.pressure, eng.mu, eng.step_sizes,
+                                eng.mwg.as_oracle_moves(), 8, 3, tape, it=it, do_re=False)
+        assert np.array_equal(out["emax"][0], o["emax"]), it
+        assert np.array_equal(out["counters"][0], o["counters"]), it
+        total = total + o["counters"]
+        d = eng.get_population()
+        assert np.array_equal(d["types"], opop.types), it
+        assert np.array_equal(d["energy"], opop.energy), it
+        assert np.array_equal(d["positions"], opop.pos), it
+    assert total[:, 2, 1].sum() > 0 and total[:, 3, 1].sum() > 0  # swaps, morphs accepted
