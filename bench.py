@@ -85,21 +85,81 @@ def build_engine(n_extra: int, R=16, K=1024, N=64, n_steps=20, seed=42,
 def launch_flops(counters: np.ndarray, eng) -> float:
     """Algorithmic FLOPs of the walk launches behind ``counters``
     (iterations, R, n_moves, 7): n_reflect E+F evaluations per galilean
-    step, one energy evaluation per volume step, N(N-1)/2 pairs each."""
-    N = int(eng.n_atoms_rep[0])
-    pairs = N * (N - 1) / 2
+    step, one energy evaluation per volume or morph step, N_r(N_r-1)/2 pairs
+    each for replica r."""
+    c = np.asarray(counters).reshape(-1, eng.n_replicas, eng.n_moves, 7)
     total = 0.0
-    for m, spec in enumerate(eng.mwg.moves):
-        steps = counters[..., m, 0].sum()
-        if spec.type == "gmc":
-            total += steps * spec.n_reflect * FLOP_EF * pairs
-        elif spec.type == "volume":
-            total += steps * FLOP_E * pairs
-        elif spec.type == "hmc":
-            total += steps * ((spec.n_leapfrog + 1) * FLOP_EF) * pairs
-        else:
-            total += steps * FLOP_E * pairs
+    for r in range(eng.n_replicas):
+        N = int(eng.n_atoms_rep[r])
+        pairs = N * (N - 1) / 2
+        for m, spec in enumerate(eng.mwg.moves):
+            steps = float(c[:, r, m, 0].sum())
+            if spec.type == "gmc":
+                total += steps * spec.n_reflect * FLOP_EF * pairs
+            elif spec.type == "hmc":
+                total += steps * ((spec.n_leapfrog + 1) * FLOP_EF) * pairs
+            else:
+                total += steps * FLOP_E * pairs
     return float(total)
+
+
+def config_throughput(n_iter_c1: int = 400, n_iter_c3: int = 20) -> dict:
+    """Throughput of the other synthetic configurations, whole iterations
+    replayed from a captured hipGraph: LJ-8 NPT (P = 1, K = 128, n_extra = 0,
+    the reference's defaults) and the binary semi-grand muPT batch (N in
+    {32, 48, 64, 96, 128} over P = 20 replicas, K = 512, whole population
+    walked, morph standing in for the identity swap, autotuned launch).
+    FLOP rates are iteration-level (all kernels of an iteration)."""
+    from jaxrens_amd.backends import LJBackend
+    from jaxrens_amd.engine import NSEngine, RunConfig
+    gv = [{"type": "galilean", "n_reflect": 10, "step_size": 0.1, "weight": 4},
+          {"type": "volume", "step_size": 0.3, "weight": 1}]
+    out = {}
+    # LJ-8 NPT, the reference's CPU-runnable configuration
+    run = RunConfig(n_replicas=1, n_walkers=128, n_atoms=8, n_mcmc_steps=20, n_extra=0,
+                    seed=42, pressures=[0.1], re_interval=0)
+    e1 = NSEngine(LJBackend(1.0, 1.0, 2.5), gv, run,
+                  cell={"min_volume_per_atom": 0.125, "max_volume_per_atom": 64.0,
+                        "flat_V_prior": True})
+    e1.init_walkers(np.random.default_rng(42), 1.0 / 8, 64.0, True)
+    e1.ns_iterations(5, graph=True)
+    t0 = time.perf_counter()
+    o1 = e1.ns_iterations(n_iter_c1, graph=True)
+    d1 = time.perf_counter() - t0
+    f1 = launch_flops(o1["counters"], e1) / d1 / 1e12
+    out["lj8_npt"] = {
+        "walker_steps_per_s": 1 * 1 * 20 * n_iter_c1 / d1,
+        "ns_iterations_per_s": n_iter_c1 / d1, "iterations": n_iter_c1,
+        "iteration_tflops": f1, "frac_of_fp64_peak": f1 / FP64_PEAK_TFLOPS,
+        "note": "one chain of 8 atoms per iteration: latency-bound, 8 of 64 lanes busy"}
+    e1.close()
+    # binary semi-grand muPT, uneven sizes in one persistent launch
+    R, K = 20, 512
+    sizes = [n for n in (32, 48, 64, 96, 128) for _ in range(4)]
+    eps = np.array([[1.0, 1.5], [1.5, 0.5]])
+    sig = np.array([[1.0, 0.8], [0.8, 0.88]])
+    mu = np.stack([np.zeros(R), np.linspace(-2.0, 2.0, R)], axis=1)
+    run = RunConfig(n_replicas=R, n_walkers=K, n_atoms=sizes, n_mcmc_steps=20,
+                    n_extra=K - 2, seed=42, pressures=[1.0] * R, chemical_potentials=mu,
+                    re_interval=1)
+    moves = gv + [{"type": "alchemical_morph", "n_species": 2, "weight": 1}]
+    e3 = NSEngine(LJBackend(pair_epsilon=eps, pair_sigma=sig, cutoff=2.5), moves, run,
+                  cell={"min_volume_per_atom": 0.5, "max_volume_per_atom": 8.0,
+                        "flat_V_prior": True})
+    e3.init_walkers(np.random.default_rng(42), 1.0, 8.0, True, species_p=[0.5, 0.5])
+    tune = e3.autotune(reps=1)
+    e3.ns_iterations(2, graph=True)
+    t0 = time.perf_counter()
+    o3 = e3.ns_iterations(n_iter_c3, graph=True)
+    d3 = time.perf_counter() - t0
+    f3 = launch_flops(o3["counters"], e3) / d3 / 1e12
+    out["binary_semigrand_mixed_n"] = {
+        "walker_steps_per_s": R * (K - 1) * 20 * n_iter_c3 / d3,
+        "ns_iterations_per_s": n_iter_c3 / d3, "iterations": n_iter_c3,
+        "iteration_tflops": f3, "frac_of_fp64_peak": f3 / FP64_PEAK_TFLOPS,
+        "launch": tune}
+    e3.close()
+    return out
 
 
 def cpu_baseline(eng, n_replicas_sample: int, threads: int, n_iter: int = 1) -> dict:
@@ -399,6 +459,7 @@ def main():
             "frac_of_fp64_peak": f32 / FP64_PEAK_TFLOPS, "launch": t32,
             "note": "float32 pair terms, float64 minimum image and accumulation"}
         e32.close()
+        secondary.update(config_throughput())
 
     surrogate = None
     if not args.skip_surrogate:

@@ -26,7 +26,18 @@ struct jr_handle {
   int max_pairs = 0;
   int A = 1;
   int cfgbits = 0;
-  int occ = 0;  // walk-kernel occupancy variant (0 or 3)
+  int occ = 0;      // walk-kernel occupancy variant (0 or 3) of the single launch
+  int occ_req = 0;  // variant asked for (a one-atom-per-lane class may use it)
+  // Replicas of different atoms-per-lane A (N <= 64, <= 128, <= 256) walk in
+  // one launch per class, each with its own kernel instantiation; n_classes
+  // is 0 when every replica shares one A.
+  struct WalkClass {
+    int A = 1, occ = 0, blocks = 1, n_rep = 0;
+    size_t lds = 0;
+  };
+  int n_classes = 0;
+  WalkClass cls[3];
+  jrh::DBuf<int> cls_order;  // [3][R] replica order of each class
   hipStream_t stream = nullptr;
   // population
   jrh::DBuf<double> pos, dir, cell, energy;

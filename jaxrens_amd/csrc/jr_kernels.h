@@ -36,6 +36,7 @@ struct WalkArgs {
   // image half-ranges (CFG bit 2); kept last: the register allocation of the
   // minimum-image walk kernels depends on the offsets of the arguments above
   int img[3];
+  int lds_atoms;  // sizes each wave's LDS region (Nmax, or 64 A of a class launch)
 };
 
 __host__ __device__ inline size_t wave_lds_bytes(int Nmax, int n_moves) {
@@ -55,7 +56,7 @@ __global__ void __launch_bounds__(256)
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
   const int nwave_blk = blockDim.x >> 6;
-  const size_t per_wave = wave_lds_bytes(a.Nmax, a.n_moves);
+  const size_t per_wave = wave_lds_bytes(a.lds_atoms, a.n_moves);
   char* base = reinterpret_cast<char*>(lds_raw) + per_wave * wave;
   double* tabs = reinterpret_cast<double*>(
       reinterpret_cast<char*>(lds_raw) + per_wave * nwave_blk);
@@ -494,7 +495,7 @@ __global__ void __launch_bounds__(1024) select_kernel(SelArgs a) {
   }
   for (int i = tid; i < a.n_moves * JR_NCOUNT; i += nt)
     a.counters[(size_t)r * a.n_moves * JR_NCOUNT + i] = 0;
-  if (r == 0 && tid == 0) *a.queue = 0;
+  if (r == 0 && tid < JR_NQUEUE) a.queue[tid] = 0;
   if (a.o_dead_pos) {
     const size_t o = (size_t)bp * a.R + r;
     for (int i = tid; i < a.Nmax * 3; i += nt)

@@ -394,6 +394,7 @@ WalkArgs walk_args(jr_handle* h) {
   a.sig2 = h->sig2.p;
   set_images(a.img, c.pot);
   a.moves = h->moves.p;
+  a.lds_atoms = c.n_atoms_max;
   return a;
 }
 
@@ -495,6 +496,27 @@ void enqueue_draws(jr_handle* h) {
                        h->re_u.p, (long long)n_re_u(h), seed, st, 0LL, 7u);
 }
 
+// The walk of one iteration: a single persistent launch, or with replicas of
+// several atoms-per-lane classes one launch per class (largest A first), each
+// with the kernel built for its A, its own queue counter and grid.  A chain's
+// results do not depend on A, so the split changes no number.
+void enqueue_walk(jr_handle* h, const WalkArgs& base, hipStream_t s) {
+  if (h->n_classes == 0) {
+    launch_walk(h->A, h->cfgbits, h->occ, base, h->blocks, h->wpb, h->lds_walk, s);
+    return;
+  }
+  const int R = h->cfg.n_replicas;
+  for (int j = 0; j < h->n_classes; ++j) {
+    const auto& k = h->cls[j];
+    WalkArgs wa = base;
+    wa.order = h->cls_order.p + (size_t)j * R;
+    wa.total = k.n_rep * base.n_walk;
+    wa.queue = h->queue.p + 1 + j;
+    wa.lds_atoms = 64 * k.A;
+    launch_walk(k.A, h->cfgbits, k.occ, wa, k.blocks, h->wpb, k.lds, s);
+  }
+}
+
 // select -> walk -> RE -> advance, all on the handle's stream.  The learned
 // model's walk is sequenced from the host (one sync for the move schedule,
 // one for its statistics); the pair-potential walk is one launch.
@@ -508,8 +530,7 @@ int enqueue_iteration_core(jr_handle* h, bool timed, int ev_idx) {
     int rc = jrl::learned_walk(h);
     if (rc) return rc;
   } else {
-    WalkArgs wa = walk_args(h);
-    launch_walk(h->A, h->cfgbits, h->occ, wa, h->blocks, h->wpb, h->lds_walk, h->stream);
+    enqueue_walk(h, walk_args(h), h->stream);
   }
   if (timed) (void)hipEventRecord(h->ev[2 * ev_idx + 1], h->stream);
   if (c.n_replicas > 1) {
@@ -542,9 +563,21 @@ namespace {
 // blocks as are resident (never more than the chains of one iteration).
 int set_geometry(jr_handle* h, int wpb, int occ) {
   const auto& c = h->cfg;
+  h->occ_req = occ;
   if (h->A != 1 || (h->cfgbits & 12)) occ = 0;
   h->wpb = wpb;
   h->occ = occ;
+  for (int j = 0; j < h->n_classes; ++j) {
+    auto& k = h->cls[j];
+    k.occ = (k.A == 1 && !(h->cfgbits & 12)) ? h->occ_req : 0;
+    k.lds = wave_lds_bytes_a(k.A) * wpb + 3 * sizeof(double) * c.pot.n_species * c.pot.n_species;
+    int pc = 0;
+    HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &pc, walk_fn(k.A, h->cfgbits, k.occ), 64 * wpb, k.lds));
+    if (pc < 1) pc = 1;
+    const int need = (k.n_rep * h->n_walk + wpb - 1) / wpb;
+    k.blocks = std::max(1, std::min(need, pc * h->n_cu));
+  }
   h->lds_walk = wave_lds_bytes(c.n_atoms_max, c.n_moves) * wpb +
                 3 * sizeof(double) * c.pot.n_species * c.pot.n_species;
   int per_cu = 0;
@@ -703,7 +736,8 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   TRY(h->cum.alloc(c.n_moves));
   TRY(h->walk.alloc((size_t)R * h->n_walk));
   TRY(h->counters.alloc((size_t)R * c.n_moves * JR_NCOUNT));
-  TRY(h->queue.alloc(1));
+  TRY(h->queue.alloc(JR_NQUEUE));
+  TRY(h->cls_order.alloc((size_t)3 * R));
   TRY(h->pair_valid.alloc(std::max(R - 1, 1)));
   TRY(h->order.alloc(R));
   TRY(h->sched.alloc(c.n_mcmc_steps));
@@ -815,8 +849,32 @@ int jr_set_population(jr_handle* h, const double* pos, const int32_t* types,
   for (int r = 0; r < R; ++r) ord[r] = byn[r].second;
   HIPCHK(h, hipMemcpyAsync(h->pair_valid.p, pv.data(), sizeof(int) * pv.size(), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->order.p, ord.data(), sizeof(int) * R, hipMemcpyHostToDevice, h->stream));
+  // size classes by atoms per lane, each in the largest-N-first order
+  h->n_classes = 0;
+  if (!h->learned) {
+    std::vector<int> corder((size_t)3 * R, 0);
+    int used = 0;
+    for (int A : {4, 2, 1}) {
+      int n = 0;
+      for (int r : ord)
+        if (atoms_per_lane(n_atoms[(size_t)r * K]) == A) corder[(size_t)used * R + n++] = r;
+      if (n > 0) {
+        h->cls[used].A = A;
+        h->cls[used].n_rep = n;
+        ++used;
+      }
+    }
+    if (used > 1) {
+      h->n_classes = used;
+      HIPCHK(h, hipMemcpyAsync(h->cls_order.p, corder.data(), sizeof(int) * 3 * R,
+                               hipMemcpyHostToDevice, h->stream));
+    }
+  }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->have_pop = true;
+  // launch shapes follow the size classes; a captured graph is dropped
+  int g = set_geometry(h, h->wpb, h->occ_req);
+  if (g) return g;
   return recompute_energies(h);
 }
 
@@ -1323,9 +1381,11 @@ int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
   HIPCHK(h, hipEventCreate(&e1));
   const int wpbs[3] = {4, 2, 1};
   const int occs[2] = {0, 3};
-  int bw = h->wpb, bo = h->occ;
+  int bw = h->wpb, bo = h->occ_req;
   double bms = 1e300;
-  for (int oi = 0; oi < (h->A == 1 ? 2 : 1); ++oi) {
+  bool one_per_lane = h->A == 1;
+  for (int j = 0; j < h->n_classes; ++j) one_per_lane = one_per_lane || h->cls[j].A == 1;
+  for (int oi = 0; oi < (one_per_lane ? 2 : 1); ++oi) {
     for (int wi = 0; wi < 3; ++wi) {
       rc = set_geometry(h, wpbs[wi], occs[oi]);
       if (rc) return rc;
@@ -1333,10 +1393,9 @@ int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
       for (int r = 0; r <= reps; ++r) {  // r == 0: warm-up launch
         HIPCHK(h, copy_pop(h->pos.p, h->dir.p, h->cell.p, h->energy.p, h->types.p,
                            h->n_atoms.p, t_pos.p, t_dir.p, t_cell.p, t_e.p, t_t.p, t_n.p));
-        HIPCHK(h, hipMemsetAsync(h->queue.p, 0, sizeof(int), s));
-        WalkArgs wa = walk_args(h);
+        HIPCHK(h, hipMemsetAsync(h->queue.p, 0, sizeof(int) * JR_NQUEUE, s));
         HIPCHK(h, hipEventRecord(e0, s));
-        launch_walk(h->A, h->cfgbits, h->occ, wa, h->blocks, h->wpb, h->lds_walk, s);
+        enqueue_walk(h, walk_args(h), s);
         HIPCHK(h, hipEventRecord(e1, s));
         HIPCHK(h, hipEventSynchronize(e1));
         float ms = 0.f;
@@ -1347,7 +1406,7 @@ int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
       if (avg < bms) {
         bms = avg;
         bw = h->wpb;
-        bo = h->occ;
+        bo = h->occ_req;
       }
     }
   }

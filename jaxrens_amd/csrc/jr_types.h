@@ -5,6 +5,9 @@
 
 namespace jr {
 
+// work-queue counters: [0] single-launch walks, [1 + j] walk of size class j
+constexpr int JR_NQUEUE = 4;
+
 struct DevMove {
   int type, n_reflect, n_leapfrog, flat_v_prior, n_species, species_mask;
   double perturb_angle, max_vpa, min_vpa, min_aspect;
