@@ -103,6 +103,29 @@ def launch_flops(counters: np.ndarray, eng) -> float:
     return float(total)
 
 
+def iteration_distribution(eng, n_iter: int) -> dict:
+    """Mean, p50 and p95 over ``n_iter`` single NS iterations, each its own
+    call ending in a host synchronisation (wall clock of the whole
+    iteration), and of the walk launch inside it (HIP events).  Run after
+    the timed region, so ``value`` does not include the per-call syncs."""
+    it_ms, walk_ms = [], []
+    eng.walk_timing(reset=True)
+    for _ in range(n_iter):
+        t0 = time.perf_counter()
+        eng.ns_iterations(1)
+        it_ms.append((time.perf_counter() - t0) * 1e3)
+        ms, n = eng.walk_timing(reset=True)
+        walk_ms.append(ms / max(n, 1))
+
+    def stats(v):
+        v = np.asarray(v)
+        return {"mean": float(v.mean()), "p50": float(np.percentile(v, 50)),
+                "p95": float(np.percentile(v, 95))}
+    return {"iterations": n_iter, "iteration_ms": stats(it_ms),
+            "walk_launch_ms": stats(walk_ms),
+            "note": "one ns_iterations(1) call per iteration, measured after the timed region"}
+
+
 def config_throughput(n_iter_c1: int = 400, n_iter_c3: int = 20) -> dict:
     """Throughput of the other synthetic configurations, whole iterations
     replayed from a captured hipGraph: LJ-8 NPT (P = 1, K = 128, n_extra = 0,
@@ -428,6 +451,7 @@ def main():
     achieved = flops / launches / (avg_launch_ms * 1e-3) / 1e12
     print(f"[bench] timed {args.steps} iterations in {dt:.3f} s; walk kernel "
           f"avg {avg_launch_ms:.3f} ms", file=sys.stderr, flush=True)
+    dist = iteration_distribution(eng, args.steps)
 
     secondary = None
     if not args.skip_secondary:
@@ -495,6 +519,7 @@ def main():
             "walker_steps_per_iteration": ws_per_iter,
             "ns_iterations_per_s": args.steps / dt,
             "launch": {**eng.launch_info(), **(tune or {})},
+            "iteration_distribution": dist,
         },
         "roofline": {
             "bound": "mfma",

@@ -154,8 +154,11 @@ def test_ns_iterations_hmc_gmc_n128(oracle):
     _run_parity(oracle, eng, 3, tape)
 
 
-def test_ns_iterations_semigrand_mixed_n(oracle):
-    print("ns semi-grand mixed N", flush=True)
+@pytest.mark.parametrize("occ", [0, 3])
+def test_ns_iterations_semigrand_mixed_n(oracle, occ):
+    """Two size classes (N <= 64 and 65..128), each in its own launch, with
+    the N <= 64 class on its default and its three-waves-per-SIMD build."""
+    print(f"ns semi-grand mixed N occupancy={occ}", flush=True)
     eps = np.array([[1.0, 1.5], [1.5, 0.5]])
     sig = np.array([[1.0, 0.8], [0.8, 0.88]])
     lj = LJBackend(pair_epsilon=eps, pair_sigma=sig, cutoff=2.5)
@@ -170,6 +173,7 @@ def test_ns_iterations_semigrand_mixed_n(oracle):
                     re_interval=1)
     eng = NSEngine(lj, moves, run, cell=CELL)
     eng.init_walkers(np.random.default_rng(5), 0.5, 8.0, True, species_p=[0.5, 0.5])
+    eng.set_launch(2, occ)
     tape = make_tape(np.random.default_rng(12), 4, R, 6, 96, 6, 3, eng.mwg.move_probs)
     pv = np.array([n_atoms[i] == n_atoms[i + 1] for i in range(R - 1)], np.int32)
     _run_parity(oracle, eng, 4, tape, pair_valid=pv)
