@@ -154,8 +154,9 @@ typedef struct {
   int32_t n_atoms_max;   /* Nmax, <= 256 */
   int32_t n_moves;
   int32_t n_mcmc_steps;
-  int32_t n_extra;       /* 0 .. K-2 */
-  int32_t n_cull;        /* must be 1 */
+  int32_t n_extra;       /* n_cull = 1: 0 .. K-2; n_cull > 1: 2*n_cull + n_extra <= K */
+  int32_t n_cull;        /* walkers culled per iteration, >= 1 (nested_sampling.py:151-212);
+                            the learned walk takes n_cull = 1 */
   int32_t n_swap_cycles;
   int32_t re_interval;   /* RE fires when i > 0 and i % re_interval == 0; 0 = never */
   int32_t device;
@@ -175,7 +176,10 @@ typedef struct {
 /* Host-side draw tape for n_iter iterations (parity / replay).  Layouts:
  *   sel_u     [n_iter][R][K]
  *   clone_u   [n_iter][R]
- *   extra_u   [n_iter][R][K]               (NULL allowed when n_extra == 0)
+ *   extra_u   [n_iter][R][K]               n_cull = 1: the extras' noise (NULL
+ *                                          allowed when n_extra == 0); n_cull > 1:
+ *                                          [0, K-n_cull) the survivors' permutation
+ *                                          keys (survivor position p takes key p)
  *   schedule  [n_iter][n_steps]            move index per MWG sub-step
  *   normals   [n_iter][R][n_walk][n_steps][3][Nmax]
  *   uniforms  [n_iter][R][n_walk][n_steps][4]
@@ -195,19 +199,21 @@ typedef struct {
   const double* re_u;
 } jr_tape;
 
-/* Per-iteration outputs for a block of n_iter iterations (any may be NULL). */
+/* Per-iteration outputs for a block of n_iter iterations (any may be NULL).
+ * C = n_cull; the dead points of an iteration are in descending energy. */
 typedef struct {
-  double* emax;         /* [n_iter][R] contour = dead energy */
-  int32_t* worst;       /* [n_iter][R] */
-  int32_t* clone;       /* [n_iter][R] */
-  double* dead_volume;  /* [n_iter][R] */
-  double* dead_pos;     /* [n_iter][R][Nmax][3] */
-  int32_t* dead_types;  /* [n_iter][R][Nmax] */
-  double* dead_cell;    /* [n_iter][R][9] */
+  double* emax;         /* [n_iter][R] contour = energy of the C-th dead point */
+  int32_t* worst;       /* [n_iter][R][C] culled slots */
+  int32_t* clone;       /* [n_iter][R][C] clone source of each culled slot */
+  double* dead_volume;  /* [n_iter][R][C] */
+  double* dead_pos;     /* [n_iter][R][C][Nmax][3] */
+  int32_t* dead_types;  /* [n_iter][R][C][Nmax] */
+  double* dead_cell;    /* [n_iter][R][C][9] */
   int32_t* counters;    /* [n_iter][R][n_moves][JR_NCOUNT] */
   int32_t* re_acc;      /* [n_iter][R-1] */
   int32_t* re_att;      /* [n_iter][R-1] */
-  int32_t* walk;        /* [n_iter][R][n_walk] */
+  int32_t* walk;        /* [n_iter][R][n_walk], n_walk = C + n_extra: culled slots first */
+  double* dead_energy;  /* [n_iter][R][C] */
 } jr_iter_out;
 
 const char* jr_global_error(void);

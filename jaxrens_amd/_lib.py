@@ -172,6 +172,7 @@ class JrIterOut(C.Structure):
         ("re_acc", _ip),
         ("re_att", _ip),
         ("walk", _ip),
+        ("dead_energy", _dp),
     ]
 
 

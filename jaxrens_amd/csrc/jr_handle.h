@@ -56,7 +56,7 @@ struct jr_handle {
   jrh::DBuf<jr::IterState> st;
   // per-block outputs
   int out_cap = 0;
-  jrh::DBuf<double> o_emax, o_dead_volume, o_dead_pos, o_dead_cell;
+  jrh::DBuf<double> o_emax, o_dead_volume, o_dead_pos, o_dead_cell, o_dead_energy;
   jrh::DBuf<int> o_worst, o_clone, o_dead_types, o_counters, o_re_acc, o_re_att,
       o_walk;
   // trial buffers

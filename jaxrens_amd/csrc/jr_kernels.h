@@ -364,11 +364,42 @@ struct SelArgs {
   double* o_dead_pos;
   int* o_dead_types;
   double* o_dead_cell;
+  // n_cull culled walkers per replica; per-block outputs with a cull axis:
+  // o_worst, o_clone, o_dead_* are [block][R][n_cull](...)
+  int n_cull;
+  double* o_dead_energy;
 };
 
 __device__ __forceinline__ bool key_before(double ka, int ia, double kb,
                                            int ib) {
   return (ka > kb) || (ka == kb && ia < ib);
+}
+
+// Block-wide bitonic sort of (key, index) pairs in LDS, Kpad a power of two,
+// into the order `before` defines (a strict total order on the pairs).
+template <class Before>
+__device__ void bitonic_sort(double* skey, int* sidx, int Kpad, Before before) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int size = 2; size <= Kpad; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < Kpad; i += nt) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = ((i & size) == 0);
+          const double ki = skey[i], kj = skey[j];
+          const int ii = sidx[i], ij = sidx[j];
+          const bool swap = up ? before(kj, ij, ki, ii) : before(ki, ii, kj, ij);
+          if (swap) {
+            skey[i] = kj;
+            skey[j] = ki;
+            sidx[i] = ij;
+            sidx[j] = ii;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
 }
 
 // nested_sampling.py:86-133,151-186 (n_cull = 1) and :486-518 (dead record,
@@ -458,27 +489,7 @@ __global__ void __launch_bounds__(1024) select_kernel(SelArgs a) {
       }
     }
     __syncthreads();
-    for (int size = 2; size <= a.Kpad; size <<= 1) {
-      for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        for (int i = tid; i < a.Kpad; i += nt) {
-          int j = i ^ stride;
-          if (j > i) {
-            bool up = ((i & size) == 0);
-            double ki = skey[i], kj = skey[j];
-            int ii = sidx[i], ij = sidx[j];
-            bool swap = up ? key_before(kj, ij, ki, ii)
-                           : key_before(ki, ii, kj, ij);
-            if (swap) {
-              skey[i] = kj;
-              skey[j] = ki;
-              sidx[i] = ij;
-              sidx[j] = ii;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
+    bitonic_sort(skey, sidx, a.Kpad, key_before);
     for (int e = tid; e < a.n_extra; e += nt)
       a.walk[(size_t)r * a.n_walk + 1 + e] = sidx[e];
   }
@@ -492,6 +503,7 @@ __global__ void __launch_bounds__(1024) select_kernel(SelArgs a) {
     a.o_worst[o] = w;
     a.o_clone[o] = cl;
     a.o_dead_volume[o] = fabs(det3(a.pop.cell + ws * 9));
+    a.o_dead_energy[o] = mx;
   }
   for (int i = tid; i < a.n_moves * JR_NCOUNT; i += nt)
     a.counters[(size_t)r * a.n_moves * JR_NCOUNT + i] = 0;
@@ -517,6 +529,107 @@ __global__ void __launch_bounds__(1024) select_kernel(SelArgs a) {
   if (tid == 0) {
     a.pop.energy[ws] = a.pop.energy[cs];
     a.pop.n_atoms[ws] = a.pop.n_atoms[cs];
+  }
+}
+
+// nested_sampling.py:188-212 (n_cull = C > 1) and :486-518.  The walkers in
+// lexicographic order: potential descending, then the noise sel_u
+// descending, then index (jnp.lexsort((-noise, -potentials)) is stable).
+// The first C are culled in that order, and the contour is the potential
+// of the C-th.  The survivors order[C:] are permuted by sorting on the keys
+// extra_u[0 .. K-C) ascending (equal keys by walker index), where survivor
+// position p takes key extra_u[p]; the first C of the permutation are the
+// clone sources of the culled slots, the next n_extra the extra chains.
+// One block per replica; LDS holds Kpad (key, index) pairs and the C culled
+// indices.
+__global__ void __launch_bounds__(1024) select_cull_kernel(SelArgs a) {
+  extern __shared__ double sh[];
+  double* skey = sh;                                      // [Kpad]
+  int* sidx = reinterpret_cast<int*>(sh + a.Kpad);        // [Kpad]
+  int* culled = sidx + a.Kpad;                            // [C]
+  const int r = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  const int K = a.K, C = a.n_cull;
+  const double* E = a.pop.energy + (size_t)r * K;
+  const double* su = a.sel_u + (size_t)r * K;
+  const double* pu = a.extra_u + (size_t)r * K;
+  const double inf = __builtin_inf();
+  // 1. lexicographic order; padding sorts last (-inf potential and noise)
+  for (int k = tid; k < a.Kpad; k += nt) {
+    skey[k] = k < K ? E[k] : -inf;
+    sidx[k] = k;
+  }
+  __syncthreads();
+  bitonic_sort(skey, sidx, a.Kpad, [&](double ka, int ia, double kb, int ib) {
+    if (ka != kb) return ka > kb;
+    const double na = ia < K ? su[ia] : -inf, nb = ib < K ? su[ib] : -inf;
+    if (na != nb) return na > nb;
+    return ia < ib;
+  });
+  const int bp = a.st->block_pos;
+  const size_t o = (size_t)bp * a.R + r;
+  if (tid == 0) {
+    a.emax[r] = skey[C - 1];
+    a.o_emax[o] = skey[C - 1];
+  }
+  for (int c = tid; c < C; c += nt) {
+    const int w = sidx[c];
+    culled[c] = w;
+    a.walk[(size_t)r * a.n_walk + c] = w;
+    a.o_worst[o * C + c] = w;
+    a.o_dead_energy[o * C + c] = skey[c];
+    a.o_dead_volume[o * C + c] = fabs(det3(a.pop.cell + ((size_t)r * K + w) * 9));
+  }
+  __syncthreads();
+  // 2. permutation of the survivors; culled and padding entries sort last
+  for (int k = tid; k < a.Kpad; k += nt) skey[k] = (k >= C && k < K) ? pu[k - C] : inf;
+  __syncthreads();
+  bitonic_sort(skey, sidx, a.Kpad, [](double ka, int ia, double kb, int ib) {
+    return (ka < kb) || (ka == kb && ia < ib);
+  });
+  for (int c = tid; c < C; c += nt) a.o_clone[o * C + c] = sidx[c];
+  for (int e = tid; e < a.n_extra; e += nt)
+    a.walk[(size_t)r * a.n_walk + C + e] = sidx[C + e];
+  for (int i = tid; i < a.n_moves * JR_NCOUNT; i += nt)
+    a.counters[(size_t)r * a.n_moves * JR_NCOUNT + i] = 0;
+  if (r == 0 && tid < JR_NQUEUE) a.queue[tid] = 0;
+  const int n3 = a.Nmax * 3;
+  const size_t base = (size_t)r * K;
+  if (a.o_dead_pos) {
+    for (int i = tid; i < C * n3; i += nt) {
+      const int c = i / n3, q = i - c * n3;
+      a.o_dead_pos[(o * C + c) * n3 + q] = a.pop.pos[(base + culled[c]) * n3 + q];
+    }
+    for (int i = tid; i < C * a.Nmax; i += nt) {
+      const int c = i / a.Nmax, q = i - c * a.Nmax;
+      a.o_dead_types[(o * C + c) * a.Nmax + q] =
+          a.pop.types[(base + culled[c]) * a.Nmax + q];
+    }
+    for (int i = tid; i < C * 9; i += nt) {
+      const int c = i / 9, q = i - c * 9;
+      a.o_dead_cell[(o * C + c) * 9 + q] = a.pop.cell[(base + culled[c]) * 9 + q];
+    }
+  }
+  __syncthreads();
+  // 3. clone each source into its culled slot (sources and slots are disjoint)
+  for (int i = tid; i < C * n3; i += nt) {
+    const int c = i / n3, q = i - c * n3;
+    const size_t ws = base + culled[c], cs = base + sidx[c];
+    a.pop.pos[ws * n3 + q] = a.pop.pos[cs * n3 + q];
+    a.pop.dir[ws * n3 + q] = a.pop.dir[cs * n3 + q];
+  }
+  for (int i = tid; i < C * a.Nmax; i += nt) {
+    const int c = i / a.Nmax, q = i - c * a.Nmax;
+    a.pop.types[(base + culled[c]) * a.Nmax + q] = a.pop.types[(base + sidx[c]) * a.Nmax + q];
+  }
+  for (int i = tid; i < C * 9; i += nt) {
+    const int c = i / 9, q = i - c * 9;
+    a.pop.cell[(base + culled[c]) * 9 + q] = a.pop.cell[(base + sidx[c]) * 9 + q];
+  }
+  for (int c = tid; c < C; c += nt) {
+    a.pop.energy[base + culled[c]] = a.pop.energy[base + sidx[c]];
+    a.pop.n_atoms[base + culled[c]] = a.pop.n_atoms[base + sidx[c]];
   }
 }
 

@@ -287,6 +287,7 @@ void release_all(jr_handle* h) {
   h->sel_u.release(); h->clone_u.release(); h->extra_u.release(); h->normals.release();
   h->uniforms.release(); h->re_u.release(); h->st.release();
   h->o_emax.release(); h->o_dead_volume.release(); h->o_dead_pos.release(); h->o_dead_cell.release();
+  h->o_dead_energy.release();
   h->o_worst.release(); h->o_clone.release(); h->o_dead_types.release(); h->o_counters.release();
   h->o_re_acc.release(); h->o_re_att.release(); h->o_walk.release();
   h->t_idx.release(); h->t_reason.release(); h->t_step.release(); h->t_emax.release();
@@ -298,13 +299,15 @@ int ensure_out(jr_handle* h, int n_iter) {
   const auto& c = h->cfg;
   const int R = c.n_replicas, Nm = c.n_atoms_max;
   size_t n = (size_t)n_iter;
+  const size_t D = n * R * c.n_cull;  // dead points in the block
   HIPCHK(h, h->o_emax.alloc(n * R));
-  HIPCHK(h, h->o_dead_volume.alloc(n * R));
-  HIPCHK(h, h->o_dead_pos.alloc(n * R * Nm * 3));
-  HIPCHK(h, h->o_dead_cell.alloc(n * R * 9));
-  HIPCHK(h, h->o_worst.alloc(n * R));
-  HIPCHK(h, h->o_clone.alloc(n * R));
-  HIPCHK(h, h->o_dead_types.alloc(n * R * Nm));
+  HIPCHK(h, h->o_dead_energy.alloc(D));
+  HIPCHK(h, h->o_dead_volume.alloc(D));
+  HIPCHK(h, h->o_dead_pos.alloc(D * Nm * 3));
+  HIPCHK(h, h->o_dead_cell.alloc(D * 9));
+  HIPCHK(h, h->o_worst.alloc(D));
+  HIPCHK(h, h->o_clone.alloc(D));
+  HIPCHK(h, h->o_dead_types.alloc(D * Nm));
   HIPCHK(h, h->o_counters.alloc(n * R * c.n_moves * JR_NCOUNT));
   HIPCHK(h, h->o_re_acc.alloc(n * std::max(R - 1, 1)));
   HIPCHK(h, h->o_re_att.alloc(n * std::max(R - 1, 1)));
@@ -409,7 +412,9 @@ SelArgs sel_args(jr_handle* h) {
   a.n_extra = c.n_extra;
   a.n_walk = h->n_walk;
   a.n_moves = c.n_moves;
-  a.Kpad = c.n_extra > 0 ? (int)pow2_at_least(c.n_walkers) : 0;
+  a.Kpad = (c.n_extra > 0 || c.n_cull > 1) ? (int)pow2_at_least(c.n_walkers) : 0;
+  a.n_cull = c.n_cull;
+  a.o_dead_energy = h->o_dead_energy.p;
   a.sel_u = h->sel_u.p;
   a.clone_u = h->clone_u.p;
   a.extra_u = h->extra_u.p;
@@ -426,6 +431,19 @@ SelArgs sel_args(jr_handle* h) {
   a.o_dead_types = h->o_dead_types.p;
   a.o_dead_cell = h->o_dead_cell.p;
   return a;
+}
+
+// Selection and cloning of one iteration, one block per replica: the
+// single-cull kernel, or with n_cull > 1 the lexicographic-order kernel
+// (LDS: Kpad (key, index) pairs, plus the culled indices).
+void launch_select(jr_handle* h, hipStream_t s) {
+  const SelArgs sa = sel_args(h);
+  const size_t pairs = sa.Kpad ? (size_t)sa.Kpad * (sizeof(double) + sizeof(int)) : 0;
+  if (sa.n_cull > 1)
+    hipLaunchKernelGGL(select_cull_kernel, dim3(sa.R), dim3(1024),
+                       pairs + sizeof(int) * sa.n_cull, s, sa);
+  else
+    hipLaunchKernelGGL(select_kernel, dim3(sa.R), dim3(1024), pairs, s, sa);
 }
 
 ReArgs re_args(jr_handle* h, IterState* st) {
@@ -449,6 +467,10 @@ ReArgs re_args(jr_handle* h, IterState* st) {
   a.o_att = h->o_re_att.p;
   return a;
 }
+
+// extra_u feeds the extras' noise (n_cull = 1) or the survivors' permutation
+// keys (n_cull > 1)
+bool uses_extra_u(const jr_config& c) { return c.n_extra > 0 || c.n_cull > 1; }
 
 size_t n_normals(const jr_handle* h) {
   const auto& c = h->cfg;
@@ -482,7 +504,7 @@ void enqueue_draws(jr_handle* h) {
                      h->sel_u.p, (long long)R * K, seed, st, 0LL, 1u);
   hipLaunchKernelGGL(fill_uniform_kernel, dim3(1), dim3(256), 0, s, h->clone_u.p,
                      (long long)R, seed, st, 0LL, 2u);
-  if (c.n_extra > 0)
+  if (uses_extra_u(c))
     hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for((size_t)R * K)), dim3(256), 0, s,
                        h->extra_u.p, (long long)R * K, seed, st, 0LL, 3u);
   hipLaunchKernelGGL(schedule_kernel, dim3(1), dim3(64), 0, s, h->sched.p,
@@ -522,9 +544,7 @@ void enqueue_walk(jr_handle* h, const WalkArgs& base, hipStream_t s) {
 // one for its statistics); the pair-potential walk is one launch.
 int enqueue_iteration_core(jr_handle* h, bool timed, int ev_idx) {
   const auto& c = h->cfg;
-  SelArgs sa = sel_args(h);
-  size_t sel_lds = sa.Kpad ? (size_t)sa.Kpad * (sizeof(double) + sizeof(int)) : 0;
-  hipLaunchKernelGGL(select_kernel, dim3(c.n_replicas), dim3(1024), sel_lds, h->stream, sa);
+  launch_select(h, h->stream);
   if (timed) (void)hipEventRecord(h->ev[2 * ev_idx], h->stream);
   if (h->learned) {
     int rc = jrl::learned_walk(h);
@@ -657,10 +677,16 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   if (c.n_moves < 1 || c.n_moves > JR_MAX_MOVES || !c.moves || !c.move_weights)
     return set_global_err(JR_E_ARG, "need 1..%d moves with weights", JR_MAX_MOVES);
   if (c.n_mcmc_steps < 1) return set_global_err(JR_E_ARG, "n_mcmc_steps must be >= 1");
-  if (c.n_cull != 1)
-    return set_global_err(JR_E_UNSUPPORTED, "n_cull=%d: only n_cull=1 is built", c.n_cull);
-  if (c.n_extra < 0 || c.n_extra > c.n_walkers - 2)
+  if (c.n_cull < 1) return set_global_err(JR_E_ARG, "n_cull must be >= 1");
+  if (c.n_cull == 1 && (c.n_extra < 0 || c.n_extra > c.n_walkers - 2))
     return set_global_err(JR_E_ARG, "n_extra must be in [0, K-2]");
+  if (c.n_cull > 1 && (c.n_extra < 0 || 2 * c.n_cull + c.n_extra > c.n_walkers))
+    return set_global_err(JR_E_ARG,
+                          "n_cull=%d, n_extra=%d: need 2*n_cull + n_extra <= K=%d (each culled "
+                          "slot is refilled from a distinct survivor, extras from the rest)",
+                          c.n_cull, c.n_extra, c.n_walkers);
+  if (c.n_cull > 1 && (c.mpnn || c.learned))
+    return set_global_err(JR_E_UNSUPPORTED, "the learned walk is built for n_cull = 1");
   if (c.n_swap_cycles < 0) return set_global_err(JR_E_ARG, "n_swap_cycles must be >= 0");
   double wsum = 0.0;
   for (int m = 0; m < c.n_moves; ++m) {
@@ -694,7 +720,7 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   h->cfg.moves = nullptr;
   h->cfg.move_weights = nullptr;
   h->cfg.pot.eps4 = h->cfg.pot.eps24 = h->cfg.pot.sig2 = nullptr;
-  h->n_walk = 1 + c.n_extra;
+  h->n_walk = c.n_cull + c.n_extra;
   h->max_pairs = c.n_replicas / 2;
   h->A = atoms_per_lane(c.n_atoms_max);
   h->cfgbits = pot_cfg(&c.pot);
@@ -978,13 +1004,15 @@ int copy_outputs(jr_handle* h, int n_iter, jr_iter_out* out) {
   auto d2h = [&](void* dst, const void* src, size_t bytes) {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
   };
+  const size_t D = n * R * c.n_cull;  // dead points in the block
   if (out->emax) HIPCHK(h, d2h(out->emax, h->o_emax.p, sizeof(double) * n * R));
-  if (out->worst) HIPCHK(h, d2h(out->worst, h->o_worst.p, sizeof(int) * n * R));
-  if (out->clone) HIPCHK(h, d2h(out->clone, h->o_clone.p, sizeof(int) * n * R));
-  if (out->dead_volume) HIPCHK(h, d2h(out->dead_volume, h->o_dead_volume.p, sizeof(double) * n * R));
-  if (out->dead_pos) HIPCHK(h, d2h(out->dead_pos, h->o_dead_pos.p, sizeof(double) * n * R * Nm * 3));
-  if (out->dead_types) HIPCHK(h, d2h(out->dead_types, h->o_dead_types.p, sizeof(int) * n * R * Nm));
-  if (out->dead_cell) HIPCHK(h, d2h(out->dead_cell, h->o_dead_cell.p, sizeof(double) * n * R * 9));
+  if (out->worst) HIPCHK(h, d2h(out->worst, h->o_worst.p, sizeof(int) * D));
+  if (out->clone) HIPCHK(h, d2h(out->clone, h->o_clone.p, sizeof(int) * D));
+  if (out->dead_volume) HIPCHK(h, d2h(out->dead_volume, h->o_dead_volume.p, sizeof(double) * D));
+  if (out->dead_pos) HIPCHK(h, d2h(out->dead_pos, h->o_dead_pos.p, sizeof(double) * D * Nm * 3));
+  if (out->dead_types) HIPCHK(h, d2h(out->dead_types, h->o_dead_types.p, sizeof(int) * D * Nm));
+  if (out->dead_cell) HIPCHK(h, d2h(out->dead_cell, h->o_dead_cell.p, sizeof(double) * D * 9));
+  if (out->dead_energy) HIPCHK(h, d2h(out->dead_energy, h->o_dead_energy.p, sizeof(double) * D));
   if (out->counters)
     HIPCHK(h, d2h(out->counters, h->o_counters.p, sizeof(int) * n * R * c.n_moves * JR_NCOUNT));
   if (R > 1) {
@@ -1017,10 +1045,10 @@ int upload_tape(jr_handle* h, const jr_tape* t, int it) {
   };
   if (!t->sel_u || !t->clone_u || !t->schedule || !t->normals || !t->uniforms)
     return h->fail(JR_E_ARG, "tape: sel_u/clone_u/schedule/normals/uniforms required");
-  if (c.n_extra > 0 && !t->extra_u) return h->fail(JR_E_ARG, "tape: extra_u required");
+  if (uses_extra_u(c) && !t->extra_u) return h->fail(JR_E_ARG, "tape: extra_u required");
   HIPCHK(h, h2d(h->sel_u.p, t->sel_u + (size_t)it * R * K, sizeof(double) * R * K));
   HIPCHK(h, h2d(h->clone_u.p, t->clone_u + (size_t)it * R, sizeof(double) * R));
-  if (c.n_extra > 0)
+  if (uses_extra_u(c))
     HIPCHK(h, h2d(h->extra_u.p, t->extra_u + (size_t)it * R * K, sizeof(double) * R * K));
   for (int s2 = 0; s2 < c.n_mcmc_steps; ++s2) {
     int m = t->schedule[(size_t)it * c.n_mcmc_steps + s2];
@@ -1120,7 +1148,7 @@ int jr_get_last_tape(jr_handle* h, double* sel_u, double* clone_u,
   };
   if (sel_u) HIPCHK(h, d2h(sel_u, h->sel_u.p, sizeof(double) * R * K));
   if (clone_u) HIPCHK(h, d2h(clone_u, h->clone_u.p, sizeof(double) * R));
-  if (extra_u && c.n_extra > 0) HIPCHK(h, d2h(extra_u, h->extra_u.p, sizeof(double) * R * K));
+  if (extra_u && uses_extra_u(c)) HIPCHK(h, d2h(extra_u, h->extra_u.p, sizeof(double) * R * K));
   if (schedule) HIPCHK(h, d2h(schedule, h->sched.p, sizeof(int) * c.n_mcmc_steps));
   if (normals) HIPCHK(h, d2h(normals, h->normals.p, sizeof(double) * n_normals(h)));
   if (uniforms) HIPCHK(h, d2h(uniforms, h->uniforms.p, sizeof(double) * n_uniforms(h)));
@@ -1371,9 +1399,7 @@ int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
   int rc = begin_block(h, 1);
   if (rc) return rc;
   enqueue_draws(h);
-  SelArgs sa = sel_args(h);
-  size_t sel_lds = sa.Kpad ? (size_t)sa.Kpad * (sizeof(double) + sizeof(int)) : 0;
-  hipLaunchKernelGGL(select_kernel, dim3(c.n_replicas), dim3(1024), sel_lds, s, sa);
+  launch_select(h, s);
   HIPCHK(h, copy_pop(t_pos.p, t_dir.p, t_cell.p, t_e.p, t_t.p, t_n.p, h->pos.p, h->dir.p,
                      h->cell.p, h->energy.p, h->types.p, h->n_atoms.p));
   hipEvent_t e0, e1;

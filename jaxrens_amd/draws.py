@@ -19,7 +19,7 @@ class Tape:
     clone_u: np.ndarray    # (n_iter, R)
     extra_u: np.ndarray    # (n_iter, R, K)
     schedule: np.ndarray   # (n_iter, n_steps) int32
-    normals: np.ndarray    # (n_iter, R, n_walk, n_steps, 3, Nmax)
+    normals: np.ndarray    # (n_iter, R, n_walk, n_steps, 3, Nmax), n_walk = n_cull + n_extra
     uniforms: np.ndarray   # (n_iter, R, n_walk, n_steps, 4)
     re_u: np.ndarray       # (n_iter, n_cycles, 2, max(R//2, 1), 2)
 
@@ -36,8 +36,11 @@ class Tape:
 
 def make_tape(rng: np.random.Generator, n_iter: int, R: int, K: int, Nmax: int,
               n_steps: int, n_extra: int, move_probs, n_swap_cycles: int = 1,
-              ties: bool = False) -> Tape:
-    n_walk = 1 + n_extra
+              ties: bool = False, n_cull: int = 1) -> Tape:
+    """``ties``: round the selection noise to quarters so that equal noise
+    occurs; with ``n_cull > 1`` also the energies' tie-break then reaches
+    the index rule when energies are equal too."""
+    n_walk = n_cull + n_extra
     probs = np.asarray(move_probs, dtype=np.float64)
     sched = rng.choice(len(probs), size=(n_iter, n_steps), p=probs / probs.sum())
     sel = rng.random((n_iter, R, K))

@@ -20,7 +20,7 @@ from . import _lib
 from .backends import Hamiltonian, LJBackend, unwrap_model
 from .draws import Tape
 from .moves import MWG, CellSpec
-from .prior_mass import dead_log_weight, final_log_evidence
+from .prior_mass import dead_log_weight, final_log_evidence, logaddexp_accumulate
 
 
 @dataclass
@@ -252,24 +252,30 @@ class NSEngine:
     def ns_iterations(self, n_iter: int, tape: Tape | None = None,
                       graph: bool = False, dead_records: bool = False) -> dict:
         """Run ``n_iter`` NS iterations; returns per-iteration outputs and
-        updates ``log_evidence`` (float64 closed-form weights)."""
-        R, Nm, nm = self.n_replicas, self.Nmax, self.n_moves
+        updates ``log_evidence`` (float64 closed-form weights).
+
+        The dead-point outputs (``worst``, ``clone``, ``dead_energy``,
+        ``dead_volume`` and the ``dead_pos/types/cell`` records) carry a cull
+        axis after the replica axis, in descending energy, when
+        ``n_cull > 1``; with ``n_cull == 1`` it is dropped (``(n_iter, R)``)."""
+        R, Nm, nm, Cc = self.n_replicas, self.Nmax, self.n_moves, self.run.n_cull
         out = {
             "emax": np.empty((n_iter, R)),
-            "worst": np.empty((n_iter, R), np.int32),
-            "clone": np.empty((n_iter, R), np.int32),
-            "dead_volume": np.empty((n_iter, R)),
+            "worst": np.empty((n_iter, R, Cc), np.int32),
+            "clone": np.empty((n_iter, R, Cc), np.int32),
+            "dead_energy": np.empty((n_iter, R, Cc)),
+            "dead_volume": np.empty((n_iter, R, Cc)),
             "counters": np.empty((n_iter, R, nm, _lib.JR_NCOUNT), np.int32),
             "re_acc": np.empty((n_iter, max(R - 1, 1)), np.int32),
             "re_att": np.empty((n_iter, max(R - 1, 1)), np.int32),
             "walk": np.empty((n_iter, R, self.n_walk), np.int32),
         }
         if dead_records:
-            out["dead_pos"] = np.empty((n_iter, R, Nm, 3))
-            out["dead_types"] = np.empty((n_iter, R, Nm), np.int32)
-            out["dead_cell"] = np.empty((n_iter, R, 3, 3))
+            out["dead_pos"] = np.empty((n_iter, R, Cc, Nm, 3))
+            out["dead_types"] = np.empty((n_iter, R, Cc, Nm), np.int32)
+            out["dead_cell"] = np.empty((n_iter, R, Cc, 3, 3))
         o = _lib.JrIterOut()
-        for k in ("emax", "dead_volume", "dead_pos", "dead_cell"):
+        for k in ("emax", "dead_energy", "dead_volume", "dead_pos", "dead_cell"):
             if k in out:
                 setattr(o, k, _lib.dptr(out[k]))
         for k in ("worst", "clone", "counters", "re_acc", "re_att", "walk",
@@ -291,9 +297,15 @@ class NSEngine:
                 self._check(self.lib.jr_ns_iterations(self.h, n_iter, None,
                                                       C.byref(o)))
         its = self.iteration + np.arange(n_iter)
-        w = dead_log_weight(its, self.n_walkers, self.run.n_cull)[:, 0]  # (n_iter,)
+        w = dead_log_weight(its, self.n_walkers, Cc)  # (n_iter, n_cull)
         for i in range(n_iter):
-            self.log_evidence = np.logaddexp(self.log_evidence, w[i] - out["emax"][i])
+            self.log_evidence = logaddexp_accumulate(
+                self.log_evidence, w[i][None, :] - out["dead_energy"][i])
+        if Cc == 1:
+            for k in ("worst", "clone", "dead_energy", "dead_volume", "dead_pos",
+                      "dead_types", "dead_cell"):
+                if k in out:
+                    out[k] = out[k][:, :, 0]
         self.iteration += n_iter
         if R == 1:
             out["re_acc"][:] = 0
@@ -304,23 +316,28 @@ class NSEngine:
         """One iteration; one reference-shaped ``info`` dict per replica
         (keys of ``sampling/nested_sampling.py:653-685``)."""
         out = self.ns_iterations(1, tape=tape, dead_records=True)
+        Cc, Nm = self.run.n_cull, self.Nmax
         infos = []
         for r in range(self.n_replicas):
             c = out["counters"][0, r]
             n_acc = c[:, 1]
             n_prop = c[:, 0]
+            worst = np.reshape(out["worst"][0, r], (Cc,))
+            clone = np.reshape(out["clone"][0, r], (Cc,))
+            dead = {"positions": np.reshape(out["dead_pos"][0, r], (Cc, Nm, 3)),
+                    "types": np.reshape(out["dead_types"][0, r], (Cc, Nm)),
+                    "energy": np.reshape(out["dead_energy"][0, r], (Cc,)),
+                    "cell": np.reshape(out["dead_cell"][0, r], (Cc, 3, 3))}
             infos.append({
                 "emax": out["emax"][0, r],
                 "hmax": out["emax"][0, r],
-                "worst_idx": int(out["worst"][0, r]),
-                "clone_idx": int(out["clone"][0, r]),
-                "worst_indices": out["worst"][0, r:r + 1],
-                "clone_indices": out["clone"][0, r:r + 1],
+                "worst_idx": int(worst[-1]),
+                "clone_idx": int(clone[-1]),
+                "worst_indices": worst,
+                "clone_indices": clone,
                 "acceptance_rate": n_acc.sum() / max(self.n_walk * self.run.n_mcmc_steps, 1),
-                "dead_walker": {"positions": out["dead_pos"][0, r],
-                                "types": out["dead_types"][0, r],
-                                "energy": out["emax"][0, r],
-                                "cell": out["dead_cell"][0, r]},
+                "dead_walker": {k: v[-1] for k, v in dead.items()},
+                "dead_walkers": dead,
                 "n_accepted_per_move": n_acc,
                 "n_proposed_per_move": n_prop,
                 "reject_reason_counts_per_move": c[:, 1:5],

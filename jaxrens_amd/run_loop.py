@@ -73,7 +73,7 @@ class RunResult:
     reason: str
     log_evidence: np.ndarray          # (R,) including the live walkers' share
     emax: np.ndarray                  # (iterations, R)
-    dead_volume: np.ndarray           # (iterations, R)
+    dead_volume: np.ndarray           # (iterations, R[, n_cull])
     step_sizes: np.ndarray            # (R, n_moves) at the end
     counters: np.ndarray              # (R, n_moves, 7) summed over iterations
     re_acc: np.ndarray
@@ -99,6 +99,10 @@ def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
     checkpoint (path or loaded
     dict) to continue from; iterations keep their global numbers, and the
     ``.energies`` files are cut back to the checkpoint and appended to.
+
+    With ``n_cull > 1`` every iteration writes its n_cull dead points, in
+    descending energy, as n_cull ``.energies`` rows and frames (the frames
+    tagged ``cull_rank``).
 
     ``traj_prefix``: write the dead walker of every iteration ``i`` with
     ``i % traj_interval == 0`` to ``<prefix>[.r].traj.extxyz`` per replica
@@ -163,19 +167,28 @@ def run_ns(engine, termination, adaptation: AdaptationConfig | None = None,
                             - (i % adaptation.adjust_interval))
                 n = max(1, min(n, _iterations_before_check(termination, i)))
             out = engine.ns_iterations(n, dead_records=bool(trajs))
+            R, Cc, Nm = engine.n_replicas, engine.run.n_cull, engine.Nmax
             for j in range(n):
                 it = i + j
                 emax_hist.append(out["emax"][j])
                 vol_hist.append(out["dead_volume"][j])
+                # one row (and frame) per dead point in descending energy; a
+                # single cull records the contour (cli/monitor.py:1331-1378)
+                de = out["emax"][j][:, None] if Cc == 1 else out["dead_energy"][j]
+                dv = np.reshape(out["dead_volume"][j], (R, Cc))
                 for r, lg in enumerate(loggers):
-                    lg.write_entry(it, float(out["emax"][j, r]),
-                                   float(out["dead_volume"][j, r]))
+                    for c in range(Cc):
+                        lg.write_entry(it, float(de[r, c]), float(dv[r, c]))
                 if trajs and it % traj_interval == 0:
+                    dp = np.reshape(out["dead_pos"][j], (R, Cc, Nm, 3))
+                    dt = np.reshape(out["dead_types"][j], (R, Cc, Nm))
+                    dc = np.reshape(out["dead_cell"][j], (R, Cc, 3, 3))
                     for r, tw in enumerate(trajs):
                         na = int(engine.n_atoms_rep[r])
-                        tw.write_dead_point(it, out["dead_pos"][j, r, :na],
-                                            out["dead_types"][j, r, :na],
-                                            out["dead_cell"][j, r], float(out["emax"][j, r]))
+                        for c in range(Cc):
+                            tw.write_dead_point(it, dp[r, c, :na], dt[r, c, :na], dc[r, c],
+                                                float(de[r, c]),
+                                                cull_rank=c if Cc > 1 else None)
             counters += out["counters"].sum(0)
             re_acc += out["re_acc"].sum(0)
             re_att += out["re_att"].sum(0)

@@ -40,12 +40,13 @@ def wrap_positions(positions, cell, eps: float = WRAP_EPS) -> np.ndarray:
 
 
 def format_frame(positions, types, cell, energy: float, iteration: int,
-                 symbol_map: dict[int, str]) -> str:
+                 symbol_map: dict[int, str], cull_rank: int | None = None) -> str:
     pos = np.asarray(positions, dtype=np.float64)
     lat = " ".join(repr(float(x)) for x in np.asarray(cell, dtype=np.float64).reshape(9))
+    rank = "" if cull_rank is None else f" cull_rank={int(cull_rank)}"
     lines = [f"{len(pos)}",
              f'Lattice="{lat}" Properties=species:S:1:pos:R:3 '
-             f'ns_energy={float(energy)!r} iter={int(iteration)} pbc="T T T"']
+             f'ns_energy={float(energy)!r} iter={int(iteration)}{rank} pbc="T T T"']
     for t, (x, y, z) in zip(np.asarray(types).reshape(-1), pos):
         lines.append(f"{symbol_map[int(t)]:<2s} {x:16.8f} {y:16.8f} {z:16.8f}")
     return "\n".join(lines) + "\n"
@@ -69,6 +70,9 @@ def read_extxyz(path) -> list[dict]:
             "ns_energy": float(re.search(r"ns_energy=(\S+)", hdr).group(1)),
             "iter": int(re.search(r"iter=(-?\d+)", hdr).group(1)),
         })
+        rank = re.search(r"cull_rank=(\d+)", hdr)
+        if rank:
+            frames[-1]["cull_rank"] = int(rank.group(1))
         i += 2 + n
     return frames
 
@@ -96,9 +100,14 @@ class ExtxyzTrajectoryWriter:
         else:
             self._f = open(self.path, "w")
 
-    def write_dead_point(self, iteration: int, positions, types, cell, energy: float):
+    def write_dead_point(self, iteration: int, positions, types, cell, energy: float,
+                         cull_rank: int | None = None):
+        """One dead point; ``cull_rank`` (0 = highest energy among the
+        iteration's culls, ``io/trajectory.py:118-139``) is written only
+        when n_cull > 1."""
         pos = wrap_positions(positions, cell) if self.wrap else np.asarray(positions)
-        self._f.write(format_frame(pos, types, cell, energy, iteration, self.symbol_map))
+        self._f.write(format_frame(pos, types, cell, energy, iteration, self.symbol_map,
+                                   cull_rank))
 
     def flush(self):
         self._f.flush()

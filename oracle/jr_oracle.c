@@ -898,6 +898,65 @@ static void select_replica(const double* E, int K, int n_extra,
   *emax = mx;
 }
 
+typedef struct {
+  double e, u;
+  int idx;
+} lex_t;
+
+/* potential descending, then noise descending, then index: the order of
+ * jnp.lexsort((-noise, -potentials)), which is stable */
+static int cmp_lex(const void* a, const void* b) {
+  const lex_t* x = (const lex_t*)a;
+  const lex_t* y = (const lex_t*)b;
+  if (x->e != y->e) return x->e > y->e ? -1 : 1;
+  if (x->u != y->u) return x->u > y->u ? -1 : 1;
+  return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+static int cmp_asc(const void* a, const void* b) {
+  const keyed_t* x = (const keyed_t*)a;
+  const keyed_t* y = (const keyed_t*)b;
+  if (x->key < y->key) return -1;
+  if (x->key > y->key) return 1;
+  return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+/* nested_sampling.py:188-212 (n_cull = C > 1): the first C walkers of the
+ * lexicographic order are culled in that order and the contour is the
+ * potential of the C-th; the survivors order[C:] are permuted
+ * (jax.random.permutation) by sorting them on the keys perm_u[p] of their
+ * positions p ascending, equal keys by walker index; the first C of the
+ * permutation are the clone sources, the next n_extra the extra chains.
+ * walk = culled slots, then extras. */
+static double select_cull(const double* E, int K, int C, int n_extra,
+                          const double* sel_u, const double* perm_u, int* worst,
+                          int* clone, int* walk, double* dead_e) {
+  lex_t* o = (lex_t*)malloc(sizeof(lex_t) * (size_t)K);
+  for (int k = 0; k < K; ++k) {
+    o[k].e = E[k];
+    o[k].u = sel_u[k];
+    o[k].idx = k;
+  }
+  qsort(o, (size_t)K, sizeof(lex_t), cmp_lex);
+  for (int c = 0; c < C; ++c) {
+    worst[c] = walk[c] = o[c].idx;
+    dead_e[c] = o[c].e;
+  }
+  const double contour = o[C - 1].e;
+  const int n_surv = K - C;
+  keyed_t* sv = (keyed_t*)malloc(sizeof(keyed_t) * (size_t)n_surv);
+  for (int p = 0; p < n_surv; ++p) {
+    sv[p].key = perm_u[p];
+    sv[p].idx = o[C + p].idx;
+  }
+  qsort(sv, (size_t)n_surv, sizeof(keyed_t), cmp_asc);
+  for (int c = 0; c < C; ++c) clone[c] = sv[c].idx;
+  for (int e = 0; e < n_extra; ++e) walk[C + e] = sv[C + e].idx;
+  free(sv);
+  free(o);
+  return contour;
+}
+
 static void rebase_pair(double E, const double* cell, const int32_t* types,
                         int N, double Pf, const double* muf, double Pt,
                         const double* mut, int S, double* out) {
@@ -980,27 +1039,43 @@ void or_replica_exchange(or_pop* pop, const or_run* run, const double* emax,
   free(tmp_t);
 }
 
-/* nested_sampling.py:391-686 for n_cull = 1, all replicas. */
+/* nested_sampling.py:391-686, all replicas; n_cull walkers culled each. */
 int or_ns_iteration(or_pop* pop, const or_pot* pot, const or_run* run,
                     const or_move* moves, int n_moves, int n_steps,
-                    int n_extra, int n_swap_cycles, const int32_t* pair_valid,
-                    const or_draws* d, or_iter_out* out, int n_threads) {
+                    int n_extra, int n_cull, int n_swap_cycles,
+                    const int32_t* pair_valid, const or_draws* d,
+                    or_iter_out* out, int n_threads) {
   int R = pop->R, K = pop->K, Nm = pop->Nmax, S = pop->S;
-  int n_walk = 1 + n_extra;
-  if (K < 2 || n_extra < 0 || n_extra > K - 2) return -1;
+  const int C = n_cull;
+  int n_walk = C + n_extra;
+  if (K < 2 || C < 1 || n_extra < 0) return -1;
+  if (C == 1 && n_extra > K - 2) return -1;
+  if (C > 1 && (2 * C + n_extra > K || !d->extra_u)) return -1;
   for (int r = 0; r < R; ++r) {
-    int w, cl;
-    double mx;
-    select_replica(pop->energy + (size_t)r * K, K, n_extra,
-                   d->sel_u + (size_t)r * K, d->clone_u[r],
-                   d->extra_u ? d->extra_u + (size_t)r * K : NULL, &w, &cl,
-                   out->walk + (size_t)r * n_walk, &mx);
-    out->worst[r] = w;
-    out->clone[r] = cl;
-    out->emax[r] = mx;
-    out->dead_energy[r] = pop->energy[(size_t)r * K + w];
-    out->dead_volume[r] = fabs(or_det3(pop->cell + 9 * ((size_t)r * K + w)));
-    copy_walker(pop, r * K + w, r * K + cl);
+    const size_t o = (size_t)r * C;
+    if (C == 1) {
+      int w, cl;
+      double mx;
+      select_replica(pop->energy + (size_t)r * K, K, n_extra,
+                     d->sel_u + (size_t)r * K, d->clone_u[r],
+                     d->extra_u ? d->extra_u + (size_t)r * K : NULL, &w, &cl,
+                     out->walk + (size_t)r * n_walk, &mx);
+      out->worst[o] = w;
+      out->clone[o] = cl;
+      out->emax[r] = mx;
+      out->dead_energy[o] = pop->energy[(size_t)r * K + w];
+    } else {
+      out->emax[r] = select_cull(pop->energy + (size_t)r * K, K, C, n_extra,
+                                 d->sel_u + (size_t)r * K, d->extra_u + (size_t)r * K,
+                                 out->worst + o, out->clone + o,
+                                 out->walk + (size_t)r * n_walk, out->dead_energy + o);
+    }
+    for (int c = 0; c < C; ++c)
+      out->dead_volume[o + c] =
+          fabs(or_det3(pop->cell + 9 * ((size_t)r * K + out->worst[o + c])));
+    /* sources and culled slots are disjoint: copies in any order */
+    for (int c = 0; c < C; ++c)
+      copy_walker(pop, r * K + out->worst[o + c], r * K + out->clone[o + c]);
   }
   memset(out->counters, 0, sizeof(int32_t) * (size_t)R * n_moves * OR_NCOUNT);
   int total = R * n_walk;

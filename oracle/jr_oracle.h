@@ -74,12 +74,12 @@ typedef struct {
 } or_draws;
 
 typedef struct {
-  double* emax;         /* [R] */
-  int32_t* worst;       /* [R] */
-  int32_t* clone;       /* [R] */
-  int32_t* walk;        /* [R*n_walk] */
-  double* dead_energy;  /* [R] */
-  double* dead_volume;  /* [R] */
+  double* emax;         /* [R] contour */
+  int32_t* worst;       /* [R][n_cull] */
+  int32_t* clone;       /* [R][n_cull] */
+  int32_t* walk;        /* [R][n_walk] */
+  double* dead_energy;  /* [R][n_cull] */
+  double* dead_volume;  /* [R][n_cull] */
   int32_t* counters;    /* [R*n_moves*7] */
   int32_t* re_acc;      /* [R-1] */
   int32_t* re_att;      /* [R-1] */
@@ -116,12 +116,14 @@ void or_walk_chain(int N, int Nmax, double* pos, int32_t* types, double* cell,
                    const double* normals, const double* uniforms,
                    int32_t* counters);
 
-/* One NS iteration (n_cull = 1) over all R replicas: select, clone, walk,
- * optional replica exchange. */
+/* One NS iteration over all R replicas: select (n_cull walkers each),
+ * clone, walk, optional replica exchange.  worst, clone, dead_energy and
+ * dead_volume of `out` are [R][n_cull]; walk is [R][n_cull + n_extra]. */
 int or_ns_iteration(or_pop* pop, const or_pot* pot, const or_run* run,
                     const or_move* moves, int n_moves, int n_steps,
-                    int n_extra, int n_swap_cycles, const int32_t* pair_valid,
-                    const or_draws* d, or_iter_out* out, int n_threads);
+                    int n_extra, int n_cull, int n_swap_cycles,
+                    const int32_t* pair_valid, const or_draws* d,
+                    or_iter_out* out, int n_threads);
 
 /* Replica exchange pass (EnsembleSwap). */
 void or_replica_exchange(or_pop* pop, const or_run* run, const double* emax,

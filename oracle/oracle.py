@@ -88,7 +88,7 @@ def lib() -> C.CDLL:
         L.or_ns_iteration.restype = C.c_int
         L.or_ns_iteration.argtypes = [C.POINTER(OrPop), C.POINTER(OrPot),
                                       C.POINTER(OrRun), C.POINTER(OrMove), C.c_int,
-                                      C.c_int, C.c_int, C.c_int, _ip,
+                                      C.c_int, C.c_int, C.c_int, C.c_int, _ip,
                                       C.POINTER(OrDraws), C.POINTER(OrIterOut), C.c_int]
         L.or_replica_exchange.restype = None
         L.or_replica_exchange.argtypes = [C.POINTER(OrPop), C.POINTER(OrRun), _dp,
@@ -195,11 +195,14 @@ class Population:
 
 def ns_iteration(pop: Population, pot: Pot, pressure, mu, step_sizes, moves, n_steps,
                  n_extra, tape, it=0, n_swap_cycles=1, pair_valid=None, do_re=True,
-                 n_threads=0):
-    """One iteration on ``pop`` (in place); ``tape`` is a draws.Tape."""
+                 n_threads=0, n_cull=1):
+    """One iteration on ``pop`` (in place); ``tape`` is a draws.Tape.
+    ``worst``, ``clone``, ``dead_energy`` and ``dead_volume`` are (R,) for
+    n_cull = 1 and (R, n_cull) otherwise."""
     R, K, Nm = pop.pos.shape[:3]
     S = pot.s.n_species
-    n_walk = 1 + n_extra
+    Cc = int(n_cull)
+    n_walk = Cc + n_extra
     n_moves = len(moves)
     pr, m, ss = f64(pressure), f64(np.asarray(mu).reshape(R, S)), f64(step_sizes)
     run = OrRun(_d(pr), _d(m), _d(ss))
@@ -209,9 +212,10 @@ def ns_iteration(pop: Population, pot: Pot, pressure, mu, step_sizes, moves, n_s
     dr = OrDraws(_d(keep[0]), _d(keep[1]), _d(keep[2]), _i(keep[3]), _d(keep[4]),
                  _d(keep[5]), _d(keep[6]) if (do_re and R > 1) else None)
     out = {
-        "emax": np.zeros(R), "worst": np.zeros(R, np.int32), "clone": np.zeros(R, np.int32),
-        "walk": np.zeros((R, n_walk), np.int32), "dead_energy": np.zeros(R),
-        "dead_volume": np.zeros(R), "counters": np.zeros((R, n_moves, 7), np.int32),
+        "emax": np.zeros(R), "worst": np.zeros((R, Cc), np.int32),
+        "clone": np.zeros((R, Cc), np.int32),
+        "walk": np.zeros((R, n_walk), np.int32), "dead_energy": np.zeros((R, Cc)),
+        "dead_volume": np.zeros((R, Cc)), "counters": np.zeros((R, n_moves, 7), np.int32),
         "re_acc": np.zeros(max(R - 1, 1), np.int32), "re_att": np.zeros(max(R - 1, 1), np.int32),
     }
     o = OrIterOut(_d(out["emax"]), _i(out["worst"]), _i(out["clone"]), _i(out["walk"]),
@@ -221,10 +225,13 @@ def ns_iteration(pop: Population, pot: Pot, pressure, mu, step_sizes, moves, n_s
     ps = pop.struct(S)
     mv = moves_array(moves)
     rc = lib().or_ns_iteration(C.byref(ps), C.byref(pot.s), C.byref(run), mv, n_moves,
-                               n_steps, n_extra, n_swap_cycles, _i(pv), C.byref(dr),
+                               n_steps, n_extra, Cc, n_swap_cycles, _i(pv), C.byref(dr),
                                C.byref(o), n_threads)
     if rc != 0:
         raise RuntimeError(f"or_ns_iteration failed: {rc}")
+    if Cc == 1:
+        for k in ("worst", "clone", "dead_energy", "dead_volume"):
+            out[k] = out[k][:, 0]
     return out
 
 
