@@ -39,6 +39,10 @@ struct jr_handle {
   WalkClass cls[3];
   jrh::DBuf<int> cls_order;  // [3][R] replica order of each class
   hipStream_t stream = nullptr;
+  // size classes after the first walk on stream2, forked from and joined
+  // back into `stream` by ev_fork / ev_join
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // population
   jrh::DBuf<double> pos, dir, cell, energy;
   jrh::DBuf<int> types, n_atoms;

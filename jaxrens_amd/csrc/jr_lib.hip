@@ -269,6 +269,10 @@ void jr_destroy(jr_handle* h) {
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
   for (auto e : h->ev) (void)hipEventDestroy(e);
+  if (h->stream2) (void)hipStreamSynchronize(h->stream2);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+  if (h->stream2) (void)hipStreamDestroy(h->stream2);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;  // DBuf members release in their owners' scope below
 }
@@ -521,13 +525,20 @@ void enqueue_draws(jr_handle* h) {
 // The walk of one iteration: a single persistent launch, or with replicas of
 // several atoms-per-lane classes one launch per class (largest A first), each
 // with the kernel built for its A, its own queue counter and grid.  A chain's
-// results do not depend on A, so the split changes no number.
-void enqueue_walk(jr_handle* h, const WalkArgs& base, hipStream_t s) {
+// results do not depend on A, so the split changes no number.  The classes
+// after the first run on stream2, forked from `s` and joined back into it,
+// so they run concurrently with the first: a class whose kernel leaves
+// registers free on a SIMD (A = 2 holds 326 of 512) shares it with waves of
+// the next class, and the classes share no data.
+hipError_t enqueue_walk(jr_handle* h, const WalkArgs& base, hipStream_t s) {
   if (h->n_classes == 0) {
     launch_walk(h->A, h->cfgbits, h->occ, base, h->blocks, h->wpb, h->lds_walk, s);
-    return;
+    return hipSuccess;
   }
   const int R = h->cfg.n_replicas;
+  hipError_t e;
+  if ((e = hipEventRecord(h->ev_fork, s)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(h->stream2, h->ev_fork, 0)) != hipSuccess) return e;
   for (int j = 0; j < h->n_classes; ++j) {
     const auto& k = h->cls[j];
     WalkArgs wa = base;
@@ -535,8 +546,10 @@ void enqueue_walk(jr_handle* h, const WalkArgs& base, hipStream_t s) {
     wa.total = k.n_rep * base.n_walk;
     wa.queue = h->queue.p + 1 + j;
     wa.lds_atoms = 64 * k.A;
-    launch_walk(k.A, h->cfgbits, k.occ, wa, k.blocks, h->wpb, k.lds, s);
+    launch_walk(k.A, h->cfgbits, k.occ, wa, k.blocks, h->wpb, k.lds, j > 0 ? h->stream2 : s);
   }
+  if ((e = hipEventRecord(h->ev_join, h->stream2)) != hipSuccess) return e;
+  return hipStreamWaitEvent(s, h->ev_join, 0);
 }
 
 // select -> walk -> RE -> advance, all on the handle's stream.  The learned
@@ -550,7 +563,7 @@ int enqueue_iteration_core(jr_handle* h, bool timed, int ev_idx) {
     int rc = jrl::learned_walk(h);
     if (rc) return rc;
   } else {
-    enqueue_walk(h, walk_args(h), h->stream);
+    HIPCHK(h, enqueue_walk(h, walk_args(h), h->stream));
   }
   if (timed) (void)hipEventRecord(h->ev[2 * ev_idx + 1], h->stream);
   if (c.n_replicas > 1) {
@@ -744,6 +757,9 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
     }                                                                    \
   } while (0)
   TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  TRY(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+  TRY(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+  TRY(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
   const size_t W = (size_t)R * K;
   TRY(h->pos.alloc(W * Nm * 3));
   TRY(h->dir.alloc(W * Nm * 3));
@@ -1421,7 +1437,7 @@ int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
                            h->n_atoms.p, t_pos.p, t_dir.p, t_cell.p, t_e.p, t_t.p, t_n.p));
         HIPCHK(h, hipMemsetAsync(h->queue.p, 0, sizeof(int) * JR_NQUEUE, s));
         HIPCHK(h, hipEventRecord(e0, s));
-        enqueue_walk(h, walk_args(h), s);
+        HIPCHK(h, enqueue_walk(h, walk_args(h), s));
         HIPCHK(h, hipEventRecord(e1, s));
         HIPCHK(h, hipEventSynchronize(e1));
         float ms = 0.f;

@@ -278,3 +278,32 @@ def test_autotune_keeps_state_and_results():
     pa, pb = a.get_population(), b.get_population()
     for k in pa:
         assert np.array_equal(pa[k], pb[k]), k
+
+
+def test_graph_replay_mixed_n_classes():
+    """Two size classes (N = 32 and 96) walked in a captured graph equal the
+    stream path."""
+    print("graph replay mixed N", flush=True)
+    eps = np.array([[1.0, 1.5], [1.5, 0.5]])
+    sig = np.array([[1.0, 0.8], [0.8, 0.88]])
+    moves = [{"type": "gmc", "n_reflect": 4, "step_size": 0.1, "weight": 4},
+             {"type": "volume", "step_size": 0.3, "weight": 1},
+             {"type": "alchemical_morph", "n_species": 2, "weight": 1}]
+    mu = np.stack([np.zeros(4), np.linspace(-1, 1, 4)], axis=1)
+
+    def make():
+        run = RunConfig(n_replicas=4, n_walkers=16, n_atoms=[32, 32, 96, 96], n_mcmc_steps=6,
+                        n_extra=8, seed=17, pressures=[1.0] * 4, chemical_potentials=mu,
+                        re_interval=1)
+        e = NSEngine(LJBackend(pair_epsilon=eps, pair_sigma=sig, cutoff=2.5), moves, run,
+                     cell=CELL)
+        e.init_walkers(np.random.default_rng(17), 0.5, 8.0, True, species_p=[0.5, 0.5])
+        return e
+    a, b = make(), make()
+    oa = a.ns_iterations(4)
+    ob = b.ns_iterations(4, graph=True)
+    for k in ("emax", "worst", "clone", "counters", "re_acc", "re_att", "walk"):
+        assert np.array_equal(oa[k], ob[k]), k
+    pa, pb = a.get_population(), b.get_population()
+    for k in pa:
+        assert np.array_equal(pa[k], pb[k]), k
