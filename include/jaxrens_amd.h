@@ -171,6 +171,17 @@ typedef struct {
    * learned configures the surrogate (NULL = JR_LEARNED_PLAIN). */
   const jr_mpnn* mpnn;
   const jr_learned* learned;
+  /* Configuration constraint (constraints/min_distance.py:25-46, gated as in
+   * sampling/mwg.py:98-141): an accepted proposal that brings two atoms i, j
+   * closer than min_distance[t_i*S + t_j] (minimum image; 0 disables a pair)
+   * is undone, positions, cell, species and energy, and counts in no reject
+   * bucket.  min_distance gates the moves that move atoms or the cell;
+   * min_distance_types (the same or another [S][S] table; NULL = not gated)
+   * gates alchemical_morph and species_swap, which the reference gates only
+   * when the floors depend on the species.  NULL min_distance: no constraint.
+   * FP64 pair walk without supercell images only. */
+  const double* min_distance;
+  const double* min_distance_types;
 } jr_config;
 
 /* Host-side draw tape for n_iter iterations (parity / replay).  Layouts:

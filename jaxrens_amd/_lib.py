@@ -144,6 +144,8 @@ class JrConfig(C.Structure):
         ("move_weights", _dp),
         ("mpnn", C.POINTER(JrMPNN)),
         ("learned", C.POINTER(JrLearned)),
+        ("min_distance", _dp),
+        ("min_distance_types", _dp),
     ]
 
 

@@ -42,6 +42,9 @@ struct jr_handle {
   // size classes after the first walk on stream2, forked from and joined
   // back into `stream` by ev_fork / ev_join
   hipStream_t stream2 = nullptr;
+  // minimum-distance floors [2][S][S] (atom/cell moves, species moves)
+  jrh::DBuf<double> dmin;
+  bool dmin_types = false;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // population
   jrh::DBuf<double> pos, dir, cell, energy;

@@ -37,6 +37,10 @@ struct WalkArgs {
   // minimum-image walk kernels depends on the offsets of the arguments above
   int img[3];
   int lds_atoms;  // sizes each wave's LDS region (Nmax, or 64 A of a class launch)
+  // minimum-distance floors [S][S] (cfg bit 4 kernels): for moves that move
+  // atoms or the cell, and for moves that change species; NULL = not gated
+  const double* dmin_pos;
+  const double* dmin_types;
 };
 
 __host__ __device__ inline size_t wave_lds_bytes(int Nmax, int n_moves) {
@@ -156,6 +160,13 @@ __global__ void __launch_bounds__(256)
         if (type == JR_MOVE_SHEAR || type == JR_MOVE_STRETCH) type = JR_MOVE_VOLUME;
         if (type == JR_MOVE_SWAP) type = JR_MOVE_MORPH;
       }
+      // configuration constraint gate (sampling/mwg.py:98-141)
+      [[maybe_unused]] const double* floors = nullptr;
+      [[maybe_unused]] WalkerRegs<A> W0;
+      if constexpr ((CFG & 16) != 0) {
+        floors = (type == JR_MOVE_MORPH) ? a.dmin_types : a.dmin_pos;
+        if (floors) W0 = W;
+      }
       switch (type) {
         case JR_MOVE_GMC:
           reason = gmc_move<A, CFG>(C, W, step, mv, xi) ? 0 : 1;
@@ -182,11 +193,32 @@ __global__ void __launch_bounds__(256)
         default:
           break;
       }
+      if constexpr ((CFG & 16) != 0) {
+        // an accepted proposal that breaks the constraint is undone:
+        // positions, cell, species and energy return, the GMC direction
+        // keeps the move's value, and the proposal counts in no reject
+        // bucket (its reason 4 falls outside the four the reference scatters)
+        if (floors && reason == 0 && !min_distance_ok<A>(C, W, floors)) {
+#pragma unroll
+          for (int s = 0; s < A; ++s) {
+            W.x[s] = W0.x[s];
+            W.y[s] = W0.y[s];
+            W.z[s] = W0.z[s];
+            W.t[s] = W0.t[s];
+          }
+#pragma unroll
+          for (int i = 0; i < 9; ++i) W.cell[i] = W0.cell[i];
+#pragma unroll
+          for (int i = 0; i < JR_MAX_SPECIES; ++i) W.counts[i] = W0.counts[i];
+          W.E = W0.E;
+          reason = 4;
+        }
+      }
       last_reason = reason;
       if (lane == 0) {
         int* c = lctr + m * JR_NCOUNT;
         c[0] += 1;
-        c[1 + reason] += 1;
+        if ((CFG & 16) == 0 || reason < 4) c[1 + reason] += 1;
         c[5] += n_eval;
         c[6] += n_grad;
       }

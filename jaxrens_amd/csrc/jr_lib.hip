@@ -51,7 +51,7 @@ template <int CFG>
 void launch_walk_c(int A, int occ, const WalkArgs& a, int blocks, int wpb,
                    size_t lds, hipStream_t s) {
   if (A == 1) {
-    if (!(CFG & 12) && occ == 3) launch_walk_t<1, CFG & 3, 3>(a, blocks, wpb, lds, s);
+    if (!(CFG & 28) && occ == 3) launch_walk_t<1, CFG & 3, 3>(a, blocks, wpb, lds, s);
     else launch_walk_t<1, CFG, 0>(a, blocks, wpb, lds, s);
   } else if (A == 2) {
     launch_walk_t<2, CFG, 0>(a, blocks, wpb, lds, s);
@@ -61,8 +61,8 @@ void launch_walk_c(int A, int occ, const WalkArgs& a, int blocks, int wpb,
 }
 
 // cfg bit 0: species table (S > 1); bit 1: FP32 pair terms; bit 2: image
-// sums; bit 3: shear, stretch or species_swap in the move table (bits 2 and 3
-// FP64 only)
+// sums; bit 3: shear, stretch or species_swap in the move table; bit 4:
+// minimum-distance constraint (bits 2-4 FP64 only, bit 4 without bit 2)
 void launch_walk(int A, int cfg, int occ, const WalkArgs& a, int blocks,
                  int wpb, size_t lds, hipStream_t s) {
   switch (cfg) {
@@ -75,14 +75,18 @@ void launch_walk(int A, int cfg, int occ, const WalkArgs& a, int blocks,
     case 8: launch_walk_c<8>(A, occ, a, blocks, wpb, lds, s); break;
     case 9: launch_walk_c<9>(A, occ, a, blocks, wpb, lds, s); break;
     case 12: launch_walk_c<12>(A, occ, a, blocks, wpb, lds, s); break;
-    default: launch_walk_c<13>(A, occ, a, blocks, wpb, lds, s); break;
+    case 13: launch_walk_c<13>(A, occ, a, blocks, wpb, lds, s); break;
+    case 16: launch_walk_c<16>(A, occ, a, blocks, wpb, lds, s); break;
+    case 17: launch_walk_c<17>(A, occ, a, blocks, wpb, lds, s); break;
+    case 24: launch_walk_c<24>(A, occ, a, blocks, wpb, lds, s); break;
+    default: launch_walk_c<25>(A, occ, a, blocks, wpb, lds, s); break;
   }
 }
 
 template <int CFG>
 const void* walk_fn_c(int A, int occ) {
   if (A == 1)
-    return (!(CFG & 12) && occ == 3) ? (const void*)walk_kernel<1, CFG & 3, 3>
+    return (!(CFG & 28) && occ == 3) ? (const void*)walk_kernel<1, CFG & 3, 3>
                                      : (const void*)walk_kernel<1, CFG, 0>;
   if (A == 2) return (const void*)walk_kernel<2, CFG, 0>;
   return (const void*)walk_kernel<4, CFG, 0>;
@@ -99,7 +103,11 @@ const void* walk_fn(int A, int cfg, int occ) {
     case 8: return walk_fn_c<8>(A, occ);
     case 9: return walk_fn_c<9>(A, occ);
     case 12: return walk_fn_c<12>(A, occ);
-    default: return walk_fn_c<13>(A, occ);
+    case 13: return walk_fn_c<13>(A, occ);
+    case 16: return walk_fn_c<16>(A, occ);
+    case 17: return walk_fn_c<17>(A, occ);
+    case 24: return walk_fn_c<24>(A, occ);
+    default: return walk_fn_c<25>(A, occ);
   }
 }
 
@@ -285,7 +293,7 @@ void release_all(jr_handle* h) {
   h->pos.release(); h->dir.release(); h->cell.release(); h->energy.release();
   h->types.release(); h->n_atoms.release();
   h->pressure.release(); h->mu.release(); h->step_sizes.release(); h->emax.release();
-  h->eps4.release(); h->eps24.release(); h->sig2.release(); h->moves.release(); h->cum.release();
+  h->eps4.release(); h->eps24.release(); h->sig2.release(); h->dmin.release(); h->moves.release(); h->cum.release();
   h->walk.release(); h->counters.release(); h->queue.release(); h->pair_valid.release();
   h->order.release(); h->sched.release();
   h->sel_u.release(); h->clone_u.release(); h->extra_u.release(); h->normals.release();
@@ -402,6 +410,11 @@ WalkArgs walk_args(jr_handle* h) {
   set_images(a.img, c.pot);
   a.moves = h->moves.p;
   a.lds_atoms = c.n_atoms_max;
+  if (h->cfgbits & 16) {
+    const int SS = c.pot.n_species * c.pot.n_species;
+    a.dmin_pos = h->dmin.p;
+    a.dmin_types = h->dmin_types ? h->dmin.p + SS : nullptr;
+  }
   return a;
 }
 
@@ -597,12 +610,12 @@ namespace {
 int set_geometry(jr_handle* h, int wpb, int occ) {
   const auto& c = h->cfg;
   h->occ_req = occ;
-  if (h->A != 1 || (h->cfgbits & 12)) occ = 0;
+  if (h->A != 1 || (h->cfgbits & 28)) occ = 0;
   h->wpb = wpb;
   h->occ = occ;
   for (int j = 0; j < h->n_classes; ++j) {
     auto& k = h->cls[j];
-    k.occ = (k.A == 1 && !(h->cfgbits & 12)) ? h->occ_req : 0;
+    k.occ = (k.A == 1 && !(h->cfgbits & 28)) ? h->occ_req : 0;
     k.lds = wave_lds_bytes_a(k.A) * wpb + 3 * sizeof(double) * c.pot.n_species * c.pot.n_species;
     int pc = 0;
     HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -700,6 +713,19 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
                           c.n_cull, c.n_extra, c.n_walkers);
   if (c.n_cull > 1 && (c.mpnn || c.learned))
     return set_global_err(JR_E_UNSUPPORTED, "the learned walk is built for n_cull = 1");
+  if (c.min_distance_types && !c.min_distance)
+    return set_global_err(JR_E_ARG, "min_distance_types needs min_distance");
+  if (c.min_distance) {
+    if (c.mpnn || c.learned || c.pot.precision != JR_PREC_FP64 || pot_images(&c.pot))
+      return set_global_err(JR_E_UNSUPPORTED,
+                            "the minimum-distance constraint is built for the FP64 pair walk "
+                            "without supercell images");
+    const int S = c.pot.n_species;
+    for (int i = 0; i < S * S; ++i)
+      if (!(c.min_distance[i] >= 0.0) ||
+          (c.min_distance_types && !(c.min_distance_types[i] >= 0.0)))
+        return set_global_err(JR_E_ARG, "minimum distances must be >= 0");
+  }
   if (c.n_swap_cycles < 0) return set_global_err(JR_E_ARG, "n_swap_cycles must be >= 0");
   double wsum = 0.0;
   for (int m = 0; m < c.n_moves; ++m) {
@@ -741,6 +767,7 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
     if (c.moves[m].type == JR_MOVE_SHEAR || c.moves[m].type == JR_MOVE_STRETCH ||
         c.moves[m].type == JR_MOVE_SWAP)
       h->cfgbits |= 8;
+  if (c.min_distance) h->cfgbits |= 16;
   const int R = c.n_replicas, K = c.n_walkers, Nm = c.n_atoms_max, S = c.pot.n_species;
   auto fail = [&](int code) {
     g_err = h->err;
@@ -772,6 +799,14 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   TRY(h->step_sizes.alloc((size_t)R * c.n_moves));
   TRY(h->emax.alloc(R));
   TRY(h->eps4.alloc((size_t)S * S));
+  if (c.min_distance) {
+    TRY(h->dmin.alloc((size_t)2 * S * S));
+    TRY(hipMemcpy(h->dmin.p, c.min_distance, sizeof(double) * S * S, hipMemcpyHostToDevice));
+    h->dmin_types = c.min_distance_types != nullptr;
+    if (h->dmin_types)
+      TRY(hipMemcpy(h->dmin.p + (size_t)S * S, c.min_distance_types, sizeof(double) * S * S,
+                    hipMemcpyHostToDevice));
+  }
   TRY(h->eps24.alloc((size_t)S * S));
   TRY(h->sig2.alloc((size_t)S * S));
   TRY(h->moves.alloc(c.n_moves));

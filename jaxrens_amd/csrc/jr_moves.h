@@ -29,6 +29,59 @@ __device__ __forceinline__ double tape_xi(const double* xi, int Nmax, int c,
   return xi[c * Nmax + a];
 }
 
+// constraints/min_distance.py:25-46: true iff no two atoms are closer than
+// their floor floors[t_i * S + t_j] (a floor of 0 disables the pair).  The
+// separation is the minimum image dr - rint(dr C^-1) C in Cartesian
+// coordinates with its Euclidean norm; every unordered pair is tested once
+// on the ring schedule (k = 1 .. N/2).  Writes the configuration into the
+// wave's LDS rows, which every energy evaluation rewrites first.
+template <int A>
+__device__ bool min_distance_ok(const WalkCtx& C, const WalkerRegs<A>& W,
+                                const double* floors) {
+  const int lane = lane_id();
+  const int N = C.N;
+  LdsView<A> L{C.L.base};
+  Geom g;
+  make_geom(W.cell, g);
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    const int a = lane + 64 * s;
+    if (a < N) {
+      L.x()[a] = L.x()[a + N] = W.x[s];
+      L.y()[a] = L.y()[a + N] = W.y[s];
+      L.z()[a] = L.z()[a + N] = W.z[s];
+      L.t()[a] = L.t()[a + N] = W.t[s];
+    }
+  }
+  wave_lds_sync();
+  bool bad = false;
+  for (int k = 1; k <= (N >> 1); ++k) {
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      const int a = lane + 64 * s;
+      if (a < N) {
+        const int j = a + k;
+        const double dx = L.x()[j] - W.x[s];
+        const double dy = L.y()[j] - W.y[s];
+        const double dz = L.z()[j] - W.z[s];
+        double f0 = fma(dz, g.inv[6], fma(dy, g.inv[3], dx * g.inv[0]));
+        double f1 = fma(dz, g.inv[7], fma(dy, g.inv[4], dx * g.inv[1]));
+        double f2 = fma(dz, g.inv[8], fma(dy, g.inv[5], dx * g.inv[2]));
+        f0 -= rint(f0);
+        f1 -= rint(f1);
+        f2 -= rint(f2);
+        const double ex = fma(f2, g.c[6], fma(f1, g.c[3], f0 * g.c[0]));
+        const double ey = fma(f2, g.c[7], fma(f1, g.c[4], f0 * g.c[1]));
+        const double ez = fma(f2, g.c[8], fma(f1, g.c[5], f0 * g.c[2]));
+        const double d2 = fma(ez, ez, fma(ey, ey, ex * ex));
+        bad = bad || (sqrt(d2) < floors[W.t[s] * C.S + L.t()[j]]);
+      }
+    }
+  }
+  wave_lds_sync();
+  return __ballot(bad) == 0;
+}
+
 // sampling/moves/galilean.py:142-300 (use_forces=True, unscoped).
 template <int A, int CFG>
 __device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,

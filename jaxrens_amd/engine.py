@@ -60,11 +60,15 @@ class NSEngine:
     """One handle of ``libjrhip``: population, run parameters and draws."""
 
     def __init__(self, backend, moves, run: RunConfig, cell: CellSpec | dict | None = None,
-                 learned=None):
+                 learned=None, constraints=None, symbol_map: dict[int, str] | None = None):
         """``learned``: :class:`~jaxrens_amd.surrogate.LearnedConfig` of the
         surrogate-screened walk.  The judge is the backend's model (the
         message-passing potential, or the pair potential); with the
-        message-passing backend and ``learned`` None the walk is plain."""
+        message-passing backend and ``learned`` None the walk is plain.
+
+        ``constraints``: the ``constraints:`` list of the reference's YAML
+        (``minimum_distance`` entries, :mod:`jaxrens_amd.constraints`);
+        ``symbol_map`` names the type codes for per-pair floors."""
         from .mpnn import MPNNBackend
         self.lib = _lib.load_library()
         self.model, p_default, mu_default = _model_of(backend)
@@ -112,6 +116,15 @@ class NSEngine:
         if learned is not None:
             self._learned_st, self._learned_keep = learned.as_struct(self.S)
             cfg.learned = C.pointer(self._learned_st)
+        from .constraints import constraint_tables
+        self.min_distance, self.min_distance_types = constraint_tables(constraints, self.S,
+                                                                       symbol_map)
+        self._dmin_keep = [None if t is None else _lib.f64(t)
+                           for t in (self.min_distance, self.min_distance_types)]
+        if self._dmin_keep[0] is not None:
+            cfg.min_distance = _lib.dptr(self._dmin_keep[0])
+        if self._dmin_keep[1] is not None:
+            cfg.min_distance_types = _lib.dptr(self._dmin_keep[1])
         h = C.c_void_p()
         _lib.check(self.lib.jr_create(C.byref(cfg), C.byref(h)))
         self.h = h

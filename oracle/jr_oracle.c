@@ -764,9 +764,41 @@ static int swap_move(const chain_ctx* c, const double* x, int32_t* types,
 
 static void count(int32_t* ctr, int reason, int n_eval, int n_grad) {
   ctr[0] += 1;
-  ctr[1 + reason] += 1;
+  if (reason < 4) ctr[1 + reason] += 1; /* 4: constraint, in no bucket */
   ctr[5] += n_eval;
   ctr[6] += n_grad;
+}
+
+int or_min_distance_ok(int N, const double* pos, const int32_t* types,
+                       const double* cell_in, const double* floors, int S) {
+  double det = or_det3(cell_in);
+  double cell[9], inv[9];
+  if (det == 0.0) { /* the stand-in cube of the energy, lj.py:138-139 */
+    memset(cell, 0, sizeof(cell));
+    cell[0] = cell[4] = cell[8] = 1.0e10;
+    det = or_det3(cell);
+  } else {
+    memcpy(cell, cell_in, sizeof(cell));
+  }
+  inv3(cell, det, inv);
+  for (int i = 0; i < N; ++i)
+    for (int j = i + 1; j < N; ++j) {
+      const double dx = pos[3 * j] - pos[3 * i];
+      const double dy = pos[3 * j + 1] - pos[3 * i + 1];
+      const double dz = pos[3 * j + 2] - pos[3 * i + 2];
+      double f0 = fma(dz, inv[6], fma(dy, inv[3], dx * inv[0]));
+      double f1 = fma(dz, inv[7], fma(dy, inv[4], dx * inv[1]));
+      double f2 = fma(dz, inv[8], fma(dy, inv[5], dx * inv[2]));
+      f0 -= rint(f0);
+      f1 -= rint(f1);
+      f2 -= rint(f2);
+      const double ex = fma(f2, cell[6], fma(f1, cell[3], f0 * cell[0]));
+      const double ey = fma(f2, cell[7], fma(f1, cell[4], f0 * cell[1]));
+      const double ez = fma(f2, cell[8], fma(f1, cell[5], f0 * cell[2]));
+      const double d2 = fma(ez, ez, fma(ey, ey, ex * ex));
+      if (sqrt(d2) < floors[types[i] * S + types[j]]) return 0;
+    }
+  return 1;
 }
 
 /* sampling/mwg.py:108-150,227-257 + nested_sampling.py:533-609 (one chain). */
@@ -779,6 +811,8 @@ void or_walk_chain(int N, int Nmax, double* pos, int32_t* types, double* cell,
                    int32_t* counters) {
   chain_ctx c = {N, Nmax, pot->n_species, pot, pressure, mu, emax};
   (void)n_moves;
+  double* pos0 = (double*)malloc(sizeof(double) * 3 * (size_t)(N > 0 ? N : 1));
+  int32_t* types0 = (int32_t*)malloc(sizeof(int32_t) * (size_t)(N > 0 ? N : 1));
   for (int t = 0; t < n_steps; ++t) {
     int m = schedule[t];
     const or_move* mv = &moves[m];
@@ -786,49 +820,58 @@ void or_walk_chain(int N, int Nmax, double* pos, int32_t* types, double* cell,
     const double* u = uniforms + (size_t)t * 4;
     double s = step_sizes[m];
     int32_t* ctr = counters + OR_NCOUNT * m;
-    switch (mv->type) {
-      case OR_MOVE_GMC: {
-        int acc = gmc_move(&c, pos, types, cell, energy, dir, s, mv, xi);
-        count(ctr, acc ? 0 : 1, mv->n_reflect, mv->n_reflect);
-        break;
-      }
-      case OR_MOVE_VOLUME: {
-        int reason = volume_move(&c, pos, types, cell, energy, s, mv, xi[0],
-                                 u[0]);
-        count(ctr, reason, 1, 0);
-        break;
-      }
-      case OR_MOVE_HMC: {
-        int acc = hmc_move(&c, pos, types, cell, energy, s, mv, xi);
-        count(ctr, acc ? 0 : 1, 2 * mv->n_leapfrog + 1, 2 * mv->n_leapfrog);
-        break;
-      }
-      case OR_MOVE_MORPH: {
-        int acc = morph_move(&c, pos, types, cell, energy, mv, u);
-        count(ctr, acc ? 0 : 1, 1, 0);
-        break;
-      }
-      case OR_MOVE_SWAP: {
-        int acc = swap_move(&c, pos, types, cell, energy, mv, u);
-        count(ctr, acc ? 0 : 1, 1, 0);
-        break;
-      }
-      case OR_MOVE_SHEAR: {
-        int reason = shear_move(&c, pos, types, cell, energy, s, mv, xi[0],
-                                xi[Nmax], u[0]);
-        count(ctr, reason, 1, 0);
-        break;
-      }
-      case OR_MOVE_STRETCH: {
-        int reason = stretch_move(&c, pos, types, cell, energy, s, mv, xi[0],
-                                  u[0]);
-        count(ctr, reason, 1, 0);
-        break;
-      }
-      default:
-        break;
+    /* sampling/mwg.py:98-141: the constraint gate of this move, if any */
+    const int species_move = mv->type == OR_MOVE_MORPH || mv->type == OR_MOVE_SWAP;
+    const double* floors = species_move ? pot->dmin_types : pot->dmin_pos;
+    double cell0[9], energy0 = *energy;
+    if (floors) {
+      memcpy(pos0, pos, sizeof(double) * 3 * (size_t)N);
+      memcpy(types0, types, sizeof(int32_t) * (size_t)N);
+      memcpy(cell0, cell, sizeof(cell0));
     }
+    int reason = 0, ne = 1, ng = 0;
+    switch (mv->type) {
+      case OR_MOVE_GMC:
+        reason = gmc_move(&c, pos, types, cell, energy, dir, s, mv, xi) ? 0 : 1;
+        ne = ng = mv->n_reflect;
+        break;
+      case OR_MOVE_VOLUME:
+        reason = volume_move(&c, pos, types, cell, energy, s, mv, xi[0], u[0]);
+        break;
+      case OR_MOVE_HMC:
+        reason = hmc_move(&c, pos, types, cell, energy, s, mv, xi) ? 0 : 1;
+        ne = 2 * mv->n_leapfrog + 1;
+        ng = 2 * mv->n_leapfrog;
+        break;
+      case OR_MOVE_MORPH:
+        reason = morph_move(&c, pos, types, cell, energy, mv, u) ? 0 : 1;
+        break;
+      case OR_MOVE_SWAP:
+        reason = swap_move(&c, pos, types, cell, energy, mv, u) ? 0 : 1;
+        break;
+      case OR_MOVE_SHEAR:
+        reason = shear_move(&c, pos, types, cell, energy, s, mv, xi[0], xi[Nmax], u[0]);
+        break;
+      case OR_MOVE_STRETCH:
+        reason = stretch_move(&c, pos, types, cell, energy, s, mv, xi[0], u[0]);
+        break;
+      default:
+        continue;
+    }
+    /* an accepted proposal that breaks the constraint is undone (positions,
+     * cell, species, energy; the GMC direction keeps the move's value) */
+    if (floors && reason == 0 &&
+        !or_min_distance_ok(N, pos, types, cell, floors, pot->n_species)) {
+      memcpy(pos, pos0, sizeof(double) * 3 * (size_t)N);
+      memcpy(types, types0, sizeof(int32_t) * (size_t)N);
+      memcpy(cell, cell0, sizeof(cell0));
+      *energy = energy0;
+      reason = 4;
+    }
+    count(ctr, reason, ne, ng);
   }
+  free(pos0);
+  free(types0);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1200,7 +1243,8 @@ void or_trial_moves(const or_pop* pop, const or_pot* pot, const or_run* run,
                     normals + (size_t)q * 3 * Nm, uniforms + (size_t)q * 4,
                     ctr);
       int32_t* c = ctr + OR_NCOUNT * move_idx;
-      int rs = c[1] ? 0 : (c[2] ? 1 : (c[3] ? 2 : 3));
+      /* no bucket: the constraint gate undid the move (reason 4) */
+      int rs = c[1] ? 0 : (c[2] ? 1 : (c[3] ? 2 : (c[4] ? 3 : 4)));
       accepted[q] = rs == 0;
       reason[q] = rs;
     }

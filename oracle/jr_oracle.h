@@ -34,6 +34,9 @@ typedef struct {
   const double* sig2;   /* [S*S] sigma_ab^2 */
   int img[3];           /* image half-ranges supercell_trafo[d] / 2; all 0:
                            minimum image only (backends/lj.py:106-113) */
+  const double* dmin_pos;   /* [S*S] minimum-distance floors gating moves
+                               that move atoms or the cell; NULL = none */
+  const double* dmin_types; /* [S*S] floors gating morph and swap; NULL = none */
 } or_pot;
 
 typedef struct {
@@ -86,6 +89,10 @@ typedef struct {
 } or_iter_out;
 
 double or_det3(const double* c);
+/* constraints/min_distance.py:25-46: 1 iff no pair i != j is closer than
+ * floors[t_i*S + t_j] (minimum image dr - rint(dr C^-1) C, Euclidean norm). */
+int or_min_distance_ok(int N, const double* pos, const int32_t* types,
+                       const double* cell, const double* floors, int S);
 double or_cbrt(double x);
 double or_exp(double x);
 double or_ipow(double x, int n);

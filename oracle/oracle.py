@@ -21,7 +21,8 @@ _ip = C.POINTER(C.c_int32)
 
 class OrPot(C.Structure):
     _fields_ = [("n_species", C.c_int), ("rc2", C.c_double), ("eps4", _dp),
-                ("eps24", _dp), ("sig2", _dp), ("img", C.c_int * 3)]
+                ("eps24", _dp), ("sig2", _dp), ("img", C.c_int * 3),
+                ("dmin_pos", _dp), ("dmin_types", _dp)]
 
 
 class OrMove(C.Structure):
@@ -141,6 +142,16 @@ class Pot:
     def from_table(cls, t):
         return cls(t.n_species, t.eps4, t.eps24, t.sig2, t.rc2,
                    getattr(t, "images", (0, 0, 0)))
+
+    def with_min_distance(self, floors_pos, floors_types=None):
+        """Gate the walk with minimum-distance floors ``[S][S]`` (NULL when
+        None): atom and cell moves by ``floors_pos``, species moves by
+        ``floors_types``."""
+        self.keep = self.keep + tuple(None if f is None else f64(f)
+                                      for f in (floors_pos, floors_types))
+        self.s.dmin_pos = None if floors_pos is None else _d(self.keep[-2])
+        self.s.dmin_types = None if floors_types is None else _d(self.keep[-1])
+        return self
 
 
 def energy_forces(pot: Pot, pos, types, cell, pressure=0.0, mu=None, forces=True):
