@@ -79,3 +79,62 @@ def test_min_distance_gate_species_moves(oracle, N):
     eng.init_walkers(np.random.default_rng(6), 1.2, 6.0, True, species_p=[0.5, 0.5])
     assert eng.min_distance_types is not None
     assert _parity(oracle, eng, 4, 33) > 0
+
+
+def _gated_engine():
+    moves = [{"type": "galilean", "n_reflect": 4, "step_size": 0.15, "weight": 3},
+             {"type": "volume", "step_size": 0.3, "weight": 1}]
+    run = RunConfig(n_replicas=2, n_walkers=16, n_atoms=24, n_mcmc_steps=4, n_extra=4,
+                    seed=9, pressures=[0.5, 2.0], re_interval=1)
+    eng = NSEngine(LJBackend(1.0, 1.0, 2.5), moves, run,
+                   cell={"min_volume_per_atom": 0.5, "max_volume_per_atom": 20.0,
+                         "flat_V_prior": True},
+                   constraints=[{"type": "minimum_distance", "d_min": 1.0}])
+    eng.init_walkers(np.random.default_rng(9), 1.5, 6.0, True)
+    return eng
+
+
+def test_min_distance_gate_trial_moves(oracle):
+    """The adaptation's trial moves report reason 4 where the gate undoes
+    a move, as the oracle does."""
+    print("min-distance gate: trial moves", flush=True)
+    from oracle.oracle import Population
+    eng = _gated_engine()
+    eng.ns_iterations(2)
+    rng = np.random.default_rng(2)
+    idx = rng.integers(0, 16, size=(2, 40)).astype(np.int32)
+    nrm = rng.standard_normal((2, 40, 3, 24))
+    uni = rng.random((2, 40, 4))
+    pop = Population.from_engine_dict(eng.get_population())
+    pot = oracle.Pot.from_table(eng.model.table).with_min_distance(eng.min_distance)
+    seen = set()
+    for m, ss in [(0, 0.3), (1, 0.8)]:
+        acc, rs = eng.trial_moves(m, [ss, ss * 2], idx, normals=nrm, uniforms=uni)
+        oacc, ors = oracle.trial_moves(pop, pot, eng.pressure, eng.mu, eng.step_sizes,
+                                       eng.mwg.as_oracle_moves(), m, [ss, ss * 2], eng.emax(),
+                                       idx, nrm, uni)
+        assert np.array_equal(acc, oacc) and np.array_equal(rs, ors), m
+        seen |= set(np.unique(rs).tolist())
+    assert 4 in seen
+
+
+def test_min_distance_gate_initial_walk(oracle):
+    print("min-distance gate: burn-in walk", flush=True)
+    from oracle.oracle import Population
+    from jaxrens_amd.burn_in import burn_in_emax
+    eng = _gated_engine()
+    pot = oracle.Pot.from_table(eng.model.table).with_min_distance(eng.min_distance)
+    opop = Population.from_engine_dict(eng.get_population())
+    emax = burn_in_emax(eng, 1.0)
+    rng = np.random.default_rng(5)
+    R, K, Nm, wl = 2, 16, 24, 6
+    sched = rng.choice(2, size=wl, p=eng.mwg.move_probs).astype(np.int32)
+    nn = rng.standard_normal((R, K, wl, 3, Nm))
+    uu = rng.random((R, K, wl, 4))
+    ctr = eng.initial_walk(wl, emax, sched, nn, uu)
+    octr = oracle.initial_walk(opop, pot, eng.pressure, eng.mu, eng.step_sizes,
+                               eng.mwg.as_oracle_moves(), emax, sched, nn, uu)
+    assert np.array_equal(ctr, octr)
+    d = eng.get_population()
+    assert np.array_equal(d["positions"], opop.pos) and np.array_equal(d["energy"], opop.energy)
+    assert (ctr[..., 0] - ctr[..., 1:5].sum(-1)).sum() > 0  # the gate fired
