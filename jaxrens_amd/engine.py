@@ -56,6 +56,25 @@ def _model_of(backend):
     return model, pressure, mu
 
 
+def grid_positions_in_cell(rng: np.random.Generator, cell, n_atoms: int,
+                           min_distance: float) -> np.ndarray:
+    """``init/positions.py:34-100``: ``n_atoms`` distinct sites of the grid
+    with floor(|a_i| / min_distance) points along cell vector i, at the
+    centres of the grid cells (fractions stored in float32, as the reference
+    stores them), drawn without replacement; Cartesian ``frac @ cell``.
+    Sites are at least ``min_distance`` apart in orthogonal cells."""
+    cell = np.asarray(cell, dtype=np.float64)
+    n_ax = [max(1, int(float(np.linalg.norm(cell[i])) / min_distance)) for i in range(3)]
+    n_grid = n_ax[0] * n_ax[1] * n_ax[2]
+    if n_grid < n_atoms:
+        raise ValueError(f"grid too coarse: {n_grid} sites available "
+                         f"({n_ax[0]}x{n_ax[1]}x{n_ax[2]}), {n_atoms} atoms requested")
+    axes = [(np.arange(m) + 0.5) / m for m in n_ax]
+    g = np.stack([x.ravel() for x in np.meshgrid(*axes, indexing="ij")], axis=1)
+    frac = g.astype(np.float32).astype(np.float64)
+    return frac[rng.choice(n_grid, size=n_atoms, replace=False)] @ cell
+
+
 class NSEngine:
     """One handle of ``libjrhip``: population, run parameters and draws."""
 
@@ -209,10 +228,12 @@ class NSEngine:
 
     def init_walkers(self, rng: np.random.Generator, v_lo_per_atom: float,
                      v_hi_per_atom: float, flat_V_prior: bool = True,
-                     species_p=None):
+                     species_p=None, grid_distance: float | None = None):
         """Cubic cells from the volume prior and uniform positions
-        (``init/cells.py:82-137``, ``init/positions.py:15-32``); species
-        Bernoulli/categorical per atom with probabilities ``species_p``."""
+        (``init/cells.py:82-137``, ``init/positions.py:15-32``), or with
+        ``grid_distance`` sites of a grid spaced by it
+        (:func:`grid_positions_in_cell`); species Bernoulli/categorical per
+        atom with probabilities ``species_p``."""
         R, K = self.n_replicas, self.n_walkers
         Nm = self.Nmax
         pos = np.zeros((R, K, Nm, 3))
@@ -230,8 +251,12 @@ class NSEngine:
                 vol = v_hi * (a + u * (1.0 - a)) ** (1.0 / p)
             lc = vol ** (1.0 / 3.0)
             cell[r] = lc[:, None, None] * np.eye(3)
-            frac = rng.random((K, n, 3))
-            pos[r, :, :n] = frac * lc[:, None, None]
+            if grid_distance is None:
+                frac = rng.random((K, n, 3))
+                pos[r, :, :n] = frac * lc[:, None, None]
+            else:
+                for k in range(K):
+                    pos[r, k, :n] = grid_positions_in_cell(rng, cell[r, k], n, grid_distance)
             if species_p is not None:
                 ty[r, :, :n] = rng.choice(len(species_p), size=(K, n), p=species_p)
         self.set_population(pos, ty, cell)

@@ -138,3 +138,31 @@ def test_min_distance_gate_initial_walk(oracle):
     d = eng.get_population()
     assert np.array_equal(d["positions"], opop.pos) and np.array_equal(d["energy"], opop.energy)
     assert (ctr[..., 0] - ctr[..., 1:5].sum(-1)).sum() > 0  # the gate fired
+
+
+def test_grid_initial_walkers_satisfy_the_floor(oracle):
+    """Walkers placed on the grid spaced by the floor start valid, then walk
+    gated (the invariant the reference's gate relies on)."""
+    print("min-distance gate: grid initial walkers", flush=True)
+    import ctypes as C
+    from oracle.oracle import _d, _i, f64, i32
+    moves = [{"type": "galilean", "n_reflect": 4, "step_size": 0.15, "weight": 3},
+             {"type": "volume", "step_size": 0.3, "weight": 1}]
+    run = RunConfig(n_replicas=2, n_walkers=16, n_atoms=27, n_mcmc_steps=4, n_extra=4,
+                    seed=12, pressures=[0.5, 2.0], re_interval=1)
+    eng = NSEngine(LJBackend(1.0, 1.0, 2.5), moves, run,
+                   cell={"min_volume_per_atom": 1.0, "max_volume_per_atom": 20.0,
+                         "flat_V_prior": True},
+                   constraints=[{"type": "minimum_distance", "d_min": 0.9}])
+    eng.init_walkers(np.random.default_rng(12), 1.0, 6.0, True, grid_distance=0.9)
+    L = oracle.lib()
+    L.or_min_distance_ok.restype = C.c_int
+    d = eng.get_population()
+    fl = f64(eng.min_distance)
+    for r in range(2):
+        for k in range(16):
+            p, t = f64(d["positions"][r, k]), i32(d["types"][r, k])
+            c = f64(np.asarray(d["cell"][r, k]).reshape(9))
+            assert L.or_min_distance_ok(27, _d(p), _i(t), _d(c), _d(fl), 1) == 1, (r, k)
+    out = eng.ns_iterations(3)
+    assert np.all(np.isfinite(out["emax"]))

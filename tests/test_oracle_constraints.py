@@ -170,3 +170,28 @@ def test_gate_restores_the_walker_and_counts_no_bucket(oracle):
     assert np.array_equal(pop.pos[0, 0], before.pos[0, 1])  # the clone, unmoved
     assert np.array_equal(pop.cell[0, 0], before.cell[0, 1])
     assert pop.energy[0, 0] == before.energy[0, 1]
+
+
+def test_grid_positions_respect_the_spacing():
+    """``init/positions.py:34-100``: in an orthogonal cell every pair of grid
+    sites is at least the spacing apart (minimum image included), sites are
+    distinct cell-centred grid points, and a too-coarse grid is an error."""
+    from jaxrens_amd.engine import grid_positions_in_cell
+    rng = np.random.default_rng(3)
+    for L, n, d in [(4.0, 20, 1.0), (6.0, 64, 1.3), (3.0, 1, 2.9)]:
+        cell = np.diag([L, L * 1.1, L * 0.9])
+        pos = grid_positions_in_cell(rng, cell, n, d)
+        assert pos.shape == (n, 3)
+        frac = pos @ np.linalg.inv(cell)
+        assert np.all((frac > 0) & (frac < 1))
+        if n > 1:
+            df = frac[:, None] - frac[None]
+            df -= np.round(df)
+            r = np.sqrt(((df @ cell) ** 2).sum(-1))[~np.eye(n, dtype=bool)]
+            assert r.min() >= d - 1e-6
+        na = [max(1, int(np.linalg.norm(cell[i]) / d)) for i in range(3)]
+        idx = np.rint(frac * na - 0.5)
+        assert np.allclose(frac * na - 0.5, idx, atol=1e-6)  # cell-centred sites
+        assert len({tuple(v) for v in idx.astype(int)}) == n
+    with pytest.raises(ValueError, match="grid too coarse"):
+        grid_positions_in_cell(rng, np.eye(3) * 2.0, 9, 1.0)
