@@ -152,3 +152,30 @@ def test_cull_config_needs_distinct_sources():
                     n_cull=4, seed=1, pressures=[0.1], re_interval=0)
     with pytest.raises(JrError, match="2\\*n_cull"):
         NSEngine(LJBackend(1.0, 1.0, 2.5), GMC_VOL, run, cell=CELL)
+
+
+def test_cull_log_z_agrees_with_single_cull():
+    """LJ-8 NPT (P = 0.1, K = 128, 16 independent replicas): log Z with
+    n_cull = 4 over 300 iterations and with n_cull = 1 over 1200 (the same
+    number of dead points), live share included, agree within 2 sigma of
+    the difference of the replica means."""
+    print("n_cull log Z vs single cull", flush=True)
+    moves = [{"type": "galilean", "n_reflect": 10, "step_size": 0.1, "weight": 4},
+             {"type": "volume", "step_size": 0.3, "weight": 1}]
+    cell = {"min_volume_per_atom": 1.0 / 8, "max_volume_per_atom": 64.0, "flat_V_prior": True}
+    R = 16
+
+    def log_z(C, iters, seed):
+        run = RunConfig(n_replicas=R, n_walkers=128, n_atoms=8, n_mcmc_steps=20, n_extra=0,
+                        n_cull=C, seed=seed, pressures=[0.1] * R, re_interval=0)
+        eng = NSEngine(LJBackend(1.0, 1.0, 2.5), moves, run, cell=cell)
+        eng.init_walkers(np.random.default_rng(seed), 1.0 / 8, 64.0, True)
+        run_ns(eng, [IterationTermination(iters)], block=50)
+        return eng.final_log_evidence()
+
+    a, b = log_z(1, 1200, 101), log_z(4, 300, 202)
+    se = np.sqrt(a.var(ddof=1) / R + b.var(ddof=1) / R)
+    z = abs(a.mean() - b.mean()) / se
+    print(f"log Z: n_cull=1 {a.mean():.4f} +- {a.std(ddof=1) / 4:.4f}, "
+          f"n_cull=4 {b.mean():.4f} +- {b.std(ddof=1) / 4:.4f}, z = {z:.2f}", flush=True)
+    assert z < 2.0
