@@ -12,7 +12,9 @@ import sys
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
-SRCS = [PKG / "csrc" / "jr_lib.hip", PKG / "csrc" / "jr_learned.hip"]
+# every translation unit under csrc/ (the walk kernels are split by configuration
+# so that they compile in parallel)
+SRCS = sorted((PKG / "csrc").glob("*.hip"))
 OUT = PKG / "libjrhip.so"
 
 HIPCC_FLAGS = [
@@ -44,20 +46,39 @@ def build(verbose: bool = False) -> Path:
     objdir = PKG / "csrc" / "build"
     objdir.mkdir(exist_ok=True)
     cc = hipcc()
-    procs = []
-    for src in SRCS:
-        obj = objdir / (src.stem + ".o")
-        cmd = [cc, *HIPCC_FLAGS, "-c", "-o", str(obj), str(src)]
-        procs.append((src, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE,
-                                                 stderr=subprocess.PIPE, text=True)))
-    objs, msgs, failed = [], [], []
-    for src, obj, p in procs:
-        out, err = p.communicate()
-        lines = [ln for ln in (out + err).splitlines() if "packed-fp32-ops" not in ln]
-        msgs += lines
+    jobs = max(1, min(len(SRCS), int(os.environ.get("MAX_JOBS", "0") or 0)
+                      or os.cpu_count() or 1, 16))
+    # largest sources first, so the longest compiles start early
+    order = sorted(SRCS, key=lambda p: -p.stat().st_size)
+    objs = [str(objdir / (src.stem + ".o")) for src in SRCS]
+    msgs, failed, running = [], [], []
+
+    def reap(entry):
+        # compiler output goes to a log file per source (no pipe can fill up
+        # while another compile is being waited for)
+        src, p, log = entry
+        p.wait()
+        text = log.read_text(errors="replace") if log.exists() else ""
+        msgs.extend(ln for ln in text.splitlines() if "packed-fp32-ops" not in ln)
         if p.returncode != 0:
             failed.append(src.name)
-        objs.append(str(obj))
+
+    for src in order:
+        while len(running) >= jobs:
+            done = next((e for e in running if e[1].poll() is not None), None)
+            if done is None:
+                running[0][1].wait()
+                continue
+            running.remove(done)
+            reap(done)
+        obj = objdir / (src.stem + ".o")
+        log = objdir / (src.stem + ".log")
+        cmd = [cc, *HIPCC_FLAGS, "-c", "-o", str(obj), str(src)]
+        with open(log, "w") as fh:
+            running.append((src, subprocess.Popen(cmd, stdout=fh, stderr=subprocess.STDOUT),
+                            log))
+    for e in running:
+        reap(e)
     if failed:
         raise RuntimeError(f"hipcc failed for {failed}:\n" + "\n".join(msgs))
     tmp = OUT.with_suffix(".so.tmp")

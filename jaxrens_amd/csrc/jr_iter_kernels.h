@@ -1,0 +1,487 @@
+// Kernels of one NS iteration around the walk: select/clone, replica
+// exchange, Philox draws and the iteration-state advance.  Included by
+// jr_lib.hip only (non-template kernels: one definition in the library).
+#pragma once
+
+#include "jr_kernels.h"
+
+namespace jr {
+
+// ------------------------------------------------------------------ select
+struct SelArgs {
+  PopPtrs pop;
+  int R, K, Nmax, n_extra, n_walk, n_moves, Kpad;
+  const double* sel_u;
+  const double* clone_u;
+  const double* extra_u;
+  double* emax;
+  int* walk;
+  int* counters;
+  int* queue;
+  IterState* st;
+  // per-block outputs (indexed by st->block_pos)
+  double* o_emax;
+  int* o_worst;
+  int* o_clone;
+  double* o_dead_volume;
+  double* o_dead_pos;
+  int* o_dead_types;
+  double* o_dead_cell;
+  // n_cull culled walkers per replica; per-block outputs with a cull axis:
+  // o_worst, o_clone, o_dead_* are [block][R][n_cull](...)
+  int n_cull;
+  double* o_dead_energy;
+};
+
+__device__ __forceinline__ bool key_before(double ka, int ia, double kb,
+                                           int ib) {
+  return (ka > kb) || (ka == kb && ia < ib);
+}
+
+// Block-wide bitonic sort of (key, index) pairs in LDS, Kpad a power of two,
+// into the order `before` defines (a strict total order on the pairs).
+template <class Before>
+__device__ void bitonic_sort(double* skey, int* sidx, int Kpad, Before before) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int size = 2; size <= Kpad; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < Kpad; i += nt) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = ((i & size) == 0);
+          const double ki = skey[i], kj = skey[j];
+          const int ii = sidx[i], ij = sidx[j];
+          const bool swap = up ? before(kj, ij, ki, ii) : before(ki, ii, kj, ij);
+          if (swap) {
+            skey[i] = kj;
+            skey[j] = ki;
+            sidx[i] = ij;
+            sidx[j] = ii;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// nested_sampling.py:86-133,151-186 (n_cull = 1) and :486-518 (dead record,
+// clone into the culled slot).  One block per replica.
+__global__ void __launch_bounds__(1024) select_kernel(SelArgs a) {
+  extern __shared__ double sh[];
+  double* skey = sh;                                   // [Kpad]
+  int* sidx = reinterpret_cast<int*>(sh + a.Kpad);     // [Kpad]
+  __shared__ double red_d[32];
+  __shared__ int red_i[32];
+  __shared__ int s_worst, s_clone;
+  __shared__ double s_mx;
+  const int r = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  const int lane = tid & 63, wv = tid >> 6, nwv = nt >> 6;
+  const double* E = a.pop.energy + (size_t)r * a.K;
+
+  // 1. contour = max energy
+  double mx = -__builtin_inf();
+  for (int k = tid; k < a.K; k += nt) mx = fmax(mx, E[k]);
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) mx = fmax(mx, __shfl_xor(mx, m, 64));
+  if (lane == 0) red_d[wv] = mx;
+  __syncthreads();
+  if (tid == 0) {
+    double v = red_d[0];
+    for (int w = 1; w < nwv; ++w) v = fmax(v, red_d[w]);
+    s_mx = v;
+  }
+  __syncthreads();
+  mx = s_mx;
+  // 2. uniform tie-break among exact maxima (first index on equal noise)
+  const double* su = a.sel_u + (size_t)r * a.K;
+  double bk = -__builtin_inf();
+  int bi = 0x7fffffff;
+  for (int k = tid; k < a.K; k += nt) {
+    if (E[k] == mx) {
+      double kk = su[k];
+      if (key_before(kk, k, bk, bi)) {
+        bk = kk;
+        bi = k;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    double ok = __shfl_xor(bk, m, 64);
+    int oi = __shfl_xor(bi, m, 64);
+    if (key_before(ok, oi, bk, bi)) {
+      bk = ok;
+      bi = oi;
+    }
+  }
+  if (lane == 0) {
+    red_d[wv] = bk;
+    red_i[wv] = bi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double k0 = red_d[0];
+    int i0 = red_i[0];
+    for (int w = 1; w < nwv; ++w)
+      if (key_before(red_d[w], red_i[w], k0, i0)) {
+        k0 = red_d[w];
+        i0 = red_i[w];
+      }
+    int w = i0;
+    int cl = (int)(a.clone_u[r] * (double)(a.K - 1));
+    if (cl > a.K - 2) cl = a.K - 2;
+    if (cl >= w) cl += 1;
+    s_worst = w;
+    s_clone = cl;
+  }
+  __syncthreads();
+  const int w = s_worst, cl = s_clone;
+  // 3. extras: stable descending sort of noise with worst/clone excluded
+  if (a.n_extra > 0) {
+    const double* eu = a.extra_u + (size_t)r * a.K;
+    for (int k = tid; k < a.Kpad; k += nt) {
+      if (k < a.K) {
+        skey[k] = (k == w || k == cl) ? -__builtin_inf() : eu[k];
+        sidx[k] = k;
+      } else {
+        skey[k] = -__builtin_inf();
+        sidx[k] = k;
+      }
+    }
+    __syncthreads();
+    bitonic_sort(skey, sidx, a.Kpad, key_before);
+    for (int e = tid; e < a.n_extra; e += nt)
+      a.walk[(size_t)r * a.n_walk + 1 + e] = sidx[e];
+  }
+  const int bp = a.st->block_pos;
+  const size_t ws = (size_t)r * a.K + w, cs = (size_t)r * a.K + cl;
+  if (tid == 0) {
+    a.walk[(size_t)r * a.n_walk] = w;
+    a.emax[r] = mx;
+    const size_t o = (size_t)bp * a.R + r;
+    a.o_emax[o] = mx;
+    a.o_worst[o] = w;
+    a.o_clone[o] = cl;
+    a.o_dead_volume[o] = fabs(det3(a.pop.cell + ws * 9));
+    a.o_dead_energy[o] = mx;
+  }
+  for (int i = tid; i < a.n_moves * JR_NCOUNT; i += nt)
+    a.counters[(size_t)r * a.n_moves * JR_NCOUNT + i] = 0;
+  if (r == 0 && tid < JR_NQUEUE) a.queue[tid] = 0;
+  if (a.o_dead_pos) {
+    const size_t o = (size_t)bp * a.R + r;
+    for (int i = tid; i < a.Nmax * 3; i += nt)
+      a.o_dead_pos[o * a.Nmax * 3 + i] = a.pop.pos[ws * a.Nmax * 3 + i];
+    for (int i = tid; i < a.Nmax; i += nt)
+      a.o_dead_types[o * a.Nmax + i] = a.pop.types[ws * a.Nmax + i];
+    for (int i = tid; i < 9; i += nt)
+      a.o_dead_cell[o * 9 + i] = a.pop.cell[ws * 9 + i];
+  }
+  __syncthreads();
+  // 4. clone the survivor into the culled slot (all walker-bound fields)
+  for (int i = tid; i < a.Nmax * 3; i += nt) {
+    a.pop.pos[ws * a.Nmax * 3 + i] = a.pop.pos[cs * a.Nmax * 3 + i];
+    a.pop.dir[ws * a.Nmax * 3 + i] = a.pop.dir[cs * a.Nmax * 3 + i];
+  }
+  for (int i = tid; i < a.Nmax; i += nt)
+    a.pop.types[ws * a.Nmax + i] = a.pop.types[cs * a.Nmax + i];
+  if (tid < 9) a.pop.cell[ws * 9 + tid] = a.pop.cell[cs * 9 + tid];
+  if (tid == 0) {
+    a.pop.energy[ws] = a.pop.energy[cs];
+    a.pop.n_atoms[ws] = a.pop.n_atoms[cs];
+  }
+}
+
+// nested_sampling.py:188-212 (n_cull = C > 1) and :486-518.  The walkers in
+// lexicographic order: potential descending, then the noise sel_u
+// descending, then index (jnp.lexsort((-noise, -potentials)) is stable).
+// The first C are culled in that order, and the contour is the potential
+// of the C-th.  The survivors order[C:] are permuted by sorting on the keys
+// extra_u[0 .. K-C) ascending (equal keys by walker index), where survivor
+// position p takes key extra_u[p]; the first C of the permutation are the
+// clone sources of the culled slots, the next n_extra the extra chains.
+// One block per replica; LDS holds Kpad (key, index) pairs and the C culled
+// indices.
+__global__ void __launch_bounds__(1024) select_cull_kernel(SelArgs a) {
+  extern __shared__ double sh[];
+  double* skey = sh;                                      // [Kpad]
+  int* sidx = reinterpret_cast<int*>(sh + a.Kpad);        // [Kpad]
+  int* culled = sidx + a.Kpad;                            // [C]
+  const int r = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  const int K = a.K, C = a.n_cull;
+  const double* E = a.pop.energy + (size_t)r * K;
+  const double* su = a.sel_u + (size_t)r * K;
+  const double* pu = a.extra_u + (size_t)r * K;
+  const double inf = __builtin_inf();
+  // 1. lexicographic order; padding sorts last (-inf potential and noise)
+  for (int k = tid; k < a.Kpad; k += nt) {
+    skey[k] = k < K ? E[k] : -inf;
+    sidx[k] = k;
+  }
+  __syncthreads();
+  bitonic_sort(skey, sidx, a.Kpad, [&](double ka, int ia, double kb, int ib) {
+    if (ka != kb) return ka > kb;
+    const double na = ia < K ? su[ia] : -inf, nb = ib < K ? su[ib] : -inf;
+    if (na != nb) return na > nb;
+    return ia < ib;
+  });
+  const int bp = a.st->block_pos;
+  const size_t o = (size_t)bp * a.R + r;
+  if (tid == 0) {
+    a.emax[r] = skey[C - 1];
+    a.o_emax[o] = skey[C - 1];
+  }
+  for (int c = tid; c < C; c += nt) {
+    const int w = sidx[c];
+    culled[c] = w;
+    a.walk[(size_t)r * a.n_walk + c] = w;
+    a.o_worst[o * C + c] = w;
+    a.o_dead_energy[o * C + c] = skey[c];
+    a.o_dead_volume[o * C + c] = fabs(det3(a.pop.cell + ((size_t)r * K + w) * 9));
+  }
+  __syncthreads();
+  // 2. permutation of the survivors; culled and padding entries sort last
+  for (int k = tid; k < a.Kpad; k += nt) skey[k] = (k >= C && k < K) ? pu[k - C] : inf;
+  __syncthreads();
+  bitonic_sort(skey, sidx, a.Kpad, [](double ka, int ia, double kb, int ib) {
+    return (ka < kb) || (ka == kb && ia < ib);
+  });
+  for (int c = tid; c < C; c += nt) a.o_clone[o * C + c] = sidx[c];
+  for (int e = tid; e < a.n_extra; e += nt)
+    a.walk[(size_t)r * a.n_walk + C + e] = sidx[C + e];
+  for (int i = tid; i < a.n_moves * JR_NCOUNT; i += nt)
+    a.counters[(size_t)r * a.n_moves * JR_NCOUNT + i] = 0;
+  if (r == 0 && tid < JR_NQUEUE) a.queue[tid] = 0;
+  const int n3 = a.Nmax * 3;
+  const size_t base = (size_t)r * K;
+  if (a.o_dead_pos) {
+    for (int i = tid; i < C * n3; i += nt) {
+      const int c = i / n3, q = i - c * n3;
+      a.o_dead_pos[(o * C + c) * n3 + q] = a.pop.pos[(base + culled[c]) * n3 + q];
+    }
+    for (int i = tid; i < C * a.Nmax; i += nt) {
+      const int c = i / a.Nmax, q = i - c * a.Nmax;
+      a.o_dead_types[(o * C + c) * a.Nmax + q] =
+          a.pop.types[(base + culled[c]) * a.Nmax + q];
+    }
+    for (int i = tid; i < C * 9; i += nt) {
+      const int c = i / 9, q = i - c * 9;
+      a.o_dead_cell[(o * C + c) * 9 + q] = a.pop.cell[(base + culled[c]) * 9 + q];
+    }
+  }
+  __syncthreads();
+  // 3. clone each source into its culled slot (sources and slots are disjoint)
+  for (int i = tid; i < C * n3; i += nt) {
+    const int c = i / n3, q = i - c * n3;
+    const size_t ws = base + culled[c], cs = base + sidx[c];
+    a.pop.pos[ws * n3 + q] = a.pop.pos[cs * n3 + q];
+    a.pop.dir[ws * n3 + q] = a.pop.dir[cs * n3 + q];
+  }
+  for (int i = tid; i < C * a.Nmax; i += nt) {
+    const int c = i / a.Nmax, q = i - c * a.Nmax;
+    a.pop.types[(base + culled[c]) * a.Nmax + q] = a.pop.types[(base + sidx[c]) * a.Nmax + q];
+  }
+  for (int i = tid; i < C * 9; i += nt) {
+    const int c = i / 9, q = i - c * 9;
+    a.pop.cell[(base + culled[c]) * 9 + q] = a.pop.cell[(base + sidx[c]) * 9 + q];
+  }
+  for (int c = tid; c < C; c += nt) {
+    a.pop.energy[base + culled[c]] = a.pop.energy[base + sidx[c]];
+    a.pop.n_atoms[base + culled[c]] = a.pop.n_atoms[base + sidx[c]];
+  }
+}
+
+// ------------------------------------------------------ replica exchange
+struct ReArgs {
+  PopPtrs pop;
+  int R, K, Nmax, S, n_cycles, max_pairs;
+  const double* pressure;
+  const double* mu;
+  const double* emax;
+  const int* pair_valid;
+  const double* re_u;
+  int* acc;  // [R-1] (per-block output row or standalone)
+  int* att;
+  IterState* st;  // NULL: always fire (standalone pass)
+  int* o_acc;     // per-block outputs
+  int* o_att;
+};
+
+// Species counts of one walker, counted by the whole block (N <= blockDim).
+__device__ __forceinline__ void block_counts(const PopPtrs& p, size_t slot,
+                                             int Nmax, int S, int* counts) {
+  const int tid = threadIdx.x;
+  const int N = p.n_atoms[slot];
+  const int ty = (tid < N) ? p.types[slot * Nmax + tid] : -1;
+#pragma unroll
+  for (int s = 0; s < JR_MAX_SPECIES; ++s)
+    counts[s] = (s < S) ? __syncthreads_count(ty == s) : 0;
+}
+
+// sampling/moves/replica_exchange.py:318-436 with EnsembleSwap (:120-164).
+__global__ void __launch_bounds__(256) re_kernel(ReArgs a) {
+  __shared__ int s_ok, s_si, s_sj;
+  __shared__ double s_ei, s_ej;
+  const int tid = threadIdx.x;
+  int* acc = a.acc;
+  int* att = a.att;
+  if (a.st) {
+    const long long i = a.st->i;
+    const bool fires = a.st->re_interval > 0 && i > 0 &&
+                       (i % a.st->re_interval) == 0;
+    const int bp = a.st->block_pos;
+    acc = a.o_acc + (size_t)bp * (a.R - 1);
+    att = a.o_att + (size_t)bp * (a.R - 1);
+    for (int k = tid; k < a.R - 1; k += blockDim.x) {
+      acc[k] = 0;
+      att[k] = 0;
+    }
+    __syncthreads();
+    if (!fires) return;
+  } else {
+    for (int k = tid; k < a.R - 1; k += blockDim.x) {
+      acc[k] = 0;
+      att[k] = 0;
+    }
+    __syncthreads();
+  }
+  for (int cyc = 0; cyc < a.n_cycles; ++cyc) {
+    for (int ph = 0; ph < 2; ++ph) {
+      int pi = 0;
+      for (int i = ph; i + 1 < a.R; i += 2, ++pi) {
+        const int j = i + 1;
+        if (!a.pair_valid[i]) continue;
+        if (tid == 0) {
+          const double* u =
+              a.re_u + (((size_t)cyc * 2 + ph) * a.max_pairs + pi) * 2;
+          int wi = (int)(u[0] * (double)a.K);
+          if (wi > a.K - 1) wi = a.K - 1;
+          int wj = (int)(u[1] * (double)a.K);
+          if (wj > a.K - 1) wj = a.K - 1;
+          s_si = (int)((size_t)i * a.K + wi);
+          s_sj = (int)((size_t)j * a.K + wj);
+        }
+        __syncthreads();
+        const size_t si = (size_t)s_si, sj = (size_t)s_sj;
+        int ci[JR_MAX_SPECIES], cj[JR_MAX_SPECIES];
+        block_counts(a.pop, si, a.Nmax, a.S, ci);
+        block_counts(a.pop, sj, a.Nmax, a.S, cj);
+        if (tid == 0) {
+          const double* mui = a.mu + (size_t)i * a.S;
+          const double* muj = a.mu + (size_t)j * a.S;
+          const double* celli = a.pop.cell + si * 9;
+          const double* cellj = a.pop.cell + sj * 9;
+          // rebase_ensemble_energy (backends/ensemble.py:62-87)
+          double ui = a.pop.energy[sj] -
+                      ensemble_term(cellj, a.pressure[j], muj, a.S, cj);
+          double into_i = ui + ensemble_term(cellj, a.pressure[i], mui, a.S, cj);
+          double uj = a.pop.energy[si] -
+                      ensemble_term(celli, a.pressure[i], mui, a.S, ci);
+          double into_j = uj + ensemble_term(celli, a.pressure[j], muj, a.S, ci);
+          int ok = (into_i < a.emax[i]) && (into_j < a.emax[j]);
+          att[i] += 1;
+          acc[i] += ok;
+          s_ok = ok;
+          s_ei = into_i;
+          s_ej = into_j;
+        }
+        __syncthreads();
+        if (s_ok) {
+          const size_t si = (size_t)s_si, sj = (size_t)s_sj;
+          for (int k = tid; k < a.Nmax * 3; k += blockDim.x) {
+            double p0 = a.pop.pos[si * a.Nmax * 3 + k];
+            a.pop.pos[si * a.Nmax * 3 + k] = a.pop.pos[sj * a.Nmax * 3 + k];
+            a.pop.pos[sj * a.Nmax * 3 + k] = p0;
+            double d0 = a.pop.dir[si * a.Nmax * 3 + k];
+            a.pop.dir[si * a.Nmax * 3 + k] = a.pop.dir[sj * a.Nmax * 3 + k];
+            a.pop.dir[sj * a.Nmax * 3 + k] = d0;
+          }
+          for (int k = tid; k < a.Nmax; k += blockDim.x) {
+            int t0 = a.pop.types[si * a.Nmax + k];
+            a.pop.types[si * a.Nmax + k] = a.pop.types[sj * a.Nmax + k];
+            a.pop.types[sj * a.Nmax + k] = t0;
+          }
+          if (tid < 9) {
+            double c0 = a.pop.cell[si * 9 + tid];
+            a.pop.cell[si * 9 + tid] = a.pop.cell[sj * 9 + tid];
+            a.pop.cell[sj * 9 + tid] = c0;
+          }
+          if (tid == 0) {
+            int n0 = a.pop.n_atoms[si];
+            a.pop.n_atoms[si] = a.pop.n_atoms[sj];
+            a.pop.n_atoms[sj] = n0;
+            a.pop.energy[si] = s_ei;
+            a.pop.energy[sj] = s_ej;
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ draws
+// Philox counter = (element-pair index lo, hi, rng iteration, stream).
+__global__ void fill_uniform_kernel(double* out, long long n,
+                                    unsigned long long seed,
+                                    const IterState* st, long long it_fixed,
+                                    unsigned stream) {
+  const long long it = st ? st->rng_iter : it_fixed;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+       2 * p < n; p += (long long)gridDim.x * blockDim.x) {
+    uint32_t c[4] = {(uint32_t)p, (uint32_t)(p >> 32), (uint32_t)it, stream};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    out[2 * p] = u53(c[0], c[1]);
+    if (2 * p + 1 < n) out[2 * p + 1] = u53(c[2], c[3]);
+  }
+}
+
+__global__ void fill_normal_kernel(double* out, long long n,
+                                   unsigned long long seed,
+                                   const IterState* st, long long it_fixed,
+                                   unsigned stream) {
+  const long long it = st ? st->rng_iter : it_fixed;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+       2 * p < n; p += (long long)gridDim.x * blockDim.x) {
+    uint32_t c[4] = {(uint32_t)p, (uint32_t)(p >> 32), (uint32_t)it, stream};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    double u1 = 1.0 - u53(c[0], c[1]);  // (0, 1]
+    double u2 = u53(c[2], c[3]);
+    double rad = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincospi(2.0 * u2, &sn, &cs);
+    out[2 * p] = rad * cs;
+    if (2 * p + 1 < n) out[2 * p + 1] = rad * sn;
+  }
+}
+
+// Shared move schedule: m = first index with u < cum[m] (sampling/mwg.py:243).
+__global__ void schedule_kernel(int* sched, int n_steps,
+                                unsigned long long seed, const IterState* st,
+                                long long it_fixed, unsigned stream,
+                                const double* cum, int n_moves) {
+  const long long it = st ? st->rng_iter : it_fixed;
+  for (int t = threadIdx.x; t < n_steps; t += blockDim.x) {
+    uint32_t c[4] = {(uint32_t)t, 0u, (uint32_t)it, stream};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    double u = u53(c[0], c[1]);
+    int m = 0;
+    while (m < n_moves - 1 && !(u < cum[m])) ++m;
+    sched[t] = m;
+  }
+}
+
+__global__ void advance_kernel(IterState* st) {
+  if (threadIdx.x == 0) {
+    st->i += 1;
+    st->rng_iter += 1;
+    st->block_pos += 1;
+  }
+}
+
+}  // namespace jr

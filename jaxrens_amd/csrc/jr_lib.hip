@@ -10,7 +10,8 @@
 #include <vector>
 
 #include "jr_handle.h"
-#include "jr_kernels.h"
+#include "jr_iter_kernels.h"
+#include "jr_launch.h"
 
 using namespace jr;
 
@@ -37,106 +38,6 @@ using namespace jrh;
 namespace {
 
 int atoms_per_lane(int nmax) { return nmax <= 64 ? 1 : (nmax <= 128 ? 2 : 4); }
-
-template <int A, int CFG, int OCC>
-void launch_walk_t(const WalkArgs& a, int blocks, int wpb, size_t lds,
-                   hipStream_t s) {
-  hipLaunchKernelGGL((walk_kernel<A, CFG, OCC>), dim3(blocks), dim3(64 * wpb),
-                     lds, s, a);
-}
-
-// occupancy variants are built for one atom per lane (N <= 64), without
-// image sums and shape moves only
-template <int CFG>
-void launch_walk_c(int A, int occ, const WalkArgs& a, int blocks, int wpb,
-                   size_t lds, hipStream_t s) {
-  if (A == 1) {
-    if (!(CFG & 28) && occ == 3) launch_walk_t<1, CFG & 3, 3>(a, blocks, wpb, lds, s);
-    else launch_walk_t<1, CFG, 0>(a, blocks, wpb, lds, s);
-  } else if (A == 2) {
-    launch_walk_t<2, CFG, 0>(a, blocks, wpb, lds, s);
-  } else {
-    launch_walk_t<4, CFG, 0>(a, blocks, wpb, lds, s);
-  }
-}
-
-// cfg bit 0: species table (S > 1); bit 1: FP32 pair terms; bit 2: image
-// sums; bit 3: shear, stretch or species_swap in the move table; bit 4:
-// minimum-distance constraint (bits 2-4 FP64 only, bit 4 without bit 2)
-void launch_walk(int A, int cfg, int occ, const WalkArgs& a, int blocks,
-                 int wpb, size_t lds, hipStream_t s) {
-  switch (cfg) {
-    case 0: launch_walk_c<0>(A, occ, a, blocks, wpb, lds, s); break;
-    case 1: launch_walk_c<1>(A, occ, a, blocks, wpb, lds, s); break;
-    case 2: launch_walk_c<2>(A, occ, a, blocks, wpb, lds, s); break;
-    case 3: launch_walk_c<3>(A, occ, a, blocks, wpb, lds, s); break;
-    case 4: launch_walk_c<4>(A, occ, a, blocks, wpb, lds, s); break;
-    case 5: launch_walk_c<5>(A, occ, a, blocks, wpb, lds, s); break;
-    case 8: launch_walk_c<8>(A, occ, a, blocks, wpb, lds, s); break;
-    case 9: launch_walk_c<9>(A, occ, a, blocks, wpb, lds, s); break;
-    case 12: launch_walk_c<12>(A, occ, a, blocks, wpb, lds, s); break;
-    case 13: launch_walk_c<13>(A, occ, a, blocks, wpb, lds, s); break;
-    case 16: launch_walk_c<16>(A, occ, a, blocks, wpb, lds, s); break;
-    case 17: launch_walk_c<17>(A, occ, a, blocks, wpb, lds, s); break;
-    case 24: launch_walk_c<24>(A, occ, a, blocks, wpb, lds, s); break;
-    default: launch_walk_c<25>(A, occ, a, blocks, wpb, lds, s); break;
-  }
-}
-
-template <int CFG>
-const void* walk_fn_c(int A, int occ) {
-  if (A == 1)
-    return (!(CFG & 28) && occ == 3) ? (const void*)walk_kernel<1, CFG & 3, 3>
-                                     : (const void*)walk_kernel<1, CFG, 0>;
-  if (A == 2) return (const void*)walk_kernel<2, CFG, 0>;
-  return (const void*)walk_kernel<4, CFG, 0>;
-}
-
-const void* walk_fn(int A, int cfg, int occ) {
-  switch (cfg) {
-    case 0: return walk_fn_c<0>(A, occ);
-    case 1: return walk_fn_c<1>(A, occ);
-    case 2: return walk_fn_c<2>(A, occ);
-    case 3: return walk_fn_c<3>(A, occ);
-    case 4: return walk_fn_c<4>(A, occ);
-    case 5: return walk_fn_c<5>(A, occ);
-    case 8: return walk_fn_c<8>(A, occ);
-    case 9: return walk_fn_c<9>(A, occ);
-    case 12: return walk_fn_c<12>(A, occ);
-    case 13: return walk_fn_c<13>(A, occ);
-    case 16: return walk_fn_c<16>(A, occ);
-    case 17: return walk_fn_c<17>(A, occ);
-    case 24: return walk_fn_c<24>(A, occ);
-    default: return walk_fn_c<25>(A, occ);
-  }
-}
-
-template <int A, int CFG>
-void launch_eval_t(const EvalArgs& a, hipStream_t s) {
-  const int wpb = 4;
-  size_t lds = wave_lds_bytes(a.Nmax, 0) * wpb + 3 * sizeof(double) * a.S * a.S;
-  int blocks = (a.B + wpb - 1) / wpb;
-  hipLaunchKernelGGL((eval_kernel<A, CFG>), dim3(blocks), dim3(64 * wpb), lds, s,
-                     a);
-}
-
-template <int CFG>
-void launch_eval_c(int A, const EvalArgs& a, hipStream_t s) {
-  if (A == 1) launch_eval_t<1, CFG>(a, s);
-  else if (A == 2) launch_eval_t<2, CFG>(a, s);
-  else launch_eval_t<4, CFG>(a, s);
-}
-
-void launch_eval(int A, int cfg, const EvalArgs& a, hipStream_t s) {
-  switch (cfg) {
-    case 0: launch_eval_c<0>(A, a, s); break;
-    case 1: launch_eval_c<1>(A, a, s); break;
-    case 2: launch_eval_c<2>(A, a, s); break;
-    case 3: launch_eval_c<3>(A, a, s); break;
-    case 4: launch_eval_c<4>(A, a, s); break;
-    default: launch_eval_c<5>(A, a, s); break;
-  }
-}
 
 bool pot_images(const jr_pot* pot) {
   return pot->images[0] > 0 || pot->images[1] > 0 || pot->images[2] > 0;
