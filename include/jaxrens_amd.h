@@ -269,6 +269,11 @@ int jr_set_iteration(jr_handle* h, int64_t i);
 int jr_get_rng_state(jr_handle* h, int64_t* state /* [3] */);
 int jr_set_rng_state(jr_handle* h, const int64_t* state /* [3] */);
 int jr_get_emax(jr_handle* h, double* emax);
+/* Set the resident contour emax [R] (the last iteration's, as a checkpoint
+ * holds it; io/checkpoint.py:347-365 restores NSState.emax).  A new handle
+ * starts at +inf; the next NS iteration replaces it, but trial moves and RE
+ * passes before that iteration read it. */
+int jr_set_emax(jr_handle* h, const double* emax);
 
 /* Run n_iter NS iterations (select -> clone -> walk -> RE).  tape == NULL
  * draws everything on the device (Philox4x32-10).  One host sync at the end. */
@@ -321,6 +326,14 @@ int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
                 int32_t* best_occ, double* best_ms);
 /* Set the walk launch shape directly (waves per block 1..4, occupancy 0|3). */
 int jr_set_launch(jr_handle* h, int32_t waves_per_block, int32_t occupancy);
+
+/* Cap the persistent walk grid (every walk launch: NS iterations, trial
+ * moves, burn-in) at max_blocks blocks; 0 removes the cap (as many blocks
+ * as are resident).  No reference counterpart (the reference leaves launch
+ * shapes to XLA).  With a small grid each wave takes many chains from the
+ * work queue in turn, so tests reach that path at sizes the oracle checks
+ * quickly; results do not depend on the cap. */
+int jr_set_walk_block_cap(jr_handle* h, int32_t max_blocks);
 
 /* Walk-kernel timing accumulated since the last reset (HIP events on the
  * handle's stream): total milliseconds and launch count. */

@@ -199,6 +199,7 @@ SIGNATURES = [
     ("jr_get_rng_state", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("jr_set_rng_state", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("jr_get_emax", C.c_int, [C.c_void_p, _dp]),
+    ("jr_set_emax", C.c_int, [C.c_void_p, _dp]),
     ("jr_ns_iterations", C.c_int,
      [C.c_void_p, C.c_int32, C.POINTER(JrTape), C.POINTER(JrIterOut)]),
     ("jr_ns_iterations_graph", C.c_int,
@@ -214,6 +215,7 @@ SIGNATURES = [
      [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
       C.POINTER(C.c_double)]),
     ("jr_set_launch", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    ("jr_set_walk_block_cap", C.c_int, [C.c_void_p, C.c_int32]),
     ("jr_walk_timing", C.c_int,
      [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]),
     ("jr_get_info", C.c_int,

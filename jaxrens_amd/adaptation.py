@@ -1,7 +1,8 @@
 """Step-size bisection of the outer loop.
 
-``process_rate`` restates ``sampling/adaptation/stepsize_handler.py:35-102``
-(``_process_rate_jax``) and ``adjust_step_size`` the loop of :110-434, with
+``process_rate`` follows the decision rule of
+``sampling/adaptation/stepsize_handler.py:35-102`` (``_process_rate_jax``) and
+``adjust_step_size`` the loop of :110-434, with
 the trial moves run on the device by ``jr_trial_moves`` (one trial move per
 walker sampled with replacement, population untouched).
 """
@@ -17,31 +18,40 @@ FLOOR_THRESHOLD = 1.0e-7
 
 def process_rate(rate, step_size, step_size_prev, rate_prev, min_rate,
                  max_rate, adjust_factor, max_step_size):
-    """Vectorised over replicas.  ``rate_prev < 0`` means no previous rate.
+    """One bisection decision per replica from the acceptance rate measured
+    at ``step_size`` (``rate_prev`` < 0: no earlier round).  The rules, the
+    first that applies wins, fix the adapted step-size sequence:
 
-    Returns ``(new_ss, converged, cap_hit, floor_hit, too_high, too_low)``.
-    """
-    rate = np.asarray(rate, dtype=np.float64)
-    step_size = np.asarray(step_size, dtype=np.float64)
-    step_size_prev = np.asarray(step_size_prev, dtype=np.float64)
-    rate_prev = np.asarray(rate_prev, dtype=np.float64)
-    target = 0.5 * (min_rate + max_rate)
-    in_window = (rate > min_rate) & (rate < max_rate)
-    has_prev = rate_prev >= 0.0
-    brackets = (has_prev & (np.minimum(rate, rate_prev) < min_rate)
-                & (np.maximum(rate, rate_prev) > max_rate))
-    prev_closer = np.abs(rate_prev - target) < np.abs(rate - target)
-    bracket_ss = np.where(prev_closer, step_size_prev, step_size)
-    too_high = rate >= max_rate
-    too_low = rate < min_rate
-    scale = np.where(too_low, 1.0 / adjust_factor,
-                     np.where(too_high, adjust_factor, 1.0))
-    proposed = step_size * scale
-    cap_hit = proposed >= max_step_size
-    floor_hit = proposed <= FLOOR_THRESHOLD
-    adjusted = np.clip(proposed, FLOOR_THRESHOLD, max_step_size)
-    new_ss = np.where(in_window, step_size, np.where(brackets, bracket_ss, adjusted))
-    return new_ss, in_window | brackets, cap_hit, floor_hit, too_high, too_low
+    1. the rate lies strictly inside (min_rate, max_rate): keep the step;
+       converged;
+    2. the current and previous rates straddle the window (the lower one
+       below min_rate, the higher one above max_rate): converged at the step
+       whose rate lies nearer the window centre (the current one on a tie);
+    3. otherwise scale the step by ``adjust_factor`` when the rate is at or
+       above max_rate, by its inverse when below min_rate (unchanged on
+       rate == min_rate) and clip to [FLOOR_THRESHOLD, max_step_size].
+
+    Flags: cap_hit / floor_hit when the unclipped step of rule 3 reaches the
+    cap / the floor, too_high = rate >= max_rate, too_low = rate < min_rate.
+    Returns ``(new_step, converged, cap_hit, floor_hit, too_high, too_low)``,
+    vectorised over replicas."""
+    r = np.asarray(rate, dtype=np.float64)
+    s_now = np.asarray(step_size, dtype=np.float64)
+    s_old = np.asarray(step_size_prev, dtype=np.float64)
+    r_old = np.asarray(rate_prev, dtype=np.float64)
+    centre = 0.5 * (min_rate + max_rate)
+    too_high, too_low = r >= max_rate, r < min_rate
+    inside = (r > min_rate) & (r < max_rate)
+    lo_rate, hi_rate = np.minimum(r, r_old), np.maximum(r, r_old)
+    straddle = (r_old >= 0.0) & (lo_rate < min_rate) & (hi_rate > max_rate)
+    nearer_old = np.abs(r_old - centre) < np.abs(r - centre)
+    factor = np.where(too_low, 1.0 / adjust_factor, np.where(too_high, adjust_factor, 1.0))
+    scaled = s_now * factor
+    new_step = np.select([inside, straddle],
+                         [s_now, np.where(nearer_old, s_old, s_now)],
+                         default=np.clip(scaled, FLOOR_THRESHOLD, max_step_size))
+    return (new_step, inside | straddle, scaled >= max_step_size,
+            scaled <= FLOOR_THRESHOLD, too_high, too_low)
 
 
 @dataclass

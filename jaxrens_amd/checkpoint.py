@@ -102,6 +102,9 @@ def restore(engine, state: dict) -> np.random.Generator | None:
                           step_sizes=state["step_sizes"])
     engine.iteration = state["iteration"]
     engine.log_evidence = _lib.f64(state["log_evidence"]).copy()
+    # the contour: the first adaptation after resume reads it (as the
+    # reference restores NSState.emax, io/checkpoint.py:347-365)
+    engine.set_emax(state["emax"])
     set_rng_state(engine, state["rng_state"])
     if "host_rng" in state:
         g = np.random.default_rng()

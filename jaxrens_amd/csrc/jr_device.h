@@ -302,4 +302,61 @@ __host__ __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
   return (double)v * 0x1.0p-53;
 }
 
+// Philox block `q` of a draw stream: counter (q lo, q hi, it, stream) under
+// the key (seed lo, seed hi).
+__device__ __forceinline__ void philox_block(unsigned long long seed, long long it,
+                                             unsigned stream, long long q, uint32_t c[4]) {
+  c[0] = (uint32_t)q;
+  c[1] = (uint32_t)((unsigned long long)q >> 32);
+  c[2] = (uint32_t)it;
+  c[3] = stream;
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// Standard normals, four per Philox block: normals 4q .. 4q+3 of a stream
+// come from block q by the Box-Muller transform of its word pairs
+// (w0, w1) -> normals 4q, 4q+1 and (w2, w3) -> 4q+2, 4q+3:
+//   u = (w + 1/2) 2^-32 in (0, 1],  r = sqrt(-2 ln u_a),
+//   z_even = r cos(2 pi u_b),  z_odd = r sin(2 pi u_b),
+// in single precision with the hardware transcendentals (v_log_f32 is log2,
+// v_sin_f32 / v_cos_f32 take the angle in revolutions), held as FP64 values.
+// |z| <= sqrt(2 ln 2 * 33) = 6.76; u rounding to 1 gives z = 0.  These are
+// proposal draws; every energy and acceptance test stays FP64.
+__device__ __forceinline__ void box_muller(uint32_t wa, uint32_t wb, double& z_cos,
+                                           double& z_sin) {
+  const float ua = fmaf((float)wa, 0x1.0p-32f, 0x1.0p-33f);
+  const float ub = fmaf((float)wb, 0x1.0p-32f, 0x1.0p-33f);
+  const float rad = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(ua));
+  z_cos = (double)(rad * __builtin_amdgcn_cosf(ub));
+  z_sin = (double)(rad * __builtin_amdgcn_sinf(ub));
+}
+
+__device__ __forceinline__ void normal_quad(unsigned long long seed, long long it,
+                                            unsigned stream, long long q, double z[4]) {
+  uint32_t c[4];
+  philox_block(seed, it, stream, q, c);
+  box_muller(c[0], c[1], z[0], z[1]);
+  box_muller(c[2], c[3], z[2], z[3]);
+}
+
+// The single normal e of a stream (one Box-Muller pair of block e / 4).
+__device__ __forceinline__ double normal_at(unsigned long long seed, long long it,
+                                            unsigned stream, long long e) {
+  uint32_t c[4];
+  philox_block(seed, it, stream, e >> 2, c);
+  const bool hi = (e & 2) != 0;
+  double zc, zs;
+  box_muller(hi ? c[2] : c[0], hi ? c[3] : c[1], zc, zs);
+  return (e & 1) ? zs : zc;
+}
+
+// Uniform e of a stream: block e / 2 gives u53(w0, w1) for even e and
+// u53(w2, w3) for odd e.
+__device__ __forceinline__ double uniform_at(unsigned long long seed, long long it,
+                                             unsigned stream, long long e) {
+  uint32_t c[4];
+  philox_block(seed, it, stream, e >> 1, c);
+  return (e & 1) ? u53(c[2], c[3]) : u53(c[0], c[1]);
+}
+
 }  // namespace jr

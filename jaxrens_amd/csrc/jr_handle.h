@@ -60,6 +60,12 @@ struct jr_handle {
   // iteration work
   jrh::DBuf<int> walk, counters, queue, pair_valid, order, sched;
   jrh::DBuf<double> sel_u, clone_u, extra_u, normals, uniforms, re_u;
+  // draws_buffered: normals/uniforms hold the current iteration's draws (a
+  // host tape, or the learned walk's device draws).  Otherwise the pair walk
+  // draws them in place and the buffers are filled only when
+  // jr_get_last_tape asks for the draws of iteration last_draw_iter.
+  bool draws_buffered = false;
+  long long last_draw_iter = 0;
   jrh::DBuf<jr::IterState> st;
   // per-block outputs
   int out_cap = 0;
@@ -88,6 +94,7 @@ struct jr_handle {
   int wpb = 4;
   int blocks = 0;
   int max_blocks = 0;  // resident blocks of the walk kernel at this geometry
+  int block_cap = 0;   // jr_set_walk_block_cap (0: none)
   size_t lds_walk = 0;
   long long iter_i = 0;
   long long rng_iter = 0;

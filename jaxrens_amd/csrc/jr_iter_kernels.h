@@ -312,28 +312,35 @@ struct ReArgs {
   int* o_att;
 };
 
-// Species counts of one walker, counted by the whole block (N <= blockDim).
-__device__ __forceinline__ void block_counts(const PopPtrs& p, size_t slot,
-                                             int Nmax, int S, int* counts) {
-  const int tid = threadIdx.x;
+// Species counts of one walker by wave ballots over its atoms.
+__device__ __forceinline__ void wave_counts(const PopPtrs& p, size_t slot, int Nmax, int S,
+                                            int* counts) {
+  const int lane = lane_id();
   const int N = p.n_atoms[slot];
-  const int ty = (tid < N) ? p.types[slot * Nmax + tid] : -1;
 #pragma unroll
-  for (int s = 0; s < JR_MAX_SPECIES; ++s)
-    counts[s] = (s < S) ? __syncthreads_count(ty == s) : 0;
+  for (int s = 0; s < JR_MAX_SPECIES; ++s) counts[s] = 0;
+  for (int base = 0; base < N; base += 64) {
+    const int at = base + lane;
+    const int ty = at < N ? p.types[slot * Nmax + at] : -1;
+#pragma unroll
+    for (int s = 0; s < JR_MAX_SPECIES; ++s)
+      if (s < S) counts[s] += __popcll(__ballot(ty == s));
+  }
 }
 
 // sampling/moves/replica_exchange.py:318-436 with EnsembleSwap (:120-164).
-__global__ void __launch_bounds__(256) re_kernel(ReArgs a) {
-  __shared__ int s_ok, s_si, s_sj;
-  __shared__ double s_ei, s_ej;
-  const int tid = threadIdx.x;
+// Per cycle the even pairs (0,1), (2,3), .. then the odd pairs; the pairs of
+// one phase are disjoint, so each wave of the block takes pairs of the phase
+// in parallel (pair pi of a phase is replicas (ph + 2 pi, ph + 2 pi + 1), its
+// draws re_u[cycle][ph][pi][2]) and the block synchronises between phases.
+__global__ void __launch_bounds__(1024) re_kernel(ReArgs a) {
+  const int tid = threadIdx.x, lane = lane_id();
+  const int wave = tid >> 6, n_waves = blockDim.x >> 6;
   int* acc = a.acc;
   int* att = a.att;
   if (a.st) {
     const long long i = a.st->i;
-    const bool fires = a.st->re_interval > 0 && i > 0 &&
-                       (i % a.st->re_interval) == 0;
+    const bool fires = a.st->re_interval > 0 && i > 0 && (i % a.st->re_interval) == 0;
     const int bp = a.st->block_pos;
     acc = a.o_acc + (size_t)bp * (a.R - 1);
     att = a.o_att + (size_t)bp * (a.R - 1);
@@ -341,86 +348,70 @@ __global__ void __launch_bounds__(256) re_kernel(ReArgs a) {
       acc[k] = 0;
       att[k] = 0;
     }
-    __syncthreads();
-    if (!fires) return;
+    if (!fires) return;  // block-uniform
   } else {
     for (int k = tid; k < a.R - 1; k += blockDim.x) {
       acc[k] = 0;
       att[k] = 0;
     }
-    __syncthreads();
   }
+  __syncthreads();
+  const size_t n3 = (size_t)a.Nmax * 3;
   for (int cyc = 0; cyc < a.n_cycles; ++cyc) {
     for (int ph = 0; ph < 2; ++ph) {
-      int pi = 0;
-      for (int i = ph; i + 1 < a.R; i += 2, ++pi) {
-        const int j = i + 1;
-        if (!a.pair_valid[i]) continue;
-        if (tid == 0) {
-          const double* u =
-              a.re_u + (((size_t)cyc * 2 + ph) * a.max_pairs + pi) * 2;
-          int wi = (int)(u[0] * (double)a.K);
-          if (wi > a.K - 1) wi = a.K - 1;
-          int wj = (int)(u[1] * (double)a.K);
-          if (wj > a.K - 1) wj = a.K - 1;
-          s_si = (int)((size_t)i * a.K + wi);
-          s_sj = (int)((size_t)j * a.K + wj);
-        }
-        __syncthreads();
-        const size_t si = (size_t)s_si, sj = (size_t)s_sj;
+      for (int pi = wave; ph + 2 * pi + 1 < a.R; pi += n_waves) {
+        const int i = ph + 2 * pi, j = i + 1;
+        if (!a.pair_valid[i]) continue;  // wave-uniform
+        const double* u = a.re_u + (((size_t)cyc * 2 + ph) * a.max_pairs + pi) * 2;
+        const int wi = min((int)(u[0] * (double)a.K), a.K - 1);
+        const int wj = min((int)(u[1] * (double)a.K), a.K - 1);
+        const size_t si = (size_t)i * a.K + wi, sj = (size_t)j * a.K + wj;
         int ci[JR_MAX_SPECIES], cj[JR_MAX_SPECIES];
-        block_counts(a.pop, si, a.Nmax, a.S, ci);
-        block_counts(a.pop, sj, a.Nmax, a.S, cj);
-        if (tid == 0) {
-          const double* mui = a.mu + (size_t)i * a.S;
-          const double* muj = a.mu + (size_t)j * a.S;
-          const double* celli = a.pop.cell + si * 9;
-          const double* cellj = a.pop.cell + sj * 9;
-          // rebase_ensemble_energy (backends/ensemble.py:62-87)
-          double ui = a.pop.energy[sj] -
-                      ensemble_term(cellj, a.pressure[j], muj, a.S, cj);
-          double into_i = ui + ensemble_term(cellj, a.pressure[i], mui, a.S, cj);
-          double uj = a.pop.energy[si] -
-                      ensemble_term(celli, a.pressure[i], mui, a.S, ci);
-          double into_j = uj + ensemble_term(celli, a.pressure[j], muj, a.S, ci);
-          int ok = (into_i < a.emax[i]) && (into_j < a.emax[j]);
+        wave_counts(a.pop, si, a.Nmax, a.S, ci);
+        wave_counts(a.pop, sj, a.Nmax, a.S, cj);
+        const double* mui = a.mu + (size_t)i * a.S;
+        const double* muj = a.mu + (size_t)j * a.S;
+        const double* celli = a.pop.cell + si * 9;
+        const double* cellj = a.pop.cell + sj * 9;
+        // rebase_ensemble_energy (backends/ensemble.py:62-87)
+        const double ui = a.pop.energy[sj] - ensemble_term(cellj, a.pressure[j], muj, a.S, cj);
+        const double into_i = ui + ensemble_term(cellj, a.pressure[i], mui, a.S, cj);
+        const double uj = a.pop.energy[si] - ensemble_term(celli, a.pressure[i], mui, a.S, ci);
+        const double into_j = uj + ensemble_term(celli, a.pressure[j], muj, a.S, ci);
+        const bool ok = (into_i < a.emax[i]) && (into_j < a.emax[j]);
+        if (lane == 0) {
           att[i] += 1;
-          acc[i] += ok;
-          s_ok = ok;
-          s_ei = into_i;
-          s_ej = into_j;
+          acc[i] += ok ? 1 : 0;
         }
-        __syncthreads();
-        if (s_ok) {
-          const size_t si = (size_t)s_si, sj = (size_t)s_sj;
-          for (int k = tid; k < a.Nmax * 3; k += blockDim.x) {
-            double p0 = a.pop.pos[si * a.Nmax * 3 + k];
-            a.pop.pos[si * a.Nmax * 3 + k] = a.pop.pos[sj * a.Nmax * 3 + k];
-            a.pop.pos[sj * a.Nmax * 3 + k] = p0;
-            double d0 = a.pop.dir[si * a.Nmax * 3 + k];
-            a.pop.dir[si * a.Nmax * 3 + k] = a.pop.dir[sj * a.Nmax * 3 + k];
-            a.pop.dir[sj * a.Nmax * 3 + k] = d0;
+        if (ok) {  // wave-uniform
+          for (size_t k = lane; k < n3; k += 64) {
+            const double p0 = a.pop.pos[si * n3 + k], p1 = a.pop.pos[sj * n3 + k];
+            const double d0 = a.pop.dir[si * n3 + k], d1 = a.pop.dir[sj * n3 + k];
+            a.pop.pos[si * n3 + k] = p1;
+            a.pop.pos[sj * n3 + k] = p0;
+            a.pop.dir[si * n3 + k] = d1;
+            a.pop.dir[sj * n3 + k] = d0;
           }
-          for (int k = tid; k < a.Nmax; k += blockDim.x) {
-            int t0 = a.pop.types[si * a.Nmax + k];
-            a.pop.types[si * a.Nmax + k] = a.pop.types[sj * a.Nmax + k];
+          for (int k = lane; k < a.Nmax; k += 64) {
+            const int t0 = a.pop.types[si * a.Nmax + k], t1 = a.pop.types[sj * a.Nmax + k];
+            a.pop.types[si * a.Nmax + k] = t1;
             a.pop.types[sj * a.Nmax + k] = t0;
           }
-          if (tid < 9) {
-            double c0 = a.pop.cell[si * 9 + tid];
-            a.pop.cell[si * 9 + tid] = a.pop.cell[sj * 9 + tid];
-            a.pop.cell[sj * 9 + tid] = c0;
+          if (lane < 9) {
+            const double c0 = a.pop.cell[si * 9 + lane], c1 = a.pop.cell[sj * 9 + lane];
+            a.pop.cell[si * 9 + lane] = c1;
+            a.pop.cell[sj * 9 + lane] = c0;
           }
-          if (tid == 0) {
-            int n0 = a.pop.n_atoms[si];
+          if (lane == 0) {
+            const int n0 = a.pop.n_atoms[si];
             a.pop.n_atoms[si] = a.pop.n_atoms[sj];
             a.pop.n_atoms[sj] = n0;
-            a.pop.energy[si] = s_ei;
-            a.pop.energy[sj] = s_ej;
+            a.pop.energy[si] = into_i;
+            a.pop.energy[sj] = into_j;
           }
         }
-        __syncthreads();
       }
+      __syncthreads();
     }
   }
 }
@@ -441,22 +432,20 @@ __global__ void fill_uniform_kernel(double* out, long long n,
   }
 }
 
+// Normals of a stream in the four-per-block layout of normal_quad (the
+// same values the walk kernel draws in place); out[e], e < n.
 __global__ void fill_normal_kernel(double* out, long long n,
                                    unsigned long long seed,
                                    const IterState* st, long long it_fixed,
                                    unsigned stream) {
   const long long it = st ? st->rng_iter : it_fixed;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-       2 * p < n; p += (long long)gridDim.x * blockDim.x) {
-    uint32_t c[4] = {(uint32_t)p, (uint32_t)(p >> 32), (uint32_t)it, stream};
-    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-    double u1 = 1.0 - u53(c[0], c[1]);  // (0, 1]
-    double u2 = u53(c[2], c[3]);
-    double rad = sqrt(-2.0 * log(u1));
-    double sn, cs;
-    sincospi(2.0 * u2, &sn, &cs);
-    out[2 * p] = rad * cs;
-    if (2 * p + 1 < n) out[2 * p + 1] = rad * sn;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+       4 * q < n; q += (long long)gridDim.x * blockDim.x) {
+    double z[4];
+    normal_quad(seed, it, stream, q, z);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (4 * q + j < n) out[4 * q + j] = z[j];
   }
 }
 

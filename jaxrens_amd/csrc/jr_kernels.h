@@ -41,7 +41,61 @@ struct WalkArgs {
   // atoms or the cell, and for moves that change species; NULL = not gated
   const double* dmin_pos;
   const double* dmin_types;
+  // normals == NULL: the walk draws its normals and uniforms in place, the
+  // values the fill kernels would write into the tape layout (Philox streams
+  // 5 and 6 of iteration st->rng_iter under seed; see step_draws)
+  const IterState* st;
+  unsigned long long seed;
 };
+
+// Draws of sub-step t of chain tq in place of the tape (normals
+// [tq][n_steps][3][Nmax] of stream 5 in the four-per-block layout,
+// uniforms [tq][n_steps][4] of stream 6).  gmc and hmc: the walker's normals
+// go to the wave's LDS rows hx..hz (free until the move's first energy
+// evaluation) as xs[c * 64A + atom]; other moves: the normals [0][0] and
+// [1][0] and the four uniforms, wave-uniform, from lanes 0..5 of one pass.
+template <int A>
+__device__ __forceinline__ void step_draws(const WalkArgs& a, long long it, int tq, int t,
+                                           int N, bool full, double* xs, double& xv0,
+                                           double& xv1, double (&u)[4]) {
+  constexpr int M = 64 * A;
+  const int lane = lane_id();
+  const long long blk = (long long)tq * a.n_steps + t;
+  const long long E0 = blk * 3 * (long long)a.Nmax;
+  if (full) {
+    const long long q0 = E0 >> 2;
+    const int nq = (int)(((E0 + 3LL * a.Nmax - 1) >> 2) - q0 + 1);
+    const int n3 = 3 * a.Nmax;
+    for (int i = lane; i < nq; i += 64) {
+      double z[4];
+      normal_quad(a.seed, it, 5u, q0 + i, z);
+      const int e0 = (int)(4 * (q0 + i) - E0);  // in [-3, 3 Nmax + 3)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int e = e0 + j;
+        const int c = (e >= a.Nmax) + (e >= 2 * a.Nmax);
+        const int at = e - c * a.Nmax;
+        if (e >= 0 && e < n3 && at < N) xs[c * M + at] = z[j];
+      }
+    }
+    wave_lds_sync();
+  } else {
+    const int l = lane & 7;
+    const bool nrm = l < 2;
+    const long long e = nrm ? E0 + (long long)l * a.Nmax : blk * 4 + (l - 2);
+    uint32_t c[4];
+    philox_block(a.seed, it, nrm ? 5u : 6u, nrm ? (e >> 2) : (e >> 1), c);
+    const bool hi = nrm ? (e & 2) != 0 : (e & 1) != 0;
+    const uint32_t w0 = hi ? c[2] : c[0], w1 = hi ? c[3] : c[1];
+    double zc, zs;
+    box_muller(w0, w1, zc, zs);
+    const double v = nrm ? ((e & 1) ? zs : zc) : u53(w0, w1);
+    xv0 = readlane_d(v, 0);
+    xv1 = readlane_d(v, 1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) u[j] = readlane_d(v, 2 + j);
+  }
+}
 
 __host__ __device__ inline size_t wave_lds_bytes(int Nmax, int n_moves) {
   (void)n_moves;
@@ -88,7 +142,9 @@ __global__ void __launch_bounds__(256)
   C.tab.img[0] = a.img[0];
   C.tab.img[1] = a.img[1];
   C.tab.img[2] = a.img[2];
-  C.Nmax = a.Nmax;
+  const bool gen = a.normals == nullptr;
+  const long long rng_it = gen ? a.st->rng_iter : 0;
+  C.xi_stride = gen ? 64 * A : a.Nmax;
   C.S = a.S;
 
   for (;;) {
@@ -151,14 +207,26 @@ __global__ void __launch_bounds__(256)
       const DevMove mv = a.moves[m];
       const double step =
           a.trial ? a.trial_step[r] : a.step_sizes[(size_t)r * a.n_moves + m];
-      const double* xi =
-          a.normals + ((size_t)tq * n_steps + t) * 3 * (size_t)a.Nmax;
-      const double* u = a.uniforms + ((size_t)tq * n_steps + t) * 4;
       int reason = 0, n_eval = 1, n_grad = 0;
       int type = mv.type;
       if constexpr ((CFG & 8) != 0) {
         if (type == JR_MOVE_SHEAR || type == JR_MOVE_STRETCH) type = JR_MOVE_VOLUME;
         if (type == JR_MOVE_SWAP) type = JR_MOVE_MORPH;
+      }
+      const double* xi;
+      const double* u;
+      double xv0, xv1, ud[4];
+      if (!gen) {
+        xi = a.normals + ((size_t)tq * n_steps + t) * 3 * (size_t)a.Nmax;
+        u = a.uniforms + ((size_t)tq * n_steps + t) * 4;
+        xv0 = xi[0];
+        xv1 = xi[a.Nmax];
+      } else {
+        double* xs = LdsView<A>{L.base}.hx();
+        step_draws<A>(a, rng_it, tq, t, N, type == JR_MOVE_GMC || type == JR_MOVE_HMC, xs,
+                      xv0, xv1, ud);
+        xi = xs;
+        u = ud;
       }
       // configuration constraint gate (sampling/mwg.py:98-141)
       [[maybe_unused]] const double* floors = nullptr;
@@ -174,9 +242,9 @@ __global__ void __launch_bounds__(256)
           break;
         case JR_MOVE_VOLUME:  // with cfg bit 3: also shear and stretch
           if constexpr ((CFG & 8) != 0)
-            reason = cell_move<A, CFG>(C, W, step, mv, xi[0], xi[a.Nmax], u[0]);
+            reason = cell_move<A, CFG>(C, W, step, mv, xv0, xv1, u[0]);
           else
-            reason = volume_move<A, CFG>(C, W, step, mv, xi[0], u[0]);
+            reason = volume_move<A, CFG>(C, W, step, mv, xv0, u[0]);
           break;
         case JR_MOVE_HMC:
           reason = hmc_move<A, CFG>(C, W, step, mv, xi) ? 0 : 1;

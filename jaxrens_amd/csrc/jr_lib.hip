@@ -311,6 +311,13 @@ WalkArgs walk_args(jr_handle* h) {
   set_images(a.img, c.pot);
   a.moves = h->moves.p;
   a.lds_atoms = c.n_atoms_max;
+  // device draws: the walk draws its normals and uniforms in place
+  a.st = h->st.p;
+  a.seed = c.seed;
+  if (!h->draws_buffered) {
+    a.normals = nullptr;
+    a.uniforms = nullptr;
+  }
   if (h->cfgbits & 16) {
     const int SS = c.pot.n_species * c.pot.n_species;
     a.dmin_pos = h->dmin.p;
@@ -386,6 +393,11 @@ ReArgs re_args(jr_handle* h, IterState* st) {
   return a;
 }
 
+// RE block: one wave per pair of a phase, at most 16 waves
+unsigned re_threads(const jr_handle* h) {
+  return 64u * (unsigned)std::min(std::max(h->max_pairs, 1), 16);
+}
+
 // extra_u feeds the extras' noise (n_cull = 1) or the survivors' permutation
 // keys (n_cull > 1)
 bool uses_extra_u(const jr_config& c) { return c.n_extra > 0 || c.n_cull > 1; }
@@ -411,8 +423,11 @@ unsigned grid_for(size_t n) {
 }
 
 // Device draws for the current iteration (Philox keyed by seed, counter
-// (index, rng_iter, stream)).
-void enqueue_draws(jr_handle* h) {
+// (index, rng_iter, stream)): selection, clone, extras, schedule and RE
+// draws into their buffers; `it` is the iteration's draw counter, kept for
+// jr_get_last_tape.
+void enqueue_draws(jr_handle* h, long long it) {
+  h->last_draw_iter = it;
   const auto& c = h->cfg;
   const int R = c.n_replicas, K = c.n_walkers;
   hipStream_t s = h->stream;
@@ -427,10 +442,17 @@ void enqueue_draws(jr_handle* h) {
                        h->extra_u.p, (long long)R * K, seed, st, 0LL, 3u);
   hipLaunchKernelGGL(schedule_kernel, dim3(1), dim3(64), 0, s, h->sched.p,
                      c.n_mcmc_steps, seed, st, 0LL, 4u, h->cum.p, c.n_moves);
-  hipLaunchKernelGGL(fill_normal_kernel, dim3(grid_for(n_normals(h))), dim3(256), 0, s,
-                     h->normals.p, (long long)n_normals(h), seed, st, 0LL, 5u);
-  hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(n_uniforms(h))), dim3(256), 0, s,
-                     h->uniforms.p, (long long)n_uniforms(h), seed, st, 0LL, 6u);
+  if (h->learned) {
+    // the learned walk's kernels read the draws from the tape buffers
+    hipLaunchKernelGGL(fill_normal_kernel, dim3(grid_for(n_normals(h))), dim3(256), 0, s,
+                       h->normals.p, (long long)n_normals(h), seed, st, 0LL, 5u);
+    hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(n_uniforms(h))), dim3(256), 0, s,
+                       h->uniforms.p, (long long)n_uniforms(h), seed, st, 0LL, 6u);
+    h->draws_buffered = true;
+  } else {
+    // the pair walk draws its normals (stream 5) and uniforms (stream 6) in place
+    h->draws_buffered = false;
+  }
   if (c.n_replicas > 1)
     hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(n_re_u(h))), dim3(256), 0, s,
                        h->re_u.p, (long long)n_re_u(h), seed, st, 0LL, 7u);
@@ -482,7 +504,7 @@ int enqueue_iteration_core(jr_handle* h, bool timed, int ev_idx) {
   if (timed) (void)hipEventRecord(h->ev[2 * ev_idx + 1], h->stream);
   if (c.n_replicas > 1) {
     ReArgs ra = re_args(h, h->st.p);
-    hipLaunchKernelGGL(re_kernel, dim3(1), dim3(256), 0, h->stream, ra);
+    hipLaunchKernelGGL(re_kernel, dim3(1), dim3(re_threads(h)), 0, h->stream, ra);
   }
   hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, h->stream, h->st.p);
   return JR_OK;
@@ -524,6 +546,7 @@ int set_geometry(jr_handle* h, int wpb, int occ) {
     if (pc < 1) pc = 1;
     const int need = (k.n_rep * h->n_walk + wpb - 1) / wpb;
     k.blocks = std::max(1, std::min(need, pc * h->n_cu));
+    if (h->block_cap > 0) k.blocks = std::min(k.blocks, h->block_cap);
   }
   h->lds_walk = wave_lds_bytes(c.n_atoms_max, c.n_moves) * wpb +
                 3 * sizeof(double) * c.pot.n_species * c.pot.n_species;
@@ -535,6 +558,7 @@ int set_geometry(jr_handle* h, int wpb, int occ) {
   const int total = c.n_replicas * h->n_walk;
   const int need = (total + wpb - 1) / wpb;
   h->max_blocks = per_cu * h->n_cu;
+  if (h->block_cap > 0) h->max_blocks = std::min(h->max_blocks, h->block_cap);
   h->blocks = std::max(1, std::min(need, h->max_blocks));
   if (h->graph_exec) {
     (void)hipGraphExecDestroy(h->graph_exec);
@@ -722,10 +746,12 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   TRY(h->sel_u.alloc(W));
   TRY(h->clone_u.alloc(R));
   TRY(h->extra_u.alloc(W));
-  TRY(h->normals.alloc(n_normals(h)));
-  TRY(h->uniforms.alloc(n_uniforms(h)));
   TRY(h->re_u.alloc(n_re_u(h)));
   TRY(h->st.alloc(1));
+  if (c.mpnn || c.learned) {
+    TRY(h->normals.alloc(n_normals(h)));
+    TRY(h->uniforms.alloc(n_uniforms(h)));
+  }
 
   if (!c.mpnn) {
     TRY(hipMemcpy(h->eps4.p, c.pot.eps4, sizeof(double) * S * S, hipMemcpyHostToDevice));
@@ -936,6 +962,13 @@ int jr_set_rng_state(jr_handle* h, const int64_t* state) {
   return JR_OK;
 }
 
+int jr_set_emax(jr_handle* h, const double* emax) {
+  if (!h || !emax) return set_global_err(JR_E_ARG, "NULL argument");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  HIPCHK(h, hipMemcpy(h->emax.p, emax, sizeof(double) * h->cfg.n_replicas, hipMemcpyHostToDevice));
+  return JR_OK;
+}
+
 int jr_get_emax(jr_handle* h, double* emax) {
   if (!h || !emax) return set_global_err(JR_E_ARG, "NULL argument");
   HIPCHK(h, hipSetDevice(h->cfg.device));
@@ -1007,6 +1040,9 @@ int upload_tape(jr_handle* h, const jr_tape* t, int it) {
     if (m < 0 || m >= c.n_moves) return h->fail(JR_E_ARG, "tape: move index %d out of range", m);
   }
   HIPCHK(h, h2d(h->sched.p, t->schedule + (size_t)it * c.n_mcmc_steps, sizeof(int) * c.n_mcmc_steps));
+  if (h->normals.n < n_normals(h)) HIPCHK(h, h->normals.alloc(n_normals(h)));
+  if (h->uniforms.n < n_uniforms(h)) HIPCHK(h, h->uniforms.alloc(n_uniforms(h)));
+  h->draws_buffered = true;
   HIPCHK(h, h2d(h->normals.p, t->normals + (size_t)it * n_normals(h), sizeof(double) * n_normals(h)));
   HIPCHK(h, h2d(h->uniforms.p, t->uniforms + (size_t)it * n_uniforms(h), sizeof(double) * n_uniforms(h)));
   if (t->re_u && R > 1)
@@ -1049,7 +1085,7 @@ int jr_ns_iterations(jr_handle* h, int32_t n_iter, const jr_tape* tape,
       rc = upload_tape(h, tape, it);
       if (rc) return rc;
     } else {
-      enqueue_draws(h);
+      enqueue_draws(h, h->rng_iter + it);
     }
     rc = enqueue_iteration_core(h, true, it);
     if (rc) return rc;
@@ -1073,12 +1109,14 @@ int jr_ns_iterations_graph(jr_handle* h, int32_t n_iter, jr_iter_out* out) {
   if (rc) return rc;
   if (!h->graph_exec) {
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_draws(h);
+    enqueue_draws(h, h->rng_iter);
     (void)enqueue_iteration_core(h, false, 0);
     hipError_t e = hipStreamEndCapture(h->stream, &h->graph);
     if (e != hipSuccess) return h->fail(JR_E_HIP, "graph capture: %s", hipGetErrorString(e));
     HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
   }
+  h->last_draw_iter = h->rng_iter + n_iter - 1;
+  h->draws_buffered = false;
   for (int it = 0; it < n_iter; ++it) {
     HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
     rc = snapshot_iter(h, it);
@@ -1102,6 +1140,20 @@ int jr_get_last_tape(jr_handle* h, double* sel_u, double* clone_u,
   if (clone_u) HIPCHK(h, d2h(clone_u, h->clone_u.p, sizeof(double) * R));
   if (extra_u && uses_extra_u(c)) HIPCHK(h, d2h(extra_u, h->extra_u.p, sizeof(double) * R * K));
   if (schedule) HIPCHK(h, d2h(schedule, h->sched.p, sizeof(int) * c.n_mcmc_steps));
+  if ((normals || uniforms) && !h->draws_buffered) {
+    // the walk drew them in place: write the same values into the tape
+    // buffers (allocated on first use)
+    if (h->normals.n < n_normals(h)) HIPCHK(h, h->normals.alloc(n_normals(h)));
+    if (h->uniforms.n < n_uniforms(h)) HIPCHK(h, h->uniforms.alloc(n_uniforms(h)));
+    const unsigned long long seed = c.seed;
+    hipLaunchKernelGGL(fill_normal_kernel, dim3(grid_for(n_normals(h))), dim3(256), 0, s,
+                       h->normals.p, (long long)n_normals(h), seed, (const IterState*)nullptr,
+                       h->last_draw_iter, 5u);
+    hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(n_uniforms(h))), dim3(256), 0, s,
+                       h->uniforms.p, (long long)n_uniforms(h), seed,
+                       (const IterState*)nullptr, h->last_draw_iter, 6u);
+    HIPCHK(h, hipGetLastError());
+  }
   if (normals) HIPCHK(h, d2h(normals, h->normals.p, sizeof(double) * n_normals(h)));
   if (uniforms) HIPCHK(h, d2h(uniforms, h->uniforms.p, sizeof(double) * n_uniforms(h)));
   if (re_u && R > 1) HIPCHK(h, d2h(re_u, h->re_u.p, sizeof(double) * n_re_u(h)));
@@ -1130,7 +1182,7 @@ int jr_replica_exchange(jr_handle* h, const double* re_u, int32_t* acc,
   ReArgs ra = re_args(h, nullptr);
   ra.acc = dacc.p;
   ra.att = datt.p;
-  hipLaunchKernelGGL(re_kernel, dim3(1), dim3(256), 0, h->stream, ra);
+  hipLaunchKernelGGL(re_kernel, dim3(1), dim3(re_threads(h)), 0, h->stream, ra);
   HIPCHK(h, hipGetLastError());
   if (acc) HIPCHK(h, hipMemcpyAsync(acc, dacc.p, sizeof(int) * (R - 1), hipMemcpyDeviceToHost, h->stream));
   if (att) HIPCHK(h, hipMemcpyAsync(att, datt.p, sizeof(int) * (R - 1), hipMemcpyDeviceToHost, h->stream));
@@ -1350,7 +1402,7 @@ int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
   // one iteration's draws and selection, then the post-selection state
   int rc = begin_block(h, 1);
   if (rc) return rc;
-  enqueue_draws(h);
+  enqueue_draws(h, h->rng_iter);
   launch_select(h, s);
   HIPCHK(h, copy_pop(t_pos.p, t_dir.p, t_cell.p, t_e.p, t_t.p, t_n.p, h->pos.p, h->dir.p,
                      h->cell.p, h->energy.p, h->types.p, h->n_atoms.p));
@@ -1412,6 +1464,16 @@ int jr_set_launch(jr_handle* h, int32_t waves_per_block, int32_t occupancy) {
     return h->fail(JR_E_ARG, "occupancy variant must be 0 or 3");
   HIPCHK(h, hipSetDevice(h->cfg.device));
   return set_geometry(h, waves_per_block, occupancy);
+}
+
+int jr_set_walk_block_cap(jr_handle* h, int32_t max_blocks) {
+  if (!h) return set_global_err(JR_E_ARG, "NULL handle");
+  if (h->learned)
+    return h->fail(JR_E_UNSUPPORTED, "%s: pair-potential walk only", __func__);
+  if (max_blocks < 0) return h->fail(JR_E_ARG, "max_blocks must be >= 0");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  h->block_cap = max_blocks;
+  return set_geometry(h, h->wpb, h->occ_req);
 }
 
 int jr_walk_timing(jr_handle* h, double* total_ms, int64_t* launches,

@@ -18,15 +18,17 @@ struct WalkerRegs {
 struct WalkCtx {
   WaveLds L;
   PairTab tab;
-  int N, Nmax, S;
+  int N;
+  int xi_stride;  // row stride of the normals a move reads (xi[c * xi_stride + atom])
+  int S;
   double pressure;
   const double* mu;  // [S]
   double emax;
 };
 
-__device__ __forceinline__ double tape_xi(const double* xi, int Nmax, int c,
+__device__ __forceinline__ double tape_xi(const double* xi, int stride, int c,
                                           int a) {
-  return xi[c * Nmax + a];
+  return xi[c * stride + a];
 }
 
 // constraints/min_distance.py:25-46: true iff no two atoms are closer than
@@ -99,9 +101,9 @@ __device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
     const int a = lane + 64 * s;
     double zx = 0.0, zy = 0.0, zz = 0.0;
     if (a < N) {
-      zx = tape_xi(xi, C.Nmax, 0, a);
-      zy = tape_xi(xi, C.Nmax, 1, a);
-      zz = tape_xi(xi, C.Nmax, 2, a);
+      zx = tape_xi(xi, C.xi_stride, 0, a);
+      zy = tape_xi(xi, C.xi_stride, 1, a);
+      zz = tape_xi(xi, C.xi_stride, 2, a);
     }
     if (fresh) {
       vx[s] = zx;
@@ -143,9 +145,9 @@ __device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
     double U = wave_eval<A, true, CFG>(C.L, N, g, C.tab, W.x, W.y, W.z, W.t, fx, fy, fz);
     double Hn = U + ens;
     if (__builtin_isnan(Hn)) Hn = __builtin_inf();
-    double fsq = wave_dot<A>(N, fx, fy, fz, fx, fy, fz);
-    double fdt = wave_dot<A>(N, fx, fy, fz, W.dx, W.dy, W.dz);
-    if (Hn >= C.emax) {
+    if (Hn >= C.emax) {  // wave-uniform: the force sums only where a reflection needs them
+      double fsq = wave_dot<A>(N, fx, fy, fz, fx, fy, fz);
+      double fdt = wave_dot<A>(N, fx, fy, fz, W.dx, W.dy, W.dz);
       double fn = fmax(sqrt(fsq), 1e-10);
       double tt = 2.0 * (fdt / fn);
 #pragma unroll
@@ -316,9 +318,9 @@ __device__ bool hmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
     const int a = lane + 64 * s;
     px[s] = py[s] = pz[s] = 0.0;
     if (a < N) {
-      px[s] = tape_xi(xi, C.Nmax, 0, a);
-      py[s] = tape_xi(xi, C.Nmax, 1, a);
-      pz[s] = tape_xi(xi, C.Nmax, 2, a);
+      px[s] = tape_xi(xi, C.xi_stride, 0, a);
+      py[s] = tape_xi(xi, C.xi_stride, 1, a);
+      pz[s] = tape_xi(xi, C.xi_stride, 2, a);
     }
     qx[s] = W.x[s];
     qy[s] = W.y[s];

@@ -396,6 +396,11 @@ class NSEngine:
         self._check(self.lib.jr_get_emax(self.h, _lib.dptr(e)))
         return e
 
+    def set_emax(self, emax):
+        """Set the resident contour (one value per replica)."""
+        e = _lib.f64(np.broadcast_to(np.asarray(emax, dtype=np.float64), (self.n_replicas,)))
+        self._check(self.lib.jr_set_emax(self.h, _lib.dptr(e)))
+
     def replica_exchange(self, re_u=None) -> dict:
         R = self.n_replicas
         acc = np.zeros(max(R - 1, 1), np.int32)
@@ -452,6 +457,11 @@ class NSEngine:
 
     def set_launch(self, waves_per_block: int, occupancy: int = 0):
         self._check(self.lib.jr_set_launch(self.h, int(waves_per_block), int(occupancy)))
+
+    def set_walk_block_cap(self, max_blocks: int):
+        """Cap the persistent walk grid at ``max_blocks`` blocks (0: none), so
+        that each wave walks many chains in turn; results do not change."""
+        self._check(self.lib.jr_set_walk_block_cap(self.h, int(max_blocks)))
 
     def walk_timing(self, reset: bool = False) -> tuple[float, int]:
         ms = C.c_double()

@@ -95,33 +95,37 @@ def test_burn_in_device_draws_and_adaptation():
     assert np.all(np.isfinite(out["emax"]))
 
 
-def test_checkpoint_resume_is_exact(tmp_path):
-    """Run 14 iterations with adaptation and device draws, checkpointing at
-    7; a fresh engine resumed from the checkpoint must end bit-identical,
-    and its .energies files must equal the uninterrupted run's."""
-    print("checkpoint resume", flush=True)
+@pytest.mark.parametrize("ck_at", [7, 10])
+def test_checkpoint_resume_is_exact(tmp_path, ck_at):
+    """Run 14 iterations with adaptation (every 5) and device draws,
+    checkpointing at ``ck_at``; a fresh engine resumed from the checkpoint
+    must end bit-identical, and its .energies files must equal the
+    uninterrupted run's.  At 10 the first iteration after resume adapts, so
+    the trial moves read the restored contour."""
+    print(f"checkpoint resume at {ck_at}", flush=True)
     ad = AdaptationConfig(adjust_interval=5, adjust_n_samples=20)
     a = _engine(seed=5)
     ra = run_ns(a, [IterationTermination(14)], adaptation=ad, seed=9,
                 energies_prefix=str(tmp_path / "a"),
-                checkpoint_path=tmp_path / "ck.npz", checkpoint_interval=7,
+                checkpoint_path=tmp_path / "ck.npz", checkpoint_interval=ck_at,
                 traj_prefix=str(tmp_path / "a"), traj_interval=3, symbol_map={0: "Ar"})
     assert ra.iterations == 14
     ck = load_checkpoint(tmp_path / "ck.npz")
-    assert ck["iteration"] == 14  # the last write
-    # rerun A's first 7 iterations to get the mid-run checkpoint
+    assert ck["iteration"] == (14 if ck_at == 7 else 10)  # the last write
+    # rerun A's first ck_at iterations to get the mid-run checkpoint
     b = _engine(seed=5)
-    run_ns(b, [IterationTermination(7)], adaptation=ad, seed=9,
+    run_ns(b, [IterationTermination(ck_at)], adaptation=ad, seed=9,
            energies_prefix=str(tmp_path / "b"),
-           checkpoint_path=tmp_path / "mid.npz", checkpoint_interval=7,
+           checkpoint_path=tmp_path / "mid.npz", checkpoint_interval=ck_at,
            traj_prefix=str(tmp_path / "b"), traj_interval=3, symbol_map={0: "Ar"})
     mid = load_checkpoint(tmp_path / "mid.npz")
-    assert mid["iteration"] == 7 and mid["n_dead"] == 7
-    # a third, fresh engine resumes at 7 and appends to b's energies
+    assert mid["iteration"] == ck_at and mid["n_dead"] == ck_at
+    assert np.array_equal(mid["emax"], b.emax()) and np.isfinite(mid["emax"]).all()
+    # a third, fresh engine resumes at ck_at and appends to b's energies
     c = _engine(seed=5)
     # stale rows past the checkpoint are cut on resume
     with open(tmp_path / "b.0.energies", "a") as f:
-        f.write("7 1.0 1.0\n8 1.0 1.0\n")
+        f.write(f"{ck_at} 1.0 1.0\n{ck_at + 1} 1.0 1.0\n")
     rc = run_ns(c, [IterationTermination(14)], adaptation=ad,
                 energies_prefix=str(tmp_path / "b"), resume=tmp_path / "mid.npz",
                 traj_prefix=str(tmp_path / "b"), traj_interval=3, symbol_map={0: "Ar"})

@@ -110,6 +110,57 @@ def test_learned_walk_matches_oracle(mode):
                                atol=1e-4 * (np.abs(pop0.energy).max() + N))
 
 
+@pytest.mark.parametrize("mode", ["plain", "screened"])
+def test_learned_walk_256_matches_oracle(mode):
+    """The 256-atom message-passing workload at its size and density (the
+    benchmark's judge, initial jittered lattice at rho = 0.9 and cell bounds;
+    ~1.2e5 edges per configuration), one tape-driven iteration of a short
+    walk against the oracle, with the tolerances of the small case."""
+    import bench
+    R, K, N, n_extra, T = 1, 3, 256, 1, 3
+    print(f"learned walk parity at N={N}: mode={mode} K={K} steps={T}", flush=True)
+    jp = init_mpnn(seed=11)
+    judge = MPNNBackend(jp)
+    base = (jp.core_eps, jp.core_sig, jp.core_rc)
+    moves = [{"type": "galilean", "n_reflect": 3, "step_size": 0.1, "weight": 4},
+             {"type": "volume", "step_size": 0.3, "weight": 1}]
+    learned = None
+    if mode != "plain":
+        learned = LearnedConfig(mode=mode, ring_capacity=256, train_batch=16, train_steps=0,
+                                min_train=32, min_heldout=8, lr=1e-3, rmse_bound=0.05,
+                                delta_k=2.0, train_max_epa=50.0, baseline=base,
+                                params=init_surrogate(1))
+    run = RunConfig(n_replicas=R, n_walkers=K, n_atoms=N, n_mcmc_steps=T, n_extra=n_extra,
+                    seed=42, pressures=[0.5], re_interval=0)
+    eng = NSEngine(Hamiltonian(judge, [EnsembleTerm(0.5)]), moves, run, cell=bench.SUR_CELL,
+                   learned=learned)
+    eng.set_population(*bench.sur_population(R, K, N, seed=7))
+    pop0 = Population.from_engine_dict(eng.get_population())
+    tape = make_tape(np.random.default_rng(17), 1, R, K, N, T, n_extra, [4, 1])
+    out = eng.ns_iterations(1, tape=tape)
+    st = eng.learned_stats()
+    pop_gpu = eng.get_population()
+    models = OW.Models(judge.params, init_surrogate(1), base=base)
+    oout, ost, recs = OW.ns_iteration(pop0, models, eng.pressure, eng.step_sizes,
+                                      _moves_dicts(eng), tape, 0, n_extra, mode,
+                                      train_max_epa=50.0)
+    for k in ("worst", "clone", "walk", "emax"):
+        assert np.array_equal(out[k][0], oout[k]), k
+    print("  counters gpu", out["counters"][0].tolist(), flush=True)
+    print("  counters ora", oout["counters"].tolist(), flush=True)
+    assert np.array_equal(out["counters"][0], oout["counters"])
+    for k in ("proposals", "stage1_pass", "accepted"):
+        assert st[k] == ost[k], k
+    assert st["exact_force_evals"] == ost["exact_fe"]
+    assert st["exact_energy_evals"] == ost["exact_e"]
+    assert st["surrogate_evals"] == ost["sur_e"]
+    L = pop_gpu["cell"][..., 0, 0].max()
+    np.testing.assert_allclose(pop_gpu["positions"], pop0.pos, atol=1e-6 * L, rtol=0)
+    np.testing.assert_allclose(pop_gpu["cell"].reshape(R, K, 9), pop0.cell, rtol=1e-12)
+    np.testing.assert_allclose(pop_gpu["energy"], pop0.energy,
+                               atol=1e-4 * (np.abs(pop0.energy).max() + N))
+
+
 # LJ judge with HMC + GMC (+ volume): the surrogate's pair baseline is the
 # r^-12 part of LJ (eps = sigma = 1, rc = 2.5), the network learns the rest
 LJ_BASE = (4.0, 1.0, 2.5)
@@ -135,19 +186,31 @@ def _lj_engine(mode, R, K, N, n_extra, n_steps, seed, pressures, vpa=(1.2, 3.0),
     return eng, lj
 
 
+@pytest.mark.parametrize("N", [32, 128])
 @pytest.mark.parametrize("mode", ["plain", "auto", "screened"])
-def test_learned_walk_lj_hmc_matches_oracle(mode):
+def test_learned_walk_lj_hmc_matches_oracle(mode, N):
     """The learned walk with the FP64 pair potential as the judge and HMC,
     GMC and volume moves: one tape-driven iteration against the oracle (the
     judge is bit-exact with the oracle's, so decisions and counters must
-    match exactly and positions to rounding)."""
-    R, K, N, n_extra, T = 2, 4, 32, 2, 8
+    match exactly and positions to rounding); N = 128 is the LJ-128
+    configuration's system size."""
+    R, K, n_extra, T = 2, 4, 2, 8
     print(f"learned walk, LJ judge, mode={mode} R={R} K={K} N={N} steps={T}", flush=True)
-    eng, lj = _lj_engine(mode, R, K, N, n_extra, T, seed=12, pressures=[0.5, 1.0])
-    # relax the random initial walkers with the pair-potential walk first, so
-    # that HMC proposals are accepted and trajectories are well conditioned
+    moves = LJ_MOVES
+    if N > 64:
+        # shorter leapfrog steps keep Delta H below the hard cut at 128 atoms
+        moves = [dict(m, step_size=0.002) if m["type"] == "hmc" else m for m in LJ_MOVES]
+    eng, lj = _lj_engine(mode, R, K, N, n_extra, T, seed=12, pressures=[0.5, 1.0],
+                         moves=moves)
+    if N > 64:
+        # a jittered lattice (density 0.75) instead of uniform positions
+        import bench
+        pos, _, cell0 = bench.sur_population(R, K, N, seed=12, rho=0.75)
+        eng.set_population(pos, np.zeros((R, K, N), np.int32), cell0)
+    # relax the initial walkers with the pair-potential walk first, so that
+    # HMC proposals are accepted and trajectories are well conditioned
     cell = {"min_volume_per_atom": 1.2, "max_volume_per_atom": 3.0, "flat_V_prior": True}
-    pre = NSEngine(Hamiltonian(lj, [EnsembleTerm(0.5)]), LJ_MOVES,
+    pre = NSEngine(Hamiltonian(lj, [EnsembleTerm(0.5)]), moves,
                    RunConfig(n_replicas=R, n_walkers=K, n_atoms=N, n_mcmc_steps=T,
                              n_extra=n_extra, seed=13, pressures=[0.5, 1.0]), cell=cell)
     pre.set_population(*(eng.get_population()[k] for k in ("positions", "types", "cell")))

@@ -1,6 +1,11 @@
 """HIP path vs the CPU oracle on identical inputs (bit-for-bit: compared
 with ``==`` on every float64 output; the two follow one canonical
-floating-point order).  Needs an MI355X."""
+floating-point order).  Needs an MI355X.
+
+``cap = 1`` runs a case on one persistent block (``set_walk_block_cap``), so
+that each wave takes several chains from the work queue in turn and reuses
+its LDS, counters and registers between them: the path the full-size
+workloads take."""
 
 import math
 
@@ -123,42 +128,48 @@ def _run_parity(oracle, eng, n_iter, tape, pair_valid=None):
         _compare_population(eng, opop)
 
 
+@pytest.mark.parametrize("cap", [0, 1])
 @pytest.mark.parametrize("n_extra", [0, 14])
-def test_ns_iterations_lj8(oracle, n_extra):
-    print(f"ns lj8 n_extra={n_extra}", flush=True)
+def test_ns_iterations_lj8(oracle, n_extra, cap):
+    print(f"ns lj8 n_extra={n_extra} cap={cap}", flush=True)
     lj = Hamiltonian(LJBackend(1.0, 1.0, 2.5), [EnsembleTerm(0.1)])
     eng = _engine(lj, GMC_VOL, 1, 16, 8, n_extra, 20, cell_spec=CELL)
+    eng.set_walk_block_cap(cap)
     tape = make_tape(np.random.default_rng(9), 6, 1, 16, 8, 20, n_extra,
                      eng.mwg.move_probs, ties=True)
     _run_parity(oracle, eng, 6, tape)
 
 
-def test_ns_iterations_lj64_replicas_with_re(oracle):
-    print("ns lj64 R=4 with RE", flush=True)
+@pytest.mark.parametrize("cap", [0, 1])
+def test_ns_iterations_lj64_replicas_with_re(oracle, cap):
+    print(f"ns lj64 R=4 with RE cap={cap}", flush=True)
     lj = LJBackend(1.0, 1.0, 2.5)
     P = list(10 ** (-2 + 3 * np.arange(4) / 3))
     eng = _engine(lj, GMC_VOL, 4, 8, 64, 6, 6, cell_spec=CELL, pressures=P)
+    eng.set_walk_block_cap(cap)
     tape = make_tape(np.random.default_rng(10), 4, 4, 8, 64, 6, 6, eng.mwg.move_probs)
     _run_parity(oracle, eng, 4, tape)
 
 
-def test_ns_iterations_hmc_gmc_n128(oracle):
-    print("ns hmc+gmc N=128", flush=True)
+@pytest.mark.parametrize("cap", [0, 1])
+def test_ns_iterations_hmc_gmc_n128(oracle, cap):
+    print(f"ns hmc+gmc N=128 cap={cap}", flush=True)
     lj = LJBackend(1.0, 1.0, 2.5)
     moves = [{"type": "hmc", "n_leapfrog": 4, "step_size": 0.005, "weight": 1},
              {"type": "gmc", "n_reflect": 3, "step_size": 0.05, "weight": 1}]
     eng = _engine(lj, moves, 2, 4, 128, 2, 4, pressures=[0.05, 0.5],
                   cell_spec={"min_volume_per_atom": 0.5, "max_volume_per_atom": 4.0},
                   v_range=(1.2, 3.0))
+    eng.set_walk_block_cap(cap)
     tape = make_tape(np.random.default_rng(11), 3, 2, 4, 128, 4, 2, eng.mwg.move_probs)
     _run_parity(oracle, eng, 3, tape)
 
 
-@pytest.mark.parametrize("occ", [0, 3])
-def test_ns_iterations_semigrand_mixed_n(oracle, occ):
+@pytest.mark.parametrize("occ,cap", [(0, 0), (3, 0), (3, 1)])
+def test_ns_iterations_semigrand_mixed_n(oracle, occ, cap):
     """Two size classes (N <= 64 and 65..128), each in its own launch, with
     the N <= 64 class on its default and its three-waves-per-SIMD build."""
-    print(f"ns semi-grand mixed N occupancy={occ}", flush=True)
+    print(f"ns semi-grand mixed N occupancy={occ} cap={cap}", flush=True)
     eps = np.array([[1.0, 1.5], [1.5, 0.5]])
     sig = np.array([[1.0, 0.8], [0.8, 0.88]])
     lj = LJBackend(pair_epsilon=eps, pair_sigma=sig, cutoff=2.5)
@@ -174,17 +185,21 @@ def test_ns_iterations_semigrand_mixed_n(oracle, occ):
     eng = NSEngine(lj, moves, run, cell=CELL)
     eng.init_walkers(np.random.default_rng(5), 0.5, 8.0, True, species_p=[0.5, 0.5])
     eng.set_launch(2, occ)
+    eng.set_walk_block_cap(cap)
     tape = make_tape(np.random.default_rng(12), 4, R, 6, 96, 6, 3, eng.mwg.move_probs)
     pv = np.array([n_atoms[i] == n_atoms[i + 1] for i in range(R - 1)], np.int32)
     _run_parity(oracle, eng, 4, tape, pair_valid=pv)
 
 
-def test_device_draws_replay(oracle):
-    """Device Philox draws, read back and replayed on the oracle."""
-    print("device draws replay", flush=True)
+@pytest.mark.parametrize("cap", [0, 1])
+def test_device_draws_replay(oracle, cap):
+    """Device Philox draws (normals and uniforms drawn in place by the walk),
+    read back and replayed on the oracle."""
+    print(f"device draws replay cap={cap}", flush=True)
     from oracle.oracle import Population
     lj = LJBackend(1.0, 1.0, 2.5)
     eng = _engine(lj, GMC_VOL, 3, 10, 20, 8, 5, cell_spec=CELL, pressures=[0.1, 0.5, 1.0])
+    eng.set_walk_block_cap(cap)
     opop = Population.from_engine_dict(eng.get_population())
     pot = oracle.Pot.from_table(lj.table)
     for it in range(3):
