@@ -241,6 +241,23 @@ int jr_eval_energy_forces(const jr_pot* pot, int32_t device, int32_t B,
                           const double* cell, const double* pressure,
                           const double* mu, double* U, double* H, double* F);
 
+/* Allocation-free variant of jr_eval_energy_forces (same arguments, same
+ * results bit for bit).  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc).  The device staging buffers are one fixed array per GPU
+ * (about 3.9 MiB, held until the process ends): B * Nmax <= 65536 atom
+ * slots and B <= 4096 configurations, larger calls fail with JR_E_ARG.
+ * Runs on the device's default stream and returns after the copies back.
+ * Calls that overlap on one GPU (from several threads) share that array and
+ * race; serialise them. */
+#define JR_NOALLOC_MAX_SLOTS 65536
+#define JR_NOALLOC_MAX_CONFIGS 4096
+int jr_eval_energy_forces_noalloc(const jr_pot* pot, int32_t device, int32_t B,
+                                  int32_t Nmax, const double* pos,
+                                  const int32_t* types, const int32_t* n_atoms,
+                                  const double* cell, const double* pressure,
+                                  const double* mu, double* U, double* H, double* F);
+
 int jr_create(const jr_config* cfg, jr_handle** out);
 void jr_destroy(jr_handle* h);
 

@@ -187,6 +187,9 @@ SIGNATURES = [
     ("jr_eval_energy_forces", C.c_int,
      [C.POINTER(JrPot), C.c_int32, C.c_int32, C.c_int32, _dp, _ip, _ip, _dp,
       _dp, _dp, _dp, _dp, _dp]),
+    ("jr_eval_energy_forces_noalloc", C.c_int,
+     [C.POINTER(JrPot), C.c_int32, C.c_int32, C.c_int32, _dp, _ip, _ip, _dp,
+      _dp, _dp, _dp, _dp, _dp]),
     ("jr_create", C.c_int, [C.POINTER(JrConfig), C.POINTER(C.c_void_p)]),
     ("jr_destroy", None, [C.c_void_p]),
     ("jr_set_population", C.c_int, [C.c_void_p, _dp, _ip, _ip, _dp, _dp]),

@@ -12,6 +12,7 @@
 #include "jr_handle.h"
 #include "jr_iter_kernels.h"
 #include "jr_launch.h"
+#include "jr_pot.h"
 
 using namespace jr;
 
@@ -37,40 +38,6 @@ using namespace jrh;
 
 namespace {
 
-int atoms_per_lane(int nmax) { return nmax <= 64 ? 1 : (nmax <= 128 ? 2 : 4); }
-
-bool pot_images(const jr_pot* pot) {
-  return pot->images[0] > 0 || pot->images[1] > 0 || pot->images[2] > 0;
-}
-
-int pot_cfg(const jr_pot* pot) {
-  return (pot->n_species > 1 ? 1 : 0) |
-         (pot->precision == JR_PREC_FP32_PAIRS ? 2 : 0) |
-         (pot_images(pot) ? 4 : 0);
-}
-
-void set_images(int* dst, const jr_pot& pot) {
-  for (int d = 0; d < 3; ++d) dst[d] = pot.images[d];
-}
-
-int validate_pot(const jr_pot* pot) {
-  if (!pot) return set_global_err(JR_E_ARG, "pot is NULL");
-  if (pot->n_species < 1 || pot->n_species > JR_MAX_SPECIES)
-    return set_global_err(JR_E_ARG, "n_species must be in [1, %d], got %d",
-                          JR_MAX_SPECIES, pot->n_species);
-  if (!pot->eps4 || !pot->eps24 || !pot->sig2)
-    return set_global_err(JR_E_ARG, "pair tables must not be NULL");
-  if (pot->precision != JR_PREC_FP64 && pot->precision != JR_PREC_FP32_PAIRS)
-    return set_global_err(JR_E_ARG, "unknown precision %d", pot->precision);
-  for (int d = 0; d < 3; ++d)
-    if (pot->images[d] < 0 || pot->images[d] > JR_MAX_IMAGES)
-      return set_global_err(JR_E_ARG, "images[%d] must be in [0, %d], got %d", d,
-                            JR_MAX_IMAGES, pot->images[d]);
-  if (pot_images(pot) && pot->precision != JR_PREC_FP64)
-    return set_global_err(JR_E_UNSUPPORTED, "image sums are built for FP64 pairs only");
-  return JR_OK;
-}
-
 long long pow2_at_least(long long n) {
   long long p = 1;
   while (p < n) p <<= 1;
@@ -92,23 +59,9 @@ int jr_eval_energy_forces(const jr_pot* pot, int32_t device, int32_t B,
                           const int32_t* types, const int32_t* n_atoms,
                           const double* cell, const double* pressure,
                           const double* mu, double* U, double* H, double* F) {
-  int rc = validate_pot(pot);
+  int rc = check_eval_args(pot, B, Nmax, pos, types, n_atoms, cell);
   if (rc) return rc;
-  if (B < 0 || Nmax < 1 || Nmax > 256)
-    return set_global_err(JR_E_ARG, "need B >= 0 and 1 <= Nmax <= 256");
-  if (!pos || !types || !n_atoms || !cell)
-    return set_global_err(JR_E_ARG, "pos/types/n_atoms/cell must be set");
   if (B == 0) return JR_OK;
-  for (int b = 0; b < B; ++b) {
-    if (n_atoms[b] < 0 || n_atoms[b] > Nmax)
-      return set_global_err(JR_E_ARG, "n_atoms[%d]=%d outside [0, %d]", b,
-                            n_atoms[b], Nmax);
-    for (int i = 0; i < n_atoms[b]; ++i) {
-      int t = types[(size_t)b * Nmax + i];
-      if (t < 0 || t >= pot->n_species)
-        return set_global_err(JR_E_ARG, "species %d out of range", t);
-    }
-  }
   HIPCHK_G(hipSetDevice(device));
   const int S = pot->n_species;
   DBuf<double> d_pos, d_cell, d_pr, d_mu, d_U, d_H, d_F, d_e4, d_e24, d_s2;
