@@ -112,9 +112,12 @@ typedef struct {
  * set).  Modes:
  *   JR_LEARNED_PLAIN     exact judge forces steer every GMC trajectory and
  *                        the judge decides every move (no surrogate);
- *   JR_LEARNED_AUTO      two-stage delayed acceptance while the surrogate's
- *                        held-out per-atom RMSE is <= rmse_bound, the plain
- *                        walk otherwise; the surrogate trains online;
+ *   JR_LEARNED_AUTO      two-stage delayed acceptance in an iteration where
+ *                        the surrogate's held-out per-atom RMSE is <=
+ *                        rmse_bound and every chain's contour bounds U/N at
+ *                        or below train_max_epa (U < Emax - P V over the
+ *                        volumes the moves allow), the plain walk otherwise;
+ *                        the surrogate trains online;
  *   JR_LEARNED_SCREENED  delayed acceptance in every iteration (the bound is
  *                        ignored); the surrogate trains online.
  * The surrogate's enthalpy carries an error model refitted after every
@@ -124,9 +127,11 @@ typedef struct {
  * + P V, GMC trajectories reflect where H_s >= Emax + N b_r, and stage 1
  * compares the verdicts [H_s < Emax + N (b_r + delta_k s_r)] at the start
  * and end of a proposal, rejecting when they differ; stage 2 accepts iff the
- * judge's H < Emax.  Both verdicts are symmetric in the two states and every
- * constant is fixed during an iteration, so the constrained prior stays
- * exact.  rmse_bound applies to the residual per-atom RMSE. */
+ * judge's H < Emax (GMC, volume) or is the Metropolis test of the judge's
+ * H + K (HMC, whose reference rule samples exp(-H) inside the contour).
+ * Both verdicts are symmetric in the two states and every constant is fixed
+ * during an iteration, so each move keeps its plain counterpart's target.
+ * rmse_bound applies to the residual per-atom RMSE. */
 #define JR_LEARNED_PLAIN 0
 #define JR_LEARNED_AUTO 1
 #define JR_LEARNED_SCREENED 2

@@ -440,6 +440,8 @@ struct LearnedState {
   DBuf<float> erow, dXg;    // [C][Nmax] row energies, [C][Nmax][64] dE/dG rows
   DBuf<unsigned> nbl;
   DBuf<int> overflow;
+  DBuf<double> dcontour;  // bound on U / N at this iteration's contours
+  double vol_lo = 0.0, vol_hi = 0.0;  // volume-per-atom range of the volume moves (0: none)
   // training ring and minibatch
   DBuf<__bf16> ring;
   DBuf<double> ringE;
@@ -591,6 +593,11 @@ int learned_create(jr_handle* h, const jr_config* c) {
   L->C = c->n_replicas * (1 + c->n_extra);
   L->n_cu = device_cus(c->device);
   L->sched.assign(c->n_mcmc_steps, 0);
+  for (const jr_move& mv : L->moves)
+    if (mv.type == JR_MOVE_VOLUME) {
+      L->vol_lo = L->vol_lo > 0.0 ? std::min(L->vol_lo, mv.min_vpa) : mv.min_vpa;
+      L->vol_hi = std::max(L->vol_hi, mv.max_vpa);
+    }
   if (net) {
     h->mpnn_cfg = *net;
     h->mpnn_cfg.emb = h->mpnn_cfg.W1 = h->mpnn_cfg.U1 = h->mpnn_cfg.c1 = nullptr;
@@ -622,6 +629,7 @@ int learned_create(jr_handle* h, const jr_config* c) {
   if (net)
     HIPCHK(h, L->nbl.alloc((size_t)kMpBlocksPerCu * L->n_cu * MP_MAXNB * MP_THREADS));
   HIPCHK(h, L->overflow.alloc(1));
+  HIPCHK(h, L->dcontour.alloc(1));
   HIPCHK(h, L->dstats.alloc(1));
   HIPCHK(h, L->rs.alloc(1));
   for (auto& e : L->ev) HIPCHK(h, hipEventCreate(&e));
@@ -960,7 +968,8 @@ hipError_t hmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
     w.ham_exact(L->idx.p, L->cnt.p, L->U.p, L->ccell.p, L->Hy.p);
     hipLaunchKernelGGL(lw_hmc_finish_kernel, dim3(gw), dim3(256), 0, s, w.ch, L->code.p,
                        L->Hy.p, L->cH.p, L->K0.p, L->mom.p, L->cem.p, L->accept.p, L->reason.p,
-                       L->cpos.p, L->x0.p, L->cHs.p, L->Hsy.p);
+                       L->cpos.p, L->x0.p, L->cHs.p, L->Hsy.p, h->uniforms.p + (size_t)t * 4,
+                       (size_t)w.n_steps * 4);
     record_exact(w, L->code.p, L->ccell.p);
     count_step(w, m, L->code.p, 0, 1, 0, 1);
     stat_add(w, &L->dstats.p->exact_e, L->cnt.p, 0, 1);
@@ -976,7 +985,7 @@ hipError_t hmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
     }
     hipLaunchKernelGGL(lw_hmc_finish_kernel, dim3(gw), dim3(256), 0, s, w.ch, nullptr, L->Hy.p,
                        L->cH.p, L->K0.p, L->mom.p, L->cem.p, L->accept.p, L->reason.p,
-                       L->cpos.p, L->x0.p, nullptr, nullptr);
+                       L->cpos.p, L->x0.p, nullptr, nullptr, nullptr, (size_t)0);
     count_step(w, m, nullptr, 2 * nl + 1, 0, 2 * nl, 0);
     stat_add(w, &L->dstats.p->exact_fe, nullptr, C, nl + 1);
   }
@@ -1078,13 +1087,6 @@ int learned_walk(jr_handle* h) {
   w.ch = LWChains{L->C, h->n_walk, L->Nmax, c.n_moves, L->cn.p};
   for (double mu : h->h_mu)
     if (mu != 0.0) return h->fail(JR_E_UNSUPPORTED, "the learned walk is NPT (mu = 0)");
-  // the shared move schedule decides which launches follow
-  HIPCHK(h, hipMemcpyAsync(L->sched.data(), h->sched.p, sizeof(int) * c.n_mcmc_steps,
-                           hipMemcpyDeviceToHost, s));
-  HIPCHK(h, hipStreamSynchronize(s));
-  const bool da = L->mode == JR_LEARNED_SCREENED ||
-                  (L->mode == JR_LEARNED_AUTO && L->rmse <= L->cfg.rmse_bound &&
-                   L->rmse_n >= L->cfg.min_heldout);
   HIPCHK(h, hipEventRecord(L->ev[0], s));
   LWGatherArgs ga;
   ga.C = L->C;
@@ -1104,6 +1106,20 @@ int learned_walk(jr_handle* h) {
   ga.ctypes = L->ctypes.p;
   ga.cn = L->cn.p;
   hipLaunchKernelGGL(lw_gather_kernel, dim3(L->C), dim3(256), 0, s, ga);
+  hipLaunchKernelGGL(lw_contour_kernel, dim3(1), dim3(256), 0, s, L->C, L->cem.p, L->cpres.p,
+                     L->ccell.p, L->cn.p, L->vol_lo, L->vol_hi, L->dcontour.p);
+  // the shared move schedule decides which launches follow
+  double contour_epa = INFINITY;
+  HIPCHK(h, hipMemcpyAsync(L->sched.data(), h->sched.p, sizeof(int) * c.n_mcmc_steps,
+                           hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipMemcpyAsync(&contour_epa, L->dcontour.p, sizeof(double), hipMemcpyDeviceToHost,
+                           s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  // auto: the held-out error meets the bound and no state the contours admit
+  // lies above the energy range the surrogate trains on
+  const bool da = L->mode == JR_LEARNED_SCREENED ||
+                  (L->mode == JR_LEARNED_AUTO && L->rmse <= L->cfg.rmse_bound &&
+                   L->rmse_n >= L->cfg.min_heldout && contour_epa <= L->cfg.train_max_epa);
   if (da) {
     // contours of this iteration from the error model of the last one
     HIPCHK(h, hipMemcpyAsync(L->dbias.p, L->bias.data(), sizeof(double) * L->bias.size(),

@@ -256,15 +256,27 @@ __global__ void lw_hmc_kick_drift_kernel(LWChains ch, double* p, double* pos, co
   if (drift) pos[i] = pos[i] + s * q;
 }
 
-// Final test: accept iff code == 1 (NULL = all), H' < emax and
-// (H' + K') - (H + K0) < 1; rejected chains return to x0.
+// The HMC decision; rejected chains return to x0.  Plain (judge forces;
+// code == NULL): the reference's rule H' < Emax and (H' + K') - (H + K) < 1
+// (hmc.py:51-152, marked unvalidated there for this reason: a hard cut is
+// no detailed-balance accept).  With unit masses, momenta N(0, 1) and a
+// small step it accepts almost every move, and the exact flow keeps
+// exp(-H - K), so the move leans toward exp(-H) inside the contour rather
+// than the flat constrained prior.  Screened (code != NULL): the
+// surrogate's forces drive the leapfrog, and the move is the exact
+// Metropolis step of the constrained prior uniform(x) N(p) under the
+// leapfrog-then-flip map, a volume-preserving involution for any force
+// field: stage 1 agrees, H' < Emax and u < exp(K - K'), u the sub-step's
+// uniform 0 (chain stride u_stride), exp_c so that every build decides
+// alike.
 __global__ void __launch_bounds__(256) lw_hmc_finish_kernel(LWChains ch, const int* code,
                                                             const double* Hy, double* Hcur,
                                                             const double* K0, const double* p,
                                                             const double* em, int* accept,
                                                             int* reason, double* pos,
                                                             const double* x0, double* Hs_cur,
-                                                            const double* Hs_new) {
+                                                            const double* Hs_new,
+                                                            const double* u, size_t u_stride) {
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c >= ch.C) return;
   const int N = ch.n_atoms[c];
@@ -276,8 +288,11 @@ __global__ void __launch_bounds__(256) lw_hmc_finish_kernel(LWChains ch, const i
   const int k = code ? code[c] : 1;
   bool acc = false;
   if (k == 1) {
-    const double dH = (Hy[c] + 0.5 * pp) - (Hcur[c] + K0[c]);
-    acc = (Hy[c] < em[c]) && (dH < 1.0);
+    const double K1 = 0.5 * pp;
+    if (code)
+      acc = (Hy[c] < em[c]) && (u[(size_t)c * u_stride] < exp_c(K0[c] - K1));
+    else
+      acc = (Hy[c] < em[c]) && ((Hy[c] + K1) - (Hcur[c] + K0[c]) < 1.0);
   }
   if (!acc)
     for (int a = (int)(threadIdx.x & 63); a < N; a += 64)
@@ -376,6 +391,31 @@ __global__ void lw_thr_kernel(int C, int n_walk, const double* em, const int* n,
   const double N = (double)n[c];
   thr_steer[c] = em[c] + N * bias[r];
   thr_screen[c] = em[c] + N * (bias[r] + k * sig[r]);
+}
+
+// Upper bound on U / N over every state the chains' contours admit:
+// U = H - P V < Emax - P V, with V fixed (no volume move, vol_lo = 0) or
+// inside [vol_lo, vol_hi] N.  One block; the maximum over chains goes to
+// *out.  The auto walk screens only while this bound stays inside the
+// range the surrogate is trained on (U / N <= train_max_epa).
+__global__ void __launch_bounds__(256) lw_contour_kernel(int C, const double* em,
+                                                         const double* pres, const double* cell,
+                                                         const int* n, double vol_lo,
+                                                         double vol_hi, double* out) {
+  __shared__ double part[256];
+  double m = -INFINITY;
+  for (int c = threadIdx.x; c < C; c += 256) {
+    const double N = (double)n[c], P = pres[c];
+    const double V = vol_lo > 0.0 ? (P >= 0.0 ? vol_lo : vol_hi) * N : fabs(det3(cell + 9 * c));
+    m = fmax(m, (em[c] - P * V) / N);
+  }
+  part[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = part[0];
 }
 
 // Final test: accept iff code == 1 (NULL = all) and H < emax.  Accepted

@@ -75,6 +75,49 @@ def grid_positions_in_cell(rng: np.random.Generator, cell, n_atoms: int,
     return frac[rng.choice(n_grid, size=n_atoms, replace=False)] @ cell
 
 
+def _aspect(cell: np.ndarray, volume: float) -> float:
+    """Smallest height of the cell over V^(1/3), over the three faces
+    (``utils/cell.py:80-102``)."""
+    vc = volume ** (1.0 / 3.0)
+    out = np.inf
+    for i, j, k in ((0, 1, 2), (1, 2, 0), (0, 2, 1)):
+        n = np.cross(cell[i], cell[j])
+        out = min(out, abs(float(np.dot(n / np.linalg.norm(n), cell[k]))) / vc)
+    return out
+
+
+def cell_shape_walk(rng: np.random.Generator, cell, n_steps: int, step: float,
+                    min_aspect: float) -> np.ndarray:
+    """Volume-preserving random walk over the cell shape (``init/cells.py:
+    140-230``): each step proposes, with equal odds, a stretch (axis pair
+    (0,1), (0,2) or (1,2) scaled by exp(+-r), r ~ N(0, step)) or a shear (one
+    cell vector moved by r1 u + r2 w in the plane of the other two, u along
+    the first of them, w the unit part of the second orthogonal to u,
+    r1, r2 ~ N(0, step)); the proposal is rescaled to the starting volume and
+    kept when its aspect ratio is at least ``min_aspect``."""
+    cell = np.array(cell, dtype=np.float64)
+    v0 = abs(np.linalg.det(cell))
+    for _ in range(n_steps):
+        new = cell.copy()
+        if rng.random() < 0.5:
+            i, j = ((0, 1), (0, 2), (1, 2))[rng.integers(3)]
+            r = step * rng.standard_normal()
+            new[:, i] *= np.exp(r)
+            new[:, j] *= np.exp(-r)
+        else:
+            idx = int(rng.integers(3))
+            a, b = [c for c in range(3) if c != idx]
+            u = cell[a] / np.linalg.norm(cell[a])
+            w = cell[b] - np.dot(cell[b], u) * u
+            w /= np.linalg.norm(w)
+            r1, r2 = step * rng.standard_normal(2)
+            new[idx] = cell[idx] + r1 * u + r2 * w
+        new *= (v0 / abs(np.linalg.det(new))) ** (1.0 / 3.0)
+        if _aspect(new, v0) >= min_aspect:
+            cell = new
+    return cell
+
+
 class NSEngine:
     """One handle of ``libjrhip``: population, run parameters and draws."""
 
@@ -228,12 +271,21 @@ class NSEngine:
 
     def init_walkers(self, rng: np.random.Generator, v_lo_per_atom: float,
                      v_hi_per_atom: float, flat_V_prior: bool = True,
-                     species_p=None, grid_distance: float | None = None):
+                     species_p=None, grid_distance: float | None = None,
+                     shared_volume: bool = False, shape_walk_steps: int = 0,
+                     shape_step: float = 0.05, min_aspect: float = 0.8):
         """Cubic cells from the volume prior and uniform positions
         (``init/cells.py:82-137``, ``init/positions.py:15-32``), or with
         ``grid_distance`` sites of a grid spaced by it
         (:func:`grid_positions_in_cell`); species Bernoulli/categorical per
-        atom with probabilities ``species_p``."""
+        atom with probabilities ``species_p``.
+
+        ``shared_volume``: the reference's ``random_initialise_cell`` path
+        (``cli/resolve.py:285-343,1020-1035``): one cubic cell per replica
+        from the volume prior, and each walker's cell from it by
+        ``shape_walk_steps`` steps of :func:`cell_shape_walk` (step
+        ``shape_step``, the run's ``min_aspect`` ratio); all walkers of a
+        replica then share one volume."""
         R, K = self.n_replicas, self.n_walkers
         Nm = self.Nmax
         pos = np.zeros((R, K, Nm, 3))
@@ -241,7 +293,7 @@ class NSEngine:
         ty = np.zeros((R, K, Nm), np.int32)
         for r in range(R):
             n = int(self.n_atoms_rep[r])
-            u = rng.random(K)
+            u = np.full(K, rng.random()) if shared_volume else rng.random(K)
             v_lo, v_hi = n * v_lo_per_atom, n * v_hi_per_atom
             if flat_V_prior:
                 vol = v_lo + u * (v_hi - v_lo)
@@ -251,7 +303,14 @@ class NSEngine:
                 vol = v_hi * (a + u * (1.0 - a)) ** (1.0 / p)
             lc = vol ** (1.0 / 3.0)
             cell[r] = lc[:, None, None] * np.eye(3)
-            if grid_distance is None:
+            if shape_walk_steps > 0:
+                for k in range(K):
+                    cell[r, k] = cell_shape_walk(rng, cell[r, k], shape_walk_steps,
+                                                 shape_step, min_aspect)
+            if grid_distance is None and shape_walk_steps > 0:
+                frac = rng.random((K, n, 3))
+                pos[r, :, :n] = np.einsum("knd,kde->kne", frac, cell[r])
+            elif grid_distance is None:
                 frac = rng.random((K, n, 3))
                 pos[r, :, :n] = frac * lc[:, None, None]
             else:

@@ -119,9 +119,11 @@ class LearnedConfig:
     """Settings of the surrogate-screened walk (``jr_learned``).
 
     mode: "plain" (the judge's forces steer and it decides every move; no
-    surrogate), "auto" (two-stage delayed acceptance while the held-out
-    per-atom RMSE after the error model is <= rmse_bound, plain otherwise)
-    or "screened" (delayed acceptance in every iteration).  The surrogate
+    surrogate), "auto" (two-stage delayed acceptance in an iteration whose
+    held-out per-atom RMSE after the error model is <= rmse_bound and whose
+    contours admit no state with U/N above train_max_epa, the range the
+    surrogate trains on; plain otherwise) or "screened" (delayed acceptance
+    in every iteration).  The surrogate
     trains online in "auto" and "screened" on the last ``ring_capacity``
     exactly evaluated configurations (those with U/N <= train_max_epa).
     delta_k: stage-1 slack in residual spreads of the error model.

@@ -144,11 +144,22 @@ def gmc(ch, mv, step, xi, models, emax, P, thr, da, stats, records, use_sur, cor
     return reason, n_eval, n_grad
 
 
-def hmc(ch, mv, step, xi, models, emax, P, thr, da, stats, records, use_sur, corr=(0.0, 0.0)):
+def _exp(x: float) -> float:
+    """exp of the C oracle (IEEE operations only, as the device's)."""
+    return float(OR.lib().or_exp(float(x)))
+
+
+def hmc(ch, mv, step, xi, models, emax, P, thr, da, stats, records, use_sur, corr=(0.0, 0.0),
+        u0=0.0):
     """One HMC move (sampling/moves/hmc.py:51-152): p = xi; n_leapfrog x
-    {p += s/2 F; x += s p; F = F(x); p += s/2 F}; accept iff H' < emax and
-    (H' + K') - (H + K) < 1.  Screened: the surrogate's forces drive the
-    leapfrog and its verdicts screen before the judge's test."""
+    {p += s/2 F; x += s p; F = F(x); p += s/2 F}.  Plain: accept iff
+    H' < emax and (H' + K') - (H + K) < 1 (the reference's rule).
+    That rule accepts almost every move at a small step and the exact flow
+    keeps exp(-H - K), so it leans toward exp(-H) inside the contour.
+    Screened: the surrogate's forces drive the leapfrog, its verdicts
+    screen, and the move is the exact Metropolis step of the constrained
+    prior uniform(x) N(p) under the leapfrog-and-flip involution: accept iff
+    H' < emax and u0 < exp(K - K') (exp as the C oracle's or_exp)."""
     N = ch["n"]
     x0 = ch["pos"][:N].copy()
     ty, cell = ch["types"][:N], ch["cell"]
@@ -175,7 +186,7 @@ def hmc(ch, mv, step, xi, models, emax, P, thr, da, stats, records, use_sur, cor
         if code == 1:
             U = models.judge(x, ty, cell, False)[0]
             Hy = _ham(U, cell, P)
-            acc = (Hy < emax) and ((Hy + K1) - (ch["H"] + K0) < 1.0)
+            acc = bool(Hy < emax) and bool(u0 < _exp(K0 - K1))
             stats["exact_e"] += 1
             stats["stage1_pass"] += 1
             records.append((U, E, N, _det(cell)))
@@ -295,7 +306,8 @@ def ns_iteration(pop, models, pressure, step_sizes, moves, tape, it, n_extra, mo
                                  stats, step_records, use_sur, corr)
             elif mv["type"] == HMC:
                 rs, ne, ng = hmc(ch, mv, step, xi, models, emax, pressure[r], thr, da,
-                                 stats, step_records, use_sur, corr)
+                                 stats, step_records, use_sur, corr,
+                                 u0=tape.uniforms[it, r, j, t, 0])
             else:
                 rs, ne, ng = volume(ch, mv, step, xi[0, 0], tape.uniforms[it, r, j, t, 0],
                                     models, emax, pressure[r], thr, da, stats,

@@ -10,11 +10,29 @@ walkers, 10 sub-steps per iteration, n_extra = 0, galilean (n_reflect 4, step
 that prior with positions on a grid of spacing 1.0, one initial walk of 5
 sub-steps at max(E) + 1 per atom after one step-size adaptation, then 200 NS
 iterations adapting every 20 (50 samples, 15 rounds, window 0.3-0.5, factor
-1.5, step cap 0.5).  The draws differ (the JAX PRNG stream cannot be
-reproduced offline), so the check is distributional: at iterations 20, 40,
-.., 200 the reference's contour lies inside the ensemble's 2.5-97.5 %
-band, and the evidence of its 200 dead points lies within 2 sigma of the
-ensemble's (sigma the run-to-run spread)."""
+1.5, step cap 0.5).  Its initial cells follow ``random_initialise_cell``:
+one cubic cell per run drawn from the volume prior, each walker's cell a
+50-step volume-preserving shear/stretch walk from it, so all 32 walkers start
+at one volume.  The draws differ (the JAX PRNG stream cannot be reproduced
+offline), so the checks are distributional:
+
+* the reference's setup, 128 independent runs (8 engines of 16 replicas,
+  each engine with its own seed and so its own move schedule): at
+  iterations 20, 40, ..,
+  200 the reference run's contour lies inside their 2.5-97.5 % band, and the
+  evidence of its 200 dead points within 2 sigma of theirs (sigma the
+  run-to-run spread);
+* the same moves started from a true prior sample (each walker's volume from
+  the prior, positions uniform) with every live walker walked each iteration
+  (n_extra = K - 2, so the clones decorrelate within an iteration; one
+  10-step walk of the clone alone lags the schedule below by a few tenths
+  after 150 iterations), 128 runs: the yardstick is the prior
+  itself, sampled directly (V with density V^N on [12, 240], positions
+  uniform; 2^21 configurations evaluated on the GPU).  Nested sampling puts
+  the contour of iteration i - 1 at the prior quantile X_i of i dead points,
+  log X_i a sum of i terms log Beta(K, 1) (mean -i/K, variance i/K^2), so
+  the runs' median contour must lie inside the band of H at log X_i +- 2 sd
+  at every checkpoint."""
 
 from pathlib import Path
 
@@ -47,29 +65,70 @@ def _log_z(h):
     return np.logaddexp.reduce(pm.log_dead_weights(len(h), K, 1) - h)
 
 
-def test_reference_lj8_run_lies_in_the_ensemble(tmp_path):
-    print(f"LJ-8 NPT reference run vs {N_RUNS} independent GPU runs", flush=True)
+def _prior_contour_band(n_samples=1 << 21, chunk=1 << 18, seed=20263):
+    """H = U + P V of configurations drawn from the prior (V ~ V^N on
+    [12, 240], positions uniform in the cubic cell), sorted."""
+    lj = LJBackend(1.0, 1.0, 2.5)
+    rng = np.random.default_rng(seed)
+    v_lo, v_hi, p = 1.5 * N, 30.0 * N, float(N + 1)
+    out = []
+    for _ in range(n_samples // chunk):
+        a = (v_lo / v_hi) ** p
+        vol = v_hi * (a + rng.random(chunk) * (1.0 - a)) ** (1.0 / p)
+        L = vol ** (1.0 / 3.0)
+        pos = rng.random((chunk, N, 3)) * L[:, None, None]
+        cell = np.eye(3)[None] * L[:, None, None]
+        _, H, _ = lj._eval(pos, np.zeros((chunk, N), np.int32), cell, pressure=0.1,
+                           forces=False)
+        out.append(H)
+    return np.sort(np.concatenate(out))
+
+
+def _runs(seed, reference_init, n_extra=0, n_engines=8):
+    """Contours and dead volumes (iterations, runs) of N_RUNS runs, spread
+    over n_engines engines with their own seeds: the move schedule is shared
+    by the replicas of one engine, so one engine alone would give its runs
+    one common sequence of move types."""
+    per = N_RUNS // n_engines
+    hs, vs = [], []
+    for e in range(n_engines):
+        sd = seed + 10 * e
+        run = RunConfig(n_replicas=per, n_walkers=K, n_atoms=N, n_mcmc_steps=10,
+                        n_extra=n_extra, seed=sd, pressures=[0.1] * per, re_interval=0)
+        eng = NSEngine(Hamiltonian(LJBackend(1.0, 1.0, 2.5), [EnsembleTerm(0.1)]), MOVES, run,
+                       cell=CELL)
+        rng = np.random.default_rng(sd + 1)
+        if reference_init:
+            eng.init_walkers(rng, 1.5, 30.0, flat_V_prior=False, grid_distance=1.0,
+                             shared_volume=True, shape_walk_steps=50, shape_step=0.05,
+                             min_aspect=0.6)
+        else:
+            eng.init_walkers(rng, 1.5, 30.0, flat_V_prior=False)
+        initial_walk(eng, n_walks=1, walklength=5, adjust_interval=1, emax_offset_per_atom=1.0,
+                     adaptation=ADAPT, rng=rng)
+        res = run_ns(eng, [IterationTermination(N_ITER)], adaptation=ADAPT, seed=sd + 2)
+        assert res.iterations == N_ITER
+        eng.close()
+        hs.append(np.asarray(res.emax))
+        vs.append(np.asarray(res.dead_volume))
+    return np.concatenate(hs, axis=1), np.concatenate(vs, axis=1)
+
+
+def test_reference_lj8_run_lies_in_the_ensemble():
+    print(f"LJ-8 NPT reference run vs {N_RUNS} independent GPU runs of its setup", flush=True)
     ref = EnergyLogger.read(FIXTURE)
     ref_h = np.asarray(ref["energies"], dtype=np.float64)
     assert len(ref_h) == N_ITER and np.array_equal(ref["iterations"], np.arange(N_ITER))
-    run = RunConfig(n_replicas=N_RUNS, n_walkers=K, n_atoms=N, n_mcmc_steps=10, n_extra=0,
-                    seed=20260, pressures=[0.1] * N_RUNS, re_interval=0)
-    eng = NSEngine(Hamiltonian(LJBackend(1.0, 1.0, 2.5), [EnsembleTerm(0.1)]), MOVES, run,
-                   cell=CELL)
-    rng = np.random.default_rng(20261)
-    eng.init_walkers(rng, 1.5, 30.0, flat_V_prior=False, grid_distance=1.0)
-    initial_walk(eng, n_walks=1, walklength=5, adjust_interval=1, emax_offset_per_atom=1.0,
-                 adaptation=ADAPT, rng=rng)
-    res = run_ns(eng, [IterationTermination(N_ITER)], adaptation=ADAPT, seed=20262)
-    assert res.iterations == N_ITER
-    h = np.asarray(res.emax)  # (iterations, runs)
+    h, vol = _runs(20260, reference_init=True)
     assert h.shape == (N_ITER, N_RUNS)
     bad = []
+    print("  iter  ensemble median [2.5, 97.5]%   reference H (rank)   median V / reference V",
+          flush=True)
     for t in range(19, N_ITER, 20):
         lo, hi = np.percentile(h[t], [2.5, 97.5])
         rank = 100.0 * np.mean(h[t] < ref_h[t])
-        print(f"  iteration {t + 1:3d}: reference H {ref_h[t]:9.4f}  ensemble 2.5-97.5% "
-              f"[{lo:9.4f}, {hi:9.4f}]  median {np.median(h[t]):9.4f}  rank {rank:5.1f}%",
+        print(f"  {t + 1:4d}  {np.median(h[t]):8.3f} [{lo:8.3f}, {hi:8.3f}]   {ref_h[t]:8.3f} "
+              f"({rank:5.1f}%)   {np.median(vol[t]):7.2f} / {ref['volumes'][t]:7.2f}",
               flush=True)
         if not lo <= ref_h[t] <= hi:
             bad.append(t + 1)
@@ -78,5 +137,27 @@ def test_reference_lj8_run_lies_in_the_ensemble(tmp_path):
     z = (lz_ref - lz.mean()) / lz.std(ddof=1)
     print(f"  log Z of 200 dead points: reference {lz_ref:.4f}, ensemble {lz.mean():.4f} +- "
           f"{lz.std(ddof=1):.4f} (run-to-run), z = {z:+.2f}", flush=True)
-    assert not bad, f"reference contour outside the 95 % band at iterations {bad}"
+    assert not bad, f"reference contour outside the ensemble's 95 % band at iterations {bad}"
     assert abs(z) < 2.0
+
+
+def test_prior_start_follows_the_prior_quantiles():
+    print(f"LJ-8 NPT from a prior sample, {N_RUNS} runs, against the prior itself", flush=True)
+    h, vol = _runs(30260, reference_init=False, n_extra=K - 2)
+    prior_h = _prior_contour_band()
+    m = len(prior_h)
+    off = []
+    print("  iter  prior band (log X +- 2 sd)   ensemble median [2.5, 97.5]%   median V",
+          flush=True)
+    for t in range(19, N_ITER, 20):
+        i = t + 1  # the contour of iteration t is dead point t: X of i = t + 1 points
+        mu, sd = -i / K, np.sqrt(i) / K
+        q_lo = prior_h[int(np.exp(mu - 2 * sd) * m)]
+        q_hi = prior_h[int(np.exp(mu + 2 * sd) * m)]
+        lo, hi = np.percentile(h[t], [2.5, 97.5])
+        med = np.median(h[t])
+        print(f"  {i:4d}  [{q_lo:8.3f}, {q_hi:8.3f}]   {med:8.3f} [{lo:8.3f}, {hi:8.3f}]   "
+              f"{np.median(vol[t]):7.2f}", flush=True)
+        if not q_lo <= med <= q_hi:
+            off.append(i)
+    assert not off, f"median contour outside the prior-quantile band at iterations {off}"

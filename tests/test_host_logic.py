@@ -202,3 +202,19 @@ def test_bisection_is_vectorised_per_replica():
                                 np.array([-1.0, -1.0, -1.0, 0.15]), 0.2, 0.7, 1.5, 10.0)
     np.testing.assert_allclose(ss, [0.5, 0.5 / 1.5, 0.75, 0.3])
     assert conv.tolist() == [True, False, False, True]
+
+
+# ------------------------------------------------------- initial cells
+def test_cell_shape_walk_keeps_volume_and_aspect():
+    from jaxrens_amd.engine import _aspect, cell_shape_walk
+    rng = np.random.default_rng(5)
+    c0 = np.eye(3) * 6.0
+    for _ in range(20):
+        c = cell_shape_walk(rng, c0, 50, 0.05, 0.6)
+        v = abs(np.linalg.det(c))
+        assert v == pytest.approx(216.0, rel=1e-12)
+        assert _aspect(c, v) >= 0.6
+        assert not np.allclose(c, c0)  # the shape moved
+    # a cube has aspect 1; a flat cell is refused
+    assert _aspect(c0, 216.0) == pytest.approx(1.0)
+    assert cell_shape_walk(rng, c0, 200, 5.0, 0.999) is not None

@@ -128,6 +128,48 @@ def test_hmc_energy_cut_uses_the_judge():
         assert rs == want
 
 
+class ConstForce(Flat):
+    """Stand-in surrogate with a uniform force f on every atom (energy
+    linear in the positions, its value irrelevant here)."""
+
+    def __init__(self, f, e_judge=0.0):
+        super().__init__(e_judge=e_judge)
+        self.f = np.asarray(f, dtype=float)
+
+    def sur(self, pos, ty, cell, forces):
+        return 0.0, (np.broadcast_to(self.f, pos.shape).copy() if forces else None)
+
+
+def test_screened_hmc_is_metropolis_on_the_kinetic_energy():
+    """Screened: the surrogate's forces drive the leapfrog and the move is
+    accepted iff the judge's H' < emax and u < exp(K - K'), whatever H' is
+    below the contour (the constrained prior is flat in x).  With a uniform
+    surrogate force f each of the L leapfrog steps adds s f to every
+    momentum, so K' = |p + L s f|^2 / 2 in closed form; u just below and
+    just above exp(K - K') decide the move, a judge H' lower than H leaves
+    the bar where it is, and a judge H' above the contour rejects whatever
+    u.  (The plain rule's hard cut on H' + K' - H - K would accept the
+    second case.)"""
+    rng = np.random.default_rng(8)
+    xi = rng.normal(size=(3, 5))
+    p = xi.T
+    f, s, L = 2.0 * p.sum(0) / np.linalg.norm(p.sum(0)), 0.01, HMC["n_leapfrog"]
+    K0 = 0.5 * np.sum(p * p)
+    K1 = 0.5 * np.sum((p + L * s * f) ** 2)
+    bar = np.exp(K0 - K1)
+    assert 0.3 < bar < 0.95  # the force raises K: a real test
+    cases = ((bar * (1 - 1e-9), 0.0, 0), (bar * (1 + 1e-9), 0.0, 1),
+             (bar * (1 - 1e-9), -0.5, 0), (bar * (1 + 1e-9), -0.5, 1),
+             (0.0, 20.0, 1))
+    for u, e_judge, want in cases:
+        ch = chain(rng, H=0.0, Hs=0.0)
+        rs, ne, ng = OW.hmc(ch, HMC, s, xi, ConstForce(f, e_judge), 10.0, 0.0,
+                            (np.inf, np.inf), True, stats(), [], True, u0=u)
+        assert rs == want, (u, e_judge)
+        assert (ne, ng) == (1, 0)  # one judge energy, no judge force
+        assert ch["H"] == (e_judge if want == 0 else 0.0)
+
+
 @pytest.mark.parametrize("move", ["gmc", "hmc"])
 def test_stage1_is_symmetric_and_screens_before_the_judge(move):
     """Screened: when the surrogate's verdicts at x and y differ the proposal
