@@ -466,7 +466,13 @@ def main():
     ap.add_argument("--lj128-only", action="store_true",
                     help="run only the LJ-128 HMC surrogate report")
     ap.add_argument("--skip-lj128", action="store_true")
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--launch", default="1,3",
+                    help="walk launch shape 'waves_per_block,occupancy' of the headline "
+                         "(the fastest in a sweep of 1/2/4 waves x occupancy 0/3)")
+    ap.add_argument("--tune", action="store_true",
+                    help="pick the headline launch shape with the autotuner instead")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="no autotuning for the secondary FP32-pair engine")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -488,7 +494,12 @@ def main():
 
     eng = build_engine(n_extra=1022)
     R, n_walk, T = eng.n_replicas, eng.n_walk, eng.run.n_mcmc_steps
-    tune = eng.autotune(reps=2) if not args.no_tune else None
+    if args.tune:
+        tune = eng.autotune(reps=2)
+    else:
+        wpb, occ = (int(x) for x in args.launch.split(","))
+        eng.set_launch(wpb, occ)
+        tune = {"waves_per_block": wpb, "occupancy": occ, "autotuned": False}
     print(f"[bench] engine ready: R={R} K={eng.n_walkers} N={eng.Nmax} "
           f"n_walk={n_walk} tune={tune} launch={eng.launch_info()}",
           file=sys.stderr, flush=True)
