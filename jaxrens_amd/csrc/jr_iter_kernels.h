@@ -465,6 +465,75 @@ __global__ void schedule_kernel(int* sched, int n_steps,
   }
 }
 
+// The iteration's selection, clone, extra, schedule and replica-exchange
+// draws in one launch: the same Philox values as fill_uniform_kernel on
+// streams 1, 2, 3, 7 and schedule_kernel on stream 4, over one flat index
+// space of uniform pairs [sel | clone | extra | re] followed by the schedule.
+struct DrawArgs {
+  double* sel_u;
+  double* clone_u;
+  double* extra_u;  // NULL: no extras drawn
+  double* re_u;     // NULL: one replica
+  int* sched;
+  const double* cum;
+  const IterState* st;
+  unsigned long long seed;
+  long long n_sel, n_clone, n_extra, n_re;  // values per buffer
+  int n_steps, n_moves;
+};
+
+__device__ __forceinline__ void draw_pair(double* out, long long n, long long p,
+                                          unsigned long long seed, long long it,
+                                          unsigned stream) {
+  uint32_t c[4] = {(uint32_t)p, (uint32_t)(p >> 32), (uint32_t)it, stream};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  out[2 * p] = u53(c[0], c[1]);
+  if (2 * p + 1 < n) out[2 * p + 1] = u53(c[2], c[3]);
+}
+
+__global__ void iter_draws_kernel(DrawArgs a) {
+  const long long it = a.st->rng_iter;
+  const long long p_sel = (a.n_sel + 1) >> 1, p_clone = (a.n_clone + 1) >> 1;
+  const long long p_extra = a.extra_u ? (a.n_extra + 1) >> 1 : 0;
+  const long long p_re = a.re_u ? (a.n_re + 1) >> 1 : 0;
+  const long long b1 = p_sel, b2 = b1 + p_clone, b3 = b2 + p_extra, b4 = b3 + p_re;
+  const long long total = b4 + a.n_steps;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (long long)gridDim.x * blockDim.x) {
+    if (g < b1) {
+      draw_pair(a.sel_u, a.n_sel, g, a.seed, it, 1u);
+    } else if (g < b2) {
+      draw_pair(a.clone_u, a.n_clone, g - b1, a.seed, it, 2u);
+    } else if (g < b3) {
+      draw_pair(a.extra_u, a.n_extra, g - b2, a.seed, it, 3u);
+    } else if (g < b4) {
+      draw_pair(a.re_u, a.n_re, g - b3, a.seed, it, 7u);
+    } else {
+      const int t = (int)(g - b4);
+      uint32_t c[4] = {(uint32_t)t, 0u, (uint32_t)it, 4u};
+      philox4x32_10(c, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+      const double u = u53(c[0], c[1]);
+      int m = 0;
+      while (m < a.n_moves - 1 && !(u < a.cum[m])) ++m;
+      a.sched[t] = m;
+    }
+  }
+}
+
+// Per-iteration output snapshot (move counters and walked indices) into the
+// block's output slot: a compute-queue copy, so no copy engine has to be
+// synchronised with the walk stream each iteration.
+__global__ void snapshot_kernel(const int* counters, int nc, const int* walk, int nw,
+                                int* o_counters, int* o_walk) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc + nw;
+       i += gridDim.x * blockDim.x) {
+    if (i < nc)
+      o_counters[i] = counters[i];
+    else
+      o_walk[i - nc] = walk[i - nc];
+  }
+}
+
 __global__ void advance_kernel(IterState* st) {
   if (threadIdx.x == 0) {
     st->i += 1;

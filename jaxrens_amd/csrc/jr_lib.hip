@@ -164,7 +164,10 @@ int ensure_out(jr_handle* h, int n_iter) {
   if (n_iter <= h->out_cap) return JR_OK;
   const auto& c = h->cfg;
   const int R = c.n_replicas, Nm = c.n_atoms_max;
-  size_t n = (size_t)n_iter;
+  // capacity in powers of two from 64 iterations, so that ordinary blocks
+  // allocate nothing once the handle exists
+  size_t n = 64;
+  while (n < (size_t)n_iter) n *= 2;
   const size_t D = n * R * c.n_cull;  // dead points in the block
   HIPCHK(h, h->o_emax.alloc(n * R));
   HIPCHK(h, h->o_dead_energy.alloc(D));
@@ -178,7 +181,7 @@ int ensure_out(jr_handle* h, int n_iter) {
   HIPCHK(h, h->o_re_acc.alloc(n * std::max(R - 1, 1)));
   HIPCHK(h, h->o_re_att.alloc(n * std::max(R - 1, 1)));
   HIPCHK(h, h->o_walk.alloc(n * R * h->n_walk));
-  h->out_cap = n_iter;
+  h->out_cap = (int)n;
   if (h->graph_exec) {
     (void)hipGraphExecDestroy(h->graph_exec);
     h->graph_exec = nullptr;
@@ -386,15 +389,27 @@ void enqueue_draws(jr_handle* h, long long it) {
   hipStream_t s = h->stream;
   IterState* st = h->st.p;
   unsigned long long seed = c.seed;
-  hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for((size_t)R * K)), dim3(256), 0, s,
-                     h->sel_u.p, (long long)R * K, seed, st, 0LL, 1u);
-  hipLaunchKernelGGL(fill_uniform_kernel, dim3(1), dim3(256), 0, s, h->clone_u.p,
-                     (long long)R, seed, st, 0LL, 2u);
-  if (uses_extra_u(c))
-    hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for((size_t)R * K)), dim3(256), 0, s,
-                       h->extra_u.p, (long long)R * K, seed, st, 0LL, 3u);
-  hipLaunchKernelGGL(schedule_kernel, dim3(1), dim3(64), 0, s, h->sched.p,
-                     c.n_mcmc_steps, seed, st, 0LL, 4u, h->cum.p, c.n_moves);
+  DrawArgs da;
+  da.sel_u = h->sel_u.p;
+  da.clone_u = h->clone_u.p;
+  da.extra_u = uses_extra_u(c) ? h->extra_u.p : nullptr;
+  da.re_u = c.n_replicas > 1 ? h->re_u.p : nullptr;
+  da.sched = h->sched.p;
+  da.cum = h->cum.p;
+  da.st = st;
+  da.seed = seed;
+  da.n_sel = (long long)R * K;
+  da.n_clone = R;
+  da.n_extra = (long long)R * K;
+  da.n_re = c.n_replicas > 1 ? (long long)n_re_u(h) : 0;
+  da.n_steps = c.n_mcmc_steps;
+  da.n_moves = c.n_moves;
+  {
+    const size_t pairs = (size_t)(da.n_sel + 1) / 2 + (size_t)(R + 1) / 2 +
+                         (da.extra_u ? (size_t)(da.n_extra + 1) / 2 : 0) +
+                         (size_t)(da.n_re + 1) / 2 + (size_t)c.n_mcmc_steps;
+    hipLaunchKernelGGL(iter_draws_kernel, dim3(grid_for(pairs)), dim3(256), 0, s, da);
+  }
   if (h->learned) {
     // the learned walk's kernels read the draws from the tape buffers
     hipLaunchKernelGGL(fill_normal_kernel, dim3(grid_for(n_normals(h))), dim3(256), 0, s,
@@ -406,9 +421,6 @@ void enqueue_draws(jr_handle* h, long long it) {
     // the pair walk draws its normals (stream 5) and uniforms (stream 6) in place
     h->draws_buffered = false;
   }
-  if (c.n_replicas > 1)
-    hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(n_re_u(h))), dim3(256), 0, s,
-                       h->re_u.p, (long long)n_re_u(h), seed, st, 0LL, 7u);
 }
 
 // The walk of one iteration: a single persistent launch, or with replicas of
@@ -465,17 +477,19 @@ int enqueue_iteration_core(jr_handle* h, bool timed, int ev_idx) {
 
 }  // namespace
 
-// counters/walk snapshots are copied device-to-device after each iteration
-// (outside the graph path they are plain async copies).
+// counters/walk snapshots go into the block's output slot after each
+// iteration (after every graph replay on the graph path), by a copy kernel
+// on the walk stream.
 static int snapshot_iter(jr_handle* h, int it) {
   const auto& c = h->cfg;
   const int R = c.n_replicas;
-  size_t nc = (size_t)R * c.n_moves * JR_NCOUNT;
-  HIPCHK(h, hipMemcpyAsync(h->o_counters.p + (size_t)it * nc, h->counters.p,
-                           nc * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
-  size_t nw = (size_t)R * h->n_walk;
-  HIPCHK(h, hipMemcpyAsync(h->o_walk.p + (size_t)it * nw, h->walk.p,
-                           nw * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
+  const size_t nc = (size_t)R * c.n_moves * JR_NCOUNT;
+  const size_t nw = (size_t)R * h->n_walk;
+  const unsigned blocks = (unsigned)std::min<size_t>(256, (nc + nw + 255) / 256);
+  hipLaunchKernelGGL(snapshot_kernel, dim3(blocks), dim3(256), 0, h->stream, h->counters.p,
+                     (int)nc, h->walk.p, (int)nw, h->o_counters.p + (size_t)it * nc,
+                     h->o_walk.p + (size_t)it * nw);
+  HIPCHK(h, hipGetLastError());
   return JR_OK;
 }
 
@@ -968,7 +982,7 @@ int begin_block(jr_handle* h, int n_iter) {
   HIPCHK(h, hipMemcpyAsync(h->st.p, &st, sizeof(st), hipMemcpyHostToDevice, h->stream));
   if ((int)h->ev.size() < 2 * n_iter) {
     size_t old = h->ev.size();
-    h->ev.resize(2 * n_iter);
+    h->ev.resize(std::max(2 * n_iter, 2 * h->out_cap));
     for (size_t i = old; i < h->ev.size(); ++i) HIPCHK(h, hipEventCreate(&h->ev[i]));
   }
   return JR_OK;

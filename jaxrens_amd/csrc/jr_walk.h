@@ -135,13 +135,13 @@ __device__ __forceinline__ void pair_lj(double q, double eps4, double eps24,
                                         double sig2, double& e, double& fs) {
   double s2 = MULTI ? sig2 * q : q;
   double s6 = (s2 * s2) * s2;
-  double s12 = s6 * s6;
+  const double el = fma(s6, s6, -s6);  // s^12 - s^6
   if (MULTI) {
-    e = eps4 * (s12 - s6);
-    if (FORCES) fs = (eps24 * fma(2.0, s12, -s6)) * q;
+    e = eps4 * el;
+    if (FORCES) fs = (eps24 * fma(s6, s6, el)) * q;  // 2 s^12 - s^6
   } else {
-    e = s12 - s6;
-    if (FORCES) fs = fma(2.0, s12, -s6) * q;
+    e = el;
+    if (FORCES) fs = fma(s6, s6, el) * q;
   }
 }
 
@@ -151,13 +151,13 @@ __device__ __forceinline__ void pair_lj_f32(float q, float eps4, float eps24,
                                             float sig2, float& e, float& fs) {
   float s2 = MULTI ? sig2 * q : q;
   float s6 = (s2 * s2) * s2;
-  float s12 = s6 * s6;
+  const float el = fmaf(s6, s6, -s6);
   if (MULTI) {
-    e = eps4 * (s12 - s6);
-    if (FORCES) fs = (eps24 * fmaf(2.0f, s12, -s6)) * q;
+    e = eps4 * el;
+    if (FORCES) fs = (eps24 * fmaf(s6, s6, el)) * q;
   } else {
-    e = s12 - s6;
-    if (FORCES) fs = fmaf(2.0f, s12, -s6) * q;
+    e = el;
+    if (FORCES) fs = fmaf(s6, s6, el) * q;
   }
 }
 

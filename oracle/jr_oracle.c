@@ -214,13 +214,13 @@ static inline void pair_lj(const pair_t* p, double q, int multi, double* e,
                            double* fs) {
   double s2 = multi ? p->sig2 * q : q;
   double s6 = (s2 * s2) * s2;
-  double s12 = s6 * s6;
+  double el = fma(s6, s6, -s6); /* s^12 - s^6 */
   if (multi) {
-    *e = p->eps4 * (s12 - s6);
-    *fs = (p->eps24 * fma(2.0, s12, -s6)) * q;
+    *e = p->eps4 * el;
+    *fs = (p->eps24 * fma(s6, s6, el)) * q; /* 2 s^12 - s^6 */
   } else {
-    *e = s12 - s6;
-    *fs = fma(2.0, s12, -s6) * q;
+    *e = el;
+    *fs = fma(s6, s6, el) * q;
   }
 }
 
