@@ -85,11 +85,16 @@ typedef struct {
   int32_t flat_v_prior;/* volume */
   int32_t n_species;   /* alchemical_morph */
   int32_t species_mask;/* species_swap: bit c set = type code c may be
-                          exchanged; 0 = every species */
+                          exchanged; gmc: bit c set = atoms of type code c
+                          move, the rest stay fixed; 0 = every species */
   double perturb_angle;/* gmc */
   double max_vpa;      /* volume, shear, stretch: cell-geometry guard */
   double min_vpa;
   double min_aspect;
+  int32_t refresh_direction; /* gmc: fresh random direction every call */
+  int32_t dir_slot;    /* gmc: persistent-direction slot; 0 = the
+                          population's direction, 1.. = extra slots (one per
+                          species-scoped move), < JR_MAX_MOVES */
 } jr_move;
 
 /* Seeded random-init invariant message-passing potential, the expensive
@@ -274,6 +279,12 @@ int jr_set_population(jr_handle* h, const double* pos, const int32_t* types,
 int jr_get_population(jr_handle* h, double* pos, int32_t* types,
                       int32_t* n_atoms, double* cell, double* energy,
                       double* dir);
+/* Extra persistent GMC directions (galilean.py:107-131 direction_field,
+ * cli/schema/moves.py GMCMoveSpec._direction_field: each species-scoped move
+ * keeps its own), slot 1 .. max dir_slot of the moves, [R*K][Nmax][3].
+ * jr_set_population zeroes them; get/set carry them across checkpoints. */
+int jr_get_direction_slot(jr_handle* h, int32_t slot, double* dir);
+int jr_set_direction_slot(jr_handle* h, int32_t slot, const double* dir);
 /* pressure [R], mu [R][S] (NULL = 0), step_sizes [R][n_moves] */
 int jr_set_run_params(jr_handle* h, const double* pressure, const double* mu,
                       const double* step_sizes);

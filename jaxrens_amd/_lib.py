@@ -58,6 +58,8 @@ class JrMove(C.Structure):
         ("max_vpa", C.c_double),
         ("min_vpa", C.c_double),
         ("min_aspect", C.c_double),
+        ("refresh_direction", C.c_int32),
+        ("dir_slot", C.c_int32),
     ]
 
 
@@ -195,6 +197,8 @@ SIGNATURES = [
     ("jr_set_population", C.c_int, [C.c_void_p, _dp, _ip, _ip, _dp, _dp]),
     ("jr_get_population", C.c_int,
      [C.c_void_p, _dp, _ip, _ip, _dp, _dp, _dp]),
+    ("jr_get_direction_slot", C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    ("jr_set_direction_slot", C.c_int, [C.c_void_p, C.c_int32, _dp]),
     ("jr_set_run_params", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     ("jr_get_step_sizes", C.c_int, [C.c_void_p, _dp]),
     ("jr_set_pair_valid", C.c_int, [C.c_void_p, _ip]),

@@ -5,7 +5,8 @@ HDF5 (h5py is not part of this stack).  The field names follow the
 reference's: ``positions``, ``types``, ``energies``, ``cells``,
 ``step_sizes``, ``log_evidence``, ``iteration``, ``emax``, ``n_dead``,
 ``n_walkers``.  Added for an exact resume of this engine: the per-walker
-``n_atoms`` and GMC ``direction``, the run parameters ``pressure`` / ``mu``,
+``n_atoms`` and GMC ``direction`` (with the persistent directions of the
+species-scoped GMC moves, ``direction_slots``), the run parameters ``pressure`` / ``mu``,
 the device draw counters ``rng_state`` (``jr_get_rng_state``; the reference
 stores its PRNG key, ``:187-197``) with the ``seed``, and optionally the
 host generator of the adaptation bisection as JSON.  Files are written and
@@ -53,6 +54,7 @@ def save_checkpoint(path, engine, host_rng: np.random.Generator | None = None) -
         "cells": pop["cell"],
         "energies": pop["energy"],
         "direction": pop["direction"],
+        "direction_slots": pop.get("direction_slots", np.zeros((0,) + pop["direction"].shape)),
         "step_sizes": engine.step_sizes,
         "pressure": engine.pressure,
         "mu": engine.mu,
@@ -98,6 +100,11 @@ def restore(engine, state: dict) -> np.random.Generator | None:
         raise ValueError("checkpoint was written under another seed")
     engine.set_population(state["positions"], state["types"], state["cells"],
                           n_atoms=state["n_atoms"], direction=state["direction"])
+    slots = state.get("direction_slots")
+    if slots is not None and len(slots) != engine.mwg.n_dir_slots - 1:
+        raise ValueError("checkpoint direction slots do not match the engine's moves")
+    if slots is not None and len(slots):
+        engine.set_direction_slots(slots)
     engine.set_run_params(pressure=state["pressure"], mu=state["mu"],
                           step_sizes=state["step_sizes"])
     engine.iteration = state["iteration"]

@@ -48,6 +48,8 @@ struct jr_handle {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // population
   jrh::DBuf<double> pos, dir, cell, energy;
+  jrh::DBuf<double> dirx;  // [n_xslots][R*K][Nmax][3] scoped-GMC directions
+  int n_xslots = 0;
   jrh::DBuf<int> types, n_atoms;
   bool have_pop = false;
   // run params

@@ -185,6 +185,10 @@ __global__ void __launch_bounds__(1024) select_kernel(SelArgs a) {
   for (int i = tid; i < a.Nmax * 3; i += nt) {
     a.pop.pos[ws * a.Nmax * 3 + i] = a.pop.pos[cs * a.Nmax * 3 + i];
     a.pop.dir[ws * a.Nmax * 3 + i] = a.pop.dir[cs * a.Nmax * 3 + i];
+    for (int e = 0; e < a.pop.n_xslots; ++e) {
+      double* X = a.pop.dirx + e * a.pop.xstride;
+      X[ws * a.Nmax * 3 + i] = X[cs * a.Nmax * 3 + i];
+    }
   }
   for (int i = tid; i < a.Nmax; i += nt)
     a.pop.types[ws * a.Nmax + i] = a.pop.types[cs * a.Nmax + i];
@@ -281,6 +285,10 @@ __global__ void __launch_bounds__(1024) select_cull_kernel(SelArgs a) {
     const size_t ws = base + culled[c], cs = base + sidx[c];
     a.pop.pos[ws * n3 + q] = a.pop.pos[cs * n3 + q];
     a.pop.dir[ws * n3 + q] = a.pop.dir[cs * n3 + q];
+    for (int e = 0; e < a.pop.n_xslots; ++e) {
+      double* X = a.pop.dirx + e * a.pop.xstride;
+      X[ws * n3 + q] = X[cs * n3 + q];
+    }
   }
   for (int i = tid; i < C * a.Nmax; i += nt) {
     const int c = i / a.Nmax, q = i - c * a.Nmax;
@@ -391,6 +399,12 @@ __global__ void __launch_bounds__(1024) re_kernel(ReArgs a) {
             a.pop.pos[sj * n3 + k] = p0;
             a.pop.dir[si * n3 + k] = d1;
             a.pop.dir[sj * n3 + k] = d0;
+            for (int e = 0; e < a.pop.n_xslots; ++e) {
+              double* X = a.pop.dirx + e * a.pop.xstride;
+              const double x0 = X[si * n3 + k], x1 = X[sj * n3 + k];
+              X[si * n3 + k] = x1;
+              X[sj * n3 + k] = x0;
+            }
           }
           for (int k = lane; k < a.Nmax; k += 64) {
             const int t0 = a.pop.types[si * a.Nmax + k], t1 = a.pop.types[sj * a.Nmax + k];

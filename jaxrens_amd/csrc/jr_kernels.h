@@ -46,6 +46,10 @@ struct WalkArgs {
   // 5 and 6 of iteration st->rng_iter under seed; see step_draws)
   const IterState* st;
   unsigned long long seed;
+  // persistent directions of the species-scoped GMC moves: slot s > 0 of
+  // walker w at dirx + ((s - 1) * n_pop + w) * Nmax * 3 (NULL: none)
+  double* dirx;
+  int n_pop;
 };
 
 // Draws of sub-step t of chain tq in place of the tape (normals
@@ -236,10 +240,25 @@ __global__ void __launch_bounds__(256)
         if (floors) W0 = W;
       }
       switch (type) {
-        case JR_MOVE_GMC:
+        case JR_MOVE_GMC: {
+          // a scoped move's own persistent direction (galilean.py:107-131
+          // direction_field); the population's stays aside in memory
+          const bool own = mv.dir_slot > 0;
+          double* Dh = a.pop.dir + (size_t)slot * a.Nmax * 3;
+          double* Ds = own ? a.dirx + ((size_t)(mv.dir_slot - 1) * a.n_pop + slot) * a.Nmax * 3
+                           : nullptr;
+          if (own) {
+            if (!a.trial) store_dir<A>(W, Dh, N);
+            load_dir<A>(W, Ds, N);
+          }
           reason = gmc_move<A, CFG>(C, W, step, mv, xi) ? 0 : 1;
+          if (own && !a.trial) {
+            store_dir<A>(W, Ds, N);
+            load_dir<A>(W, Dh, N);
+          }
           n_eval = n_grad = mv.n_reflect;
           break;
+        }
         case JR_MOVE_VOLUME:  // with cfg bit 3: also shear and stretch
           if constexpr ((CFG & 8) != 0)
             reason = cell_move<A, CFG>(C, W, step, mv, xv0, xv1, u[0]);

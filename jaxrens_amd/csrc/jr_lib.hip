@@ -236,11 +236,19 @@ int recompute_energies(jr_handle* h) {
   return JR_OK;
 }
 
+PopPtrs pop_ptrs(jr_handle* h) {
+  PopPtrs p{h->pos.p, h->types.p, h->n_atoms.p, h->cell.p, h->energy.p, h->dir.p};
+  p.dirx = h->dirx.p;
+  p.n_xslots = h->n_xslots;
+  p.xstride = (size_t)h->cfg.n_replicas * h->cfg.n_walkers * h->cfg.n_atoms_max * 3;
+  return p;
+}
+
 WalkArgs walk_args(jr_handle* h) {
   const auto& c = h->cfg;
   WalkArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.pop = PopPtrs{h->pos.p, h->types.p, h->n_atoms.p, h->cell.p, h->energy.p, h->dir.p};
+  a.pop = pop_ptrs(h);
   a.pressure = h->pressure.p;
   a.mu = h->mu.p;
   a.step_sizes = h->step_sizes.p;
@@ -270,6 +278,8 @@ WalkArgs walk_args(jr_handle* h) {
   // device draws: the walk draws its normals and uniforms in place
   a.st = h->st.p;
   a.seed = c.seed;
+  a.dirx = h->dirx.p;
+  a.n_pop = c.n_replicas * c.n_walkers;
   if (!h->draws_buffered) {
     a.normals = nullptr;
     a.uniforms = nullptr;
@@ -286,7 +296,7 @@ SelArgs sel_args(jr_handle* h) {
   const auto& c = h->cfg;
   SelArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.pop = PopPtrs{h->pos.p, h->types.p, h->n_atoms.p, h->cell.p, h->energy.p, h->dir.p};
+  a.pop = pop_ptrs(h);
   a.R = c.n_replicas;
   a.K = c.n_walkers;
   a.Nmax = c.n_atoms_max;
@@ -331,7 +341,7 @@ ReArgs re_args(jr_handle* h, IterState* st) {
   const auto& c = h->cfg;
   ReArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.pop = PopPtrs{h->pos.p, h->types.p, h->n_atoms.p, h->cell.p, h->energy.p, h->dir.p};
+  a.pop = pop_ptrs(h);
   a.R = c.n_replicas;
   a.K = c.n_walkers;
   a.Nmax = c.n_atoms_max;
@@ -628,6 +638,19 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
       return set_global_err(JR_E_ARG, "alchemical_morph needs 2 <= n_species <= S");
     if (mv.type == JR_MOVE_GMC && mv.n_reflect < 1)
       return set_global_err(JR_E_ARG, "gmc needs n_reflect >= 1");
+    if (mv.type == JR_MOVE_GMC) {
+      if (mv.species_mask & ~((1 << c.pot.n_species) - 1))
+        return set_global_err(JR_E_ARG, "gmc species scope names codes >= S = %d",
+                              c.pot.n_species);
+      if (mv.dir_slot < 0 || mv.dir_slot >= JR_MAX_MOVES)
+        return set_global_err(JR_E_ARG, "gmc dir_slot must be in [0, %d)", JR_MAX_MOVES);
+      if ((c.mpnn || c.learned) && (mv.species_mask || mv.refresh_direction || mv.dir_slot))
+        return set_global_err(JR_E_UNSUPPORTED,
+                              "species-scoped gmc and refresh_direction are built for the "
+                              "pair-potential walk");
+    } else if (mv.dir_slot != 0 || mv.refresh_direction) {
+      return set_global_err(JR_E_ARG, "move %d: dir_slot and refresh_direction are gmc-only", m);
+    }
     if ((mv.type == JR_MOVE_SHEAR || mv.type == JR_MOVE_STRETCH || mv.type == JR_MOVE_SWAP) &&
         !c.mpnn && c.pot.precision != JR_PREC_FP64)
       return set_global_err(JR_E_UNSUPPORTED,
@@ -682,6 +705,12 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   const size_t W = (size_t)R * K;
   TRY(h->pos.alloc(W * Nm * 3));
   TRY(h->dir.alloc(W * Nm * 3));
+  for (int m = 0; m < c.n_moves; ++m)
+    if (c.moves[m].type == JR_MOVE_GMC) h->n_xslots = std::max(h->n_xslots, c.moves[m].dir_slot);
+  if (h->n_xslots) {
+    TRY(h->dirx.alloc((size_t)h->n_xslots * W * Nm * 3));
+    TRY(hipMemset(h->dirx.p, 0, sizeof(double) * h->n_xslots * W * Nm * 3));
+  }
   TRY(h->cell.alloc(W * 9));
   TRY(h->energy.alloc(W));
   TRY(h->types.alloc(W * Nm));
@@ -731,8 +760,10 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
   for (int m = 0; m < c.n_moves; ++m) {
     const jr_move& mv = c.moves[m];
     dm[m] = DevMove{mv.type, mv.n_reflect, mv.n_leapfrog, mv.flat_v_prior, mv.n_species,
-                    mv.type == JR_MOVE_SWAP ? mv.species_mask : 0,
-                    mv.perturb_angle, mv.max_vpa, mv.min_vpa, mv.min_aspect};
+                    (mv.type == JR_MOVE_SWAP || mv.type == JR_MOVE_GMC) ? mv.species_mask : 0,
+                    mv.perturb_angle, mv.max_vpa, mv.min_vpa, mv.min_aspect,
+                    mv.type == JR_MOVE_GMC ? mv.refresh_direction : 0,
+                    mv.type == JR_MOVE_GMC ? mv.dir_slot : 0};
     acc += c.move_weights[m] / wsum;
     cum[m] = acc;
   }
@@ -809,6 +840,9 @@ int jr_set_population(jr_handle* h, const double* pos, const int32_t* types,
     HIPCHK(h, hipMemcpyAsync(h->dir.p, dir, sizeof(double) * W * Nm * 3, hipMemcpyHostToDevice, h->stream));
   else
     HIPCHK(h, hipMemsetAsync(h->dir.p, 0, sizeof(double) * W * Nm * 3, h->stream));
+  if (h->n_xslots)
+    HIPCHK(h, hipMemsetAsync(h->dirx.p, 0, sizeof(double) * h->n_xslots * W * Nm * 3,
+                             h->stream));
   // default RE validity: equal atom counts per replica (first walker)
   const int R = c.n_replicas, K = c.n_walkers;
   std::vector<int> pv(std::max(R - 1, 1), 1);
@@ -863,6 +897,32 @@ int jr_get_population(jr_handle* h, double* pos, int32_t* types,
   if (cell) HIPCHK(h, hipMemcpyAsync(cell, h->cell.p, sizeof(double) * W * 9, hipMemcpyDeviceToHost, h->stream));
   if (energy) HIPCHK(h, hipMemcpyAsync(energy, h->energy.p, sizeof(double) * W, hipMemcpyDeviceToHost, h->stream));
   if (dir) HIPCHK(h, hipMemcpyAsync(dir, h->dir.p, sizeof(double) * W * Nm * 3, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return JR_OK;
+}
+
+int jr_get_direction_slot(jr_handle* h, int32_t slot, double* dir) {
+  if (!h || !dir) return set_global_err(JR_E_ARG, "NULL argument");
+  if (slot < 1 || slot > h->n_xslots)
+    return h->fail(JR_E_ARG, "direction slot %d: the moves use slots 1..%d", slot, h->n_xslots);
+  const auto& c = h->cfg;
+  const size_t n = (size_t)c.n_replicas * c.n_walkers * c.n_atoms_max * 3;
+  HIPCHK(h, hipSetDevice(c.device));
+  HIPCHK(h, hipMemcpyAsync(dir, h->dirx.p + (size_t)(slot - 1) * n, sizeof(double) * n,
+                           hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return JR_OK;
+}
+
+int jr_set_direction_slot(jr_handle* h, int32_t slot, const double* dir) {
+  if (!h || !dir) return set_global_err(JR_E_ARG, "NULL argument");
+  if (slot < 1 || slot > h->n_xslots)
+    return h->fail(JR_E_ARG, "direction slot %d: the moves use slots 1..%d", slot, h->n_xslots);
+  const auto& c = h->cfg;
+  const size_t n = (size_t)c.n_replicas * c.n_walkers * c.n_atoms_max * 3;
+  HIPCHK(h, hipSetDevice(c.device));
+  HIPCHK(h, hipMemcpyAsync(h->dirx.p + (size_t)(slot - 1) * n, dir, sizeof(double) * n,
+                           hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return JR_OK;
 }
@@ -1366,6 +1426,12 @@ int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
   HIPCHK(h, copy_pop(s_pos.p, s_dir.p, s_cell.p, s_e.p, s_t.p, s_n.p, h->pos.p, h->dir.p,
                      h->cell.p, h->energy.p, h->types.p, h->n_atoms.p));
   HIPCHK(h, hipMemcpyAsync(s_emax.p, h->emax.p, c.n_replicas * 8, hipMemcpyDeviceToDevice, s));
+  DBuf<double> s_dirx;
+  const size_t nx = (size_t)h->n_xslots * W * Nm * 3;
+  if (nx) {
+    HIPCHK(h, s_dirx.alloc(nx));
+    HIPCHK(h, hipMemcpyAsync(s_dirx.p, h->dirx.p, nx * 8, hipMemcpyDeviceToDevice, s));
+  }
   // one iteration's draws and selection, then the post-selection state
   int rc = begin_block(h, 1);
   if (rc) return rc;
@@ -1414,6 +1480,7 @@ int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
   HIPCHK(h, copy_pop(h->pos.p, h->dir.p, h->cell.p, h->energy.p, h->types.p, h->n_atoms.p,
                      s_pos.p, s_dir.p, s_cell.p, s_e.p, s_t.p, s_n.p));
   HIPCHK(h, hipMemcpyAsync(h->emax.p, s_emax.p, c.n_replicas * 8, hipMemcpyDeviceToDevice, s));
+  if (nx) HIPCHK(h, hipMemcpyAsync(h->dirx.p, s_dirx.p, nx * 8, hipMemcpyDeviceToDevice, s));
   HIPCHK(h, hipStreamSynchronize(s));
   if (best_wpb) *best_wpb = bw;
   if (best_occ) *best_occ = bo;

@@ -84,7 +84,38 @@ __device__ bool min_distance_ok(const WalkCtx& C, const WalkerRegs<A>& W,
   return __ballot(bad) == 0;
 }
 
-// sampling/moves/galilean.py:142-300 (use_forces=True, unscoped).
+// A walker's GMC direction to / from memory (active atoms only).
+template <int A>
+__device__ __forceinline__ void store_dir(const WalkerRegs<A>& W, double* D, int N) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    const int at = lane + 64 * s;
+    if (at < N) {
+      D[3 * at + 0] = W.dx[s];
+      D[3 * at + 1] = W.dy[s];
+      D[3 * at + 2] = W.dz[s];
+    }
+  }
+}
+
+template <int A>
+__device__ __forceinline__ void load_dir(WalkerRegs<A>& W, const double* D, int N) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    const int at = lane + 64 * s;
+    W.dx[s] = at < N ? D[3 * at + 0] : 0.0;
+    W.dy[s] = at < N ? D[3 * at + 1] : 0.0;
+    W.dz[s] = at < N ? D[3 * at + 2] : 0.0;
+  }
+}
+
+// sampling/moves/galilean.py:142-300 (use_forces=True).  A species scope
+// (mv.species_mask != 0) multiplies the direction before each normalisation
+// and the reflection force by the moving-atom mask (1 for types in scope,
+// else 0; galilean.py:58-72,171-180,233-240), so the other atoms stay fixed;
+// refresh_direction draws a fresh direction on every call.
 template <int A, int CFG>
 __device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
                          const DevMove& mv, const double* xi) {
@@ -93,8 +124,11 @@ __device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
   (void)F32;
   const int lane = lane_id();
   const int N = C.N;
+  const bool scoped = mv.species_mask != 0;
+  // moving-atom mask m_a of a scoped move
+  auto msk = [&](int s) -> double { return ((mv.species_mask >> W.t[s]) & 1) ? 1.0 : 0.0; };
   double dn = sqrt(wave_dot<A>(N, W.dx, W.dy, W.dz, W.dx, W.dy, W.dz));
-  const bool fresh = dn < 1e-8;
+  const bool fresh = dn < 1e-8 || mv.refresh_direction;
   double vx[A], vy[A], vz[A];
 #pragma unroll
   for (int s = 0; s < A; ++s) {
@@ -113,6 +147,12 @@ __device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
       vx[s] = W.dx[s] + mv.perturb_angle * zx;
       vy[s] = W.dy[s] + mv.perturb_angle * zy;
       vz[s] = W.dz[s] + mv.perturb_angle * zz;
+    }
+    if (scoped) {
+      const double m = msk(s);
+      vx[s] = vx[s] * m;
+      vy[s] = vy[s] * m;
+      vz[s] = vz[s] * m;
     }
   }
   double vn = sqrt(wave_dot<A>(N, vx, vy, vz, vx, vy, vz));
@@ -146,6 +186,15 @@ __device__ bool gmc_move(const WalkCtx& C, WalkerRegs<A>& W, double step,
     double Hn = U + ens;
     if (__builtin_isnan(Hn)) Hn = __builtin_inf();
     if (Hn >= C.emax) {  // wave-uniform: the force sums only where a reflection needs them
+      if (scoped) {
+#pragma unroll
+        for (int s = 0; s < A; ++s) {
+          const double m = msk(s);
+          fx[s] = fx[s] * m;
+          fy[s] = fy[s] * m;
+          fz[s] = fz[s] * m;
+        }
+      }
       double fsq = wave_dot<A>(N, fx, fy, fz, fx, fy, fz);
       double fdt = wave_dot<A>(N, fx, fy, fz, W.dx, W.dy, W.dz);
       double fn = fmax(sqrt(fsq), 1e-10);

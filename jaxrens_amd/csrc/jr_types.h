@@ -11,6 +11,7 @@ constexpr int JR_NQUEUE = 4;
 struct DevMove {
   int type, n_reflect, n_leapfrog, flat_v_prior, n_species, species_mask;
   double perturb_angle, max_vpa, min_vpa, min_aspect;
+  int refresh_direction, dir_slot;  // gmc
 };
 
 // Device-resident iteration state, read by every kernel of an iteration so a
@@ -29,6 +30,11 @@ struct PopPtrs {
   double* cell;
   double* energy;
   double* dir;
+  // directions of the species-scoped GMC moves, n_xslots slots of xstride
+  // doubles ([R*K][Nmax][3] each); they travel with the walker
+  double* dirx = nullptr;
+  int n_xslots = 0;
+  size_t xstride = 0;
 };
 
 }  // namespace jr

@@ -266,8 +266,28 @@ class NSEngine:
         self._check(self.lib.jr_get_population(
             self.h, _lib.dptr(pos), _lib.iptr(ty), _lib.iptr(na), _lib.dptr(cl),
             _lib.dptr(e), _lib.dptr(d)))
-        return {"positions": pos, "types": ty, "n_atoms": na, "cell": cl,
-                "energy": e, "direction": d}
+        out = {"positions": pos, "types": ty, "n_atoms": na, "cell": cl,
+               "energy": e, "direction": d}
+        if self.mwg.n_dir_slots > 1:
+            out["direction_slots"] = self.direction_slots()
+        return out
+
+    def direction_slots(self) -> np.ndarray:
+        """Persistent directions of the species-scoped GMC moves,
+        (n_slots - 1, R, K, Nmax, 3); slot s + 1 belongs to the moves with
+        ``mwg.dir_slots == s + 1``."""
+        R, K, Nm = self.n_replicas, self.n_walkers, self.Nmax
+        n = self.mwg.n_dir_slots - 1
+        out = np.empty((n, R, K, Nm, 3))
+        for s in range(n):
+            self._check(self.lib.jr_get_direction_slot(self.h, s + 1, _lib.dptr(out[s])))
+        return out
+
+    def set_direction_slots(self, dirs):
+        dirs = _lib.f64(dirs)
+        for s in range(self.mwg.n_dir_slots - 1):
+            self._check(self.lib.jr_set_direction_slot(self.h, s + 1,
+                                                       _lib.dptr(_lib.f64(dirs[s]))))
 
     def init_walkers(self, rng: np.random.Generator, v_lo_per_atom: float,
                      v_hi_per_atom: float, flat_V_prior: bool = True,

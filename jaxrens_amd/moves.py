@@ -16,7 +16,7 @@ from __future__ import annotations
 from typing import Annotated, Literal, Union
 
 import numpy as np
-from pydantic import BaseModel, ConfigDict, Field, TypeAdapter
+from pydantic import BaseModel, ConfigDict, Field, TypeAdapter, field_validator
 
 from . import _lib
 
@@ -49,22 +49,51 @@ class BaseMoveSpec(BaseModel):
 
 
 class GMCMoveSpec(BaseMoveSpec):
-    """Galilean MC with force reflection (``sampling/moves/galilean.py``)."""
+    """Galilean MC with force reflection (``sampling/moves/galilean.py``).
+
+    ``species`` scopes the move to the atoms of the listed type codes (the
+    reference names elements, ``cli/schema/moves.py`` GMCMoveSpec; here the
+    integer codes of the pair table); every other atom stays fixed, the
+    direction and the reflection normal are projected onto the moving
+    atoms, and the move keeps a persistent direction of its own (auto-named
+    ``gmc_<codes>``).  ``refresh_direction`` draws a fresh direction on
+    every call instead of perturbing the stored one."""
 
     type: Literal["gmc"] = "gmc"
     n_reflect: int = 5
-    species: tuple[str, ...] | None = None
+    species: tuple[int, ...] | None = None
     refresh_direction: bool = False
 
+    @field_validator("species", mode="before")
+    @classmethod
+    def _wrap_bare_code(cls, v):
+        return (v,) if isinstance(v, (int, str)) else v
+
+    def effective_name(self) -> str:
+        if self.name is not None:
+            return self.name
+        if self.species:
+            return "gmc_" + "_".join(str(c) for c in self.species)
+        return self.type
+
+    @property
+    def scoped(self) -> bool:
+        return bool(self.species)
+
     def device_move(self, cell: CellSpec) -> _lib.JrMove:
-        if self.species is not None:
-            raise NotImplementedError("species-scoped gmc is not built")
-        if self.refresh_direction:
-            raise NotImplementedError("refresh_direction is not built")
         if self.n_reflect < 1:
             raise ValueError("n_reflect must be >= 1")
+        mask = 0
+        if self.species is not None:
+            codes = set(int(c) for c in self.species)
+            if not codes:
+                raise ValueError("gmc species must name at least one type code")
+            if min(codes) < 0 or max(codes) >= _lib.JR_MAX_SPECIES:
+                raise ValueError(f"species codes must be in [0, {_lib.JR_MAX_SPECIES})")
+            mask = sum(1 << c for c in codes)
         return _lib.JrMove(type=_lib.JR_MOVE_GMC, n_reflect=self.n_reflect,
-                           perturb_angle=0.1)
+                           perturb_angle=0.1, species_mask=mask,
+                           refresh_direction=int(self.refresh_direction))
 
 
 class HMCMoveSpec(BaseMoveSpec):
@@ -201,16 +230,32 @@ class MWG:
         self.default_step_sizes = np.array([m.step_size for m in self.moves])
         self.names = [m.effective_name() for m in self.moves]
         self.n_moves = len(self.moves)
+        # persistent-direction slots: unscoped GMC moves share the population's
+        # direction (slot 0), each species-scoped one keeps its own
+        # (cli/schema/moves.py GMCMoveSpec._direction_field)
+        self.dir_slots, n = [], 0
+        for m in self.moves:
+            if getattr(m, "scoped", False):
+                n += 1
+                self.dir_slots.append(n)
+            else:
+                self.dir_slots.append(0)
+        self.n_dir_slots = n + 1
+
+    def _device_move(self, i):
+        mv = self.moves[i].device_move(self.cell)
+        mv.dir_slot = self.dir_slots[i]
+        return mv
 
     def device_table(self):
         arr = (_lib.JrMove * self.n_moves)()
-        for i, m in enumerate(self.moves):
-            arr[i] = m.device_move(self.cell)
+        for i in range(self.n_moves):
+            arr[i] = self._device_move(i)
         return arr
 
     def as_oracle_moves(self):
         """Plain tuples for the CPU oracle (same fields as the device table)."""
-        return [m.device_move(self.cell) for m in self.moves]
+        return [self._device_move(i) for i in range(self.n_moves)]
 
 
 def build_mwg(backend, move_descriptors, cell=None) -> MWG:

@@ -455,12 +455,17 @@ static double wave_sum_dot(const double* a, const double* b, int N,
   return wave_sum_atoms(scratch, N);
 }
 
-/* sampling/moves/galilean.py:142-300 (use_forces=True, no species mask).
- * Returns accepted. */
+/* sampling/moves/galilean.py:142-300 (use_forces=True).  A species scope
+ * (mv->species_mask != 0) multiplies the direction, before each
+ * normalisation, and the reflection force by the moving-atom mask m_a = 1
+ * for types in the scope, else 0 (galilean.py:58-72,171-180,233-240);
+ * refresh_direction draws a fresh direction on every call.  Returns
+ * accepted. */
 static int gmc_move(const chain_ctx* c, double* x, const int32_t* types,
                     const double* cell, double* E, double* d, double s,
                     const or_move* mv, const double* xi) {
   int N = c->N, Nmax = c->Nmax;
+  const int scoped = mv->species_mask != 0;
   double* scr = (double*)malloc(sizeof(double) * (size_t)(N + 1));
   double* v = (double*)malloc(sizeof(double) * 3 * (size_t)(N + 1));
   double* x0 = (double*)malloc(sizeof(double) * 3 * (size_t)(N + 1));
@@ -470,12 +475,15 @@ static int gmc_move(const chain_ctx* c, double* x, const int32_t* types,
 
   /* direction: fresh if |d_prev| < 1e-8, else perturbed (galilean.py:160-168) */
   double dn = sqrt(wave_sum_dot(d, d, N, scr));
-  int fresh = dn < 1e-8;
-  for (int a = 0; a < N; ++a)
+  int fresh = dn < 1e-8 || mv->refresh_direction;
+  for (int a = 0; a < N; ++a) {
+    const double m = ((mv->species_mask >> types[a]) & 1) ? 1.0 : 0.0;
     for (int q = 0; q < 3; ++q) {
       double z = xi[q * Nmax + a];
       v[3 * a + q] = fresh ? z : d[3 * a + q] + mv->perturb_angle * z;
+      if (scoped) v[3 * a + q] = v[3 * a + q] * m;
     }
+  }
   double vn = sqrt(wave_sum_dot(v, v, N, scr));
   double den = fmax(vn, 1e-10);
   for (int i = 0; i < 3 * N; ++i) d[i] = v[i] / den;
@@ -489,6 +497,11 @@ static int gmc_move(const chain_ctx* c, double* x, const int32_t* types,
     or_energy_forces(N, x, types, cell, c->pot, c->pressure, c->mu, &U, &Hn,
                      F);
     if (isnan(Hn)) Hn = INFINITY;
+    if (scoped)
+      for (int a = 0; a < N; ++a) {
+        const double m = ((mv->species_mask >> types[a]) & 1) ? 1.0 : 0.0;
+        for (int q = 0; q < 3; ++q) F[3 * a + q] = F[3 * a + q] * m;
+      }
     for (int a = 0; a < N; ++a) {
       fq[a] = dot3(F[3 * a], F[3 * a + 1], F[3 * a + 2], F[3 * a],
                    F[3 * a + 1], F[3 * a + 2]);
@@ -803,7 +816,8 @@ int or_min_distance_ok(int N, const double* pos, const int32_t* types,
 
 /* sampling/mwg.py:108-150,227-257 + nested_sampling.py:533-609 (one chain). */
 void or_walk_chain(int N, int Nmax, double* pos, int32_t* types, double* cell,
-                   double* energy, double* dir, const or_pot* pot,
+                   double* energy, double* dir, double* dirx, size_t xstride,
+                   const or_pot* pot,
                    double pressure, const double* mu, double emax,
                    const double* step_sizes, const or_move* moves,
                    int n_moves, const int32_t* schedule, int n_steps,
@@ -831,10 +845,12 @@ void or_walk_chain(int N, int Nmax, double* pos, int32_t* types, double* cell,
     }
     int reason = 0, ne = 1, ng = 0;
     switch (mv->type) {
-      case OR_MOVE_GMC:
-        reason = gmc_move(&c, pos, types, cell, energy, dir, s, mv, xi) ? 0 : 1;
+      case OR_MOVE_GMC: {
+        double* dd = (mv->dir_slot > 0 && dirx) ? dirx + (size_t)(mv->dir_slot - 1) * xstride : dir;
+        reason = gmc_move(&c, pos, types, cell, energy, dd, s, mv, xi) ? 0 : 1;
         ne = ng = mv->n_reflect;
         break;
+      }
       case OR_MOVE_VOLUME:
         reason = volume_move(&c, pos, types, cell, energy, s, mv, xi[0], u[0]);
         break;
@@ -884,6 +900,10 @@ static void copy_walker(or_pop* pop, int dst, int src) {
          sizeof(double) * 3 * (size_t)Nm);
   memcpy(pop->dir + (size_t)dst * Nm * 3, pop->dir + (size_t)src * Nm * 3,
          sizeof(double) * 3 * (size_t)Nm);
+  const size_t xs = (size_t)pop->R * pop->K * Nm * 3;
+  for (int e = 0; e < pop->n_xslots; ++e)
+    memcpy(pop->dirx + e * xs + (size_t)dst * Nm * 3, pop->dirx + e * xs + (size_t)src * Nm * 3,
+           sizeof(double) * 3 * (size_t)Nm);
   memcpy(pop->types + (size_t)dst * Nm, pop->types + (size_t)src * Nm,
          sizeof(int32_t) * (size_t)Nm);
   memcpy(pop->cell + (size_t)dst * 9, pop->cell + (size_t)src * 9,
@@ -1058,6 +1078,13 @@ void or_replica_exchange(or_pop* pop, const or_run* run, const double* emax,
                  sizeof(double) * 3 * (size_t)Nm);
           memcpy(pop->dir + (size_t)sj * Nm * 3, tmp_dir,
                  sizeof(double) * 3 * (size_t)Nm);
+          for (int e = 0; e < pop->n_xslots; ++e) {
+            double* X = pop->dirx + (size_t)e * pop->R * pop->K * Nm * 3;
+            memcpy(tmp_dir, X + (size_t)si * Nm * 3, sizeof(double) * 3 * (size_t)Nm);
+            memcpy(X + (size_t)si * Nm * 3, X + (size_t)sj * Nm * 3,
+                   sizeof(double) * 3 * (size_t)Nm);
+            memcpy(X + (size_t)sj * Nm * 3, tmp_dir, sizeof(double) * 3 * (size_t)Nm);
+          }
           memcpy(tmp_t, pop->types + (size_t)si * Nm,
                  sizeof(int32_t) * (size_t)Nm);
           memcpy(pop->types + (size_t)si * Nm, pop->types + (size_t)sj * Nm,
@@ -1139,7 +1166,9 @@ int or_ns_iteration(or_pop* pop, const or_pot* pot, const or_run* run,
       const double* u = d->uniforms + ((size_t)q * n_steps) * 4;
       or_walk_chain(pop->n_atoms[slot], Nm, pop->pos + (size_t)slot * Nm * 3,
                     pop->types + (size_t)slot * Nm, pop->cell + 9 * (size_t)slot,
-                    pop->energy + slot, pop->dir + (size_t)slot * Nm * 3, pot,
+                    pop->energy + slot, pop->dir + (size_t)slot * Nm * 3,
+                    pop->n_xslots ? pop->dirx + (size_t)slot * Nm * 3 : NULL,
+                    (size_t)R * K * Nm * 3, pot,
                     run->pressure[r], run->mu + (size_t)r * S, out->emax[r],
                     run->step_sizes + (size_t)r * n_moves, moves, n_moves,
                     d->schedule, n_steps, xi, u,
@@ -1190,7 +1219,9 @@ void or_initial_walk(or_pop* pop, const or_pot* pot, const or_run* run,
       const double* u = uniforms + (size_t)q * walklength * 4;
       or_walk_chain(pop->n_atoms[q], Nm, pop->pos + (size_t)q * Nm * 3,
                     pop->types + (size_t)q * Nm, pop->cell + 9 * (size_t)q,
-                    pop->energy + q, pop->dir + (size_t)q * Nm * 3, pot,
+                    pop->energy + q, pop->dir + (size_t)q * Nm * 3,
+                    pop->n_xslots ? pop->dirx + (size_t)q * Nm * 3 : NULL,
+                    (size_t)R * K * Nm * 3, pot,
                     run->pressure[r], run->mu + (size_t)r * S, emax[r],
                     run->step_sizes + (size_t)r * n_moves, moves, n_moves,
                     schedule, walklength, xi, u,
@@ -1221,6 +1252,7 @@ void or_trial_moves(const or_pop* pop, const or_pot* pot, const or_run* run,
   {
     double* pos = (double*)malloc(sizeof(double) * 3 * (size_t)Nm);
     double* dir = (double*)malloc(sizeof(double) * 3 * (size_t)Nm);
+    double* dx = (double*)malloc(sizeof(double) * 3 * (size_t)Nm * (size_t)(pop->n_xslots + 1));
     int32_t* ty = (int32_t*)malloc(sizeof(int32_t) * (size_t)Nm);
     double* ss = (double*)malloc(sizeof(double) * (size_t)n_moves);
 #pragma omp for schedule(dynamic, 1)
@@ -1230,6 +1262,9 @@ void or_trial_moves(const or_pop* pop, const or_pot* pot, const or_run* run,
       double cell[9];
       memcpy(pos, pop->pos + (size_t)slot * Nm * 3, sizeof(double) * 3 * (size_t)Nm);
       memcpy(dir, pop->dir + (size_t)slot * Nm * 3, sizeof(double) * 3 * (size_t)Nm);
+      for (int e = 0; e < pop->n_xslots; ++e)
+        memcpy(dx + (size_t)e * Nm * 3,
+               pop->dirx + ((size_t)e * R * K + slot) * Nm * 3, sizeof(double) * 3 * (size_t)Nm);
       memcpy(ty, pop->types + (size_t)slot * Nm, sizeof(int32_t) * (size_t)Nm);
       memcpy(cell, pop->cell + 9 * (size_t)slot, sizeof(cell));
       double E = pop->energy[slot];
@@ -1237,7 +1272,7 @@ void or_trial_moves(const or_pop* pop, const or_pot* pot, const or_run* run,
       int32_t ctr[OR_NCOUNT * 16];
       memset(ctr, 0, sizeof(ctr));
       int32_t sched = move_idx;
-      or_walk_chain(pop->n_atoms[slot], Nm, pos, ty, cell, &E, dir, pot,
+      or_walk_chain(pop->n_atoms[slot], Nm, pos, ty, cell, &E, dir, dx, (size_t)Nm * 3, pot,
                     run->pressure[r], run->mu + (size_t)r * S, emax[r], ss,
                     moves, n_moves, &sched, 1,
                     normals + (size_t)q * 3 * Nm, uniforms + (size_t)q * 4,
@@ -1250,6 +1285,7 @@ void or_trial_moves(const or_pop* pop, const or_pot* pot, const or_run* run,
     }
     free(pos);
     free(dir);
+    free(dx);
     free(ty);
     free(ss);
   }

@@ -13,6 +13,7 @@
 #ifndef JR_ORACLE_H
 #define JR_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -45,9 +46,11 @@ typedef struct {
   int n_leapfrog;
   int flat_v_prior;
   int n_species;
-  int species_mask; /* species_swap scope, 0 = all */
+  int species_mask; /* species_swap / gmc scope, 0 = all */
   double perturb_angle;
   double max_vpa, min_vpa, min_aspect;
+  int refresh_direction; /* gmc: fresh direction every call */
+  int dir_slot;          /* gmc: 0 = dir, s > 0 = extra slot s */
 } or_move;
 
 typedef struct {
@@ -58,6 +61,9 @@ typedef struct {
   double* cell;    /* [R*K][9] rows = lattice vectors */
   double* energy;  /* [R*K] full Hamiltonian U + P V - mu.N */
   double* dir;     /* [R*K][Nmax][3] GMC persistent direction */
+  double* dirx;    /* [n_xslots][R*K][Nmax][3] directions of the
+                      species-scoped GMC moves (slots 1..n_xslots) */
+  int n_xslots;
 } or_pop;
 
 typedef struct {
@@ -115,8 +121,11 @@ void or_initial_walk(or_pop* pop, const or_pot* pot, const or_run* run,
                      int32_t* counters, int n_threads);
 
 /* Walk one chain for n_steps MWG sub-steps (state updated in place). */
+/* dirx: the chain's slot-1 direction (NULL without scoped moves), slot s at
+ * dirx + (s - 1) * xstride. */
 void or_walk_chain(int N, int Nmax, double* pos, int32_t* types, double* cell,
-                   double* energy, double* dir, const or_pot* pot,
+                   double* energy, double* dir, double* dirx, size_t xstride,
+                   const or_pot* pot,
                    double pressure, const double* mu, double emax,
                    const double* step_sizes, const or_move* moves,
                    int n_moves, const int32_t* schedule, int n_steps,

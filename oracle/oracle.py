@@ -29,13 +29,14 @@ class OrMove(C.Structure):
     _fields_ = [("type", C.c_int), ("n_reflect", C.c_int), ("n_leapfrog", C.c_int),
                 ("flat_v_prior", C.c_int), ("n_species", C.c_int), ("species_mask", C.c_int),
                 ("perturb_angle", C.c_double), ("max_vpa", C.c_double),
-                ("min_vpa", C.c_double), ("min_aspect", C.c_double)]
+                ("min_vpa", C.c_double), ("min_aspect", C.c_double),
+                ("refresh_direction", C.c_int), ("dir_slot", C.c_int)]
 
 
 class OrPop(C.Structure):
     _fields_ = [("R", C.c_int), ("K", C.c_int), ("Nmax", C.c_int), ("S", C.c_int),
                 ("pos", _dp), ("types", _ip), ("n_atoms", _ip), ("cell", _dp),
-                ("energy", _dp), ("dir", _dp)]
+                ("energy", _dp), ("dir", _dp), ("dirx", _dp), ("n_xslots", C.c_int)]
 
 
 class OrRun(C.Structure):
@@ -82,8 +83,8 @@ def lib() -> C.CDLL:
         L.or_energy_forces.argtypes = [C.c_int, _dp, _ip, _dp, C.POINTER(OrPot),
                                        C.c_double, _dp, _dp, _dp, _dp]
         L.or_walk_chain.restype = None
-        L.or_walk_chain.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, _dp, _dp,
-                                    C.POINTER(OrPot), C.c_double, _dp, C.c_double,
+        L.or_walk_chain.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, _dp, _dp, _dp,
+                                    C.c_size_t, C.POINTER(OrPot), C.c_double, _dp, C.c_double,
                                     _dp, C.POINTER(OrMove), C.c_int, _ip, C.c_int,
                                     _dp, _dp, _ip]
         L.or_ns_iteration.restype = C.c_int
@@ -174,34 +175,40 @@ def moves_array(jr_moves):
     for i, m in enumerate(jr_moves):
         arr[i] = OrMove(m.type, m.n_reflect, m.n_leapfrog, m.flat_v_prior,
                         m.n_species, m.species_mask, m.perturb_angle, m.max_vpa,
-                        m.min_vpa, m.min_aspect)
+                        m.min_vpa, m.min_aspect, getattr(m, "refresh_direction", 0),
+                        getattr(m, "dir_slot", 0))
     return arr
 
 
 class Population:
     """Host copy of the population in the device layout."""
 
-    def __init__(self, pos, types, n_atoms, cell, energy, direction):
+    def __init__(self, pos, types, n_atoms, cell, energy, direction, dirx=None):
         self.pos = f64(pos)
         self.types = i32(types)
         self.n_atoms = i32(n_atoms)
         self.cell = f64(np.asarray(cell).reshape(self.pos.shape[0], self.pos.shape[1], 9))
         self.energy = f64(energy)
         self.dir = f64(direction)
+        # directions of the species-scoped GMC moves, (n_xslots, R, K, Nmax, 3)
+        self.dirx = f64(np.zeros((0,) + self.dir.shape) if dirx is None else dirx)
 
     @classmethod
     def from_engine_dict(cls, d):
         return cls(d["positions"], d["types"], d["n_atoms"], d["cell"], d["energy"],
-                   d["direction"])
+                   d["direction"], d.get("direction_slots"))
 
     def copy(self):
         return Population(self.pos.copy(), self.types.copy(), self.n_atoms.copy(),
-                          self.cell.copy(), self.energy.copy(), self.dir.copy())
+                          self.cell.copy(), self.energy.copy(), self.dir.copy(),
+                          self.dirx.copy())
 
     def struct(self, S):
         R, K, Nm = self.pos.shape[:3]
+        nx = self.dirx.shape[0]
         return OrPop(R, K, Nm, S, _d(self.pos), _i(self.types), _i(self.n_atoms),
-                     _d(self.cell), _d(self.energy), _d(self.dir))
+                     _d(self.cell), _d(self.energy), _d(self.dir),
+                     _d(self.dirx) if nx else None, nx)
 
 
 def ns_iteration(pop: Population, pot: Pot, pressure, mu, step_sizes, moves, n_steps,

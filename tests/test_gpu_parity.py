@@ -101,6 +101,8 @@ def _compare_population(eng, opop):
     assert np.array_equal(d["types"], opop.types)
     assert np.array_equal(d["cell"].reshape(opop.cell.shape), opop.cell)
     assert np.array_equal(d["n_atoms"], opop.n_atoms)
+    if "direction_slots" in d:
+        assert np.array_equal(d["direction_slots"], opop.dirx)
 
 
 def _run_parity(oracle, eng, n_iter, tape, pair_valid=None):
@@ -216,6 +218,77 @@ def test_device_draws_replay(oracle, cap):
     assert 0.0 <= tape.uniforms.min() and tape.uniforms.max() < 1.0
     z = tape.normals.ravel()
     assert abs(z.mean()) < 0.05 and abs(z.std() - 1.0) < 0.05
+
+
+# species-scoped GMC (each with its own persistent direction), a refreshed
+# scoped one, a refreshed unscoped one, and a volume move
+SCOPED_MOVES = [
+    {"type": "gmc", "species": 0, "n_reflect": 4, "step_size": 0.1, "weight": 2},
+    {"type": "gmc", "species": [1], "n_reflect": 4, "step_size": 0.1, "weight": 2,
+     "refresh_direction": True},
+    {"type": "gmc", "n_reflect": 3, "step_size": 0.08, "weight": 1, "refresh_direction": True},
+    {"type": "volume", "step_size": 0.3, "weight": 1}]
+BINARY = dict(pair_epsilon=np.array([[1.0, 1.5], [1.5, 0.5]]),
+              pair_sigma=np.array([[1.0, 0.8], [0.8, 0.88]]), cutoff=2.5)
+
+
+def _scoped_engine(cap, seed=21):
+    lj = LJBackend(**BINARY)
+    eng = _engine(lj, SCOPED_MOVES, 4, 8, 32, 4, 8, seed=seed, cell_spec=CELL,
+                  pressures=[0.1, 0.3, 1.0, 3.0], species_p=[0.5, 0.5], v_range=(0.5, 8.0))
+    eng.set_walk_block_cap(cap)
+    return eng, lj
+
+
+@pytest.mark.parametrize("cap", [0, 1])
+def test_ns_iterations_species_scoped_gmc(oracle, cap):
+    """Tape-driven iterations with species-scoped GMC moves, bit for bit
+    against the oracle, including each scoped move's persistent direction
+    through cloning and replica exchange."""
+    print(f"ns species-scoped gmc cap={cap}", flush=True)
+    eng, _ = _scoped_engine(cap)
+    assert eng.mwg.n_dir_slots == 3
+    tape = make_tape(np.random.default_rng(13), 6, 4, 8, 32, 8, 4, eng.mwg.move_probs)
+    _run_parity(oracle, eng, 6, tape)
+    slots = eng.direction_slots()
+    assert slots[0].any() and slots[1].any()
+    pop = eng.get_population()
+    for k, code in ((0, 0), (1, 1)):  # a scoped direction is zero off its species
+        assert not slots[k][pop["types"] != code].any()
+
+
+def test_species_scoped_gmc_device_draws_and_trials(oracle):
+    """The in-place device draws replayed on the oracle, then trial moves of
+    a scoped move (step-size adaptation) leave the population and the
+    scoped directions untouched."""
+    print("species-scoped gmc: device draws and trial moves", flush=True)
+    from oracle.oracle import Population
+    eng, lj = _scoped_engine(0, seed=22)
+    opop = Population.from_engine_dict(eng.get_population())
+    pot = oracle.Pot.from_table(lj.table)
+    for it in range(3):
+        out = eng.ns_iterations(1)
+        tape = eng.last_tape()
+        fires = (eng.iteration - 1) > 0
+        o = oracle.ns_iteration(opop, pot, eng.pressure, eng.mu, eng.step_sizes,
+                                eng.mwg.as_oracle_moves(), 8, 4, tape, it=0, do_re=fires)
+        assert np.array_equal(out["counters"][0], o["counters"]), it
+        _compare_population(eng, opop)
+    rng = np.random.default_rng(23)
+    idx = rng.integers(0, 8, size=(4, 20)).astype(np.int32)
+    nrm = rng.standard_normal((4, 20, 3, 32))
+    uni = rng.random((4, 20, 4))
+    before = eng.get_population()
+    pop = Population.from_engine_dict(before)
+    for m in (0, 1):
+        acc, rs = eng.trial_moves(m, [0.1] * 4, idx, normals=nrm, uniforms=uni)
+        oacc, ors = oracle.trial_moves(pop, pot, eng.pressure, eng.mu, eng.step_sizes,
+                                       eng.mwg.as_oracle_moves(), m, [0.1] * 4, eng.emax(),
+                                       idx, nrm, uni)
+        assert np.array_equal(acc, oacc) and np.array_equal(rs, ors), m
+    after = eng.get_population()
+    for k in ("positions", "direction", "direction_slots"):
+        assert np.array_equal(after[k], before[k]), k
 
 
 def test_graph_replay_equals_stream_path():

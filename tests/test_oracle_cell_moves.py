@@ -206,7 +206,7 @@ class CellChain:
         xi[0, 0], xi[1, 0] = xi0, xi1
         ctr = np.zeros((1, 7), np.int32)
         O.lib().or_walk_chain(self.N, self.N, O._d(self.pos), O._i(self.types),
-                              O._d(self.cell), O._d(self.E), O._d(self.dir),
+                              O._d(self.cell), O._d(self.E), O._d(self.dir), None, 0,
                               C.byref(self.pot.s), self.pressure, O._d(self.mu), emax,
                               O._d(O.f64([step])), O.moves_array(self.mwg.as_oracle_moves()),
                               1, O._i(np.zeros(1, np.int32)), 1, O._d(O.f64(xi)),

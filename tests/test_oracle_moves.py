@@ -45,6 +45,8 @@ class Chain:
                                  {"type": "volume", "step_size": 0.3},
                                  {"type": "hmc", "n_leapfrog": 5, "step_size": 0.002},
                                  {"type": "alchemical_morph", "n_species": 2}], CELL)
+        nx = self.mwg.n_dir_slots - 1
+        self.dirx = np.zeros((nx, self.N, 3)) if nx else None
 
     def step(self, m, emax, xi, u=(0.5, 0.5, 0.5, 0.5), step=None):
         O = self.O
@@ -56,7 +58,8 @@ class Chain:
         sched = np.array([m], np.int32)
         O.lib().or_walk_chain(self.N, self.N, O._d(self.pos), O._i(self.types),
                               O._d(self.cell), O._d(self.E), O._d(self.dir),
-                              C.byref(self.pot.s), self.pressure, O._d(self.mu), emax,
+                              O._d(self.dirx) if self.dirx is not None else None,
+                              self.N * 3, C.byref(self.pot.s), self.pressure, O._d(self.mu), emax,
                               O._d(ss), mv, self.mwg.n_moves, O._i(sched), 1,
                               O._d(xi), O._d(O.f64(u)), O._i(ctr))
         return ctr[m]
@@ -324,3 +327,83 @@ def test_replica_exchange_rebase_and_schedule(oracle):
     acc, att = oracle.replica_exchange(pop2, P, mu, np.full(R, -1e9), re_u, 1, S=S)
     assert acc.tolist() == [0, 0] and att.tolist() == [1, 1]
     assert np.array_equal(pop2.pos, before.pos)
+
+
+BINARY = dict(pair_epsilon=[[1.0, 1.5], [1.5, 0.5]], pair_sigma=[[1.0, 0.8], [0.8, 0.88]],
+              cutoff=2.5)
+SCOPED = [{"type": "gmc", "species": 0, "n_reflect": 3, "step_size": 0.05},
+          {"type": "gmc", "species": [1], "n_reflect": 3, "step_size": 0.05,
+           "refresh_direction": True},
+          {"type": "gmc", "n_reflect": 3, "step_size": 0.05}]
+
+
+def _binary_chain(oracle):
+    pos = lattice(2, 1.3) + 2.0
+    types = np.array([0, 1, 0, 1, 1, 0, 0, 1])
+    return Chain(oracle, pos, np.eye(3) * 6.0, types=types, lj=LJBackend(**BINARY),
+                 moves=SCOPED)
+
+
+def test_scoped_gmc_moves_only_its_species(oracle):
+    """galilean.py:58-72,171-180: the direction is the normalised masked
+    normal draw; atoms outside the scope never move, and the move keeps its
+    own persistent direction (slot 1), leaving the shared one untouched."""
+    ch = _binary_chain(oracle)
+    assert ch.mwg.dir_slots == [1, 2, 0] and ch.mwg.names[:2] == ["gmc_0", "gmc_1"]
+    xi = np.random.default_rng(3).standard_normal((3, 8))
+    x0 = ch.pos.copy()
+    c = ch.step(0, np.inf, xi)
+    assert c[1] == 1
+    m = (ch.types == 0)[:, None].astype(float)
+    d = xi.T * m / np.linalg.norm(xi.T * m)
+    np.testing.assert_allclose(ch.dirx[0], d, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(ch.pos, x0 + 3 * 0.05 * d, rtol=0, atol=1e-14)
+    assert np.array_equal(ch.pos[ch.types == 1], x0[ch.types == 1])  # bit for bit
+    assert not ch.dir.any() and not ch.dirx[1].any()
+    # the second call perturbs the stored slot-1 direction (masked again)
+    xi2 = np.random.default_rng(4).standard_normal((3, 8))
+    ch.step(0, np.inf, xi2)
+    v = (d + 0.1 * xi2.T) * m
+    np.testing.assert_allclose(ch.dirx[0], v / np.linalg.norm(v), rtol=0, atol=1e-15)
+
+
+def test_refresh_direction_ignores_the_stored_direction(oracle):
+    """refresh_direction draws a fresh direction on every call
+    (galilean.py:160-168), so two calls with the same normals give the same
+    direction whatever the first call left behind."""
+    ch = _binary_chain(oracle)
+    xi = np.random.default_rng(5).standard_normal((3, 8))
+    ch.step(1, np.inf, xi)
+    m = (ch.types == 1)[:, None].astype(float)
+    d = xi.T * m / np.linalg.norm(xi.T * m)
+    np.testing.assert_allclose(ch.dirx[1], d, rtol=0, atol=1e-15)
+    ch.dirx[1] = np.random.default_rng(6).standard_normal((8, 3))  # any stored value
+    ch.step(1, np.inf, xi)
+    np.testing.assert_allclose(ch.dirx[1], d, rtol=0, atol=1e-15)
+
+
+def test_scoped_reflection_uses_the_projected_force(oracle):
+    """On a violated step the direction is mirrored off the force projected
+    onto the moving atoms; frozen rows of the direction stay exactly zero."""
+    ch = _binary_chain(oracle)
+    ch.mwg = MWG([{"type": "gmc", "species": 0, "n_reflect": 2, "step_size": 1e-4}], CELL)
+    ch.dirx = np.zeros((1, 8, 3))
+    xi = np.random.default_rng(7).standard_normal((3, 8))
+    m = (ch.types == 0)[:, None].astype(float)
+    d = xi.T * m / np.linalg.norm(xi.T * m)
+    x1 = ch.pos + 1e-4 * d
+    _, H1, F1 = oracle.energy_forces(ch.pot, x1, ch.types, ch.cell)
+    emax = H1 - 1e-12  # the first step violates
+    ch.step(0, emax, xi)
+    fm = F1 * m
+    f = fm / np.linalg.norm(fm)
+    d_ref = d - 2 * np.sum(f * d) * f
+    x2 = x1 + 1e-4 * d_ref
+    _, H2, _ = oracle.energy_forces(ch.pot, x2, ch.types, ch.cell)
+    if H2 < emax:
+        np.testing.assert_allclose(ch.pos, x2, atol=1e-13)
+        np.testing.assert_allclose(ch.dirx[0], d_ref, atol=1e-12)
+    else:
+        np.testing.assert_allclose(ch.dirx[0], -d, atol=1e-14)
+    assert not ch.dirx[0][ch.types == 1].any()
+    assert np.array_equal(ch.pos[ch.types == 1], (lattice(2, 1.3) + 2.0)[ch.types == 1])

@@ -25,7 +25,7 @@ def swap_once(oracle, lj, pos, types, cell, u, emax=np.inf, species=None):
     E = np.array([O.energy_forces(pot, p, t, c, 0.0, mu)[1]])
     ctr = np.zeros((1, 7), np.int32)
     O.lib().or_walk_chain(N, N, O._d(p), O._i(t), O._d(c), O._d(E), O._d(np.zeros((N, 3))),
-                          C.byref(pot.s), 0.0, O._d(mu), emax, O._d(O.f64([0.1])),
+                          None, 0, C.byref(pot.s), 0.0, O._d(mu), emax, O._d(O.f64([0.1])),
                           O.moves_array(mw.as_oracle_moves()), 1, O._i(np.zeros(1, np.int32)),
                           1, O._d(np.zeros((3, N))), O._d(O.f64(list(u) + [0.5])), O._i(ctr))
     return p, t, E[0], ctr[0], pot
