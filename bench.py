@@ -382,12 +382,15 @@ def lj128_hmc_report(R=8, K=2048, N=128, T=20, n_timed=1) -> dict:
     P = list(np.geomspace(0.05, 5.0, R))
     moves = [{"type": "hmc", "n_leapfrog": 10, "step_size": 0.005, "weight": 1},
              {"type": "galilean", "n_reflect": 10, "step_size": 0.1, "weight": 1}]
-    pos, ty, cell = sur_population(R, K, N, seed=9)
+    # 128 of the 6^3 simple-cubic sites at rho = 0.45 (spacing 1.10 sigma, near
+    # the LJ minimum): the contours start inside the range the surrogate
+    # trains on (U/N <= 0), so the auto walk can screen from its first records
+    pos, ty, cell = sur_population(R, K, N, seed=9, rho=0.45)
     ty[:] = 0
     ws_it = R * (K - 1) * T
     out = {"workload": f"LJ-{N} (eps = sigma = 1, rc 2.5, FP64 judge), P = {R} pressures "
                        f"0.05-5, K = {K}, {T} sub-steps hmc(10 x 0.005):galilean(10) 1:1, "
-                       "n_extra = K-2, initial jittered lattice at density 0.9",
+                       "n_extra = K-2, initial jittered lattice at density 0.45 (site spacing 1.10)",
            "baseline": "pair term 4 (1/r)^12 below 2.5 (the r^-12 part of LJ)"}
 
     def run(mode, max_warm, n):
@@ -415,6 +418,7 @@ def lj128_hmc_report(R=8, K=2048, N=128, T=20, n_timed=1) -> dict:
             warm += 1
             st = eng.learned_stats()
             print(f"[bench] lj128 warm-up {warm}: held-out rmse {st['heldout_rmse']:.4f}, "
+                  f"contour max {np.max(eng.emax()):.1f}, "
                   f"screened iterations {st['iterations_screened']}", file=sys.stderr,
                   flush=True)
             if st["iterations_screened"] > 0:
@@ -439,7 +443,7 @@ def lj128_hmc_report(R=8, K=2048, N=128, T=20, n_timed=1) -> dict:
                       "surrogate_evals": st["surrogate_evals"]})
         return r
 
-    out["screened"] = run("auto", 8, n_timed)
+    out["screened"] = run("auto", 20, n_timed)
     out["plain"] = run("plain", out["screened"]["warm_up_iterations"], n_timed)
     # log Z: the screened HMC is the exact constrained-prior move, checked
     # against the plain random-walk moves at LJ-8 (at LJ-128 from a lattice
