@@ -65,10 +65,10 @@ def copy_calibration(n_bytes: int = 1 << 30) -> dict:
 
 
 def build_engine(n_extra: int, R=16, K=1024, N=64, n_steps=20, seed=42,
-                 precision="fp64"):
+                 precision="fp64", eval_chunks=1):
     from jaxrens_amd.backends import LJBackend
     from jaxrens_amd.engine import NSEngine, RunConfig
-    lj = LJBackend(1.0, 1.0, 2.5, precision=precision)
+    lj = LJBackend(1.0, 1.0, 2.5, precision=precision, eval_chunks=eval_chunks)
     moves = [{"type": "galilean", "n_reflect": 10, "step_size": 0.1, "weight": 4},
              {"type": "volume", "step_size": 0.3, "weight": 1}]
     cell = {"min_volume_per_atom": 0.125, "max_volume_per_atom": 64.0,

@@ -75,7 +75,14 @@ typedef struct {
   const double* eps24; /* [S*S] 24*eps_ab */
   const double* sig2;  /* [S*S] sigma_ab^2 */
   int32_t images[3];   /* image half-ranges per lattice vector */
-  int32_t pad_;
+  int32_t eval_chunks; /* 0 or 1: each evaluation sums every atom's pair
+                          terms in one running sum over the ring steps;
+                          C = 2, 4 or 8: the steps are split into C
+                          consecutive chunks summed on their own and added
+                          in chunk order (a fixed evaluation order; FP64
+                          without images).  With C > 1 a team of C
+                          wavefronts walks each chain, one chunk each, for
+                          walks with few chains per iteration (n_extra = 0). */
 } jr_pot;
 
 typedef struct {

@@ -42,7 +42,7 @@ class JrPot(C.Structure):
         ("eps24", _dp),
         ("sig2", _dp),
         ("images", C.c_int32 * 3),
-        ("pad_", C.c_int32),
+        ("eval_chunks", C.c_int32),
     ]
 
 

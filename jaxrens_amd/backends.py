@@ -53,6 +53,7 @@ class PairTable(NamedTuple):
     sig2: np.ndarray   # (S, S) sigma_ab ** 2
     rc2: float
     images: tuple[int, int, int] = (0, 0, 0)  # supercell_trafo // 2
+    eval_chunks: int = 1  # evaluation chunks (summation order, team size)
 
     def as_struct(self, precision: int = _lib.JR_PREC_FP64):
         keep = (_lib.f64(self.eps4), _lib.f64(self.eps24), _lib.f64(self.sig2))
@@ -61,6 +62,7 @@ class PairTable(NamedTuple):
                          _lib.dptr(keep[2]))
         for d in range(3):
             pot.images[d] = int(self.images[d])
+        pot.eval_chunks = int(self.eval_chunks)
         return pot, keep
 
 
@@ -80,6 +82,14 @@ class LJBackend:
     ``r^2 < rc^2`` and unshifted.  ``supercell_trafo`` adds the lattice
     translations ``-(s//2) .. s//2`` per cell vector on top of the minimum
     image, as ``lj.py:106-113,152-166`` does (entries up to 9, FP64 only).
+
+    ``eval_chunks`` (1, 2, 4 or 8) fixes the summation order of every energy
+    and force evaluation: the ring steps of the pair sum are split into that
+    many consecutive chunks, each summed on its own and the chunk sums added
+    in order.  With more than one chunk a team of that many wavefronts
+    walks each chain, one chunk each, which cuts the latency of a walk with
+    few chains per iteration (``n_extra = 0``).  1 (the default) is one
+    running sum per atom.  FP64 minimum image only.
     """
 
     def __init__(
@@ -93,6 +103,7 @@ class LJBackend:
         pair_sigma=None,
         device: int = 0,
         precision: str = "fp64",
+        eval_chunks: int = 1,
     ):
         sc = tuple(int(x) for x in supercell_trafo)
         if min(sc) < 1:
@@ -162,8 +173,15 @@ class LJBackend:
                 sig2 = np.array([[self.sigma ** 2]])
         if self.n_species > _lib.JR_MAX_SPECIES:
             raise ValueError(f"at most {_lib.JR_MAX_SPECIES} species")
+        eval_chunks = int(eval_chunks)
+        if eval_chunks not in (1, 2, 4, 8):
+            raise ValueError("eval_chunks must be 1, 2, 4 or 8")
+        if eval_chunks > 1 and (max(images) > 0 or precision != "fp64"):
+            raise NotImplementedError(
+                "eval_chunks > 1 is built for the fp64 minimum-image potential")
+        self.eval_chunks = eval_chunks
         self.table = PairTable(self.n_species, 4.0 * eps, 24.0 * eps, sig2, rc2,
-                               images)
+                               images, eval_chunks)
 
     # -- evaluation ---------------------------------------------------------
     def _eval(self, positions, species, cell, pressure=None, mu=None,

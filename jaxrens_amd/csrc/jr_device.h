@@ -27,6 +27,16 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
+// LDS pointers with the address space explicit, so that every access is a
+// ds_* instruction whatever the compiler can infer about where a pointer came
+// from (a generic pointer to LDS is accessed by slower flat instructions).
+template <class T>
+using lds_ptr = __attribute__((address_space(3))) T*;
+template <class T>
+__device__ __forceinline__ lds_ptr<T> to_lds(T* p) {
+  return (lds_ptr<T>)p;
+}
+
 // DPP move of both 32-bit halves (CTRL: quad_perm, row_mirror, ...).
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {

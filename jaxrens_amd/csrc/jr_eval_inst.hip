@@ -9,9 +9,10 @@ namespace {
 
 template <int A, int CFG>
 void launch_eval_t(const EvalArgs& a, hipStream_t s) {
-  const int wpb = 4;
-  size_t lds = wave_lds_bytes(a.Nmax, 0) * wpb + 3 * sizeof(double) * a.S * a.S;
-  int blocks = (a.B + wpb - 1) / wpb;
+  constexpr bool team = (CFG & 32) != 0;
+  const int wpb = team ? a.team : 4;
+  const size_t lds = block_lds_bytes(A, wpb, a.S, team ? a.chunks : 1);
+  const int blocks = team ? a.B : (a.B + wpb - 1) / wpb;
   hipLaunchKernelGGL((eval_kernel<A, CFG>), dim3(blocks), dim3(64 * wpb), lds, s, a);
 }
 
@@ -31,7 +32,9 @@ void launch_eval(int A, int cfg, const EvalArgs& a, hipStream_t s) {
     case 2: launch_eval_c<2>(A, a, s); break;
     case 3: launch_eval_c<3>(A, a, s); break;
     case 4: launch_eval_c<4>(A, a, s); break;
-    default: launch_eval_c<5>(A, a, s); break;
+    case 5: launch_eval_c<5>(A, a, s); break;
+    case 32: launch_eval_c<32>(A, a, s); break;
+    default: launch_eval_c<33>(A, a, s); break;
   }
 }
 
@@ -51,7 +54,11 @@ void launch_walk(int A, int cfg, int occ, const WalkArgs& a, int blocks, int wpb
     case 16: launch_walk_cfg<16>(A, occ, a, blocks, wpb, lds, s); break;
     case 17: launch_walk_cfg<17>(A, occ, a, blocks, wpb, lds, s); break;
     case 24: launch_walk_cfg<24>(A, occ, a, blocks, wpb, lds, s); break;
-    default: launch_walk_cfg<25>(A, occ, a, blocks, wpb, lds, s); break;
+    case 25: launch_walk_cfg<25>(A, occ, a, blocks, wpb, lds, s); break;
+    case 32: launch_walk_cfg<32>(A, occ, a, blocks, wpb, lds, s); break;
+    case 33: launch_walk_cfg<33>(A, occ, a, blocks, wpb, lds, s); break;
+    case 40: launch_walk_cfg<40>(A, occ, a, blocks, wpb, lds, s); break;
+    default: launch_walk_cfg<41>(A, occ, a, blocks, wpb, lds, s); break;
   }
 }
 
@@ -70,7 +77,11 @@ const void* walk_fn(int A, int cfg, int occ) {
     case 16: return walk_fn_cfg<16>(A, occ);
     case 17: return walk_fn_cfg<17>(A, occ);
     case 24: return walk_fn_cfg<24>(A, occ);
-    default: return walk_fn_cfg<25>(A, occ);
+    case 25: return walk_fn_cfg<25>(A, occ);
+    case 32: return walk_fn_cfg<32>(A, occ);
+    case 33: return walk_fn_cfg<33>(A, occ);
+    case 40: return walk_fn_cfg<40>(A, occ);
+    default: return walk_fn_cfg<41>(A, occ);
   }
 }
 

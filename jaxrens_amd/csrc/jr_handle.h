@@ -33,6 +33,7 @@ struct jr_handle {
   // is 0 when every replica shares one A.
   struct WalkClass {
     int A = 1, occ = 0, blocks = 1, n_rep = 0;
+    int wpb = 4;  // waves per block (an evaluation team's size depends on A)
     size_t lds = 0;
   };
   int n_classes = 0;
@@ -94,6 +95,7 @@ struct jr_handle {
   // launch geometry
   int n_cu = 0;
   int wpb = 4;
+  int ipb = 4;  // chains a block walks at once (wpb, or 1 for an evaluation team)
   int blocks = 0;
   int max_blocks = 0;  // resident blocks of the walk kernel at this geometry
   int block_cap = 0;   // jr_set_walk_block_cap (0: none)

@@ -50,6 +50,7 @@ struct WalkArgs {
   // walker w at dirx + ((s - 1) * n_pop + w) * Nmax * 3 (NULL: none)
   double* dirx;
   int n_pop;
+  int chunks;  // evaluation chunks of a team kernel (cfg bit 5)
 };
 
 // Draws of sub-step t of chain tq in place of the tape (normals
@@ -60,7 +61,7 @@ struct WalkArgs {
 // [1][0] and the four uniforms, wave-uniform, from lanes 0..5 of one pass.
 template <int A>
 __device__ __forceinline__ void step_draws(const WalkArgs& a, long long it, int tq, int t,
-                                           int N, bool full, double* xs, double& xv0,
+                                           int N, bool full, lds_ptr<double> xs, double& xv0,
                                            double& xv1, double (&u)[4]) {
   constexpr int M = 64 * A;
   const int lane = lane_id();
@@ -106,13 +107,28 @@ __host__ __device__ inline size_t wave_lds_bytes(int Nmax, int n_moves) {
   return wave_lds_bytes_a(Nmax <= 64 ? 1 : (Nmax <= 128 ? 2 : 4));
 }
 
+// Dynamic LDS of a walk or evaluation block: per-wave regions, the pair
+// table, and for an evaluation team (cfg bit 5, the block is one team of
+// `wpb` wavefronts evaluating `chunks` chunks) its exchange rows and the
+// work-item broadcast slot.
+__host__ __device__ inline size_t team_xbuf_offset(int A, int wpb, int S) {
+  return (wave_lds_bytes_a(A) * wpb + 3 * sizeof(double) * S * S + 15) & ~(size_t)15;
+}
+
+__host__ __device__ inline size_t block_lds_bytes(int A, int wpb, int S, int chunks) {
+  if (chunks <= 1) return wave_lds_bytes_a(A) * wpb + 3 * sizeof(double) * S * S;
+  return team_xbuf_offset(A, wpb, S) + team_xbuf_bytes(A, chunks) + 16;
+}
+
 // OCC > 0 asks the register allocator for at least OCC waves per SIMD.
+// A team kernel (cfg bit 5) runs blocks of up to four waves, one per SIMD.
 template <int A, int CFG, int OCC>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8)))
     walk_kernel(WalkArgs a) {
   constexpr bool MULTI = (CFG & 1) != 0;
   constexpr bool F32 = (CFG & 2) != 0;
+  constexpr bool TEAM = (CFG & 32) != 0;
   (void)F32;
   extern __shared__ double lds_raw[];
   const int lane = lane_id();
@@ -124,7 +140,22 @@ __global__ void __launch_bounds__(256)
       reinterpret_cast<char*>(lds_raw) + per_wave * nwave_blk);
   WaveLds L;
   L.base = reinterpret_cast<double*>(base);
-  int* lctr = wave_counters(L, A);
+  int* qslot = nullptr;
+  if constexpr (TEAM) {
+    // one team per block: every wave walks the block's chain, evaluating
+    // its own chunks of the ring steps; wave 0 writes the results
+    const size_t off = ((per_wave * nwave_blk + 3 * sizeof(double) * a.S * a.S) + 15) &
+                       ~(size_t)15;
+    L.xbuf = reinterpret_cast<double*>(reinterpret_cast<char*>(lds_raw) + off);
+    L.rank = wave;
+    L.size = nwave_blk;
+    L.chunks = a.chunks;
+    qslot = reinterpret_cast<int*>(reinterpret_cast<char*>(L.xbuf) +
+                                   team_xbuf_bytes(A, a.chunks));
+  }
+  // the wave that writes results and counters (every wave without a team)
+  const bool writer = !TEAM || wave == 0;
+  const lds_ptr<int> lctr = to_lds(wave_counters(L, A));
 
   // pair table in LDS (block-shared)
   const int SS = a.S * a.S;
@@ -157,9 +188,23 @@ __global__ void __launch_bounds__(256)
     // branch may guard the fetch: with `if (lane == 0) q = atomicAdd(..)`
     // the compiler threads lanes 1..63 from an earlier `if (lane == 0)`
     // straight back to the loop head with q = 0 and no atomic, and those
-    // lanes then repeat item 0 forever.
-    int q = atomicAdd(a.queue, lane == 0 ? 1 : 0);
-    q = __builtin_amdgcn_readfirstlane(q);
+    // lanes then repeat item 0 forever.  A team takes wave 0's item through
+    // LDS, between two block barriers, so every wave of the block walks the
+    // same chain and leaves the loop together.
+    int q;
+    if constexpr (TEAM) {
+      if (wave == 0) {
+        int v = atomicAdd(a.queue, lane == 0 ? 1 : 0);
+        v = __builtin_amdgcn_readfirstlane(v);
+        if (lane == 0) *qslot = v;
+      }
+      __syncthreads();
+      q = __builtin_amdgcn_readfirstlane(*qslot);
+      __syncthreads();
+    } else {
+      q = atomicAdd(a.queue, lane == 0 ? 1 : 0);
+      q = __builtin_amdgcn_readfirstlane(q);
+    }
     if (q >= a.total) break;
     int r, slot, tq;
     if (a.trial) {
@@ -175,7 +220,18 @@ __global__ void __launch_bounds__(256)
     const int N = a.pop.n_atoms[slot];
     C.N = N;
     C.pressure = a.pressure[r];
-    C.mu = a.mu + (size_t)r * a.S;
+    // per-chain constants into registers: every lookup in the step loop
+    // would otherwise wait on a global load
+#pragma unroll
+    for (int sp = 0; sp < JR_MAX_SPECIES; ++sp)
+      C.mu[sp] = a.mu[(size_t)r * a.S + (sp < a.S ? sp : 0)];
+#pragma unroll
+    for (int sp = 0; sp < JR_MAX_SPECIES; ++sp)
+      if (sp >= a.S) C.mu[sp] = 0.0;
+    double stepv[JR_MAX_MOVES];
+#pragma unroll
+    for (int j = 0; j < JR_MAX_MOVES; ++j)
+      stepv[j] = a.trial ? 0.0 : a.step_sizes[(size_t)r * a.n_moves + (j < a.n_moves ? j : 0)];
     C.emax = a.emax[r];
 
     WalkerRegs<A> W;
@@ -206,11 +262,16 @@ __global__ void __launch_bounds__(256)
 
     const int n_steps = a.trial ? 1 : a.n_steps;
     int last_reason = 0;
+    // the next sub-step's move index is fetched a step ahead
+    int m_next = a.trial ? a.trial_move : a.schedule[0];
     for (int t = 0; t < n_steps; ++t) {
-      const int m = a.trial ? a.trial_move : a.schedule[t];
+      const int m = m_next;
+      if (t + 1 < n_steps) m_next = a.schedule[t + 1];
       const DevMove mv = a.moves[m];
-      const double step =
-          a.trial ? a.trial_step[r] : a.step_sizes[(size_t)r * a.n_moves + m];
+      double step = stepv[0];
+#pragma unroll
+      for (int j = 1; j < JR_MAX_MOVES; ++j) step = (m == j) ? stepv[j] : step;
+      if (a.trial) step = a.trial_step[r];
       int reason = 0, n_eval = 1, n_grad = 0;
       int type = mv.type;
       if constexpr ((CFG & 8) != 0) {
@@ -226,10 +287,10 @@ __global__ void __launch_bounds__(256)
         xv0 = xi[0];
         xv1 = xi[a.Nmax];
       } else {
-        double* xs = LdsView<A>{L.base}.hx();
+        const lds_ptr<double> xs = LdsView<A>{L.base}.hx();
         step_draws<A>(a, rng_it, tq, t, N, type == JR_MOVE_GMC || type == JR_MOVE_HMC, xs,
                       xv0, xv1, ud);
-        xi = xs;
+        xi = (const double*)xs;
         u = ud;
       }
       // configuration constraint gate (sampling/mwg.py:98-141)
@@ -302,8 +363,8 @@ __global__ void __launch_bounds__(256)
         }
       }
       last_reason = reason;
-      if (lane == 0) {
-        int* c = lctr + m * JR_NCOUNT;
+      if (writer && lane == 0) {
+        const lds_ptr<int> c = lctr + m * JR_NCOUNT;
         c[0] += 1;
         if ((CFG & 16) == 0 || reason < 4) c[1 + reason] += 1;
         c[5] += n_eval;
@@ -311,9 +372,10 @@ __global__ void __launch_bounds__(256)
       }
     }
     if (a.trial) {
-      if (lane == 0) a.trial_reason[q] = last_reason;
+      if (writer && lane == 0) a.trial_reason[q] = last_reason;
       continue;
     }
+    if (!writer) continue;
     double* Po = a.pop.pos + (size_t)slot * a.Nmax * 3;
     double* Do = a.pop.dir + (size_t)slot * a.Nmax * 3;
     int* To = a.pop.types + (size_t)slot * a.Nmax;
@@ -349,7 +411,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Batched energy (+forces): one wave per configuration.
 struct EvalArgs {
   const double* pos;
   const int* types;
@@ -368,12 +429,17 @@ struct EvalArgs {
   const double* eps24;
   const double* sig2;
   int img[3];  // image half-ranges (CFG bit 2)
+  int chunks;  // evaluation chunks of a team kernel (cfg bit 5)
+  int team;    // waves per block of a team kernel
 };
 
+// Batched energy (+forces): one wave per configuration, or (cfg bit 5) one
+// block per configuration whose waves evaluate one chunk each.
 template <int A, int CFG>
 __global__ void __launch_bounds__(256) eval_kernel(EvalArgs a) {
   constexpr bool MULTI = (CFG & 1) != 0;
   constexpr bool F32 = (CFG & 2) != 0;
+  constexpr bool TEAM = (CFG & 32) != 0;
   (void)F32;
   extern __shared__ double lds_raw[];
   const int lane = lane_id();
@@ -385,6 +451,14 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs a) {
       reinterpret_cast<char*>(lds_raw) + per_wave * nwave_blk);
   WaveLds L;
   L.base = reinterpret_cast<double*>(base);
+  if constexpr (TEAM) {
+    const size_t off = ((per_wave * nwave_blk + 3 * sizeof(double) * a.S * a.S) + 15) &
+                       ~(size_t)15;
+    L.xbuf = reinterpret_cast<double*>(reinterpret_cast<char*>(lds_raw) + off);
+    L.rank = wave;
+    L.size = nwave_blk;
+    L.chunks = a.chunks;
+  }
   const int SS = a.S * a.S;
   for (int i = threadIdx.x; i < SS; i += blockDim.x) {
     tabs[i] = a.eps4[i];
@@ -402,9 +476,10 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs a) {
   tab.img[1] = a.img[1];
   tab.img[2] = a.img[2];
 
-  const int item = blockIdx.x * nwave_blk + wave;
+  const int item = TEAM ? (int)blockIdx.x : (int)blockIdx.x * nwave_blk + wave;
   const int n_items = a.count ? min(*a.count, a.B) : a.B;
-  if (item >= n_items) return;  // wave-uniform; no block barrier follows
+  // wave-uniform (block-uniform for a team); no block barrier follows
+  if (item >= n_items) return;
   const int b = a.cfg_index ? a.cfg_index[item] : item;
   const int N = a.n_atoms[b];
   double x[A], y[A], z[A];
@@ -443,6 +518,7 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs a) {
   } else {
     Hv = U + pr * fabs(det3(cell));
   }
+  if (TEAM && wave != 0) return;
   if (lane == 0) {
     if (a.U) a.U[b] = U;
     if (a.H) a.H[b] = Hv;

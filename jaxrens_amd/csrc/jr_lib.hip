@@ -111,6 +111,8 @@ int jr_eval_energy_forces(const jr_pot* pot, int32_t device, int32_t B,
   a.eps24 = d_e24.p;
   a.sig2 = d_s2.p;
   set_images(a.img, *pot);
+  a.chunks = pot_chunks(pot);
+  a.team = team_waves(atoms_per_lane(Nmax), a.chunks, S, device);
   launch_eval(atoms_per_lane(Nmax), pot_cfg(pot), a, s);
   HIPCHK_G(hipGetLastError());
   if (U) HIPCHK_G(hipMemcpyAsync(U, d_U.p, sizeof(double) * B, hipMemcpyDeviceToHost, s));
@@ -230,7 +232,9 @@ int recompute_energies(jr_handle* h) {
   a.eps24 = h->eps24.p;
   a.sig2 = h->sig2.p;
   set_images(a.img, c.pot);
-  launch_eval(h->A, h->cfgbits & 7, a, h->stream);
+  a.chunks = pot_chunks(&c.pot);
+  a.team = team_waves(h->A, a.chunks, S, c.device);
+  launch_eval(h->A, h->cfgbits & 39, a, h->stream);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return JR_OK;
@@ -280,6 +284,7 @@ WalkArgs walk_args(jr_handle* h) {
   a.seed = c.seed;
   a.dirx = h->dirx.p;
   a.n_pop = c.n_replicas * c.n_walkers;
+  a.chunks = pot_chunks(&c.pot);
   if (!h->draws_buffered) {
     a.normals = nullptr;
     a.uniforms = nullptr;
@@ -457,7 +462,7 @@ hipError_t enqueue_walk(jr_handle* h, const WalkArgs& base, hipStream_t s) {
     wa.total = k.n_rep * base.n_walk;
     wa.queue = h->queue.p + 1 + j;
     wa.lds_atoms = 64 * k.A;
-    launch_walk(k.A, h->cfgbits, k.occ, wa, k.blocks, h->wpb, k.lds, j > 0 ? h->stream2 : s);
+    launch_walk(k.A, h->cfgbits, k.occ, wa, k.blocks, k.wpb, k.lds, j > 0 ? h->stream2 : s);
   }
   if ((e = hipEventRecord(h->ev_join, h->stream2)) != hipSuccess) return e;
   return hipStreamWaitEvent(s, h->ev_join, 0);
@@ -506,34 +511,43 @@ static int snapshot_iter(jr_handle* h, int it) {
 namespace {
 
 // Persistent walk grid: waves per block, occupancy variant, and as many
-// blocks as are resident (never more than the chains of one iteration).
+// blocks as are resident (never more than the chains of one iteration).  An
+// evaluation team (cfg bit 5) is one block of eval_chunks waves per chain.
 int set_geometry(jr_handle* h, int wpb, int occ) {
   const auto& c = h->cfg;
+  const bool team = (h->cfgbits & 32) != 0;
+  const int S = c.pot.n_species;
+  const int C = team ? pot_chunks(&c.pot) : 1;
+  if (team) {
+    wpb = team_waves(h->A, C, S, c.device);
+    occ = 0;
+  }
   h->occ_req = occ;
-  if (h->A != 1 || (h->cfgbits & 28)) occ = 0;
+  if (h->A != 1 || (h->cfgbits & 60)) occ = 0;
   h->wpb = wpb;
+  h->ipb = team ? 1 : wpb;
   h->occ = occ;
   for (int j = 0; j < h->n_classes; ++j) {
     auto& k = h->cls[j];
-    k.occ = (k.A == 1 && !(h->cfgbits & 28)) ? h->occ_req : 0;
-    k.lds = wave_lds_bytes_a(k.A) * wpb + 3 * sizeof(double) * c.pot.n_species * c.pot.n_species;
+    k.occ = (k.A == 1 && !(h->cfgbits & 60)) ? h->occ_req : 0;
+    k.wpb = team ? team_waves(k.A, C, S, c.device) : wpb;
+    k.lds = block_lds_bytes(k.A, k.wpb, S, C);
     int pc = 0;
     HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &pc, walk_fn(k.A, h->cfgbits, k.occ), 64 * wpb, k.lds));
+                  &pc, walk_fn(k.A, h->cfgbits, k.occ), 64 * k.wpb, k.lds));
     if (pc < 1) pc = 1;
-    const int need = (k.n_rep * h->n_walk + wpb - 1) / wpb;
+    const int need = (k.n_rep * h->n_walk + h->ipb - 1) / h->ipb;
     k.blocks = std::max(1, std::min(need, pc * h->n_cu));
     if (h->block_cap > 0) k.blocks = std::min(k.blocks, h->block_cap);
   }
-  h->lds_walk = wave_lds_bytes(c.n_atoms_max, c.n_moves) * wpb +
-                3 * sizeof(double) * c.pot.n_species * c.pot.n_species;
+  h->lds_walk = block_lds_bytes(h->A, wpb, S, C);
   int per_cu = 0;
   HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &per_cu, walk_fn(h->A, h->cfgbits, h->occ), 64 * wpb,
                 h->lds_walk));
   if (per_cu < 1) per_cu = 1;
   const int total = c.n_replicas * h->n_walk;
-  const int need = (total + wpb - 1) / wpb;
+  const int need = (total + h->ipb - 1) / h->ipb;
   h->max_blocks = per_cu * h->n_cu;
   if (h->block_cap > 0) h->max_blocks = std::min(h->max_blocks, h->block_cap);
   h->blocks = std::max(1, std::min(need, h->max_blocks));
@@ -573,7 +587,9 @@ hipError_t lj_eval(jr_handle* h, const double* pos, const int* types, const int*
   a.eps24 = h->eps24.p;
   a.sig2 = h->sig2.p;
   set_images(a.img, h->cfg.pot);
-  launch_eval(h->A, h->cfgbits & 7, a, s);
+  a.chunks = pot_chunks(&h->cfg.pot);
+  a.team = team_waves(h->A, a.chunks, a.S, h->cfg.device);
+  launch_eval(h->A, h->cfgbits & 39, a, s);
   return hipGetLastError();
 }
 
@@ -617,6 +633,10 @@ int jr_create(const jr_config* cfg, jr_handle** out) {
     return set_global_err(JR_E_UNSUPPORTED, "the learned walk is built for n_cull = 1");
   if (c.min_distance_types && !c.min_distance)
     return set_global_err(JR_E_ARG, "min_distance_types needs min_distance");
+  if (pot_chunks(&c.pot) > 1 && (c.learned || c.min_distance))
+    return set_global_err(JR_E_UNSUPPORTED,
+                          "eval_chunks > 1 is built for the pair-potential walk without the "
+                          "minimum-distance constraint");
   if (c.min_distance) {
     if (c.mpnn || c.learned || c.pot.precision != JR_PREC_FP64 || pot_images(&c.pot))
       return set_global_err(JR_E_UNSUPPORTED,
@@ -1278,7 +1298,7 @@ int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
   a.trial_step = h->t_step.p;
   a.trial_reason = h->t_reason.p;
   a.counters = nullptr;
-  int blocks = std::max(1, std::min((int)((T + h->wpb - 1) / h->wpb), h->blocks));
+  int blocks = std::max(1, std::min((int)((T + h->ipb - 1) / h->ipb), h->blocks));
   launch_walk(h->A, h->cfgbits, h->occ, a, blocks, h->wpb, h->lds_walk, s);
   HIPCHK(h, hipGetLastError());
   std::vector<int> rs(T);
@@ -1380,7 +1400,7 @@ int jr_initial_walk(jr_handle* h, int32_t walklength, const double* emax,
     a.uniforms = h->b_uniforms.p;
     a.counters = h->b_ctr.p;
     a.total = R * ck;
-    const int blocks = std::max(1, std::min((R * ck + h->wpb - 1) / h->wpb, h->max_blocks));
+    const int blocks = std::max(1, std::min((R * ck + h->ipb - 1) / h->ipb, h->max_blocks));
     launch_walk(h->A, h->cfgbits, h->occ, a, blocks, h->wpb, h->lds_walk, s);
     HIPCHK(h, hipGetLastError());
   }
@@ -1448,8 +1468,10 @@ int jr_autotune(jr_handle* h, int32_t reps, int32_t* best_wpb,
   double bms = 1e300;
   bool one_per_lane = h->A == 1;
   for (int j = 0; j < h->n_classes; ++j) one_per_lane = one_per_lane || h->cls[j].A == 1;
-  for (int oi = 0; oi < (one_per_lane ? 2 : 1); ++oi) {
-    for (int wi = 0; wi < 3; ++wi) {
+  // an evaluation team has one geometry (eval_chunks waves per block)
+  const bool team = (h->cfgbits & 32) != 0;
+  for (int oi = 0; oi < (one_per_lane && !team ? 2 : 1); ++oi) {
+    for (int wi = 0; wi < (team ? 1 : 3); ++wi) {
       rc = set_geometry(h, wpbs[wi], occs[oi]);
       if (rc) return rc;
       double tot = 0.0;

@@ -138,6 +138,8 @@ extern "C" int jr_eval_energy_forces_noalloc(const jr_pot* pot, int32_t device, 
   a.eps24 = reinterpret_cast<const double*>(at(L.e24));
   a.sig2 = reinterpret_cast<const double*>(at(L.s2));
   set_images(a.img, *pot);
+  a.chunks = pot_chunks(pot);
+  a.team = team_waves(atoms_per_lane(Nmax), a.chunks, S, device);
   launch_eval(atoms_per_lane(Nmax), pot_cfg(pot), a, s);
   HIPCHK_G(hipGetLastError());
   auto d2h = [&](void* dst, size_t off, size_t bytes) {

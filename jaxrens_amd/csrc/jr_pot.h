@@ -4,6 +4,7 @@
 #pragma once
 
 #include "jr_host.h"
+#include "jr_launch.h"
 
 namespace jrh {
 
@@ -13,11 +14,38 @@ inline bool pot_images(const jr_pot* pot) {
   return pot->images[0] > 0 || pot->images[1] > 0 || pot->images[2] > 0;
 }
 
-// kernel configuration bits 0-2 (jr_launch.h)
+inline int pot_chunks(const jr_pot* pot) { return pot->eval_chunks > 1 ? pot->eval_chunks : 1; }
+
+// kernel configuration bits 0-2 and 5 (jr_launch.h)
 inline int pot_cfg(const jr_pot* pot) {
   return (pot->n_species > 1 ? 1 : 0) |
          (pot->precision == JR_PREC_FP32_PAIRS ? 2 : 0) |
-         (pot_images(pot) ? 4 : 0);
+         (pot_images(pot) ? 4 : 0) | (pot_chunks(pot) > 1 ? 32 : 0);
+}
+
+// LDS one workgroup may allocate on `device` (queried once per device).
+inline size_t lds_per_block(int device) {
+  static size_t cache[64] = {0};
+  const int d = device >= 0 && device < 64 ? device : 0;
+  if (!cache[d]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+            hipSuccess ||
+        v <= 0)
+      v = 65536;
+    cache[d] = (size_t)v;
+  }
+  return cache[d];
+}
+
+// Waves of an evaluation team for `chunks` chunks at A atoms per lane: the
+// largest divisor of chunks up to 4 (one wave per SIMD of a CU) whose block
+// LDS fits; each wave evaluates chunks / waves consecutive chunks.
+inline int team_waves(int A, int chunks, int S, int device) {
+  const size_t cap = lds_per_block(device);
+  int w = chunks < 4 ? chunks : 4;
+  while (w > 1 && jr::block_lds_bytes(A, w, S, chunks) > cap) w >>= 1;
+  return w;
 }
 
 inline void set_images(int* dst, const jr_pot& pot) {
@@ -39,6 +67,12 @@ inline int validate_pot(const jr_pot* pot) {
                             JR_MAX_IMAGES, pot->images[d]);
   if (pot_images(pot) && pot->precision != JR_PREC_FP64)
     return set_global_err(JR_E_UNSUPPORTED, "image sums are built for FP64 pairs only");
+  const int ch = pot->eval_chunks;
+  if (ch != 0 && ch != 1 && ch != 2 && ch != 4 && ch != 8)
+    return set_global_err(JR_E_ARG, "eval_chunks must be 0, 1, 2, 4 or 8, got %d", ch);
+  if (ch > 1 && (pot_images(pot) || pot->precision != JR_PREC_FP64))
+    return set_global_err(JR_E_UNSUPPORTED,
+                          "eval_chunks > 1 is built for FP64 pairs without images");
   return JR_OK;
 }
 

@@ -21,7 +21,19 @@ struct PairTab {
 
 struct WaveLds {
   double* base;
+  // Evaluation team (cfg bit 5 kernels): the block's `size` wavefronts walk
+  // one chain together; of the evaluation's `chunks` chunks of ring steps
+  // wave `rank` evaluates chunks/size consecutive ones.  xbuf holds the
+  // block's exchange rows, [chunks][7][64 A] doubles (energy, own force,
+  // travelling reaction per chunk).
+  double* xbuf = nullptr;
+  int rank = 0, size = 1, chunks = 1;
 };
+
+// bytes of a team's exchange rows
+__host__ __device__ inline size_t team_xbuf_bytes(int A, int chunks) {
+  return (size_t)chunks * 7 * 64 * A * sizeof(double);
+}
 
 // Per-wave LDS view, M = 64*A ring positions.  Positions (and types) are
 // stored twice, entries j and j + N both holding atom j, so the partner of
@@ -32,14 +44,14 @@ template <int A>
 struct LdsView {
   static constexpr int M = 64 * A;
   double* b;
-  __device__ __forceinline__ double* x() const { return b; }
-  __device__ __forceinline__ double* y() const { return b + 2 * M; }
-  __device__ __forceinline__ double* z() const { return b + 4 * M; }
-  __device__ __forceinline__ double* hx() const { return b + 6 * M; }
-  __device__ __forceinline__ double* hy() const { return b + 7 * M; }
-  __device__ __forceinline__ double* hz() const { return b + 8 * M; }
-  __device__ __forceinline__ int* t() const {
-    return reinterpret_cast<int*>(b + 9 * M);
+  __device__ __forceinline__ lds_ptr<double> x() const { return to_lds(b); }
+  __device__ __forceinline__ lds_ptr<double> y() const { return to_lds(b + 2 * M); }
+  __device__ __forceinline__ lds_ptr<double> z() const { return to_lds(b + 4 * M); }
+  __device__ __forceinline__ lds_ptr<double> hx() const { return to_lds(b + 6 * M); }
+  __device__ __forceinline__ lds_ptr<double> hy() const { return to_lds(b + 7 * M); }
+  __device__ __forceinline__ lds_ptr<double> hz() const { return to_lds(b + 8 * M); }
+  __device__ __forceinline__ lds_ptr<int> t() const {
+    return to_lds(reinterpret_cast<int*>(b + 9 * M));
   }
 };
 
@@ -203,14 +215,16 @@ __device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int j,
   pair_dr<GEO>(xs, ys, zs, L.x()[j], L.y()[j], L.z()[j], g, P.dx, P.dy, P.dz,
                P.r2);
   P.ok = active && (P.r2 < K.rc2);
-  P.eps4 = tab.eps4[0];
-  P.eps24 = tab.eps24[0];
-  P.sig2 = tab.sig2[0];
+  const lds_ptr<const double> e4 = to_lds(tab.eps4), e24 = to_lds(tab.eps24),
+                              s2 = to_lds(tab.sig2);
+  P.eps4 = e4[0];
+  P.eps24 = e24[0];
+  P.sig2 = s2[0];
   if (MULTI) {
     const int ab = ts * tab.S + L.t()[j];
-    P.eps4 = tab.eps4[ab];
-    P.eps24 = tab.eps24[ab];
-    P.sig2 = tab.sig2[ab];
+    P.eps4 = e4[ab];
+    P.eps24 = e24[ab];
+    P.sig2 = s2[ab];
   }
   return P;
 }
@@ -235,14 +249,16 @@ __device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int j,
 // wave_rol per 32-bit half.
 template <int A, bool FORCES, int CFG, int GEO, bool FULL>
 __device__ __forceinline__ void wave_eval_impl(
-    const LdsView<A>& L, int N, const Geom& g, const PairTab& tab,
+    const WaveLds& WL, int chunk, const LdsView<A>& L, int N, const Geom& g, const PairTab& tab,
     const EvalConst& K, const double (&x)[A], const double (&y)[A], const double (&z)[A],
     const int (&t)[A], double& e_lane, double (&fx)[A], double (&fy)[A],
     double (&fz)[A]) {
   constexpr bool MULTI = (CFG & 1) != 0;
   constexpr bool F32 = (CFG & 2) != 0;
   constexpr bool IMG = (CFG & 4) != 0;
+  constexpr bool TEAM = (CFG & 32) != 0;
   static_assert(!(IMG && F32), "image sums are FP64 only");
+  static_assert(!(TEAM && (IMG || F32)), "evaluation teams are FP64 minimum image only");
   const int lane = lane_id();
   double e[A], ox[A], oy[A], oz[A], tx[A], ty[A], tz[A];
 #pragma unroll
@@ -296,7 +312,20 @@ __device__ __forceinline__ void wave_eval_impl(
     }
   };
   const int kmax = (N - 1) >> 1;
-  int k = 1;
+  // Steps of this evaluation: groups of four from kfirst to kend4, then (on
+  // the last chunk) the group of 2, the lone step and the k = N/2 pairs.
+  // Chunk c of C takes groups [ceil(c ng / C), ceil((c+1) ng / C)) of the
+  // ng = kmax / 4 groups; one wave alone takes them all.
+  int kfirst = 1, kend4 = 1 + 4 * (kmax >> 2);
+  bool last = true;
+  if constexpr (TEAM) {
+    // C is a power of two: ceil(x / C) = (x + C - 1) >> log2(C)
+    const int ng = kmax >> 2, C = WL.chunks, lc = __builtin_ctz(C);
+    kfirst = 1 + 4 * ((chunk * ng + C - 1) >> lc);
+    kend4 = 1 + 4 * (((chunk + 1) * ng + C - 1) >> lc);
+    last = chunk == C - 1;
+  }
+  int k = kfirst;
   if constexpr (IMG) {
     // backends/lj.py:152-197 with supercell images: each ring pair is summed
     // over the translations T = i0 c0 + i1 c1 + i2 c2 (i_d ascending, i2
@@ -353,8 +382,8 @@ __device__ __forceinline__ void wave_eval_impl(
             image_shift(g, i0, i1, i2, Tx, Ty, Tz);
             const double r2 = dot3(Tx, Ty, Tz, Tx, Ty, Tz);
             double ep, fs;
-            pair_lj<false, MULTI>(mask_q(r2 < K.rc2, num / r2), tab.eps4[ab],
-                                  tab.eps24[ab], tab.sig2[ab], ep, fs);
+            pair_lj<false, MULTI>(mask_q(r2 < K.rc2, num / r2), to_lds(tab.eps4)[ab],
+                                  to_lds(tab.eps24)[ab], to_lds(tab.sig2)[ab], ep, fs);
             e[s] = e[s] + 0.5 * ep;
           }
     }
@@ -363,7 +392,7 @@ __device__ __forceinline__ void wave_eval_impl(
   // kept rolled: with N known (N == 64*A) a full unroll multiplies the
   // register pressure and the code size of every call site
 #pragma unroll 1
-  for (; !IMG && k + 3 <= kmax; k += 4) {
+  for (; !IMG && k < kend4; k += 4) {
     PairIn P[4][A];
     QT q[4][A];
 #pragma unroll
@@ -390,12 +419,12 @@ __device__ __forceinline__ void wave_eval_impl(
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      if (FORCES && k + i > 1) rotate();
+      if (FORCES && k + i > kfirst) rotate();
 #pragma unroll
       for (int s = 0; s < A; ++s) accumulate(s, P[i][s], q[i][s], true, 1.0);
     }
   }
-  if (!IMG && k + 1 <= kmax) {
+  if (!IMG && last && k + 1 <= kmax) {
     PairIn P0[A], P1[A];
     QT q0[A], q1[A];
 #pragma unroll
@@ -415,7 +444,7 @@ __device__ __forceinline__ void wave_eval_impl(
         q1[s] = mask_q(P1[s].ok, P0[s].r2 * d);
       }
     }
-    if (FORCES && k > 1) rotate();
+    if (FORCES && k > kfirst) rotate();
 #pragma unroll
     for (int s = 0; s < A; ++s) accumulate(s, P0[s], q0[s], true, 1.0);
     if (FORCES) rotate();
@@ -423,8 +452,8 @@ __device__ __forceinline__ void wave_eval_impl(
     for (int s = 0; s < A; ++s) accumulate(s, P1[s], q1[s], true, 1.0);
     k += 2;
   }
-  if (!IMG && k <= kmax) {  // lone last step
-    if (FORCES && k > 1) rotate();
+  if (!IMG && last && k <= kmax) {  // lone last step
+    if (FORCES && k > kfirst) rotate();
 #pragma unroll
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
@@ -434,7 +463,7 @@ __device__ __forceinline__ void wave_eval_impl(
       accumulate(s, P, q_single(P), true, 1.0);
     }
   }
-  if (!IMG && N >= 2 && !(N & 1)) {
+  if (!IMG && last && N >= 2 && !(N & 1)) {
     const int h = N >> 1;
 #pragma unroll
     for (int s = 0; s < A; ++s) {
@@ -445,6 +474,32 @@ __device__ __forceinline__ void wave_eval_impl(
       accumulate(s, P, q_single(P), false, 0.5);
     }
   }
+  if constexpr (TEAM) {
+    // the chunk's sums into its exchange rows (the travelling reactions at
+    // their atoms: position p holds atom (p + klast) mod N); wave_eval adds
+    // the chunks once the whole team has written them
+    constexpr int M = 64 * A;
+    const int klast = last ? kmax : kend4 - 1;
+    const lds_ptr<double> mine = to_lds(WL.xbuf + (size_t)chunk * 7 * M);
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      const int a = lane + 64 * s;
+      if (FULL || a < N) {
+        mine[a] = e[s];
+        if (FORCES) {
+          int dst = a + klast;
+          if (dst >= N) dst -= N;
+          mine[M + a] = ox[s];
+          mine[2 * M + a] = oy[s];
+          mine[3 * M + a] = oz[s];
+          mine[4 * M + dst] = tx[s];
+          mine[5 * M + dst] = ty[s];
+          mine[6 * M + dst] = tz[s];
+        }
+      }
+    }
+    return;
+  }
   double el = e[0];
 #pragma unroll
   for (int s = 1; s < A; ++s) {
@@ -452,11 +507,9 @@ __device__ __forceinline__ void wave_eval_impl(
     if (FULL || a < N) el = el + e[s];
   }
   e_lane = el;
-  if (FORCES) {
+  if (FORCES && !TEAM) {
     // position p holds the reaction of atom (p + kmax) mod N
-    double* HX = L.hx();
-    double* HY = L.hy();
-    double* HZ = L.hz();
+    const lds_ptr<double> HX = L.hx(), HY = L.hy(), HZ = L.hz();
 #pragma unroll
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
@@ -488,6 +541,50 @@ __device__ __forceinline__ void wave_eval_impl(
   }
 }
 
+// Sum of a team's chunk rows in chunk order, ((x_0 + x_1) + x_2) + ..., for
+// the energy partial of each ring position and the force on each atom (own
+// plus travelling reaction, times cf where the evaluation scales).  C is a
+// compile-time constant so every load is issued before the first add.
+template <int A, bool FORCES, int C>
+__device__ __forceinline__ void team_combine(lds_ptr<const double> xb, int N, bool scale,
+                                             double cf, double (&e)[A], double (&fx)[A],
+                                             double (&fy)[A], double (&fz)[A]) {
+  constexpr int M = 64 * A;
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    const int a = lane + 64 * s;
+    e[s] = 0.0;
+    if (a < N) {
+      double v[C][7];
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int r = 0; r < (FORCES ? 7 : 1); ++r) v[c][r] = xb[(c * 7 + r) * M + a];
+      double acc[7];
+#pragma unroll
+      for (int r = 0; r < (FORCES ? 7 : 1); ++r) {
+        acc[r] = v[0][r];
+#pragma unroll
+        for (int c = 1; c < C; ++c) acc[r] = acc[r] + v[c][r];
+      }
+      e[s] = acc[0];
+      if (FORCES) {
+        fx[s] = acc[1] + acc[4];
+        fy[s] = acc[2] + acc[5];
+        fz[s] = acc[3] + acc[6];
+        if (scale) {
+          fx[s] = cf * fx[s];
+          fy[s] = cf * fy[s];
+          fz[s] = cf * fz[s];
+        }
+      }
+    } else if (FORCES) {
+      fx[s] = fy[s] = fz[s] = 0.0;
+    }
+  }
+}
+
 // Model energy U (wave-uniform) and forces of the configuration in
 // registers; writes the configuration (twice) into the wave's LDS first.
 // A cubic cell is evaluated in fractional coordinates u = x * inv[0] with
@@ -504,12 +601,13 @@ __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
                                             double (&fz)[A]) {
   constexpr bool MULTI = (CFG & 1) != 0;
   constexpr bool IMG = (CFG & 4) != 0;
+  constexpr bool TEAM = (CFG & 32) != 0;
   const int lane = lane_id();
   const LdsView<A> L{W.base};
   EvalConst K;
-  K.num = MULTI ? 1.0 : tab.sig2[0];
+  K.num = MULTI ? 1.0 : to_lds(tab.sig2)[0];
   K.rc2 = tab.rc2;
-  K.cf = MULTI ? 1.0 : tab.eps24[0] / tab.sig2[0];
+  K.cf = MULTI ? 1.0 : to_lds(tab.eps24)[0] / to_lds(tab.sig2)[0];
   double u[A], v[A], w[A];
   if (!IMG && g.cubic) {
     const double len = g.c[0], len2 = len * len;
@@ -548,36 +646,65 @@ __device__ __forceinline__ double wave_eval(const WaveLds& W, int N,
     }
   }
   wave_lds_sync();
-  double el;
+  double el = 0.0;
   const bool full = (N == 64 * A);
-  if constexpr (IMG) {
-    wave_eval_impl<A, FORCES, CFG, GEO_TRI, false>(L, N, g, tab, K, u, v, w, t,
-                                                   el, fx, fy, fz);
-  } else if (g.cubic) {
-    if (full)
-      wave_eval_impl<A, FORCES, CFG, GEO_CUBIC, true>(L, N, g, tab, K, u, v, w,
+  auto eval_chunk = [&](int chunk) {
+    if constexpr (IMG) {
+      wave_eval_impl<A, FORCES, CFG, GEO_TRI, false>(W, chunk, L, N, g, tab, K, u, v, w,
+                                                     t, el, fx, fy, fz);
+    } else if (g.cubic) {
+      if (full)
+        wave_eval_impl<A, FORCES, CFG, GEO_CUBIC, true>(W, chunk, L, N, g, tab, K, u, v,
+                                                        w, t, el, fx, fy, fz);
+      else
+        wave_eval_impl<A, FORCES, CFG, GEO_CUBIC, false>(W, chunk, L, N, g, tab, K, u, v,
+                                                         w, t, el, fx, fy, fz);
+    } else if (g.diag) {
+      if (full)
+        wave_eval_impl<A, FORCES, CFG, GEO_DIAG, true>(W, chunk, L, N, g, tab, K, u, v, w,
+                                                       t, el, fx, fy, fz);
+      else
+        wave_eval_impl<A, FORCES, CFG, GEO_DIAG, false>(W, chunk, L, N, g, tab, K, u, v,
+                                                        w, t, el, fx, fy, fz);
+    } else {
+      if (full)
+        wave_eval_impl<A, FORCES, CFG, GEO_TRI, true>(W, chunk, L, N, g, tab, K, u, v, w,
                                                       t, el, fx, fy, fz);
-    else
-      wave_eval_impl<A, FORCES, CFG, GEO_CUBIC, false>(L, N, g, tab, K, u, v,
+      else
+        wave_eval_impl<A, FORCES, CFG, GEO_TRI, false>(W, chunk, L, N, g, tab, K, u, v,
                                                        w, t, el, fx, fy, fz);
-  } else if (g.diag) {
-    if (full)
-      wave_eval_impl<A, FORCES, CFG, GEO_DIAG, true>(L, N, g, tab, K, u, v, w,
-                                                     t, el, fx, fy, fz);
-    else
-      wave_eval_impl<A, FORCES, CFG, GEO_DIAG, false>(L, N, g, tab, K, u, v, w,
-                                                      t, el, fx, fy, fz);
+    }
+  };
+  if constexpr (TEAM) {
+    // Each wave evaluates its chunks into the exchange rows (the first
+    // barrier keeps them from overwriting rows the team still reads from the
+    // previous evaluation), then every wave adds the chunks in chunk order,
+    // so the whole team ends with the same energy partials and forces.
+    constexpr int M = 64 * A;
+    const int per = W.chunks / W.size;
+    __syncthreads();
+    for (int j = 0; j < per; ++j) eval_chunk(W.rank * per + j);
+    __syncthreads();
+    const lds_ptr<const double> xb = to_lds(static_cast<const double*>(W.xbuf));
+    const bool scale = g.cubic || !MULTI;
+    double e[A];
+    switch (W.chunks) {
+      case 2: team_combine<A, FORCES, 2>(xb, N, scale, K.cf, e, fx, fy, fz); break;
+      case 4: team_combine<A, FORCES, 4>(xb, N, scale, K.cf, e, fx, fy, fz); break;
+      default: team_combine<A, FORCES, 8>(xb, N, scale, K.cf, e, fx, fy, fz); break;
+    }
+    el = e[0];
+#pragma unroll
+    for (int s = 1; s < A; ++s) {
+      const int a = lane + 64 * s;
+      if (a < N) el = el + e[s];
+    }
   } else {
-    if (full)
-      wave_eval_impl<A, FORCES, CFG, GEO_TRI, true>(L, N, g, tab, K, u, v, w,
-                                                    t, el, fx, fy, fz);
-    else
-      wave_eval_impl<A, FORCES, CFG, GEO_TRI, false>(L, N, g, tab, K, u, v, w,
-                                                     t, el, fx, fy, fz);
+    eval_chunk(0);
   }
   wave_lds_sync();
   double U = wave_sum((lane < N) ? el : 0.0);
-  if (!MULTI) U = tab.eps4[0] * U;
+  if (!MULTI) U = to_lds(tab.eps4)[0] * U;
   return U;
 }
 

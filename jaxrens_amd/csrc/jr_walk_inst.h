@@ -15,8 +15,13 @@ template <int CFG>
 void launch_walk_cfg(int A, int occ, const WalkArgs& a, int blocks, int wpb, size_t lds,
                      hipStream_t s) {
   if (A == 1) {
-    if (!(CFG & 28) && occ == 3) launch_walk_t<1, CFG & 3, 3>(a, blocks, wpb, lds, s);
-    else launch_walk_t<1, CFG, 0>(a, blocks, wpb, lds, s);
+    if constexpr (!(CFG & 60)) {
+      if (occ == 3) {
+        launch_walk_t<1, CFG & 3, 3>(a, blocks, wpb, lds, s);
+        return;
+      }
+    }
+    launch_walk_t<1, CFG, 0>(a, blocks, wpb, lds, s);
   } else if (A == 2) {
     launch_walk_t<2, CFG, 0>(a, blocks, wpb, lds, s);
   } else {
@@ -26,9 +31,12 @@ void launch_walk_cfg(int A, int occ, const WalkArgs& a, int blocks, int wpb, siz
 
 template <int CFG>
 const void* walk_fn_cfg(int A, int occ) {
-  if (A == 1)
-    return (!(CFG & 28) && occ == 3) ? (const void*)walk_kernel<1, CFG & 3, 3>
-                                     : (const void*)walk_kernel<1, CFG, 0>;
+  if (A == 1) {
+    if constexpr (!(CFG & 60)) {
+      if (occ == 3) return (const void*)walk_kernel<1, CFG & 3, 3>;
+    }
+    return (const void*)walk_kernel<1, CFG, 0>;
+  }
   if (A == 2) return (const void*)walk_kernel<2, CFG, 0>;
   return (const void*)walk_kernel<4, CFG, 0>;
 }

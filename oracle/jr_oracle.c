@@ -14,6 +14,7 @@
  */
 #include "jr_oracle.h"
 
+#include <assert.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -323,56 +324,25 @@ static void image_pairs(int N, const double* pos, const int32_t* types,
  * and F = (24 eps / sigma^2) * (own + trav).  A cubic cell (diagonal, equal
  * lengths L) is evaluated in fractional coordinates u = x * inv[0]:
  * dr = du - rint(du), num and rc^2 divided by L^2, and F = cf * (own + trav)
- * with cf = 24 eps / sigma^2 * L (species table: cf = L). */
-void or_energy_forces(int N, const double* pos, const int32_t* types,
-                      const double* cell_in, const or_pot* pot,
-                      double pressure, const double* mu, double* U_out,
-                      double* H_out, double* F) {
-  double det = or_det3(cell_in);
-  double cell[9];
-  if (det == 0.0) { /* lj.py:138-139: non-periodic stand-in */
-    memset(cell, 0, sizeof(cell));
-    cell[0] = cell[4] = cell[8] = 1.0e10;
-    det = or_det3(cell);
-  } else {
-    memcpy(cell, cell_in, sizeof(cell));
-  }
-  double inv[9];
-  inv3(cell, det, inv);
-  const int multi = pot->n_species > 1;
-  const int diag = cell[1] == 0.0 && cell[2] == 0.0 && cell[3] == 0.0 &&
-                   cell[5] == 0.0 && cell[6] == 0.0 && cell[7] == 0.0;
-  const int images = pot->img[0] > 0 || pot->img[1] > 0 || pot->img[2] > 0;
-  const int cubic = !images && diag && cell[0] == cell[4] && cell[0] == cell[8];
-  double num = multi ? 1.0 : pot->sig2[0];
-  double rc2 = pot->rc2;
-  double cf = multi ? 1.0 : pot->eps24[0] / pot->sig2[0];
-
-  size_t nn = (size_t)(N > 0 ? N : 1);
-  double* e = (double*)calloc(nn, sizeof(double));
-  double* own = (double*)calloc(3 * nn, sizeof(double));
-  double* trav = (double*)calloc(3 * nn, sizeof(double));
-  double* frac = NULL;
-  if (cubic) { /* fractional coordinates; lengths folded into the constants */
-    const double len = cell[0], len2 = len * len;
-    num = num / len2;
-    rc2 = rc2 / len2;
-    cf = cf * len;
-    frac = (double*)malloc(3 * nn * sizeof(double));
-    for (int i = 0; i < 3 * N; ++i) frac[i] = pos[i] * inv[0];
-    pos = frac;
-  }
-  pair_t* P = (pair_t*)malloc(4 * nn * sizeof(pair_t)); /* [4][N] */
-  double* q = (double*)malloc(4 * nn * sizeof(double));
-  int kmax = (N - 1) / 2;
-  int half = (N >= 2) && (N % 2 == 0);
-  int k = 1;
-  if (images) {
-    image_pairs(N, pos, types, cell, inv, pot, num, rc2, multi, e, own, trav);
-    k = kmax + 1;
-    half = 0;
-  }
-  while (k <= kmax) {
+ * with cf = 24 eps / sigma^2 * L (species table: cf = L).
+ *
+ * Evaluation chunks (pot->chunks = C > 1, minimum image only): the ng =
+ * floor(kmax / 4) groups of four steps are split into C consecutive chunks,
+ * chunk c holding groups [ceil(c ng / C), ceil((c+1) ng / C)); the last
+ * chunk also takes the tail (the group of 2, the lone step, the k = N/2
+ * pairs).  Each chunk keeps its own running sums e_c[a], own_c[a],
+ * trav_c[a] (k order within the chunk, each starting at 0), and the totals
+ * are ((x_0 + x_1) + x_2) + ... in chunk order.  C = 1 is the single running
+ * sum above.  A team of C wavefronts evaluates one configuration in this
+ * order, one chunk each. */
+static void lj_steps(int N, const double* pos, const int32_t* types,
+                     const double* cell, const double* inv, const or_pot* pot,
+                     double num, double rc2, int multi, int cubic, int k,
+                     int kend, int tail, double* e, double* own, double* trav,
+                     pair_t* P, double* q) {
+  const int kmax = (N - 1) / 2;
+  const int klim = tail ? kmax : kend - 1;
+  while (k <= klim) {
     const int G = (k + 3 <= kmax) ? 4 : ((k + 1 <= kmax) ? 2 : 1);
     for (int a = 0; a < N; ++a) {
       pair_t* p = P + a; /* p[i * N] is step k + i */
@@ -405,7 +375,7 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
                  1.0, e, own, trav);
     k += G;
   }
-  if (half) {
+  if (tail && N >= 2 && N % 2 == 0) {
     int h = N / 2;
     for (int a = 0; a < N; ++a) {
       int j = (a + h) % N;
@@ -414,6 +384,80 @@ void or_energy_forces(int N, const double* pos, const int32_t* types,
       double qh = p.ok ? num / p.r2 : 0.0;
       acc_pair(a, j, &p, qh, multi, 0, 0.5, e, own, trav);
     }
+  }
+}
+
+void or_energy_forces(int N, const double* pos, const int32_t* types,
+                      const double* cell_in, const or_pot* pot,
+                      double pressure, const double* mu, double* U_out,
+                      double* H_out, double* F) {
+  double det = or_det3(cell_in);
+  double cell[9];
+  if (det == 0.0) { /* lj.py:138-139: non-periodic stand-in */
+    memset(cell, 0, sizeof(cell));
+    cell[0] = cell[4] = cell[8] = 1.0e10;
+    det = or_det3(cell);
+  } else {
+    memcpy(cell, cell_in, sizeof(cell));
+  }
+  double inv[9];
+  inv3(cell, det, inv);
+  const int multi = pot->n_species > 1;
+  const int diag = cell[1] == 0.0 && cell[2] == 0.0 && cell[3] == 0.0 &&
+                   cell[5] == 0.0 && cell[6] == 0.0 && cell[7] == 0.0;
+  const int images = pot->img[0] > 0 || pot->img[1] > 0 || pot->img[2] > 0;
+  const int cubic = !images && diag && cell[0] == cell[4] && cell[0] == cell[8];
+  const int C = pot->chunks > 1 ? pot->chunks : 1;
+  assert(!(images && C > 1));
+  double num = multi ? 1.0 : pot->sig2[0];
+  double rc2 = pot->rc2;
+  double cf = multi ? 1.0 : pot->eps24[0] / pot->sig2[0];
+
+  size_t nn = (size_t)(N > 0 ? N : 1);
+  double* e = (double*)calloc(nn, sizeof(double));
+  double* own = (double*)calloc(3 * nn, sizeof(double));
+  double* trav = (double*)calloc(3 * nn, sizeof(double));
+  double* frac = NULL;
+  if (cubic) { /* fractional coordinates; lengths folded into the constants */
+    const double len = cell[0], len2 = len * len;
+    num = num / len2;
+    rc2 = rc2 / len2;
+    cf = cf * len;
+    frac = (double*)malloc(3 * nn * sizeof(double));
+    for (int i = 0; i < 3 * N; ++i) frac[i] = pos[i] * inv[0];
+    pos = frac;
+  }
+  pair_t* P = (pair_t*)malloc(4 * nn * sizeof(pair_t)); /* [4][N] */
+  double* q = (double*)malloc(4 * nn * sizeof(double));
+  const int kmax = (N - 1) / 2;
+  if (images) {
+    image_pairs(N, pos, types, cell, inv, pot, num, rc2, multi, e, own, trav);
+  } else if (C == 1) {
+    lj_steps(N, pos, types, cell, inv, pot, num, rc2, multi, cubic, 1, 0, 1,
+             e, own, trav, P, q);
+  } else {
+    double* ec = (double*)malloc(nn * sizeof(double));
+    double* oc = (double*)malloc(3 * nn * sizeof(double));
+    double* tc = (double*)malloc(3 * nn * sizeof(double));
+    const int ng = kmax / 4;
+    for (int c = 0; c < C; ++c) {
+      const int g0 = (c * ng + C - 1) / C, g1 = ((c + 1) * ng + C - 1) / C;
+      memset(ec, 0, nn * sizeof(double));
+      memset(oc, 0, 3 * nn * sizeof(double));
+      memset(tc, 0, 3 * nn * sizeof(double));
+      lj_steps(N, pos, types, cell, inv, pot, num, rc2, multi, cubic,
+               1 + 4 * g0, 1 + 4 * g1, c == C - 1, ec, oc, tc, P, q);
+      for (int a = 0; a < N; ++a) {
+        e[a] = c == 0 ? ec[a] : e[a] + ec[a];
+        for (int d = 0; d < 3; ++d) {
+          own[3 * a + d] = c == 0 ? oc[3 * a + d] : own[3 * a + d] + oc[3 * a + d];
+          trav[3 * a + d] = c == 0 ? tc[3 * a + d] : trav[3 * a + d] + tc[3 * a + d];
+        }
+      }
+    }
+    free(ec);
+    free(oc);
+    free(tc);
   }
   double U = wave_sum_atoms(e, N);
   if (!multi) U = pot->eps4[0] * U;

@@ -38,6 +38,8 @@ typedef struct {
   const double* dmin_pos;   /* [S*S] minimum-distance floors gating moves
                                that move atoms or the cell; NULL = none */
   const double* dmin_types; /* [S*S] floors gating morph and swap; NULL = none */
+  int chunks;           /* evaluation chunks C (0 or 1: one running sum per
+                           atom over all ring steps; see or_energy_forces) */
 } or_pot;
 
 typedef struct {

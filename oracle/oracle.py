@@ -22,7 +22,7 @@ _ip = C.POINTER(C.c_int32)
 class OrPot(C.Structure):
     _fields_ = [("n_species", C.c_int), ("rc2", C.c_double), ("eps4", _dp),
                 ("eps24", _dp), ("sig2", _dp), ("img", C.c_int * 3),
-                ("dmin_pos", _dp), ("dmin_types", _dp)]
+                ("dmin_pos", _dp), ("dmin_types", _dp), ("chunks", C.c_int)]
 
 
 class OrMove(C.Structure):
@@ -133,16 +133,17 @@ def i32(a):
 class Pot:
     """Pair table as the oracle takes it (from an LJ backend's table)."""
 
-    def __init__(self, n_species, eps4, eps24, sig2, rc2, images=(0, 0, 0)):
+    def __init__(self, n_species, eps4, eps24, sig2, rc2, images=(0, 0, 0), chunks=1):
         self.keep = (f64(eps4), f64(eps24), f64(sig2))
         self.s = OrPot(int(n_species), float(rc2), _d(self.keep[0]),
                        _d(self.keep[1]), _d(self.keep[2]),
-                       (C.c_int * 3)(*(int(h) for h in images)))
+                       (C.c_int * 3)(*(int(h) for h in images)), None, None,
+                       int(chunks))
 
     @classmethod
     def from_table(cls, t):
         return cls(t.n_species, t.eps4, t.eps24, t.sig2, t.rc2,
-                   getattr(t, "images", (0, 0, 0)))
+                   getattr(t, "images", (0, 0, 0)), getattr(t, "eval_chunks", 1))
 
     def with_min_distance(self, floors_pos, floors_types=None):
         """Gate the walk with minimum-distance floors ``[S][S]`` (NULL when
