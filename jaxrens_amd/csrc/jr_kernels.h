@@ -132,7 +132,8 @@ __global__ void __launch_bounds__(256)
   (void)F32;
   extern __shared__ double lds_raw[];
   const int lane = lane_id();
-  const int wave = threadIdx.x >> 6;
+  // wave index within the block, wave-uniform (kept in a scalar register)
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nwave_blk = blockDim.x >> 6;
   const size_t per_wave = wave_lds_bytes(a.lds_atoms, a.n_moves);
   char* base = reinterpret_cast<char*>(lds_raw) + per_wave * wave;
@@ -443,7 +444,8 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs a) {
   (void)F32;
   extern __shared__ double lds_raw[];
   const int lane = lane_id();
-  const int wave = threadIdx.x >> 6;
+  // wave index within the block, wave-uniform (kept in a scalar register)
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nwave_blk = blockDim.x >> 6;
   const size_t per_wave = wave_lds_bytes(a.Nmax, 0);
   char* base = reinterpret_cast<char*>(lds_raw) + per_wave * wave;

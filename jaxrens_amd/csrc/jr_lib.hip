@@ -522,14 +522,16 @@ int set_geometry(jr_handle* h, int wpb, int occ) {
     wpb = team_waves(h->A, C, S, c.device);
     occ = 0;
   }
+  // occupancy request 3: the build held to three waves per SIMD, which
+  // exists for one atom per lane without cfg bits 2-5
+  auto occ_for = [&](int A) { return (A == 1 && !(h->cfgbits & 60)) ? occ : 0; };
   h->occ_req = occ;
-  if (h->A != 1 || (h->cfgbits & 60)) occ = 0;
   h->wpb = wpb;
   h->ipb = team ? 1 : wpb;
-  h->occ = occ;
+  h->occ = occ_for(h->A);
   for (int j = 0; j < h->n_classes; ++j) {
     auto& k = h->cls[j];
-    k.occ = (k.A == 1 && !(h->cfgbits & 60)) ? h->occ_req : 0;
+    k.occ = occ_for(k.A);
     k.wpb = team ? team_waves(k.A, C, S, c.device) : wpb;
     k.lds = block_lds_bytes(k.A, k.wpb, S, C);
     int pc = 0;
