@@ -250,7 +250,7 @@ def sur_population(R, K, N, seed=7, rho=0.9):
 SUR_BASE = (1.0, 0.8, 1.0)
 # committed record of the matched-log-Z runs (tests/test_gpu_learned_logz.py and
 # tests/test_gpu_learned_walk.py)
-LOGZ_FILE = os.path.join(ROOT, "profiles", "surrogate_log_z_mpnn256_lj8_hmc.txt")
+LOGZ_FILE = os.path.join(ROOT, "profiles", "surrogate_log_z_mpnn256_lj128_lj8.txt")
 
 
 def logz_record(config: str) -> dict | None:
@@ -446,9 +446,11 @@ def lj128_hmc_report(R=8, K=2048, N=128, T=20, n_timed=1) -> dict:
     out["screened"] = run("auto", 20, n_timed)
     out["plain"] = run("plain", out["screened"]["warm_up_iterations"], n_timed)
     # log Z: the screened HMC is the exact constrained-prior move, checked
-    # against the plain random-walk moves at LJ-8 (at LJ-128 from a lattice
-    # start the plain walk's own log Z has not converged in walk length)
-    out["log_z_check"] = {"screened_hmc_vs_random_walk": logz_record("lj8_hmc"),
+    # against the plain pair-potential random walk (GMC + volume) at LJ-128
+    # (screened in every iteration, 96 sub-steps) and at LJ-8, where the
+    # reference's HMC rule is also compared
+    out["log_z_check"] = {"lj128_screened_hmc_vs_random_walk": logz_record("lj128_hmc"),
+                          "screened_hmc_vs_random_walk": logz_record("lj8_hmc"),
                           "reference_hmc_vs_random_walk": logz_record("lj8_reference_hmc")}
     return out
 
@@ -524,17 +526,27 @@ def main():
 
     secondary = None
     if not args.skip_secondary:
-        e0 = build_engine(n_extra=0)
-        e0.ns_iterations(5, graph=True)
-        n0 = 200
-        t1 = time.perf_counter()
-        e0.ns_iterations(n0, graph=True)
-        d0 = time.perf_counter() - t1
-        secondary = {"n_extra_0": {
-            "walker_steps_per_s": R * 1 * T * n0 / d0,
-            "ns_iterations_per_s": n0 / d0, "iterations": n0,
-            "mode": "hipGraph replay of one iteration"}}
-        e0.close()
+        secondary = {}
+        for key, chunks in (("n_extra_0", 1), ("n_extra_0_teams", 4)):
+            e0 = build_engine(n_extra=0, eval_chunks=chunks)
+            e0.ns_iterations(5, graph=True)
+            n0 = 200
+            t1 = time.perf_counter()
+            e0.ns_iterations(n0, graph=True)
+            d0 = time.perf_counter() - t1
+            secondary[key] = {
+                "walker_steps_per_s": R * 1 * T * n0 / d0,
+                "ns_iterations_per_s": n0 / d0, "iterations": n0,
+                "mode": "hipGraph replay of one iteration",
+                "eval_chunks": chunks, "launch": e0.launch_info()}
+            if chunks > 1:
+                secondary[key]["note"] = (
+                    "evaluation teams: every chain walked by a block of 4 waves, each "
+                    "evaluating one of 4 chunks of the ring steps (chunked FP64 order)")
+            e0.close()
+        secondary["n_extra_0_teams"]["speedup_vs_one_wave"] = (
+            secondary["n_extra_0_teams"]["walker_steps_per_s"]
+            / secondary["n_extra_0"]["walker_steps_per_s"])
         e32 = build_engine(n_extra=1022, precision="fp32_pairs")
         t32 = e32.autotune(reps=2) if not args.no_tune else None
         e32.ns_iterations(2)

@@ -74,3 +74,56 @@ def test_mpnn256_screened_matches_plain_log_z():
                 "stage1_pass": acc["stage1_pass"] / max(acc["proposals"], 1)})
     assert acc["iterations_screened"] == M * N_ITER
     assert abs(t) < T_MAX
+
+
+def test_lj128_screened_hmc_matches_random_walk_log_z():
+    """LJ-128 (the HMC configuration's system): the surrogate-screened walk
+    of HMC + GMC + volume (screened in every iteration, online training; the
+    screened HMC is the exact constrained-prior move) against the plain
+    pair-potential walk
+    of GMC + volume on the same system: 6 engines per walk of 8 replicas
+    (P = 1, no exchange), K = 8, 120 iterations, Welch's t over engine
+    means, |t| < 3.  The walks are long enough to decorrelate a 128-atom
+    walker between iterations (96 sub-steps, galilean step scaled with
+    sqrt(N) from the 8-atom test's); at a few dozen sub-steps both walks
+    leave their clones correlated and their log Z differ for that reason."""
+    from jaxrens_amd.backends import LJBackend
+    import os
+    N, P, K, R, M_E, N_IT = 128, 1.0, 8, 8, 6, 120
+    T = int(os.environ.get("JR_LJ128_STEPS", "96"))
+    cell = {"min_volume_per_atom": 4.0, "max_volume_per_atom": 40.0, "flat_V_prior": True}
+    hmc = {"type": "hmc", "n_leapfrog": 5, "step_size": 0.01, "weight": 1}
+    gal = {"type": "galilean", "n_reflect": 4, "step_size": 0.6, "weight": 2}
+    vol = {"type": "volume", "step_size": 1.0, "weight": 1}
+    means = {"random_walk": [], "screened_hmc": []}
+    acc = {}
+    for name, base in (("random_walk", 810), ("screened_hmc", 820)):
+        for m in range(M_E):
+            lc = None
+            moves = [gal, vol]
+            if name == "screened_hmc":
+                moves = [hmc, gal, vol]
+                lc = LearnedConfig(mode="screened", ring_capacity=4096, train_batch=16,
+                                   train_steps=4, min_train=64, min_heldout=8, lr=2e-3,
+                                   rmse_bound=0.2, delta_k=2.0, baseline=(4.0, 1.0, 2.5),
+                                   params=init_surrogate(6, S=1))
+            run = RunConfig(n_replicas=R, n_walkers=K, n_atoms=N, n_mcmc_steps=T, n_extra=0,
+                            seed=base + m, pressures=[P] * R, re_interval=0)
+            eng = NSEngine(Hamiltonian(LJBackend(1.0, 1.0, 2.5), [EnsembleTerm(P)]), moves,
+                           run, cell=cell, learned=lc)
+            eng.init_walkers(np.random.default_rng(base + m), 4.0 * 1.001, 40.0, True)
+            eng.ns_iterations(N_IT)
+            lz = eng.final_log_evidence()
+            means[name].append(float(lz.mean()))
+            if lc is not None:
+                _stats_sum(acc, eng.learned_stats())
+            print(f"  N=128 {name} engine {m}: mean log Z {lz.mean():.3f}", flush=True)
+            eng.close()
+    t = report("lj128_hmc", means["random_walk"], means["screened_hmc"],
+               {"iterations": N_IT, "replicas_per_engine": R, "n_walkers": K,
+                "n_mcmc_steps": T, "iterations_screened": acc["iterations_screened"],
+                "stage1_pass": acc["stage1_pass"] / max(acc["proposals"], 1),
+                "accepted_per_judge_eval": acc["accepted"] / max(acc["exact_energy_evals"], 1)},
+               labels=("random_walk", "screened_hmc"))
+    assert acc["iterations_screened"] == M_E * N_IT
+    assert abs(t) < T_MAX
