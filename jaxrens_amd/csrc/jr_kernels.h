@@ -221,18 +221,20 @@ __global__ void __launch_bounds__(256)
     const int N = a.pop.n_atoms[slot];
     C.N = N;
     C.pressure = a.pressure[r];
-    // per-chain constants into registers: every lookup in the step loop
-    // would otherwise wait on a global load
-#pragma unroll
-    for (int sp = 0; sp < JR_MAX_SPECIES; ++sp)
-      C.mu[sp] = a.mu[(size_t)r * a.S + (sp < a.S ? sp : 0)];
-#pragma unroll
-    for (int sp = 0; sp < JR_MAX_SPECIES; ++sp)
-      if (sp >= a.S) C.mu[sp] = 0.0;
-    double stepv[JR_MAX_MOVES];
-#pragma unroll
-    for (int j = 0; j < JR_MAX_MOVES; ++j)
-      stepv[j] = a.trial ? 0.0 : a.step_sizes[(size_t)r * a.n_moves + (j < a.n_moves ? j : 0)];
+    // the chain's chemical potentials and step sizes into the wave's LDS:
+    // every lookup in the step loop would otherwise wait on a global load
+    {
+      const lds_ptr<double> cst = to_lds(wave_consts(L, A));
+      wave_lds_sync();
+      if (lane < JR_MAX_SPECIES)
+        cst[lane] = lane < a.S ? a.mu[(size_t)r * a.S + lane] : 0.0;
+      else if (lane < JR_MAX_SPECIES + JR_MAX_MOVES)
+        cst[lane] = (!a.trial && lane - JR_MAX_SPECIES < a.n_moves)
+                        ? a.step_sizes[(size_t)r * a.n_moves + lane - JR_MAX_SPECIES]
+                        : 0.0;
+      wave_lds_sync();
+      C.mu = cst;
+    }
     C.emax = a.emax[r];
 
     WalkerRegs<A> W;
@@ -269,10 +271,7 @@ __global__ void __launch_bounds__(256)
       const int m = m_next;
       if (t + 1 < n_steps) m_next = a.schedule[t + 1];
       const DevMove mv = a.moves[m];
-      double step = stepv[0];
-#pragma unroll
-      for (int j = 1; j < JR_MAX_MOVES; ++j) step = (m == j) ? stepv[j] : step;
-      if (a.trial) step = a.trial_step[r];
+      const double step = a.trial ? a.trial_step[r] : C.mu[JR_MAX_SPECIES + m];
       int reason = 0, n_eval = 1, n_grad = 0;
       int type = mv.type;
       if constexpr ((CFG & 8) != 0) {

@@ -22,7 +22,7 @@ struct WalkCtx {
   int xi_stride;  // row stride of the normals a move reads (xi[c * xi_stride + atom])
   int S;
   double pressure;
-  double mu[JR_MAX_SPECIES];  // the replica's chemical potentials (0 past S)
+  lds_ptr<const double> mu;  // the replica's chemical potentials (LDS, JR_MAX_SPECIES)
   double emax;
 };
 

@@ -55,16 +55,25 @@ struct LdsView {
   }
 };
 
-// bytes of one wave's LDS region (arrays + JR_MAX_MOVES*JR_NCOUNT counters)
+// bytes of one wave's LDS region: arrays, JR_MAX_MOVES*JR_NCOUNT counters,
+// then the walked chain's constants (JR_MAX_SPECIES chemical potentials and
+// JR_MAX_MOVES step sizes, doubles)
 __host__ __device__ inline size_t wave_lds_bytes_a(int A) {
   size_t b = (size_t)9 * 64 * A * sizeof(double) +
              (size_t)2 * 64 * A * sizeof(int) +
-             (size_t)JR_MAX_MOVES * JR_NCOUNT * sizeof(int);
+             (size_t)JR_MAX_MOVES * JR_NCOUNT * sizeof(int) +
+             (size_t)(JR_MAX_SPECIES + JR_MAX_MOVES) * sizeof(double);
   return (b + 15) & ~(size_t)15;
 }
 
 __device__ __forceinline__ int* wave_counters(const WaveLds& L, int A) {
   return reinterpret_cast<int*>(L.base + 9 * 64 * A) + 2 * 64 * A;
+}
+
+// the walked chain's chemical potentials [JR_MAX_SPECIES], then its step
+// sizes [JR_MAX_MOVES] (8-byte aligned: the counters end on one)
+__device__ __forceinline__ double* wave_consts(const WaveLds& L, int A) {
+  return reinterpret_cast<double*>(wave_counters(L, A) + JR_MAX_MOVES * JR_NCOUNT);
 }
 
 struct Geom {
@@ -750,6 +759,19 @@ __device__ __forceinline__ void species_counts(int N, int S, const int (&t)[A],
 __device__ __forceinline__ double ensemble_term(const double* cell,
                                                 double pressure,
                                                 const double* mu, int S,
+                                                const int* counts) {
+  double e = pressure * fabs(det3(cell));
+  double acc = 0.0;
+#pragma unroll
+  for (int sp = 0; sp < JR_MAX_SPECIES; ++sp)
+    if (sp < S) acc = acc + mu[sp] * (double)counts[sp];
+  return e - acc;
+}
+
+// The same with the chemical potentials in the wave's LDS.
+__device__ __forceinline__ double ensemble_term(const double* cell,
+                                                double pressure,
+                                                lds_ptr<const double> mu, int S,
                                                 const int* counts) {
   double e = pressure * fabs(det3(cell));
   double acc = 0.0;
