@@ -34,15 +34,16 @@ def _cfgs(rng, B, Nm, S):
     return pos, ty, na, cell
 
 
-@pytest.mark.parametrize("species", [1, 2])
-def test_noalloc_matches_allocating_entry_point(species):
-    print(f"noalloc eval vs jr_eval_energy_forces, S={species}", flush=True)
+@pytest.mark.parametrize("species,chunks", [(1, 1), (2, 1), (1, 4)])
+def test_noalloc_matches_allocating_entry_point(species, chunks):
+    print(f"noalloc eval vs jr_eval_energy_forces, S={species} eval_chunks={chunks}", flush=True)
     lib = _lib.load_library()
     if species == 1:
-        lj = LJBackend(1.0, 1.0, 2.5)
+        lj = LJBackend(1.0, 1.0, 2.5, eval_chunks=chunks)
     else:
         lj = LJBackend(pair_epsilon=np.array([[1.0, 1.5], [1.5, 0.5]]),
-                       pair_sigma=np.array([[1.0, 0.8], [0.8, 0.88]]), cutoff=2.5)
+                       pair_sigma=np.array([[1.0, 0.8], [0.8, 0.88]]), cutoff=2.5,
+                       eval_chunks=chunks)
     pot, keep = lj.table.as_struct(lj._prec_code)
     rng = np.random.default_rng(40 + species)
     # the first call fills most of the scratch, the second is smaller
