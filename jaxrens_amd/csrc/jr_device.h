@@ -91,10 +91,13 @@ __device__ __forceinline__ void ring_rot1_full(double (&v)[A], int lane) {
   }
 }
 
-// Ring rotation over positions p < N: new[p] = old[(p + 1) % N].
+// Ring rotation over positions p < N: new[p] = old[(p + 1) % N].  The wave
+// rotation carries lane 63 of a slot into lane 0's value of the next slot;
+// when the ring is shorter than 64 A (N = 64 under A = 2 included) position
+// N - 1 then takes position 0's value instead.
 template <int A>
 __device__ __forceinline__ void ring_rot1(double (&v)[A], int lane, int N) {
-  const bool partial = (N & 63) != 0;
+  const bool partial = N != 64 * A;
   double w0 = 0.0;
   if (partial) w0 = readlane_d(v[0], 0);
   double r[A];

@@ -6,6 +6,8 @@ order, for the batched evaluator, NS iterations on host tapes and on device
 draws (the reference-default n_extra = 0 setting among them), trial moves,
 the burn-in walk and graph replay.  Needs an MI355X."""
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -227,3 +229,55 @@ def test_team_graph_replay_equals_stream_path():
     pa, pb = a.get_population(), b.get_population()
     for k in pa:
         assert np.array_equal(pa[k], pb[k]), k
+
+
+def test_team_eval_mixed_sizes_in_one_call(oracle):
+    """One batched call over configurations of different atom counts (one
+    team per configuration, the kernel built for the largest)."""
+    print("team eval mixed sizes", flush=True)
+    rng = np.random.default_rng(77)
+    lj = LJBackend(**BINARY, eval_chunks=4)
+    pot = oracle.Pot.from_table(lj.table)
+    sizes = [1, 7, 40, 64, 65, 100, 128]
+    Nm = max(sizes)
+    B = len(sizes)
+    pos = np.zeros((B, Nm, 3))
+    ty = np.zeros((B, Nm), np.int32)
+    cell = np.zeros((B, 3, 3))
+    for b, n in enumerate(sizes):
+        p, t, c = _random_cfgs(rng, 1, n, b % 2 == 0, n_species=2, dense=0.7)
+        pos[b, :n], ty[b, :n], cell[b] = p[0], t[0], c[0]
+    from jaxrens_amd import _lib
+    lib = _lib.load_library()
+    st, keep = lj.table.as_struct(lj._prec_code)
+    U, H, F = np.zeros(B), np.zeros(B), np.zeros((B, Nm, 3))
+    na = np.array(sizes, np.int32)
+    rc = lib.jr_eval_energy_forces(C.byref(st), 0, B, Nm, _lib.dptr(_lib.f64(pos)),
+                                   _lib.iptr(_lib.i32(ty)), _lib.iptr(na),
+                                   _lib.dptr(_lib.f64(cell)), None, None, _lib.dptr(U),
+                                   _lib.dptr(H), _lib.dptr(F))
+    assert rc == 0, lib.jr_global_error()
+    for b, n in enumerate(sizes):
+        u, h, f = oracle.energy_forces(pot, pos[b, :n], ty[b, :n], cell[b])
+        assert U[b] == u and H[b] == h, (b, n)
+        assert np.array_equal(F[b, :n], f), (b, n)
+        assert not F[b, n:].any()
+
+
+def test_team_autotune_keeps_state_and_results():
+    """The autotuner on a team handle times its one geometry and leaves the
+    population and contour alone; results equal an untuned engine's."""
+    print("team autotune", flush=True)
+    lj = LJBackend(1.0, 1.0, 2.5, eval_chunks=4)
+    a = _engine(lj, GMC_VOL, 3, 32, 64, 0, 4, cell_spec=CELL, pressures=[0.1, 1.0, 5.0], seed=8)
+    b = _engine(lj, GMC_VOL, 3, 32, 64, 0, 4, cell_spec=CELL, pressures=[0.1, 1.0, 5.0], seed=8)
+    before = a.get_population()
+    t = a.autotune(reps=1)
+    assert t["waves_per_block"] == 4
+    after = a.get_population()
+    for k in before:
+        assert np.array_equal(before[k], after[k]), k
+    oa = a.ns_iterations(3)
+    ob = b.ns_iterations(3)
+    for k in ("emax", "counters", "walk"):
+        assert np.array_equal(oa[k], ob[k]), k

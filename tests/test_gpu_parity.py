@@ -395,3 +395,62 @@ def test_graph_replay_mixed_n_classes():
     pa, pb = a.get_population(), b.get_population()
     for k in pa:
         assert np.array_equal(pa[k], pb[k]), k
+
+
+def test_eval_sizes_below_the_kernel_ring(oracle):
+    """Batched evaluation where the kernel's ring (64 atoms per lane slot,
+    sized by Nmax) is longer than a configuration's N, N a multiple of 64
+    among them: N = 64 under two atoms per lane, N = 64 and 128 under four."""
+    print("eval N below the kernel ring", flush=True)
+    rng = np.random.default_rng(91)
+    lj = LJBackend(1.0, 1.0, 2.5)
+    pot = oracle.Pot.from_table(lj.table)
+    st, keep = lj.table.as_struct(lj._prec_code)
+    lib = _lib.load_library()
+    for Nm, sizes in ((128, [64, 63, 65, 1, 100, 128]), (256, [64, 128, 192, 129, 200, 256])):
+        B = len(sizes)
+        pos = np.zeros((B, Nm, 3))
+        ty = np.zeros((B, Nm), np.int32)
+        cell = np.zeros((B, 3, 3))
+        for b, n in enumerate(sizes):
+            p, t, c = _random_cfgs(rng, 1, n, b % 2 == 1)
+            pos[b, :n], ty[b, :n], cell[b] = p[0], t[0], c[0]
+        na = np.array(sizes, np.int32)
+        U, H, F = np.zeros(B), np.zeros(B), np.full((B, Nm, 3), np.nan)
+        rc = lib.jr_eval_energy_forces(st, 0, B, Nm, _lib.dptr(pos), _lib.iptr(ty),
+                                       _lib.iptr(na), _lib.dptr(cell), None, None,
+                                       _lib.dptr(U), _lib.dptr(H), _lib.dptr(F))
+        assert rc == 0, rc
+        for b, n in enumerate(sizes):
+            u, h, f = oracle.energy_forces(pot, pos[b, :n], ty[b, :n], cell[b])
+            assert U[b] == u and H[b] == h, (Nm, n)
+            assert np.array_equal(F[b, :n], f), (Nm, n, np.abs(F[b, :n] - f).max())
+
+
+def test_trial_moves_mixed_sizes(oracle):
+    """Trial moves of a mixed-size population run every chain in the kernel
+    built for the largest N (here two atoms per lane), N = 64 and 32 among
+    them."""
+    print("trial moves, mixed sizes", flush=True)
+    from oracle.oracle import Population
+    lj = LJBackend(**BINARY)
+    R, K = 4, 12
+    sizes = [32, 64, 64, 128]
+    run = RunConfig(n_replicas=R, n_walkers=K, n_atoms=sizes, n_mcmc_steps=4, n_extra=4,
+                    seed=31, pressures=[1.0] * R, re_interval=0)
+    eng = NSEngine(lj, GMC_VOL, run, cell=CELL)
+    eng.init_walkers(np.random.default_rng(31), 0.5, 8.0, True, species_p=[0.5, 0.5])
+    eng.ns_iterations(1)
+    rng = np.random.default_rng(32)
+    idx = rng.integers(0, K, size=(R, 16)).astype(np.int32)
+    nrm = rng.standard_normal((R, 16, 3, 128))
+    uni = rng.random((R, 16, 4))
+    emax = eng.emax()
+    pop = Population.from_engine_dict(eng.get_population())
+    pot = oracle.Pot.from_table(lj.table)
+    for m, ss in [(0, 0.05), (1, 0.3)]:
+        acc, rs = eng.trial_moves(m, [ss] * R, idx, normals=nrm, uniforms=uni)
+        oacc, ors = oracle.trial_moves(pop, pot, eng.pressure, eng.mu, eng.step_sizes,
+                                       eng.mwg.as_oracle_moves(), m, [ss] * R, emax,
+                                       idx, nrm, uni)
+        assert np.array_equal(acc, oacc) and np.array_equal(rs, ors), m
