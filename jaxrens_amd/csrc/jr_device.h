@@ -75,6 +75,16 @@ __device__ __forceinline__ double wave_rol1(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// One atom per lane, N < 64: new[p] = old[(p + 1) mod N] through the LDS
+// crossbar; addr = 4 ((p + 1) mod N) for the active lanes (any lane for the
+// idle ones, which then never feed an active position).  Every lane of the
+// wave must be active.
+__device__ __forceinline__ double ring_perm(double v, int addr) {
+  int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
+  int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
 // Ring rotation when N == 64*A (every ring position occupied): DPP rotate
 // each slot, lane 63 takes lane 0 of the next slot.
 template <int A>

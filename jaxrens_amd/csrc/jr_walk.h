@@ -277,11 +277,17 @@ __device__ __forceinline__ void wave_eval_impl(
     tx[s] = ty[s] = tz[s] = 0.0;
   }
   const double num = K.num;
+  // successor (p + 1) mod N as a crossbar byte address (A = 1, N < 64)
+  const int rot_addr = (lane + 1 == N ? 0 : (lane + 1) & 63) << 2;
   auto rotate = [&]() {
     if (FULL) {
       ring_rot1_full<A>(tx, lane);
       ring_rot1_full<A>(ty, lane);
       ring_rot1_full<A>(tz, lane);
+    } else if constexpr (A == 1) {
+      tx[0] = ring_perm(tx[0], rot_addr);
+      ty[0] = ring_perm(ty[0], rot_addr);
+      tz[0] = ring_perm(tz[0], rot_addr);
     } else {
       ring_rot1<A>(tx, lane, N);
       ring_rot1<A>(ty, lane, N);
