@@ -85,6 +85,21 @@ __device__ __forceinline__ double ring_perm(double v, int addr) {
   return __hiloint2double(hi, lo);
 }
 
+// Ring rotation with slot s of a lane at ring position A lane + s:
+// new[s] = old[s + 1] (registers only) and slot A - 1 takes slot 0 of the
+// next lane.  FULL_ (N == 64 A): one DPP wave_rol, whose lane 63 takes
+// lane 0.  Otherwise slot 0 moves through the crossbar at byte address addr
+// (lane + 1, or lane 0 into the lane holding position N - 1), and the slot
+// below A - 1 holding position N - 1 (wrap[s]) takes that value too.
+template <int A, bool FULL_>
+__device__ __forceinline__ void ring_rot1_ilv(double (&v)[A], int addr,
+                                              const bool (&wrap)[A]) {
+  const double b = FULL_ ? wave_rol1(v[0]) : ring_perm(v[0], addr);
+#pragma unroll
+  for (int s = 0; s + 1 < A; ++s) v[s] = (!FULL_ && wrap[s]) ? b : v[s + 1];
+  v[A - 1] = b;
+}
+
 // Ring rotation when N == 64*A (every ring position occupied): DPP rotate
 // each slot, lane 63 takes lane 0 of the next slot.
 template <int A>

@@ -256,6 +256,13 @@ __device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int j,
 // with num = sigma^2 (one species) or 1.
 // FULL: N == 64*A, so no ring position is idle and the rotation is one DPP
 // wave_rol per 32-bit half.
+// Slot layout of the steps: with several atoms per lane (A > 1, no image
+// sums) slot s of a lane walks ring position p = A lane + s, so the
+// rotation moves slot s + 1 into slot s (a register renaming) and only slot 0
+// crosses lanes, into slot A - 1 of the lane below; the reactions then rotate
+// after every step.  Positions are read from the LDS copy, and the energy
+// partials and forces go back to the p = lane + 64 s layout through LDS.
+// Sums per ring position are the same in either layout.
 template <int A, bool FORCES, int CFG, int GEO, bool FULL>
 __device__ __forceinline__ void wave_eval_impl(
     const WaveLds& WL, int chunk, const LdsView<A>& L, int N, const Geom& g, const PairTab& tab,
@@ -268,7 +275,28 @@ __device__ __forceinline__ void wave_eval_impl(
   constexpr bool TEAM = (CFG & 32) != 0;
   static_assert(!(IMG && F32), "image sums are FP64 only");
   static_assert(!(TEAM && (IMG || F32)), "evaluation teams are FP64 minimum image only");
+  constexpr bool ILV = A > 1 && !IMG;
+  constexpr int LA = A == 1 ? 0 : A == 2 ? 1 : 2;  // log2 A
   const int lane = lane_id();
+  auto pos_of = [&](int s) { return ILV ? A * lane + s : lane + 64 * s; };
+  // own positions (and types) of the slots
+  double xo[A], yo[A], zo[A];
+  int to[A];
+#pragma unroll
+  for (int s = 0; s < A; ++s) {
+    if constexpr (ILV) {
+      const int a = pos_of(s);
+      xo[s] = L.x()[a];
+      yo[s] = L.y()[a];
+      zo[s] = L.z()[a];
+      to[s] = MULTI ? L.t()[a] : 0;
+    } else {
+      xo[s] = x[s];
+      yo[s] = y[s];
+      zo[s] = z[s];
+      to[s] = t[s];
+    }
+  }
   double e[A], ox[A], oy[A], oz[A], tx[A], ty[A], tz[A];
 #pragma unroll
   for (int s = 0; s < A; ++s) {
@@ -277,10 +305,26 @@ __device__ __forceinline__ void wave_eval_impl(
     tx[s] = ty[s] = tz[s] = 0.0;
   }
   const double num = K.num;
-  // successor (p + 1) mod N as a crossbar byte address (A = 1, N < 64)
-  const int rot_addr = (lane + 1 == N ? 0 : (lane + 1) & 63) << 2;
+  // Partial rings: ring position N - 1 (lane lw, slot sw of the layout)
+  // takes position 0.  Slot 0 moves through the crossbar from lane + 1, or
+  // from lane 0 into lane lw (A = 1, or interleaved slots).
+  const int lw = (N - 1) >> (ILV ? LA : 0), sw = (N - 1) & (ILV ? A - 1 : 0);
+  const int rot_addr = (lane == lw ? 0 : (lane + 1) & 63) << 2;
+  bool wrap[A];
+#pragma unroll
+  for (int s = 0; s < A; ++s) wrap[s] = (s == sw) && (lane == lw);
   auto rotate = [&]() {
-    if (FULL) {
+    if constexpr (ILV) {
+      if (FULL) {
+        ring_rot1_ilv<A, true>(tx, rot_addr, wrap);
+        ring_rot1_ilv<A, true>(ty, rot_addr, wrap);
+        ring_rot1_ilv<A, true>(tz, rot_addr, wrap);
+      } else {
+        ring_rot1_ilv<A, false>(tx, rot_addr, wrap);
+        ring_rot1_ilv<A, false>(ty, rot_addr, wrap);
+        ring_rot1_ilv<A, false>(tz, rot_addr, wrap);
+      }
+    } else if (FULL) {
       ring_rot1_full<A>(tx, lane);
       ring_rot1_full<A>(ty, lane);
       ring_rot1_full<A>(tz, lane);
@@ -368,10 +412,10 @@ __device__ __forceinline__ void wave_eval_impl(
       if (FORCES && k > 1) rotate();
 #pragma unroll
       for (int s = 0; s < A; ++s) {
-        const int a = lane + 64 * s;
+        const int a = pos_of(s);
         const bool active = FULL || a < N;
-        image_terms(s, load_pair<A, CFG, GEO>(L, a + k, active, g, tab, K, x[s],
-                                             y[s], z[s], t[s]),
+        image_terms(s, load_pair<A, CFG, GEO>(L, a + k, active, g, tab, K, xo[s],
+                                             yo[s], zo[s], to[s]),
                     active, true, 1.0);
       }
     }
@@ -379,10 +423,10 @@ __device__ __forceinline__ void wave_eval_impl(
       const int hN = N >> 1;
 #pragma unroll
       for (int s = 0; s < A; ++s) {
-        const int a = lane + 64 * s;
+        const int a = pos_of(s);
         const bool active = FULL || a < N;
         image_terms(s, load_pair<A, CFG, GEO>(L, a + hN, active, g, tab, K,
-                                             x[s], y[s], z[s], t[s]),
+                                             xo[s], yo[s], zo[s], to[s]),
                     active, false, 0.5);
       }
     }
@@ -412,12 +456,12 @@ __device__ __forceinline__ void wave_eval_impl(
     QT q[4][A];
 #pragma unroll
     for (int s = 0; s < A; ++s) {
-      const int a = lane + 64 * s;
+      const int a = pos_of(s);
       const bool active = FULL || a < N;
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        P[i][s] = load_pair<A, CFG, GEO>(L, a + k + i, active, g, tab, K, x[s],
-                                          y[s], z[s], t[s]);
+        P[i][s] = load_pair<A, CFG, GEO>(L, a + k + i, active, g, tab, K, xo[s],
+                                          yo[s], zo[s], to[s]);
       if constexpr (F32) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) q[i][s] = q_single(P[i][s]);
@@ -434,9 +478,10 @@ __device__ __forceinline__ void wave_eval_impl(
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      if (FORCES && k + i > kfirst) rotate();
+      if (FORCES && !ILV && k + i > kfirst) rotate();
 #pragma unroll
       for (int s = 0; s < A; ++s) accumulate(s, P[i][s], q[i][s], true, 1.0);
+      if (FORCES && ILV) rotate();
     }
   }
   if (!IMG && last && k + 1 <= kmax) {
@@ -444,12 +489,12 @@ __device__ __forceinline__ void wave_eval_impl(
     QT q0[A], q1[A];
 #pragma unroll
     for (int s = 0; s < A; ++s) {
-      const int a = lane + 64 * s;
+      const int a = pos_of(s);
       const bool active = FULL || a < N;
-      P0[s] = load_pair<A, CFG, GEO>(L, a + k, active, g, tab, K, x[s], y[s],
-                                      z[s], t[s]);
-      P1[s] = load_pair<A, CFG, GEO>(L, a + k + 1, active, g, tab, K, x[s],
-                                      y[s], z[s], t[s]);
+      P0[s] = load_pair<A, CFG, GEO>(L, a + k, active, g, tab, K, xo[s], yo[s],
+                                      zo[s], to[s]);
+      P1[s] = load_pair<A, CFG, GEO>(L, a + k + 1, active, g, tab, K, xo[s],
+                                      yo[s], zo[s], to[s]);
       if constexpr (F32) {
         q0[s] = q_single(P0[s]);
         q1[s] = q_single(P1[s]);
@@ -459,46 +504,49 @@ __device__ __forceinline__ void wave_eval_impl(
         q1[s] = mask_q(P1[s].ok, P0[s].r2 * d);
       }
     }
-    if (FORCES && k > kfirst) rotate();
+    if (FORCES && !ILV && k > kfirst) rotate();
 #pragma unroll
     for (int s = 0; s < A; ++s) accumulate(s, P0[s], q0[s], true, 1.0);
     if (FORCES) rotate();
 #pragma unroll
     for (int s = 0; s < A; ++s) accumulate(s, P1[s], q1[s], true, 1.0);
+    if (FORCES && ILV) rotate();
     k += 2;
   }
   if (!IMG && last && k <= kmax) {  // lone last step
-    if (FORCES && k > kfirst) rotate();
+    if (FORCES && !ILV && k > kfirst) rotate();
 #pragma unroll
     for (int s = 0; s < A; ++s) {
-      const int a = lane + 64 * s;
+      const int a = pos_of(s);
       const bool active = FULL || a < N;
-      PairIn P = load_pair<A, CFG, GEO>(L, a + k, active, g, tab, K, x[s], y[s],
-                                         z[s], t[s]);
+      PairIn P = load_pair<A, CFG, GEO>(L, a + k, active, g, tab, K, xo[s], yo[s],
+                                         zo[s], to[s]);
       accumulate(s, P, q_single(P), true, 1.0);
     }
+    if (FORCES && ILV) rotate();
   }
   if (!IMG && last && N >= 2 && !(N & 1)) {
     const int h = N >> 1;
 #pragma unroll
     for (int s = 0; s < A; ++s) {
-      const int a = lane + 64 * s;
+      const int a = pos_of(s);
       const bool active = FULL || a < N;
-      PairIn P = load_pair<A, CFG, GEO>(L, a + h, active, g, tab, K, x[s], y[s],
-                                         z[s], t[s]);
+      PairIn P = load_pair<A, CFG, GEO>(L, a + h, active, g, tab, K, xo[s], yo[s],
+                                         zo[s], to[s]);
       accumulate(s, P, q_single(P), false, 0.5);
     }
   }
   if constexpr (TEAM) {
     // the chunk's sums into its exchange rows (the travelling reactions at
-    // their atoms: position p holds atom (p + klast) mod N); wave_eval adds
-    // the chunks once the whole team has written them
+    // their atoms: position p holds atom (p + klast) mod N, p + klast + 1
+    // when they rotate after every step); wave_eval adds the chunks once the
+    // whole team has written them
     constexpr int M = 64 * A;
-    const int klast = last ? kmax : kend4 - 1;
+    const int klast = (last ? kmax : kend4 - 1) + (ILV ? 1 : 0);
     const lds_ptr<double> mine = to_lds(WL.xbuf + (size_t)chunk * 7 * M);
 #pragma unroll
     for (int s = 0; s < A; ++s) {
-      const int a = lane + 64 * s;
+      const int a = pos_of(s);
       if (FULL || a < N) {
         mine[a] = e[s];
         if (FORCES) {
@@ -515,19 +563,79 @@ __device__ __forceinline__ void wave_eval_impl(
     }
     return;
   }
-  double el = e[0];
+  const lds_ptr<double> HX = L.hx(), HY = L.hy(), HZ = L.hz();
+  if constexpr (ILV) {
+    // energy partials of the positions to the p = lane + 64 s layout (an
+    // idle position's partial is +0)
+    wave_lds_sync();
 #pragma unroll
-  for (int s = 1; s < A; ++s) {
-    const int a = lane + 64 * s;
-    if (FULL || a < N) el = el + e[s];
+    for (int s = 0; s < A; ++s) HX[pos_of(s)] = e[s];
+    wave_lds_sync();
+    double el = HX[lane];
+#pragma unroll
+    for (int s = 1; s < A; ++s) {
+      const int a = lane + 64 * s;
+      if (FULL || a < N) el = el + HX[a];
+    }
+    e_lane = el;
+    wave_lds_sync();
+  } else {
+    double el = e[0];
+#pragma unroll
+    for (int s = 1; s < A; ++s) {
+      const int a = lane + 64 * s;
+      if (FULL || a < N) el = el + e[s];
+    }
+    e_lane = el;
   }
-  e_lane = el;
-  if (FORCES && !TEAM) {
-    // position p holds the reaction of atom (p + kmax) mod N
-    const lds_ptr<double> HX = L.hx(), HY = L.hy(), HZ = L.hz();
+  if (FORCES && ILV) {
+    // position p holds the reaction of atom (p + kmax + 1) mod N; the sums
+    // own + reaction go back through the same rows to p = lane + 64 s
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      const int a = pos_of(s);
+      if (FULL || a < N) {
+        int dst = a + kmax + 1;
+        if (dst >= N) dst -= N;
+        HX[dst] = tx[s];
+        HY[dst] = ty[s];
+        HZ[dst] = tz[s];
+      }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      const int a = pos_of(s);
+      if (FULL || a < N) {
+        const double sx = ox[s] + HX[a], sy = oy[s] + HY[a], sz = oz[s] + HZ[a];
+        HX[a] = sx;
+        HY[a] = sy;
+        HZ[a] = sz;
+      }
+    }
+    wave_lds_sync();
 #pragma unroll
     for (int s = 0; s < A; ++s) {
       const int a = lane + 64 * s;
+      if (FULL || a < N) {
+        fx[s] = HX[a];
+        fy[s] = HY[a];
+        fz[s] = HZ[a];
+        if (GEO == GEO_CUBIC || !MULTI) {
+          fx[s] = K.cf * fx[s];
+          fy[s] = K.cf * fy[s];
+          fz[s] = K.cf * fz[s];
+        }
+      } else {
+        fx[s] = fy[s] = fz[s] = 0.0;
+      }
+    }
+  }
+  if (FORCES && !ILV) {
+    // position p holds the reaction of atom (p + kmax) mod N
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+      const int a = pos_of(s);
       if (FULL || a < N) {
         int dst = a + kmax;
         if (dst >= N) dst -= N;
@@ -539,7 +647,7 @@ __device__ __forceinline__ void wave_eval_impl(
     wave_lds_sync();
 #pragma unroll
     for (int s = 0; s < A; ++s) {
-      const int a = lane + 64 * s;
+      const int a = pos_of(s);
       if (FULL || a < N) {
         fx[s] = ox[s] + HX[a];
         fy[s] = oy[s] + HY[a];
