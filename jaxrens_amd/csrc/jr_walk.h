@@ -212,13 +212,14 @@ struct EvalConst {
   double num, rc2, cf;
 };
 
-// Pair of the atom in slot s (ring position a) with LDS entry j = a + k.
+// Pair of the atom in slot s (ring position a) with LDS entry j = a + k,
+// whose species tj (species table kernels) the caller has read.
 template <int A, int CFG, int GEO>
-__device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int j,
-                                            bool active, const Geom& g,
-                                            const PairTab& tab,
-                                            const EvalConst& K, double xs,
-                                            double ys, double zs, int ts) {
+__device__ __forceinline__ PairIn load_pair_t(const LdsView<A>& L, int j,
+                                              bool active, const Geom& g,
+                                              const PairTab& tab,
+                                              const EvalConst& K, double xs,
+                                              double ys, double zs, int ts, int tj) {
   constexpr bool MULTI = (CFG & 1) != 0;
   PairIn P;
   pair_dr<GEO>(xs, ys, zs, L.x()[j], L.y()[j], L.z()[j], g, P.dx, P.dy, P.dz,
@@ -230,12 +231,23 @@ __device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int j,
   P.eps24 = e24[0];
   P.sig2 = s2[0];
   if (MULTI) {
-    const int ab = ts * tab.S + L.t()[j];
+    const int ab = ts * tab.S + tj;
     P.eps4 = e4[ab];
     P.eps24 = e24[ab];
     P.sig2 = s2[ab];
   }
   return P;
+}
+
+template <int A, int CFG, int GEO>
+__device__ __forceinline__ PairIn load_pair(const LdsView<A>& L, int j,
+                                            bool active, const Geom& g,
+                                            const PairTab& tab,
+                                            const EvalConst& K, double xs,
+                                            double ys, double zs, int ts) {
+  constexpr bool MULTI = (CFG & 1) != 0;
+  return load_pair_t<A, CFG, GEO>(L, j, active, g, tab, K, xs, ys, zs, ts,
+                                  MULTI ? L.t()[j] : 0);
 }
 
 // Energy partial (per lane) and forces of the configuration whose positions
@@ -448,6 +460,16 @@ __device__ __forceinline__ void wave_eval_impl(
     }
     k = kmax + 1;
   }
+  // Species table: the partner species of a group are read one group ahead
+  // (entries up to 64 A + kmax + 3 < 2 M), so that the table reads of a
+  // group do not wait on an LDS round trip of their own.
+  int tn[4][A];
+  if constexpr (MULTI && !IMG) {
+#pragma unroll
+    for (int s = 0; s < A; ++s)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) tn[i][s] = L.t()[pos_of(s) + k + i];
+  }
   // kept rolled: with N known (N == 64*A) a full unroll multiplies the
   // register pressure and the code size of every call site
 #pragma unroll 1
@@ -458,10 +480,14 @@ __device__ __forceinline__ void wave_eval_impl(
     for (int s = 0; s < A; ++s) {
       const int a = pos_of(s);
       const bool active = FULL || a < N;
+      // interleaved slots read their own positions again per group rather
+      // than hold them through the loop (registers for the species table)
+      const double xs = ILV ? L.x()[a] : xo[s], ys = ILV ? L.y()[a] : yo[s],
+                   zs = ILV ? L.z()[a] : zo[s];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        P[i][s] = load_pair<A, CFG, GEO>(L, a + k + i, active, g, tab, K, xo[s],
-                                          yo[s], zo[s], to[s]);
+        P[i][s] = load_pair_t<A, CFG, GEO>(L, a + k + i, active, g, tab, K, xs, ys,
+                                            zs, to[s], MULTI ? tn[i][s] : 0);
       if constexpr (F32) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) q[i][s] = q_single(P[i][s]);
@@ -475,6 +501,12 @@ __device__ __forceinline__ void wave_eval_impl(
         q[2][s] = mask_q(P[2][s].ok, P[3][s].r2 * d23);
         q[3][s] = mask_q(P[3][s].ok, P[2][s].r2 * d23);
       }
+    }
+    if constexpr (MULTI) {
+#pragma unroll
+      for (int s = 0; s < A; ++s)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) tn[i][s] = L.t()[pos_of(s) + k + 4 + i];
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
