@@ -194,6 +194,30 @@ def test_ns_iterations_semigrand_mixed_n(oracle, occ, cap):
 
 
 @pytest.mark.parametrize("cap", [0, 1])
+def test_ns_iterations_odd_and_four_per_lane(oracle, cap):
+    """Interleaved slots on odd rings (N = 97 under two atoms per lane, the
+    wrap in slot 0) and under four atoms per lane (N = 150 partial, N = 256
+    full), species table with mu, gmc + volume + morph."""
+    print(f"ns N = 97, 150, 256 cap={cap}", flush=True)
+    lj = LJBackend(**BINARY)
+    moves = [{"type": "gmc", "n_reflect": 3, "step_size": 0.05, "weight": 3},
+             {"type": "volume", "step_size": 0.2, "weight": 1},
+             {"type": "alchemical_morph", "n_species": 2, "weight": 1}]
+    R = 4
+    n_atoms = [97, 97, 150, 256]
+    mu = np.stack([np.zeros(R), np.linspace(-1, 1, R)], axis=1)
+    run = RunConfig(n_replicas=R, n_walkers=4, n_atoms=n_atoms, n_mcmc_steps=4,
+                    n_extra=2, seed=13, pressures=[1.0] * R, chemical_potentials=mu,
+                    re_interval=1)
+    eng = NSEngine(lj, moves, run, cell=CELL)
+    eng.init_walkers(np.random.default_rng(13), 0.8, 4.0, True, species_p=[0.5, 0.5])
+    eng.set_walk_block_cap(cap)
+    tape = make_tape(np.random.default_rng(14), 3, R, 4, 256, 4, 2, eng.mwg.move_probs)
+    pv = np.array([n_atoms[i] == n_atoms[i + 1] for i in range(R - 1)], np.int32)
+    _run_parity(oracle, eng, 3, tape, pair_valid=pv)
+
+
+@pytest.mark.parametrize("cap", [0, 1])
 def test_device_draws_replay(oracle, cap):
     """Device Philox draws (normals and uniforms drawn in place by the walk),
     read back and replayed on the oracle."""
