@@ -444,8 +444,9 @@ void enqueue_draws(jr_handle* h, long long it) {
 // results do not depend on A, so the split changes no number.  The classes
 // after the first run on stream2, forked from `s` and joined back into it,
 // so they run concurrently with the first: a class whose kernel leaves
-// registers free on a SIMD (A = 2 holds 326 of 512) shares it with waves of
-// the next class, and the classes share no data.
+// registers free on a SIMD (the A = 2 species-table kernel holds 344 of 512,
+// exactly enough for one 168-register A = 1 wave beside it) shares it with
+// waves of the next class, and the classes share no data.
 hipError_t enqueue_walk(jr_handle* h, const WalkArgs& base, hipStream_t s) {
   if (h->n_classes == 0) {
     launch_walk(h->A, h->cfgbits, h->occ, base, h->blocks, h->wpb, h->lds_walk, s);
