@@ -335,7 +335,10 @@ int jr_replica_exchange(jr_handle* h, const double* re_u, int32_t* acc,
 /* One trial move per sample (population untouched).  sample_idx [R][n],
  * trial_step [R], emax [R] (NULL = resident emax), normals
  * [R][n][3][Nmax], uniforms [R][n][4] (both NULL = device draws).
- * accepted/reason [R][n]. */
+ * accepted/reason [R][n].  On a learned-model handle the move runs as the
+ * learned walk would run it (judge-steered, or surrogate-steered and
+ * screened when the walk screens), without counters, statistics or
+ * training records. */
 int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
                    const double* emax, int32_t n_samples,
                    const int32_t* sample_idx, const double* normals,

@@ -136,6 +136,14 @@ int learned_check_population(jr_handle* h, const int32_t* types, const int32_t* 
 int learned_recompute_energies(jr_handle* h);
 // The walk of one iteration (after select, before RE), on h->stream.
 int learned_walk(jr_handle* h);
+// Trial moves for step-size adaptation: move m from each of the n sampled
+// walkers per replica (sample_idx [R][n], host) at the trial step sizes
+// tstep [R] under the contours temax [R] with the device draws tnorm
+// [R][n][3][Nmax] and tunif [R][n][4]; reject reasons [R][n] (0 = accepted)
+// to the host; the population is not changed.
+int learned_trial_moves(jr_handle* h, int m, int n, const int32_t* sample_idx,
+                        const double* tstep, const double* temax, const double* tnorm,
+                        const double* tunif, int32_t* reason);
 // Pair-potential judge of the learned walk (jr_lib.hip): model energies U and
 // forces F (NULL: energy only) of the configurations listed in idx (NULL:
 // all B) with the device count (NULL: B); outputs indexed by configuration.

@@ -448,6 +448,11 @@ struct LearnedState {
   DBuf<int> ringN;
   DBuf<RingState> rs;
   DBuf<int> tidx, tn;
+  // trial moves: sampled walkers, their draws and the step sizes with the
+  // trial move's column replaced (allocated on the first trial)
+  DBuf<int> trial_idx;
+  DBuf<double> trial_norm, trial_unif, trial_steps;
+  std::vector<int> trial_host;
   DBuf<__bf16> tX;
   DBuf<double> tE, tsq;
   DBuf<float> te, tc, gpart, grad;
@@ -518,16 +523,16 @@ hipError_t judge_eval(jr_handle* h, LearnedState* L, const double* pos, const do
   return run_judge(L, judge_args(L, pos, cell, B, types, n, idx, count, U, F), s);
 }
 
-// The surrogate on chain arrays (items idx/count, or all C): energies E and
-// forces F (NULL: energy only); the descriptor rows land in xstage and the
-// baseline energies in Ub for the training ring.
-hipError_t run_sur(LearnedState* L, const double* pos, const double* cell, const int* idx,
-                   const int* count, double* E, double* F, hipStream_t s) {
+// The surrogate on the first B chains of the chain arrays (items idx/count,
+// or all B): energies E and forces F (NULL: energy only); the descriptor rows
+// land in xstage and the baseline energies in Ub for the training ring.
+hipError_t run_sur(LearnedState* L, int B, const double* pos, const double* cell,
+                   const int* idx, const int* count, double* E, double* F, hipStream_t s) {
   SurCfgArgs c;
   std::memset(&c, 0, sizeof(c));
   const double base[3] = {L->cfg.base_eps, L->cfg.base_sig, L->cfg.base_rc};
   (void)set_baseline(c, base);  // validated at creation
-  c.B = L->C;
+  c.B = B;
   c.Nmax = L->Nmax;
   c.pos = pos;
   c.types = L->ctypes.p;
@@ -747,6 +752,13 @@ struct WalkCtx {
   int C, Nm, n_steps;
   bool use_sur;
   double cap;
+  // draws [C][n_steps][3][Nm] and [C][n_steps][4]; step sizes [R][n_moves]
+  const double* normals;
+  const double* uniforms;
+  const double* step_sizes;
+  // trial moves: no counters, statistics or training records, and no
+  // surrogate prediction that only feeds the held-out record
+  bool trial = false;
 
   // H of the judge's energies U (all chains, or the listed ones)
   void ham_exact(const int* idx, const int* cnt, const double* U, const double* cell,
@@ -765,6 +777,7 @@ struct WalkCtx {
 // Held-out error and ring append over the chains listed in (idx2, cnt[1]);
 // cell holds the evaluated configurations' cells.
 void record_exact(WalkCtx& w, const int* code, const double* cell) {
+  if (w.trial) return;
   LearnedState* L = w.L;
   hipStream_t s = w.s;
   hipLaunchKernelGGL(lw_compact_kernel, dim3(1), dim3(1024), 0, s, w.C, code, 1, 1, L->U.p,
@@ -779,6 +792,7 @@ void record_exact(WalkCtx& w, const int* code, const double* cell) {
 
 void count_step(WalkCtx& w, int m, const int* code, int eval_const, int eval_code1,
                 int grad_const, int screened) {
+  if (w.trial) return;
   LWCountArgs a;
   a.C = w.C;
   a.n_walk = w.h->n_walk;
@@ -797,6 +811,7 @@ void count_step(WalkCtx& w, int m, const int* code, int eval_const, int eval_cod
 
 void stat_add(WalkCtx& w, unsigned long long* dst, const int* count, long long n,
               long long mult) {
+  if (w.trial) return;
   hipLaunchKernelGGL(lw_stat_add_kernel, dim3(1), dim3(64), 0, w.s, dst, count, n, mult);
 }
 
@@ -807,7 +822,7 @@ hipError_t gmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
   const int C = w.C, Nm = w.Nm;
   const size_t xs = (size_t)w.n_steps * 3 * Nm;
   hipLaunchKernelGGL(lw_gmc_begin_kernel, dim3(blocks_for(C, 4)), dim3(256), 0, s, w.ch,
-                     L->cdir.p, h->normals.p + (size_t)t * 3 * Nm, xs, mv.perturb_angle,
+                     L->cdir.p, w.normals + (size_t)t * 3 * Nm, xs, mv.perturb_angle,
                      L->cpos.p, L->x0.p, L->d0.p);
   const unsigned gc = blocks_for((size_t)C * Nm * 3);
   const int nr = mv.n_reflect;
@@ -815,9 +830,9 @@ hipError_t gmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
   for (int k = 0; k < nr; ++k) {
     const bool last = k == nr - 1;
     hipLaunchKernelGGL(lw_advance_kernel, dim3(gc), dim3(256), 0, s, w.ch, L->cpos.p,
-                       L->cdir.p, h->step_sizes.p, m);
+                       L->cdir.p, w.step_sizes, m);
     if (da) {
-      e = run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, L->F.p, s);
+      e = run_sur(L, C, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, L->F.p, s);
       if (e != hipSuccess) return e;
       w.ham_sur(nullptr, nullptr, L->Us.p, L->ccell.p, L->Hsy.p);
       if (!last)
@@ -853,9 +868,9 @@ hipError_t gmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
   } else {
     hipLaunchKernelGGL(lw_accept_kernel, dim3(gb), dim3(256), 0, s, C, nullptr, L->Hy.p,
                        L->cem.p, L->accept.p, L->reason.p, L->cH.p, nullptr, nullptr);
-    if (w.use_sur) {
+    if (w.use_sur && !w.trial) {
       // the surrogate's prediction at the judged end point: held-out error
-      e = run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s);
+      e = run_sur(L, C, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s);
       if (e != hipSuccess) return e;
       record_exact(w, nullptr, L->ccell.p);
       stat_add(w, &L->dstats.p->sur_e, nullptr, C, 1);
@@ -877,11 +892,11 @@ hipError_t volume_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
   va.ch = w.ch;
   va.pos = L->cpos.p;
   va.cell = L->ccell.p;
-  va.step_sizes = h->step_sizes.p;
+  va.step_sizes = w.step_sizes;
   va.m = m;
-  va.xi = h->normals.p + (size_t)t * 3 * Nm;
+  va.xi = w.normals + (size_t)t * 3 * Nm;
   va.xi_stride = (size_t)w.n_steps * 3 * Nm;
-  va.u = h->uniforms.p + (size_t)t * 4;
+  va.u = w.uniforms + (size_t)t * 4;
   va.u_stride = (size_t)w.n_steps * 4;
   va.npos = L->npos.p;
   va.ncell = L->ncell.p;
@@ -896,8 +911,8 @@ hipError_t volume_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
                      nullptr, 0.0, L->idx.p, L->cnt.p);
   hipError_t e;
   const int* fin = L->code.p;  // codes of the final test
-  if (da || w.use_sur) {
-    e = run_sur(L, L->npos.p, L->ncell.p, L->idx.p, L->cnt.p, L->Us.p, nullptr, s);
+  if (da || (w.use_sur && !w.trial)) {
+    e = run_sur(L, C, L->npos.p, L->ncell.p, L->idx.p, L->cnt.p, L->Us.p, nullptr, s);
     if (e != hipSuccess) return e;
     stat_add(w, &L->dstats.p->sur_e, L->cnt.p, 0, 1);
   }
@@ -937,24 +952,24 @@ hipError_t hmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
   const size_t xs = (size_t)w.n_steps * 3 * Nm;
   const unsigned gw = blocks_for(C, 4), gc = blocks_for((size_t)C * Nm * 3);
   hipLaunchKernelGGL(lw_hmc_begin_kernel, dim3(gw), dim3(256), 0, s, w.ch,
-                     h->normals.p + (size_t)t * 3 * Nm, xs, L->cpos.p, L->x0.p, L->mom.p,
+                     w.normals + (size_t)t * 3 * Nm, xs, L->cpos.p, L->x0.p, L->mom.p,
                      L->K0.p);
   const int nl = mv.n_leapfrog;
   hipError_t e;
   auto eval = [&]() -> hipError_t {
     if (da)
-      return run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, L->F.p, s);
+      return run_sur(L, C, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, L->F.p, s);
     return judge_eval(h, L, L->cpos.p, L->ccell.p, L->ctypes.p, L->cn.p, nullptr, nullptr, C,
                       L->U.p, L->F.p, s);
   };
   if ((e = eval()) != hipSuccess) return e;
   for (int l = 0; l < nl; ++l) {
     hipLaunchKernelGGL(lw_hmc_kick_drift_kernel, dim3(gc), dim3(256), 0, s, w.ch, L->mom.p,
-                       L->cpos.p, L->F.p, h->step_sizes.p, m, l == 0 ? 1 : 2, 1);
+                       L->cpos.p, L->F.p, w.step_sizes, m, l == 0 ? 1 : 2, 1);
     if ((e = eval()) != hipSuccess) return e;
   }
   hipLaunchKernelGGL(lw_hmc_kick_drift_kernel, dim3(gc), dim3(256), 0, s, w.ch, L->mom.p,
-                     L->cpos.p, L->F.p, h->step_sizes.p, m, 1, 0);
+                     L->cpos.p, L->F.p, w.step_sizes, m, 1, 0);
   const unsigned gb = blocks_for(C);
   if (da) {
     w.ham_sur(nullptr, nullptr, L->Us.p, L->ccell.p, L->Hsy.p);
@@ -968,7 +983,7 @@ hipError_t hmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
     w.ham_exact(L->idx.p, L->cnt.p, L->U.p, L->ccell.p, L->Hy.p);
     hipLaunchKernelGGL(lw_hmc_finish_kernel, dim3(gw), dim3(256), 0, s, w.ch, L->code.p,
                        L->Hy.p, L->cH.p, L->K0.p, L->mom.p, L->cem.p, L->accept.p, L->reason.p,
-                       L->cpos.p, L->x0.p, L->cHs.p, L->Hsy.p, h->uniforms.p + (size_t)t * 4,
+                       L->cpos.p, L->x0.p, L->cHs.p, L->Hsy.p, w.uniforms + (size_t)t * 4,
                        (size_t)w.n_steps * 4);
     record_exact(w, L->code.p, L->ccell.p);
     count_step(w, m, L->code.p, 0, 1, 0, 1);
@@ -976,9 +991,9 @@ hipError_t hmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
     stat_add(w, &L->dstats.p->sur_e, nullptr, C, nl + 1);
   } else {
     w.ham_exact(nullptr, nullptr, L->U.p, L->ccell.p, L->Hy.p);
-    if (w.use_sur) {
+    if (w.use_sur && !w.trial) {
       // the surrogate's prediction at the judged end point: held-out error
-      e = run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s);
+      e = run_sur(L, C, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s);
       if (e != hipSuccess) return e;
       record_exact(w, nullptr, L->ccell.p);
       stat_add(w, &L->dstats.p->sur_e, nullptr, C, 1);
@@ -1085,6 +1100,9 @@ int learned_walk(jr_handle* h) {
   w.use_sur = L->mode != JR_LEARNED_PLAIN;
   w.cap = L->cfg.train_max_epa;
   w.ch = LWChains{L->C, h->n_walk, L->Nmax, c.n_moves, L->cn.p};
+  w.normals = h->normals.p;
+  w.uniforms = h->uniforms.p;
+  w.step_sizes = h->step_sizes.p;
   for (double mu : h->h_mu)
     if (mu != 0.0) return h->fail(JR_E_UNSUPPORTED, "the learned walk is NPT (mu = 0)");
   HIPCHK(h, hipEventRecord(L->ev[0], s));
@@ -1131,7 +1149,7 @@ int learned_walk(jr_handle* h) {
                        L->thr_steer.p, L->thr_screen.p);
     // the surrogate's verdict on every chain's current state, with the
     // parameters this iteration uses throughout
-    HIPCHK(h, run_sur(L, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s));
+    HIPCHK(h, run_sur(L, L->C, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s));
     w.ham_sur(nullptr, nullptr, L->Us.p, L->ccell.p, L->cHs.p);
     hipLaunchKernelGGL(lw_stat_add_kernel, dim3(1), dim3(64), 0, s, &L->dstats.p->sur_e,
                        nullptr, (long long)L->C, 1LL);
@@ -1186,6 +1204,128 @@ int learned_walk(jr_handle* h) {
   a.stored = rs.size;
   fit_error_model(L, hs);
   a.heldout_rmse = L->rmse;
+  return JR_OK;
+}
+
+// Trial moves of the learned walk (sampling/mwg.py per_move_fns as the
+// step-size adaptation uses them): move m once from each sampled walker
+// (sample_idx [R][n] on the host) at trial step size tstep [R] under the
+// contours temax [R], with the draws tnorm [R][n][3][Nmax] and tunif
+// [R][n][4] (device), in the walk's mode (the surrogate steers and screens
+// when the walk would); reasons [R][n] to the host (0 = accepted).  The
+// population, the counters, the statistics and the training ring are left
+// as they are.  Passes of at most the walk's chain count C.
+int learned_trial_moves(jr_handle* h, int m, int n, const int32_t* sample_idx,
+                        const double* tstep, const double* temax, const double* tnorm,
+                        const double* tunif, int32_t* reason) {
+  LearnedState* L = h->learned;
+  const auto& c = h->cfg;
+  hipStream_t s = h->stream;
+  const int R = c.n_replicas, Nm = L->Nmax, nm = c.n_moves;
+  for (double mu : h->h_mu)
+    if (mu != 0.0) return h->fail(JR_E_UNSUPPORTED, "the learned walk is NPT (mu = 0)");
+  const size_t C = (size_t)L->C;
+  if (!L->trial_idx.p) {
+    HIPCHK(h, L->trial_idx.alloc(C));
+    HIPCHK(h, L->trial_norm.alloc(C * 3 * Nm));
+    HIPCHK(h, L->trial_unif.alloc(C * 4));
+    HIPCHK(h, L->trial_steps.alloc((size_t)R * nm));
+  }
+  // the step sizes with the trial move's column replaced
+  HIPCHK(h, hipMemcpyAsync(L->trial_steps.p, h->step_sizes.p, sizeof(double) * R * nm,
+                           hipMemcpyDeviceToDevice, s));
+  HIPCHK(h, hipMemcpy2DAsync(L->trial_steps.p + m, sizeof(double) * nm, tstep, sizeof(double),
+                             sizeof(double), R, hipMemcpyDeviceToDevice, s));
+  const jr_move& mv = L->moves[m];
+  const int per = (int)(C / R);  // sampled walkers per replica in one pass
+  std::vector<int> rs(C);
+  L->trial_host.resize(C);
+  for (int j0 = 0; j0 < n; j0 += per) {
+    const int nc = std::min(per, n - j0), Ct = R * nc;
+    for (int r = 0; r < R; ++r)
+      for (int j = 0; j < nc; ++j)
+        L->trial_host[(size_t)r * nc + j] = sample_idx[(size_t)r * n + j0 + j];
+    HIPCHK(h, hipMemcpyAsync(L->trial_idx.p, L->trial_host.data(), sizeof(int) * Ct,
+                             hipMemcpyHostToDevice, s));
+    const size_t row = (size_t)3 * Nm;
+    HIPCHK(h, hipMemcpy2DAsync(L->trial_norm.p, sizeof(double) * row * nc, tnorm + j0 * row,
+                               sizeof(double) * row * n, sizeof(double) * row * nc, R,
+                               hipMemcpyDeviceToDevice, s));
+    HIPCHK(h, hipMemcpy2DAsync(L->trial_unif.p, sizeof(double) * 4 * nc,
+                               tunif + (size_t)j0 * 4, sizeof(double) * 4 * n,
+                               sizeof(double) * 4 * nc, R, hipMemcpyDeviceToDevice, s));
+    LWGatherArgs ga;
+    ga.C = Ct;
+    ga.n_walk = nc;
+    ga.K = c.n_walkers;
+    ga.Nmax = Nm;
+    ga.walk = L->trial_idx.p;
+    ga.pop = PopPtrs{h->pos.p, h->types.p, h->n_atoms.p, h->cell.p, h->energy.p, h->dir.p};
+    ga.emax = temax;
+    ga.pressure = h->pressure.p;
+    ga.cpos = L->cpos.p;
+    ga.cdir = L->cdir.p;
+    ga.ccell = L->ccell.p;
+    ga.cH = L->cH.p;
+    ga.cem = L->cem.p;
+    ga.cpres = L->cpres.p;
+    ga.ctypes = L->ctypes.p;
+    ga.cn = L->cn.p;
+    hipLaunchKernelGGL(lw_gather_kernel, dim3(Ct), dim3(256), 0, s, ga);
+    hipLaunchKernelGGL(lw_contour_kernel, dim3(1), dim3(256), 0, s, Ct, L->cem.p, L->cpres.p,
+                       L->ccell.p, L->cn.p, L->vol_lo, L->vol_hi, L->dcontour.p);
+    double contour_epa = INFINITY;
+    HIPCHK(h, hipMemcpyAsync(&contour_epa, L->dcontour.p, sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    // the walk's rule for screening
+    const bool da = L->mode == JR_LEARNED_SCREENED ||
+                    (L->mode == JR_LEARNED_AUTO && L->rmse <= L->cfg.rmse_bound &&
+                     L->rmse_n >= L->cfg.min_heldout && contour_epa <= L->cfg.train_max_epa);
+    WalkCtx w;
+    w.h = h;
+    w.L = L;
+    w.s = s;
+    w.C = Ct;
+    w.Nm = Nm;
+    w.n_steps = 1;
+    w.use_sur = L->mode != JR_LEARNED_PLAIN;
+    w.cap = L->cfg.train_max_epa;
+    w.ch = LWChains{Ct, nc, Nm, nm, L->cn.p};
+    w.normals = L->trial_norm.p;
+    w.uniforms = L->trial_unif.p;
+    w.step_sizes = L->trial_steps.p;
+    w.trial = true;
+    if (da) {
+      HIPCHK(h, hipMemcpyAsync(L->dbias.p, L->bias.data(), sizeof(double) * L->bias.size(),
+                               hipMemcpyHostToDevice, s));
+      HIPCHK(h, hipMemcpyAsync(L->dsig.p, L->sig.data(), sizeof(double) * L->sig.size(),
+                               hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(lw_thr_kernel, dim3(blocks_for(Ct)), dim3(256), 0, s, Ct, nc,
+                         L->cem.p, L->cn.p, L->dbias.p, L->dsig.p, L->cfg.delta_k,
+                         L->thr_steer.p, L->thr_screen.p);
+      HIPCHK(h, run_sur(L, Ct, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s));
+      w.ham_sur(nullptr, nullptr, L->Us.p, L->ccell.p, L->cHs.p);
+    }
+    hipError_t e = mv.type == JR_MOVE_GMC   ? gmc_step(w, 0, m, mv, da)
+                   : mv.type == JR_MOVE_HMC ? hmc_step(w, 0, m, mv, da)
+                                            : volume_step(w, 0, m, mv, da);
+    if (e != hipSuccess)
+      return h->fail(JR_E_HIP, "learned trial move: %s", hipGetErrorString(e));
+    int of = 0;
+    HIPCHK(h, hipMemcpyAsync(rs.data(), L->reason.p, sizeof(int) * Ct, hipMemcpyDeviceToHost,
+                             s));
+    HIPCHK(h, hipMemcpyAsync(&of, L->overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    if (of) {
+      HIPCHK(h, hipMemsetAsync(L->overflow.p, 0, sizeof(int), s));
+      HIPCHK(h, hipStreamSynchronize(s));
+      return h->fail(JR_E_UNSUPPORTED, "judge neighbour list overflow (> %d)", MP_MAXNB);
+    }
+    for (int r = 0; r < R; ++r)
+      for (int j = 0; j < nc; ++j)
+        reason[(size_t)r * n + j0 + j] = rs[(size_t)r * nc + j];
+  }
   return JR_OK;
 }
 

@@ -1246,8 +1246,6 @@ int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
                    const double* uniforms, int32_t* accepted,
                    int32_t* reason) {
   if (!h) return set_global_err(JR_E_ARG, "NULL handle");
-  if (h->learned)
-    return h->fail(JR_E_UNSUPPORTED, "%s: pair-potential walk only", __func__);
   const auto& c = h->cfg;
   if (!h->have_pop) return h->fail(JR_E_STATE, "population not set");
   if (move_idx < 0 || move_idx >= c.n_moves) return h->fail(JR_E_ARG, "bad move index");
@@ -1287,6 +1285,18 @@ int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
     hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(T * 4)), dim3(256), 0, s,
                        h->t_uniforms.p, (long long)(T * 4), (unsigned long long)c.seed,
                        (const IterState*)nullptr, it, 11u);
+  }
+  if (h->learned) {
+    std::vector<int> rs(T);
+    const int rc = jrl::learned_trial_moves(h, move_idx, n_samples, sample_idx, h->t_step.p,
+                                            h->t_emax.p, h->t_normals.p, h->t_uniforms.p,
+                                            rs.data());
+    if (rc) return rc;
+    for (size_t q = 0; q < T; ++q) {
+      if (accepted) accepted[q] = rs[q] == 0;
+      if (reason) reason[q] = rs[q];
+    }
+    return JR_OK;
   }
   HIPCHK(h, hipMemsetAsync(h->queue.p, 0, sizeof(int), s));
   WalkArgs a = walk_args(h);

@@ -327,3 +327,47 @@ def ns_iteration(pop, models, pressure, step_sizes, moves, tape, it, n_extra, mo
         pop.cell[r, k] = ch["cell"].reshape(-1)
         pop.energy[r, k] = ch["H"]
     return out, stats, records
+
+
+def trial_moves(pop, models, pressure, moves, move_idx, trial_step, emax, sample_idx,
+                normals, uniforms, mode, corr=(0.0, 0.0), bias=None, sig=None, k_slack=2.0):
+    """Trial moves of the learned walk (the step-size adaptation's
+    per-move function, sampling/adaptation/stepsize_handler.py:215-261):
+    move ``move_idx`` once from each sampled walker (sample_idx [R, n]) at
+    trial_step [R] under the contours emax [R], with the draws normals
+    [R, n, 3, Nmax] and uniforms [R, n, 4], in ``mode`` as ns_iteration
+    takes it; ``pop`` is not changed.  Returns the reasons [R, n]
+    (0 = accepted)."""
+    R = pop.pos.shape[0]
+    bias = np.zeros(R) if bias is None else np.asarray(bias, dtype=float)
+    sig = np.zeros(R) if sig is None else np.asarray(sig, dtype=float)
+    da = mode == "screened"
+    use_sur = mode != "plain"
+    mv = moves[move_idx]
+    n = sample_idx.shape[1]
+    reason = np.zeros((R, n), int)
+    stats = dict(proposals=0, stage1_pass=0, accepted=0, exact_fe=0, exact_e=0, sur_e=0)
+    for r in range(R):
+        for j in range(n):
+            k = int(sample_idx[r, j])
+            ch = {"pos": pop.pos[r, k].copy(), "dir": pop.dir[r, k].copy(),
+                  "types": pop.types[r, k], "cell": pop.cell[r, k].reshape(3, 3).copy(),
+                  "H": pop.energy[r, k], "n": int(pop.n_atoms[r, k]), "Hs": None}
+            N = ch["n"]
+            if da:
+                Us = models.sur(ch["pos"][:N], ch["types"][:N], ch["cell"], False)[0]
+                ch["Hs"] = _ham_s(Us, ch["cell"], pressure[r], N, corr)
+            e = emax[r]
+            thr = (e + N * bias[r], e + N * (bias[r] + k_slack * sig[r]))
+            xi = normals[r, j]
+            if mv["type"] == GMC:
+                rs = gmc(ch, mv, trial_step[r], xi, models, e, pressure[r], thr, da, stats,
+                         [], use_sur, corr)[0]
+            elif mv["type"] == HMC:
+                rs = hmc(ch, mv, trial_step[r], xi, models, e, pressure[r], thr, da, stats,
+                         [], use_sur, corr, u0=uniforms[r, j, 0])[0]
+            else:
+                rs = volume(ch, mv, trial_step[r], xi[0, 0], uniforms[r, j, 0], models, e,
+                            pressure[r], thr, da, stats, [], use_sur, corr)[0]
+            reason[r, j] = rs
+    return reason

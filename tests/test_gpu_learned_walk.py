@@ -460,3 +460,77 @@ def test_trained_surrogate_gain():
     assert eff["trained"] > 2.0 * eff["plain"]
     assert rows["trained"]["heldout_rmse"] < 0.7 * rows["untrained"]["heldout_rmse"]
     assert eff["trained"] >= 0.98 * eff["untrained"]
+
+
+def _relaxed_lj_engine(mode, R, K, N, n_extra, T, seed):
+    """The LJ-judge learned engine of the parity test, its walkers relaxed by
+    the pair-potential walk first."""
+    eng, lj = _lj_engine(mode, R, K, N, n_extra, T, seed=seed, pressures=[0.5, 1.0])
+    cell = {"min_volume_per_atom": 1.2, "max_volume_per_atom": 3.0, "flat_V_prior": True}
+    pre = NSEngine(Hamiltonian(lj, [EnsembleTerm(0.5)]), LJ_MOVES,
+                   RunConfig(n_replicas=R, n_walkers=K, n_atoms=N, n_mcmc_steps=T,
+                             n_extra=n_extra, seed=seed + 1, pressures=[0.5, 1.0]), cell=cell)
+    pre.set_population(*(eng.get_population()[k] for k in ("positions", "types", "cell")))
+    pre.ns_iterations(40)
+    rel = pre.get_population()
+    eng.set_population(rel["positions"], rel["types"], rel["cell"], direction=rel["direction"])
+    return eng, lj
+
+
+@pytest.mark.parametrize("mode", ["plain", "screened"])
+def test_learned_trial_moves_match_oracle(mode):
+    """Trial moves of the learned walk (the step-size adaptation's moves):
+    HMC, GMC and volume from sampled walkers at trial step sizes, from host
+    draws, against the oracle's moves; seven samples per replica take three
+    passes of the walk's six chains.  The FP64 pair-potential judge is
+    bit-exact with the oracle's, so every reason must match.  The population,
+    the learned statistics and the training ring stay as they were."""
+    R, K, N, n_extra, T = 2, 4, 32, 2, 4
+    print(f"learned trial moves mode={mode} R={R} K={K} N={N}", flush=True)
+    eng, lj = _relaxed_lj_engine(mode, R, K, N, n_extra, T, seed=21)
+    before = eng.get_population()
+    st0 = eng.learned_stats()
+    pop = Population.from_engine_dict(before)
+    models = OW.Models(Pot.from_table(lj.table), init_surrogate(6, S=1), S=1, base=LJ_BASE)
+    omode = {"plain": "plain", "screened": "screened"}[mode]
+    rng = np.random.default_rng(22)
+    n = 7
+    emax = pop.energy.max(axis=1)
+    seen = set()
+    for m in range(len(LJ_MOVES)):
+        idx = rng.integers(0, K, size=(R, n)).astype(np.int32)
+        nrm = rng.standard_normal((R, n, 3, N))
+        uni = rng.random((R, n, 4))
+        ts = eng.step_sizes[:, m] * np.array([1.0, 1.5])
+        acc, rs = eng.trial_moves(m, ts, idx, emax=emax, normals=nrm, uniforms=uni)
+        ors = OW.trial_moves(pop, models, eng.pressure, _moves_dicts(eng), m, ts, emax, idx,
+                             nrm, uni, omode)
+        print(f"  move {m}: gpu {rs.tolist()} oracle {ors.tolist()}", flush=True)
+        assert np.array_equal(rs, ors), m
+        assert np.array_equal(acc, ors == 0), m
+        seen.update(np.unique(rs).tolist())
+    assert 0 in seen and len(seen) > 1, seen  # accepted and rejected proposals
+    after = eng.get_population()
+    for k in before:
+        assert np.array_equal(before[k], after[k]), k
+    st1 = eng.learned_stats()
+    for k in ("proposals", "accepted", "exact_force_evals", "exact_energy_evals",
+              "surrogate_evals", "stored"):
+        assert st1[k] == st0[k], k
+
+
+def test_learned_run_with_adaptation():
+    """run_ns with step-size adaptation on a learned handle (device draws):
+    the trial moves run, the step sizes move, the population stays finite."""
+    from jaxrens_amd.adaptation import AdaptationConfig
+    from jaxrens_amd.run_loop import run_ns
+    from jaxrens_amd.termination import IterationTermination
+    print("learned run with adaptation", flush=True)
+    eng, _ = _relaxed_lj_engine("auto", 2, 8, 32, 2, 4, seed=31)
+    s0 = eng.step_sizes.copy()
+    res = run_ns(eng, [IterationTermination(6)], adaptation=AdaptationConfig(
+        adjust_interval=3, adjust_n_samples=10, adjust_max_rounds=4), seed=3)
+    assert res.iterations == 6
+    assert len(res.adaptation) >= 2
+    assert not np.array_equal(eng.step_sizes, s0)
+    assert np.all(np.isfinite(eng.get_population()["energy"]))
