@@ -355,7 +355,10 @@ int jr_trial_moves(jr_handle* h, int32_t move_idx, const double* trial_step,
  * Walkers go in launches of max_chunk per replica (<= 0: as many as keep the
  * draws of one launch within 1 GiB).  counters_out [R][n_moves][JR_NCOUNT]
  * (may be NULL) sums this walk's counters.  The NS contour, iteration state
- * and per-iteration outputs are left unchanged. */
+ * and per-iteration outputs are left unchanged.  On a learned-model handle
+ * the walkers take the learned walk's moves (judge-steered, or surrogate-
+ * steered and screened when the walk screens) in passes of at most the walk's
+ * chain count per launch, without statistics or training records. */
 int jr_initial_walk(jr_handle* h, int32_t walklength, const double* emax,
                     const int32_t* schedule, const double* normals,
                     const double* uniforms, int32_t max_chunk,

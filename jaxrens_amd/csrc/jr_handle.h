@@ -144,6 +144,16 @@ int learned_walk(jr_handle* h);
 int learned_trial_moves(jr_handle* h, int m, int n, const int32_t* sample_idx,
                         const double* tstep, const double* temax, const double* tnorm,
                         const double* tunif, int32_t* reason);
+// Chains the learned walk's buffers hold (R n_walk).
+int learned_chains(jr_handle* h);
+// Burn-in walk of the walkers walk [R][nw] (device, R nw <= learned_chains):
+// wl sub-steps of the move schedule sched (host) at the fixed contours emax
+// [R] (device) with the draws normals [R][nw][wl][3][Nmax] and uniforms
+// [R][nw][wl][4] (device); counters added into ctr [R][n_moves][JR_NCOUNT];
+// the walkers are written back.
+int learned_burn(jr_handle* h, int wl, const int* sched, int nw, const int* walk,
+                 const double* emax, const double* normals, const double* uniforms,
+                 int* ctr);
 // Pair-potential judge of the learned walk (jr_lib.hip): model energies U and
 // forces F (NULL: energy only) of the configurations listed in idx (NULL:
 // all B) with the device count (NULL: B); outputs indexed by configuration.

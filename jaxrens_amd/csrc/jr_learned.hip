@@ -756,9 +756,10 @@ struct WalkCtx {
   const double* normals;
   const double* uniforms;
   const double* step_sizes;
-  // trial moves: no counters, statistics or training records, and no
-  // surrogate prediction that only feeds the held-out record
-  bool trial = false;
+  // move counters [R][n_moves][JR_NCOUNT] (NULL: none); the walk statistics;
+  // training records and the surrogate predictions that only feed them
+  int* counters = nullptr;
+  bool stats = true, record = true;
 
   // H of the judge's energies U (all chains, or the listed ones)
   void ham_exact(const int* idx, const int* cnt, const double* U, const double* cell,
@@ -777,7 +778,7 @@ struct WalkCtx {
 // Held-out error and ring append over the chains listed in (idx2, cnt[1]);
 // cell holds the evaluated configurations' cells.
 void record_exact(WalkCtx& w, const int* code, const double* cell) {
-  if (w.trial) return;
+  if (!w.record) return;
   LearnedState* L = w.L;
   hipStream_t s = w.s;
   hipLaunchKernelGGL(lw_compact_kernel, dim3(1), dim3(1024), 0, s, w.C, code, 1, 1, L->U.p,
@@ -792,10 +793,10 @@ void record_exact(WalkCtx& w, const int* code, const double* cell) {
 
 void count_step(WalkCtx& w, int m, const int* code, int eval_const, int eval_code1,
                 int grad_const, int screened) {
-  if (w.trial) return;
+  if (!w.counters) return;
   LWCountArgs a;
   a.C = w.C;
-  a.n_walk = w.h->n_walk;
+  a.n_walk = w.ch.n_walk;
   a.n_moves = w.h->cfg.n_moves;
   a.m = m;
   a.reason = w.L->reason.p;
@@ -803,15 +804,15 @@ void count_step(WalkCtx& w, int m, const int* code, int eval_const, int eval_cod
   a.eval_const = eval_const;
   a.eval_code1 = eval_code1;
   a.grad_const = grad_const;
-  a.counters = w.h->counters.p;
-  a.stats = w.L->dstats.p;
+  a.counters = w.counters;
+  a.stats = w.stats ? w.L->dstats.p : nullptr;
   a.screened = screened;
   hipLaunchKernelGGL(lw_count_kernel, dim3(blocks_for(w.C)), dim3(256), 0, w.s, a);
 }
 
 void stat_add(WalkCtx& w, unsigned long long* dst, const int* count, long long n,
               long long mult) {
-  if (w.trial) return;
+  if (!w.stats) return;
   hipLaunchKernelGGL(lw_stat_add_kernel, dim3(1), dim3(64), 0, w.s, dst, count, n, mult);
 }
 
@@ -868,7 +869,7 @@ hipError_t gmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
   } else {
     hipLaunchKernelGGL(lw_accept_kernel, dim3(gb), dim3(256), 0, s, C, nullptr, L->Hy.p,
                        L->cem.p, L->accept.p, L->reason.p, L->cH.p, nullptr, nullptr);
-    if (w.use_sur && !w.trial) {
+    if (w.use_sur && w.record) {
       // the surrogate's prediction at the judged end point: held-out error
       e = run_sur(L, C, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s);
       if (e != hipSuccess) return e;
@@ -911,7 +912,7 @@ hipError_t volume_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
                      nullptr, 0.0, L->idx.p, L->cnt.p);
   hipError_t e;
   const int* fin = L->code.p;  // codes of the final test
-  if (da || (w.use_sur && !w.trial)) {
+  if (da || (w.use_sur && w.record)) {
     e = run_sur(L, C, L->npos.p, L->ncell.p, L->idx.p, L->cnt.p, L->Us.p, nullptr, s);
     if (e != hipSuccess) return e;
     stat_add(w, &L->dstats.p->sur_e, L->cnt.p, 0, 1);
@@ -991,7 +992,7 @@ hipError_t hmc_step(WalkCtx& w, int t, int m, const jr_move& mv, bool da) {
     stat_add(w, &L->dstats.p->sur_e, nullptr, C, nl + 1);
   } else {
     w.ham_exact(nullptr, nullptr, L->U.p, L->ccell.p, L->Hy.p);
-    if (w.use_sur && !w.trial) {
+    if (w.use_sur && w.record) {
       // the surrogate's prediction at the judged end point: held-out error
       e = run_sur(L, C, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s);
       if (e != hipSuccess) return e;
@@ -1103,6 +1104,7 @@ int learned_walk(jr_handle* h) {
   w.normals = h->normals.p;
   w.uniforms = h->uniforms.p;
   w.step_sizes = h->step_sizes.p;
+  w.counters = h->counters.p;
   for (double mu : h->h_mu)
     if (mu != 0.0) return h->fail(JR_E_UNSUPPORTED, "the learned walk is NPT (mu = 0)");
   HIPCHK(h, hipEventRecord(L->ev[0], s));
@@ -1207,6 +1209,114 @@ int learned_walk(jr_handle* h) {
   return JR_OK;
 }
 
+namespace {
+
+// Gathers the walkers walk [R][nw] (indices within their replica) into the
+// first R nw chains under the contours emax [R], then sets up the walk's
+// screening for them: whether it screens (the walk's rule), and if so the
+// thresholds and the surrogate's verdict on their current states.  One sync.
+int gather_chains(jr_handle* h, int nw, const int* walk, const double* emax, bool* da_out) {
+  LearnedState* L = h->learned;
+  const auto& c = h->cfg;
+  hipStream_t s = h->stream;
+  const int Ct = c.n_replicas * nw;
+  LWGatherArgs ga;
+  ga.C = Ct;
+  ga.n_walk = nw;
+  ga.K = c.n_walkers;
+  ga.Nmax = L->Nmax;
+  ga.walk = walk;
+  ga.pop = PopPtrs{h->pos.p, h->types.p, h->n_atoms.p, h->cell.p, h->energy.p, h->dir.p};
+  ga.emax = emax;
+  ga.pressure = h->pressure.p;
+  ga.cpos = L->cpos.p;
+  ga.cdir = L->cdir.p;
+  ga.ccell = L->ccell.p;
+  ga.cH = L->cH.p;
+  ga.cem = L->cem.p;
+  ga.cpres = L->cpres.p;
+  ga.ctypes = L->ctypes.p;
+  ga.cn = L->cn.p;
+  hipLaunchKernelGGL(lw_gather_kernel, dim3(Ct), dim3(256), 0, s, ga);
+  hipLaunchKernelGGL(lw_contour_kernel, dim3(1), dim3(256), 0, s, Ct, L->cem.p, L->cpres.p,
+                     L->ccell.p, L->cn.p, L->vol_lo, L->vol_hi, L->dcontour.p);
+  double contour_epa = INFINITY;
+  HIPCHK(h, hipMemcpyAsync(&contour_epa, L->dcontour.p, sizeof(double), hipMemcpyDeviceToHost,
+                           s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  const bool da = L->mode == JR_LEARNED_SCREENED ||
+                  (L->mode == JR_LEARNED_AUTO && L->rmse <= L->cfg.rmse_bound &&
+                   L->rmse_n >= L->cfg.min_heldout && contour_epa <= L->cfg.train_max_epa);
+  if (da) {
+    HIPCHK(h, hipMemcpyAsync(L->dbias.p, L->bias.data(), sizeof(double) * L->bias.size(),
+                             hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(L->dsig.p, L->sig.data(), sizeof(double) * L->sig.size(),
+                             hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(lw_thr_kernel, dim3(blocks_for(Ct)), dim3(256), 0, s, Ct, nw, L->cem.p,
+                       L->cn.p, L->dbias.p, L->dsig.p, L->cfg.delta_k, L->thr_steer.p,
+                       L->thr_screen.p);
+    HIPCHK(h, run_sur(L, Ct, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s));
+    hipLaunchKernelGGL(lw_ham_kernel, dim3(blocks_for(Ct)), dim3(256), 0, s, Ct, nullptr,
+                       nullptr, L->Us.p, L->ccell.p, L->cpres.p, L->cn.p, L->ca, L->cv,
+                       L->cHs.p);
+    HIPCHK(h, hipGetLastError());
+  }
+  *da_out = da;
+  return JR_OK;
+}
+
+// A walk context over the first R nw chains with the given draws, step sizes
+// and counters, without statistics or training records.
+WalkCtx side_ctx(jr_handle* h, int nw, int n_steps, const double* normals,
+                 const double* uniforms, const double* step_sizes, int* counters) {
+  LearnedState* L = h->learned;
+  WalkCtx w;
+  w.h = h;
+  w.L = L;
+  w.s = h->stream;
+  w.C = h->cfg.n_replicas * nw;
+  w.Nm = L->Nmax;
+  w.n_steps = n_steps;
+  w.use_sur = L->mode != JR_LEARNED_PLAIN;
+  w.cap = L->cfg.train_max_epa;
+  w.ch = LWChains{w.C, nw, L->Nmax, h->cfg.n_moves, L->cn.p};
+  w.normals = normals;
+  w.uniforms = uniforms;
+  w.step_sizes = step_sizes;
+  w.counters = counters;
+  w.stats = w.record = false;
+  return w;
+}
+
+hipError_t run_step(WalkCtx& w, int t, int m, bool da) {
+  const jr_move& mv = w.L->moves[m];
+  return mv.type == JR_MOVE_GMC   ? gmc_step(w, t, m, mv, da)
+         : mv.type == JR_MOVE_HMC ? hmc_step(w, t, m, mv, da)
+                                  : volume_step(w, t, m, mv, da);
+}
+
+// The judge's neighbour-list overflow flag (read and cleared; one sync).
+int check_overflow(jr_handle* h) {
+  LearnedState* L = h->learned;
+  int of = 0;
+  HIPCHK(h, hipMemcpyAsync(&of, L->overflow.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (!of) return JR_OK;
+  HIPCHK(h, hipMemsetAsync(L->overflow.p, 0, sizeof(int), h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return h->fail(JR_E_UNSUPPORTED, "judge neighbour list overflow (> %d)", MP_MAXNB);
+}
+
+int npt_only(jr_handle* h) {
+  for (double mu : h->h_mu)
+    if (mu != 0.0) return h->fail(JR_E_UNSUPPORTED, "the learned walk is NPT (mu = 0)");
+  return JR_OK;
+}
+
+}  // namespace
+
+int learned_chains(jr_handle* h) { return h->learned->C; }
+
 // Trial moves of the learned walk (sampling/mwg.py per_move_fns as the
 // step-size adaptation uses them): move m once from each sampled walker
 // (sample_idx [R][n] on the host) at trial step size tstep [R] under the
@@ -1222,8 +1332,7 @@ int learned_trial_moves(jr_handle* h, int m, int n, const int32_t* sample_idx,
   const auto& c = h->cfg;
   hipStream_t s = h->stream;
   const int R = c.n_replicas, Nm = L->Nmax, nm = c.n_moves;
-  for (double mu : h->h_mu)
-    if (mu != 0.0) return h->fail(JR_E_UNSUPPORTED, "the learned walk is NPT (mu = 0)");
+  if (int rc = npt_only(h)) return rc;
   const size_t C = (size_t)L->C;
   if (!L->trial_idx.p) {
     HIPCHK(h, L->trial_idx.alloc(C));
@@ -1236,7 +1345,6 @@ int learned_trial_moves(jr_handle* h, int m, int n, const int32_t* sample_idx,
                            hipMemcpyDeviceToDevice, s));
   HIPCHK(h, hipMemcpy2DAsync(L->trial_steps.p + m, sizeof(double) * nm, tstep, sizeof(double),
                              sizeof(double), R, hipMemcpyDeviceToDevice, s));
-  const jr_move& mv = L->moves[m];
   const int per = (int)(C / R);  // sampled walkers per replica in one pass
   std::vector<int> rs(C);
   L->trial_host.resize(C);
@@ -1254,79 +1362,62 @@ int learned_trial_moves(jr_handle* h, int m, int n, const int32_t* sample_idx,
     HIPCHK(h, hipMemcpy2DAsync(L->trial_unif.p, sizeof(double) * 4 * nc,
                                tunif + (size_t)j0 * 4, sizeof(double) * 4 * n,
                                sizeof(double) * 4 * nc, R, hipMemcpyDeviceToDevice, s));
-    LWGatherArgs ga;
-    ga.C = Ct;
-    ga.n_walk = nc;
-    ga.K = c.n_walkers;
-    ga.Nmax = Nm;
-    ga.walk = L->trial_idx.p;
-    ga.pop = PopPtrs{h->pos.p, h->types.p, h->n_atoms.p, h->cell.p, h->energy.p, h->dir.p};
-    ga.emax = temax;
-    ga.pressure = h->pressure.p;
-    ga.cpos = L->cpos.p;
-    ga.cdir = L->cdir.p;
-    ga.ccell = L->ccell.p;
-    ga.cH = L->cH.p;
-    ga.cem = L->cem.p;
-    ga.cpres = L->cpres.p;
-    ga.ctypes = L->ctypes.p;
-    ga.cn = L->cn.p;
-    hipLaunchKernelGGL(lw_gather_kernel, dim3(Ct), dim3(256), 0, s, ga);
-    hipLaunchKernelGGL(lw_contour_kernel, dim3(1), dim3(256), 0, s, Ct, L->cem.p, L->cpres.p,
-                       L->ccell.p, L->cn.p, L->vol_lo, L->vol_hi, L->dcontour.p);
-    double contour_epa = INFINITY;
-    HIPCHK(h, hipMemcpyAsync(&contour_epa, L->dcontour.p, sizeof(double),
-                             hipMemcpyDeviceToHost, s));
-    HIPCHK(h, hipStreamSynchronize(s));
-    // the walk's rule for screening
-    const bool da = L->mode == JR_LEARNED_SCREENED ||
-                    (L->mode == JR_LEARNED_AUTO && L->rmse <= L->cfg.rmse_bound &&
-                     L->rmse_n >= L->cfg.min_heldout && contour_epa <= L->cfg.train_max_epa);
-    WalkCtx w;
-    w.h = h;
-    w.L = L;
-    w.s = s;
-    w.C = Ct;
-    w.Nm = Nm;
-    w.n_steps = 1;
-    w.use_sur = L->mode != JR_LEARNED_PLAIN;
-    w.cap = L->cfg.train_max_epa;
-    w.ch = LWChains{Ct, nc, Nm, nm, L->cn.p};
-    w.normals = L->trial_norm.p;
-    w.uniforms = L->trial_unif.p;
-    w.step_sizes = L->trial_steps.p;
-    w.trial = true;
-    if (da) {
-      HIPCHK(h, hipMemcpyAsync(L->dbias.p, L->bias.data(), sizeof(double) * L->bias.size(),
-                               hipMemcpyHostToDevice, s));
-      HIPCHK(h, hipMemcpyAsync(L->dsig.p, L->sig.data(), sizeof(double) * L->sig.size(),
-                               hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(lw_thr_kernel, dim3(blocks_for(Ct)), dim3(256), 0, s, Ct, nc,
-                         L->cem.p, L->cn.p, L->dbias.p, L->dsig.p, L->cfg.delta_k,
-                         L->thr_steer.p, L->thr_screen.p);
-      HIPCHK(h, run_sur(L, Ct, L->cpos.p, L->ccell.p, nullptr, nullptr, L->Us.p, nullptr, s));
-      w.ham_sur(nullptr, nullptr, L->Us.p, L->ccell.p, L->cHs.p);
-    }
-    hipError_t e = mv.type == JR_MOVE_GMC   ? gmc_step(w, 0, m, mv, da)
-                   : mv.type == JR_MOVE_HMC ? hmc_step(w, 0, m, mv, da)
-                                            : volume_step(w, 0, m, mv, da);
+    bool da = false;
+    if (int rc = gather_chains(h, nc, L->trial_idx.p, temax, &da)) return rc;
+    WalkCtx w = side_ctx(h, nc, 1, L->trial_norm.p, L->trial_unif.p, L->trial_steps.p,
+                         nullptr);
+    hipError_t e = run_step(w, 0, m, da);
     if (e != hipSuccess)
       return h->fail(JR_E_HIP, "learned trial move: %s", hipGetErrorString(e));
-    int of = 0;
     HIPCHK(h, hipMemcpyAsync(rs.data(), L->reason.p, sizeof(int) * Ct, hipMemcpyDeviceToHost,
                              s));
-    HIPCHK(h, hipMemcpyAsync(&of, L->overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(h, hipStreamSynchronize(s));
-    if (of) {
-      HIPCHK(h, hipMemsetAsync(L->overflow.p, 0, sizeof(int), s));
-      HIPCHK(h, hipStreamSynchronize(s));
-      return h->fail(JR_E_UNSUPPORTED, "judge neighbour list overflow (> %d)", MP_MAXNB);
-    }
+    if (int rc = check_overflow(h)) return rc;
     for (int r = 0; r < R; ++r)
       for (int j = 0; j < nc; ++j)
         reason[(size_t)r * n + j0 + j] = rs[(size_t)r * nc + j];
   }
   return JR_OK;
+}
+
+// Burn-in walk of the learned handle (init/burn_in.py:48-148) for the
+// walkers walk [R][nw] (device, nw <= C / R): wl sub-steps with the move
+// schedule sched (host) at the fixed contours emax [R], the draws normals
+// [R][nw][wl][3][Nmax] and uniforms [R][nw][wl][4] and the handle's step
+// sizes, as the walk takes them (judge-steered, or surrogate-steered and
+// screened when the walk screens); move counters added into ctr
+// [R][n_moves][JR_NCOUNT]; the walkers are written back.  No statistics or
+// training records.
+int learned_burn(jr_handle* h, int wl, const int* sched, int nw, const int* walk,
+                 const double* emax, const double* normals, const double* uniforms,
+                 int* ctr) {
+  LearnedState* L = h->learned;
+  if (int rc = npt_only(h)) return rc;
+  if (nw < 1 || (size_t)h->cfg.n_replicas * nw > (size_t)L->C)
+    return h->fail(JR_E_ARG, "burn-in pass of %d walkers per replica exceeds the chains", nw);
+  bool da = false;
+  if (int rc = gather_chains(h, nw, walk, emax, &da)) return rc;
+  WalkCtx w = side_ctx(h, nw, wl, normals, uniforms, h->step_sizes.p, ctr);
+  for (int t = 0; t < wl; ++t) {
+    hipError_t e = run_step(w, t, sched[t], da);
+    if (e != hipSuccess)
+      return h->fail(JR_E_HIP, "learned burn-in step %d: %s", t, hipGetErrorString(e));
+  }
+  LWGatherArgs ga;
+  std::memset(&ga, 0, sizeof(ga));
+  ga.C = w.C;
+  ga.n_walk = nw;
+  ga.K = h->cfg.n_walkers;
+  ga.Nmax = L->Nmax;
+  ga.walk = walk;
+  ga.pop = PopPtrs{h->pos.p, h->types.p, h->n_atoms.p, h->cell.p, h->energy.p, h->dir.p};
+  ga.cpos = L->cpos.p;
+  ga.cdir = L->cdir.p;
+  ga.ccell = L->ccell.p;
+  ga.cH = L->cH.p;
+  ga.cn = L->cn.p;
+  hipLaunchKernelGGL(lw_scatter_kernel, dim3(w.C), dim3(256), 0, h->stream, ga);
+  HIPCHK(h, hipGetLastError());
+  return check_overflow(h);
 }
 
 }  // namespace jrl

@@ -1329,8 +1329,6 @@ int jr_initial_walk(jr_handle* h, int32_t walklength, const double* emax,
                     const double* uniforms, int32_t max_chunk,
                     int32_t* counters_out) {
   if (!h) return set_global_err(JR_E_ARG, "NULL handle");
-  if (h->learned)
-    return h->fail(JR_E_UNSUPPORTED, "%s: pair-potential walk only", __func__);
   const auto& c = h->cfg;
   if (!h->have_pop) return h->fail(JR_E_STATE, "population not set");
   if (walklength < 1 || !emax) return h->fail(JR_E_ARG, "need walklength >= 1 and emax");
@@ -1350,6 +1348,8 @@ int jr_initial_walk(jr_handle* h, int32_t walklength, const double* emax,
   int chunk = (int)std::min<size_t>(
       (size_t)K, std::max<size_t>(1, cap / ((size_t)R * per_walker)));
   if (max_chunk > 0) chunk = std::min(chunk, (int)max_chunk);
+  // the learned walk takes passes of at most the chains its buffers hold
+  if (h->learned) chunk = std::max(1, std::min(chunk, jrl::learned_chains(h) / R));
   if (h->b_chunk != chunk || h->b_steps != wl) {
     HIPCHK(h, h->b_normals.alloc((size_t)R * chunk * per_walker));
     HIPCHK(h, h->b_uniforms.alloc((size_t)R * chunk * wl * 4));
@@ -1372,6 +1372,14 @@ int jr_initial_walk(jr_handle* h, int32_t walklength, const double* emax,
     hipLaunchKernelGGL(schedule_kernel, dim3(1), dim3(64), 0, s, h->b_sched.p, wl,
                        (unsigned long long)c.seed, (const IterState*)nullptr,
                        (long long)h->burn_rng, 14u, h->cum.p, c.n_moves);
+  }
+  // the learned walk dispatches its launches by move type from the host
+  std::vector<int> sched_host;
+  if (h->learned) {
+    sched_host.resize(wl);
+    HIPCHK(h, hipMemcpyAsync(sched_host.data(), h->b_sched.p, sizeof(int) * wl,
+                             hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
   }
   std::vector<int> widx((size_t)R * chunk);
   for (int k0 = 0; k0 < K; k0 += chunk) {
@@ -1401,6 +1409,12 @@ int jr_initial_walk(jr_handle* h, int32_t walklength, const double* emax,
       hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(nu)), dim3(256), 0, s,
                          h->b_uniforms.p, (long long)nu, (unsigned long long)c.seed,
                          (const IterState*)nullptr, it * 4096 + k0 / chunk, 13u);
+    }
+    if (h->learned) {
+      const int rc = jrl::learned_burn(h, wl, sched_host.data(), ck, h->b_walk.p, h->t_emax.p,
+                                       h->b_normals.p, h->b_uniforms.p, h->b_ctr.p);
+      if (rc) return rc;
+      continue;
     }
     HIPCHK(h, hipMemsetAsync(h->queue.p, 0, sizeof(int), s));
     WalkArgs a = walk_args(h);

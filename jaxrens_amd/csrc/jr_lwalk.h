@@ -460,6 +460,7 @@ __global__ void lw_count_kernel(LWCountArgs a) {
   const int ne = a.eval_const + (k == 1 ? a.eval_code1 : 0);
   if (ne) atomicAdd(ct + 5, ne);
   if (a.grad_const) atomicAdd(ct + 6, a.grad_const);
+  if (!a.stats) return;
   atomicAdd(&a.stats->proposals, 1ull);
   if (rs == 0) atomicAdd(&a.stats->accepted, 1ull);
   if (a.screened && k == 1) atomicAdd(&a.stats->stage1_pass, 1ull);

@@ -371,3 +371,52 @@ def trial_moves(pop, models, pressure, moves, move_idx, trial_step, emax, sample
                             pressure[r], thr, da, stats, [], use_sur, corr)[0]
             reason[r, j] = rs
     return reason
+
+
+def initial_walk(pop, models, pressure, step_sizes, moves, emax, schedule, normals, uniforms,
+                 mode, corr=(0.0, 0.0), bias=None, sig=None, k_slack=2.0):
+    """Burn-in walk of the learned walk (init/burn_in.py:48-148): every
+    walker of replica r takes the sub-steps of ``schedule`` [wl] at the
+    fixed contour emax[r] with the draws normals [R, K, wl, 3, Nmax] and
+    uniforms [R, K, wl, 4], in ``mode`` as ns_iteration takes it; ``pop`` is
+    walked in place.  Returns the counters [R, n_moves, 7]."""
+    R, K = pop.pos.shape[:2]
+    bias = np.zeros(R) if bias is None else np.asarray(bias, dtype=float)
+    sig = np.zeros(R) if sig is None else np.asarray(sig, dtype=float)
+    da = mode == "screened"
+    use_sur = mode != "plain"
+    counters = np.zeros((R, len(moves), 7), np.int64)
+    stats = dict(proposals=0, stage1_pass=0, accepted=0, exact_fe=0, exact_e=0, sur_e=0)
+    for r in range(R):
+        for k in range(K):
+            ch = {"pos": pop.pos[r, k].copy(), "dir": pop.dir[r, k].copy(),
+                  "types": pop.types[r, k], "cell": pop.cell[r, k].reshape(3, 3).copy(),
+                  "H": pop.energy[r, k], "n": int(pop.n_atoms[r, k]), "Hs": None}
+            N = ch["n"]
+            if da:
+                Us = models.sur(ch["pos"][:N], ch["types"][:N], ch["cell"], False)[0]
+                ch["Hs"] = _ham_s(Us, ch["cell"], pressure[r], N, corr)
+            e = emax[r]
+            thr = (e + N * bias[r], e + N * (bias[r] + k_slack * sig[r]))
+            for t, m in enumerate(schedule):
+                mv = moves[m]
+                xi = normals[r, k, t]
+                if mv["type"] == GMC:
+                    rs, ne, ng = gmc(ch, mv, step_sizes[r, m], xi, models, e, pressure[r], thr,
+                                     da, stats, [], use_sur, corr)
+                elif mv["type"] == HMC:
+                    rs, ne, ng = hmc(ch, mv, step_sizes[r, m], xi, models, e, pressure[r], thr,
+                                     da, stats, [], use_sur, corr, u0=uniforms[r, k, t, 0])
+                else:
+                    rs, ne, ng = volume(ch, mv, step_sizes[r, m], xi[0, 0], uniforms[r, k, t, 0],
+                                        models, e, pressure[r], thr, da, stats, [], use_sur,
+                                        corr)
+                cnt = counters[r, m]
+                cnt[0] += 1
+                cnt[1 + rs] += 1
+                cnt[5] += ne
+                cnt[6] += ng
+            pop.pos[r, k], pop.dir[r, k] = ch["pos"], ch["dir"]
+            pop.cell[r, k] = ch["cell"].reshape(-1)
+            pop.energy[r, k] = ch["H"]
+    return counters

@@ -534,3 +534,43 @@ def test_learned_run_with_adaptation():
     assert len(res.adaptation) >= 2
     assert not np.array_equal(eng.step_sizes, s0)
     assert np.all(np.isfinite(eng.get_population()["energy"]))
+
+
+@pytest.mark.parametrize("mode", ["plain", "screened"])
+def test_learned_burn_in_matches_oracle(mode):
+    """The burn-in walk on a learned handle (init/burn_in.py:48-148): every
+    walker takes three sub-steps of one schedule at fixed contours, in passes
+    of the walk's six chains (eight walkers per replica: 3 + 3 + 2), against
+    the oracle's walk with the FP64 pair-potential judge: counters exactly,
+    the walked population to rounding; the learned statistics untouched.
+    The surrogate (BF16 rows) matches its oracle only to rounding, and an
+    untrained one drives HMC's leapfrog far enough that the rounding decides
+    its stage-1 verdicts, so the screened walk takes gmc and volume moves
+    only, well inside the contour."""
+    R, K, N, n_extra, T, wl = 2, 8, 32, 2, 4, 3
+    print(f"learned burn-in mode={mode} R={R} K={K} N={N} walklength={wl}", flush=True)
+    eng, lj = _relaxed_lj_engine(mode, R, K, N, n_extra, T, seed=41)
+    pop0 = Population.from_engine_dict(eng.get_population())
+    st0 = eng.learned_stats()
+    rng = np.random.default_rng(42)
+    sched = rng.integers(0 if mode == "plain" else 1, len(LJ_MOVES), size=wl).astype(np.int32)
+    nrm = rng.standard_normal((R, K, wl, 3, N))
+    uni = rng.random((R, K, wl, 4))
+    emax = pop0.energy.max(axis=1) + (0.5 if mode == "plain" else 50.0)
+    ctr = eng.initial_walk(wl, emax, schedule=sched, normals=nrm, uniforms=uni)
+    pop_gpu = eng.get_population()
+    models = OW.Models(Pot.from_table(lj.table), init_surrogate(6, S=1), S=1, base=LJ_BASE)
+    octr = OW.initial_walk(pop0, models, eng.pressure, eng.step_sizes, _moves_dicts(eng), emax,
+                           sched, nrm, uni, mode)
+    print("  counters gpu", ctr.tolist(), flush=True)
+    print("  counters ora", octr.tolist(), flush=True)
+    assert np.array_equal(ctr, octr)
+    assert ctr[:, :, 1].sum() > 0  # some moves accepted
+    L = pop_gpu["cell"][..., 0, 0].max()
+    np.testing.assert_allclose(pop_gpu["positions"], pop0.pos, atol=1e-6 * L, rtol=0)
+    np.testing.assert_allclose(pop_gpu["cell"].reshape(R, K, 9), pop0.cell, rtol=1e-12)
+    np.testing.assert_allclose(pop_gpu["energy"], pop0.energy,
+                               atol=1e-9 * (np.abs(pop0.energy).max() + N))
+    st1 = eng.learned_stats()
+    for k in ("proposals", "accepted", "exact_force_evals", "exact_energy_evals", "stored"):
+        assert st1[k] == st0[k], k
