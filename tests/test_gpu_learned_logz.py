@@ -7,12 +7,15 @@ schedule is shared inside an engine), compared by Welch's t over the engine
 means, |t| < 3 (about 1 % two-sided at these counts).
 
 The 256-atom configuration (its judge, density and cell bounds): 8 engines
-per walk of 8 replicas (P = 0.5, no exchange), K = 8, 8 sub-steps
+(5 unless JR_LOGZ_FULL=1, which reproduces the recorded run; every test
+then stays far below three minutes) per walk of 8 replicas (P = 0.5, no exchange), K = 8, 8 sub-steps
 gmc(n_reflect 4):volume 3:1, 100 iterations, each engine from its own
 jittered lattice; plain against screened with the surrogate training online
 from a seeded network, on a generic r^-12 wall baseline of its own (not the
 judge's core term).  Prints a ``LOGZ_RESULT`` JSON line with delta log Z,
 sigma and t."""
+
+import os
 
 import numpy as np
 import pytest
@@ -25,7 +28,8 @@ from logz_stats import report
 
 pytestmark = pytest.mark.gpu
 
-M, R, K, N_ITER = 8, 8, 8, 100
+FULL = os.environ.get("JR_LOGZ_FULL") == "1"
+M, R, K, N_ITER = (8 if FULL else 5), 8, 8, 100
 T_MAX = 3.0
 # the surrogate's own wall: eps ((sig/r)^12 - (sig/rc)^12) below rc
 GENERIC_WALL = (1.0, 0.8, 1.0)
@@ -81,15 +85,15 @@ def test_lj128_screened_hmc_matches_random_walk_log_z():
     of HMC + GMC + volume (screened in every iteration, online training; the
     screened HMC is the exact constrained-prior move) against the plain
     pair-potential walk
-    of GMC + volume on the same system: 6 engines per walk of 8 replicas
+    of GMC + volume on the same system: 6 engines (4 unless JR_LOGZ_FULL=1)
+    per walk of 8 replicas
     (P = 1, no exchange), K = 8, 120 iterations, Welch's t over engine
     means, |t| < 3.  The walks are long enough to decorrelate a 128-atom
     walker between iterations (96 sub-steps, galilean step scaled with
     sqrt(N) from the 8-atom test's); at a few dozen sub-steps both walks
     leave their clones correlated and their log Z differ for that reason."""
     from jaxrens_amd.backends import LJBackend
-    import os
-    N, P, K, R, M_E, N_IT = 128, 1.0, 8, 8, 6, 120
+    N, P, K, R, M_E, N_IT = 128, 1.0, 8, 8, (6 if FULL else 4), 120
     T = int(os.environ.get("JR_LJ128_STEPS", "96"))
     cell = {"min_volume_per_atom": 4.0, "max_volume_per_atom": 40.0, "flat_V_prior": True}
     hmc = {"type": "hmc", "n_leapfrog": 5, "step_size": 0.01, "weight": 1}
