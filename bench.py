@@ -126,7 +126,7 @@ def iteration_distribution(eng, n_iter: int) -> dict:
             "note": "one ns_iterations(1) call per iteration, measured after the timed region"}
 
 
-def config_throughput(n_iter_c1: int = 400, n_iter_c3: int = 20) -> dict:
+def config_throughput(n_iter_c1: int = 400, n_iter_c3: int = 20, cpu: bool = True) -> dict:
     """Throughput of the other synthetic configurations, whole iterations
     replayed from a captured hipGraph: LJ-8 NPT (P = 1, K = 128, n_extra = 0,
     the reference's defaults) and the binary semi-grand muPT batch (N in
@@ -155,6 +155,10 @@ def config_throughput(n_iter_c1: int = 400, n_iter_c3: int = 20) -> dict:
         "ns_iterations_per_s": n_iter_c1 / d1, "iterations": n_iter_c1,
         "iteration_tflops": f1, "frac_of_fp64_peak": f1 / FP64_PEAK_TFLOPS,
         "note": "one chain of 8 atoms per iteration: latency-bound, 8 of 64 lanes busy"}
+    if cpu:
+        # the same configuration on one host core (its one chain per
+        # iteration leaves nothing for more threads)
+        out["lj8_npt"]["cpu_baseline"] = cpu_baseline(e1, 1, 1, n_iter=200)
     e1.close()
     # binary semi-grand muPT, uneven sizes in one persistent launch
     R, K = 20, 512
@@ -215,7 +219,7 @@ def cpu_baseline(eng, n_replicas_sample: int, threads: int, n_iter: int = 1) -> 
             "kind": "port",
             "sample": (f"{n_iter} NS iteration(s) of {Rs} of {eng.n_replicas} replicas "
                        f"x {eng.n_walk} walked chains x {T} sub-steps = {ws} "
-                       f"walker-steps in {dt:.1f} s; FP64 C oracle, OpenMP "
+                       f"walker-steps in {dt:.3g} s; FP64 C oracle, OpenMP "
                        f"{threads} threads"),
             "flops_per_s": flops / dt}
 
@@ -564,7 +568,7 @@ def main():
             "frac_of_fp64_peak": f32 / FP64_PEAK_TFLOPS, "launch": t32,
             "note": "float32 pair terms, float64 minimum image and accumulation"}
         e32.close()
-        secondary.update(config_throughput())
+        secondary.update(config_throughput(cpu=not args.no_cpu_baseline))
 
     surrogate = None
     if not args.skip_surrogate:
