@@ -220,3 +220,118 @@ def test_volume_codes_and_prior():
     assert rs == 0 and ne == 1 and m.judge_calls == 1
     np.testing.assert_allclose(ch["cell"], c0 * s, rtol=1e-15)
     np.testing.assert_allclose(ch["pos"], x0 * s, rtol=1e-15)
+
+
+def _flat_pop(oracle, rng, R=2, K=3, N=5, L=6.0, H=0.0):
+    return oracle.Population(rng.random((R, K, N, 3)) * L, np.zeros((R, K, N), np.int32),
+                             np.full((R, K), N, np.int32), np.tile(np.eye(3) * L, (R, K, 1, 1)),
+                             np.full((R, K), H), np.zeros((R, K, N, 3)))
+
+
+VOL = {"type": 1, "flat_v_prior": True, "max_vpa": 64.0, "min_vpa": 0.125, "min_aspect": 0.8}
+
+
+def test_trial_moves_decide_without_touching_the_population(oracle):
+    """Trial moves with a flat judge: every GMC proposal is accepted under a
+    contour above the judge's energy and rejected (reason 1) under one below,
+    a volume step that leaves the cell bounds is a cell reject (reason 2),
+    and the population is left as it was; a walker sampled twice with the
+    same draws gets the same reason."""
+    rng = np.random.default_rng(5)
+    pop = _flat_pop(oracle, rng)
+    before = pop.pos.copy(), pop.dir.copy(), pop.energy.copy()
+    R, n, N = 2, 4, 5
+    idx = np.array([[0, 1, 1, 2], [2, 2, 0, 1]], np.int32)
+    nrm = rng.standard_normal((R, n, 3, N))
+    nrm[:, 2] = nrm[:, 1]
+    uni = rng.random((R, n, 4))
+    P = np.zeros(2)  # H = U: the flat judge sits at 0
+    m = Flat()
+    for emax, want in ((1.0, 0), (-1.0, 1)):
+        rs = OW.trial_moves(pop, m, P, [GMC], 0, np.array([0.1, 0.2]),
+                            np.array([emax, emax]), idx, nrm, uni, "plain")
+        assert np.all(rs == want)
+    # a volume step of 1e4 per atom leaves the cell bounds
+    nrm[..., 0, 0] = 1.0
+    rs = OW.trial_moves(pop, m, P, [GMC, VOL], 1, np.array([1e4, 1e4]),
+                        np.array([1.0, 1.0]), idx, nrm, uni, "plain")
+    assert np.all(rs == 2)
+    assert rs[0, 1] == rs[0, 2]
+    for a, b in zip(before, (pop.pos, pop.dir, pop.energy)):
+        assert np.array_equal(a, b)
+
+
+def test_initial_walk_is_the_moves_in_schedule_order(oracle):
+    """The burn-in walk with a flat judge above the contour: every GMC step
+    is the straight line of the normalised draw (fresh at a zero direction,
+    then perturbed), applied to each walker in schedule order, and the
+    counters count every proposal as accepted; one sub-step from a walker
+    equals that walker's trial move from the same state and draws."""
+    rng = np.random.default_rng(6)
+    R, K, N, wl = 2, 3, 5, 3
+    pop = _flat_pop(oracle, rng, R, K, N)
+    pop0 = _flat_pop(oracle, np.random.default_rng(6), R, K, N)
+    nrm = rng.standard_normal((R, K, wl, 3, N))
+    uni = rng.random((R, K, wl, 4))
+    steps = np.array([[0.1], [0.2]])
+    emax = np.array([1.0, 1.0])
+    P = np.zeros(2)  # H = U: the flat judge sits at 0
+    ctr = OW.initial_walk(pop, Flat(), P, steps, [GMC], emax, np.zeros(wl, np.int32), nrm,
+                          uni, "plain")
+    assert np.array_equal(ctr[:, 0, :2], np.full((R, 2), K * wl))
+    for r in range(R):
+        for k in range(K):
+            x, d = pop0.pos[r, k].copy(), np.zeros((N, 3))
+            for t in range(wl):
+                z = nrm[r, k, t].T
+                v = z if np.sqrt(np.sum(d * d)) < 1e-8 else d + GMC["perturb_angle"] * z
+                d = v / max(np.sqrt(np.sum(v * v)), 1e-10)
+                for _ in range(GMC["n_reflect"]):
+                    x = x + steps[r, 0] * d
+            np.testing.assert_allclose(pop.pos[r, k], x, rtol=0, atol=1e-12)
+    # one sub-step equals the trial move of each walker
+    pop1 = _flat_pop(oracle, np.random.default_rng(6), R, K, N)
+    idx = np.tile(np.arange(K, dtype=np.int32), (R, 1))
+    rs = OW.trial_moves(pop1, Flat(), P, [GMC], 0, steps[:, 0], np.array([-1.0, 1.0]), idx,
+                        nrm[:, :, 0], uni[:, :, 0], "plain")
+    c1 = OW.initial_walk(pop1, Flat(), P, steps, [GMC], np.array([-1.0, 1.0]),
+                         np.zeros(1, np.int32), nrm[:, :, :1], uni[:, :, :1], "plain")
+    assert np.array_equal(c1[:, 0, 1], (rs == 0).sum(axis=1))
+    assert np.array_equal(c1[:, 0, 2], (rs == 1).sum(axis=1))
+
+
+class Tilted(Flat):
+    """Stand-in judge E = sum of the x coordinates, zero forces."""
+
+    def judge(self, pos, ty, cell, forces):
+        self.judge_calls += 1
+        return float(np.sum(pos[:, 0])), (np.zeros_like(pos) if forces else None)
+
+
+def test_trial_moves_take_the_sampled_walker_and_its_draws(oracle):
+    """With E = sum x and no forces each trial GMC move ends at
+    x0 + n_reflect s d for the sampled walker's x0 and the sample's own draw,
+    so the reason follows from that end point against the contour; a wrong
+    walker or draw index changes it."""
+    rng = np.random.default_rng(8)
+    R, K, N, n = 2, 6, 5, 8
+    pop = _flat_pop(oracle, rng, R, K, N)
+    idx = rng.integers(0, K, size=(R, n)).astype(np.int32)
+    nrm = rng.standard_normal((R, n, 3, N))
+    uni = rng.random((R, n, 4))
+    steps = np.array([0.3, 0.5])
+    emax = np.array([np.median(pop.pos[0, :, :, 0].sum(-1)),
+                     np.median(pop.pos[1, :, :, 0].sum(-1))])
+    rs = OW.trial_moves(pop, Tilted(), np.zeros(2), [GMC], 0, steps, emax, idx, nrm, uni,
+                        "plain")
+    want = np.zeros((R, n), int)
+    for r in range(R):
+        for j in range(n):
+            z = nrm[r, j].T
+            d = z / max(np.sqrt(np.sum(z * z)), 1e-10)
+            x = pop.pos[r, idx[r, j]].copy()
+            for _ in range(GMC["n_reflect"]):
+                x = x + steps[r] * d
+            want[r, j] = 0 if np.sum(x[:, 0]) < emax[r] else 1
+    assert 0 < (want == 0).sum() < want.size  # both outcomes occur
+    assert np.array_equal(rs, want)
